@@ -1,0 +1,113 @@
+"""Oracle: CPU restatement of the reference's VTI path (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker.  The product (libvtiamd.so and its
+binding) never loads it.
+
+Parity status per function (what pins it) is documented in PINS below and
+checked by tests/test_oracle_*.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import c_double, c_float, c_int, c_long, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+F32, REF, F64 = 0, 1, 2
+
+# What pins each oracle function (the checks live in tests/).
+PINS = {
+    "orc3d_init_damping": "bit-exact vs compiled reference abc3d.cu host code (tests/golden) + closed form",
+    "orc3d_boundary": "bit-exact vs compiled reference boundary.cpp (tests/golden) + round trip",
+    "orc3d_forward": "FP64 mode: plane-wave dispersion closed form, time reversibility, energy; "
+    "REF/F32 modes within 1e-5 of FP64",
+    "orc3d_gradient": "FP64 mode: adjoint-state Taylor test of J; F32/REF within 1e-4 of FP64",
+    "orc3d_rtm": "FP64 mode: linearity in the data, F32/REF within 1e-4 of FP64",
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built at {LIB_PATH}; run make -C oracle")
+        L = ctypes.CDLL(LIB_PATH)
+        P = c_void_p
+        L.orc3d_init_damping.argtypes = [P, c_int, c_int, c_int, c_int]
+        L.orc3d_boundary.argtypes = [c_int, P, P, c_int, c_int, c_int, c_int]
+        L.orc3d_nboundary.restype = c_long
+        L.orc3d_nboundary.argtypes = [c_int] * 4
+        L.orc3d_lmax.argtypes = [c_int, c_int]
+        L.orc3d_forward.argtypes = [
+            c_int, P, P, P, c_int, c_int, c_int, c_int, c_float, c_float, c_float, c_float, c_int, P, c_long,
+            c_int, P, c_int, c_int, c_int, P, P, P, P, P, P,
+        ]
+        L.orc3d_steps_f32.argtypes = [P, P, P, P, P, P, P] + [c_int] * 4 + [c_float] * 4 + [c_int, c_int]
+        L.orc3d_gradient.restype = c_double
+        L.orc3d_gradient.argtypes = (
+            [c_int, P, P, P, P, c_int, P, P, P, P, P]
+            + [c_int] * 5
+            + [c_float] * 9
+            + [c_int, P]
+        )
+        L.orc3d_rtm.argtypes = [c_int, P, P, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 7 + [c_int, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def init_damping3d(nx, ny, nz, npad):
+    d = np.zeros(nx * ny + nz, np.float32)
+    lib().orc3d_init_damping(_p(d), nx, ny, nz, npad)
+    return d
+
+
+def lmax3d(nt, ngpu=1):
+    return lib().orc3d_lmax(nt, ngpu)
+
+
+def forward3d(mode, c11, c13, c33, npad, dx, dy, dz, dt, lmax, wavelet, sou_index, rec, sts, nnt,
+              absorbing=True, want_state=False):
+    """Returns (traces float32 (nr, nnt), traces float64 (nr, nnt), state or None).
+
+    state = (cur_x, cur_z, prev_x, prev_z) as stored by the reference's
+    block pipeline (cur damped once, prev damped twice in absorbing runs)."""
+    nz, ny, nx = c11.shape
+    c11, c13, c33 = (np.ascontiguousarray(a, np.float32) for a in (c11, c13, c33))
+    w = np.ascontiguousarray(wavelet, np.float32)
+    rec = np.ascontiguousarray(rec, np.int64)
+    tr = np.zeros((rec.size, nnt), np.float32)
+    tr64 = np.zeros((rec.size, nnt), np.float64)
+    st = None
+    sp = [None] * 4
+    if want_state:
+        dt_ = np.float64 if mode == F64 else np.float32
+        st = [np.zeros((nz, ny, nx), dt_) for _ in range(4)]
+        sp = [_p(a) for a in st]
+    rc = lib().orc3d_forward(
+        mode, _p(c11), _p(c13), _p(c33), nx, ny, nz, npad, dx, dy, dz, dt, lmax, _p(w), int(sou_index), rec.size,
+        _p(rec), sts, nnt, int(absorbing), _p(tr), _p(tr64), *sp,
+    )
+    if rc != 0:
+        raise RuntimeError(f"orc3d_forward failed {rc}")
+    return tr, tr64, st
+
+
+def steps3d_f32(state, c11, c13, c33, npad, dx, dy, dz, dt, nsteps, absorbing=True):
+    """Advance (cur_x, prev_x, cur_z, prev_z) float32 arrays in place."""
+    cx, px, cz, pz = state
+    nz, ny, nx = c11.shape
+    rc = lib().orc3d_steps_f32(_p(cx), _p(px), _p(cz), _p(pz), _p(c11), _p(c13), _p(c33), nx, ny, nz, npad,
+                               dx, dy, dz, dt, nsteps, int(absorbing))
+    if rc != 0:
+        raise RuntimeError("orc3d_steps_f32 failed")

@@ -1,0 +1,270 @@
+// 3D VTI kernels for gfx950 (header-only templates; included by vti3d.hip).
+//
+// Device layout: the reference's x-fastest order (j = ix + iy*nx + iz*nx*ny,
+// lib/vti-fwi-3d/gpu/objFuncGradient3d_f.cu:560-567) with the row pitch
+// padded to a multiple of 64 floats, so every 64-lane wavefront row load is
+// one aligned 256-byte segment.  Interior x,y: [4, n-4); z: every plane of the
+// first 4*(nz/4), with zero ghosts outside, as the reference's top/bottom
+// block kernels use (kernel3d.cu:76-220).
+//
+// Blocking (2.5D): a workgroup owns a 64 x BY tile of (x, y) columns and
+// marches through a z-chunk.  sigma_x needs x and y derivatives: each plane of
+// the tile plus a 4-point halo lives in LDS (double buffered, one barrier per
+// plane).  sigma_z needs only the z derivative: each lane keeps a 9-deep
+// register queue of its own column.
+//
+// Arithmetic is FP32 with contraction off, in the operation order of the
+// reference's forwardKernel / gradientKernel expressions, so results are
+// reproducible bit for bit by the oracle's FP32 mode.
+#pragma once
+
+#include "vti_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace vti {
+
+struct Geo3 {
+    int nx, ny, nz;   // logical dims (reference layout)
+    int zmax;         // 4*(nz/4): planes the reference updates
+    int pitch;        // device row pitch in floats (multiple of 64)
+    long long plane;  // pitch*ny
+};
+
+// Maps the hardware block order (round-robin over the 8 XCDs) to a logical
+// tile order in which each XCD owns a contiguous range of tiles, so halo
+// re-reads between neighbouring tiles hit the same L2.
+__device__ __forceinline__ int xcd_logical_block(int nblocks)
+{
+    const int i = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int q = nblocks >> 3, r = nblocks & 7;
+    const int xcd = i & 7, pos = i >> 3;
+    return (xcd < r) ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
+}
+
+template <int BY>
+struct Tile {
+    // LDS row width: 64 columns + 4 halo on each side
+    static constexpr int W = 72;
+    static constexpr int R = BY + 8;
+    static constexpr int NYH = 512 / (64 * BY);  // y-halo elements per thread
+};
+
+// Per-thread description of what it loads into the x/y tile each plane.
+template <int BY>
+struct TileLoader {
+    long long main_off;  // column offset of own point
+    bool main_ok;
+    int main_lds;
+    long long yh_off[Tile<BY>::NYH];
+    bool yh_ok[Tile<BY>::NYH];
+    int yh_lds[Tile<BY>::NYH];
+    long long xh_off;
+    bool xh_ok;
+    int xh_lds;
+    bool has_xh;
+
+    __device__ __forceinline__ void init(int tx, int ty, int x0, int y0, const Geo3 &g)
+    {
+        constexpr int W = Tile<BY>::W;
+        const int tid = tx + 64 * ty;
+        const int ix = x0 + tx, iy = y0 + ty;
+        main_ok = ix < g.nx && iy < g.ny;
+        main_off = (long long)iy * g.pitch + ix;
+        main_lds = (ty + 4) * W + tx + 4;
+#pragma unroll
+        for (int h = 0; h < Tile<BY>::NYH; ++h) {
+            const int e = tid + h * 64 * BY;
+            const int r = e >> 6, c = e & 63;
+            const int hy = r < 4 ? y0 - 4 + r : y0 + BY + r - 4;
+            const int hx = x0 + c;
+            yh_ok[h] = hy < g.ny && hx < g.nx;
+            yh_off[h] = (long long)hy * g.pitch + hx;
+            yh_lds[h] = (r < 4 ? r : r + BY) * W + c + 4;
+        }
+        has_xh = tid < 8 * BY;
+        {
+            const int r = tid >> 3, c = tid & 7;
+            const int hx = c < 4 ? x0 - 4 + c : x0 + 64 + c - 4;
+            const int hy = y0 + r;
+            xh_ok = has_xh && hx >= 0 && hx < g.nx && hy < g.ny;
+            xh_off = (long long)hy * g.pitch + hx;
+            xh_lds = (r + 4) * W + (c < 4 ? c : 64 + c);
+        }
+    }
+};
+
+template <int BY>
+struct PlaneRegs {
+    float m;
+    float yh[Tile<BY>::NYH];
+    float xh;
+
+    __device__ __forceinline__ void load(const float *__restrict__ p, const TileLoader<BY> &L)
+    {
+        m = L.main_ok ? p[L.main_off] : 0.f;
+#pragma unroll
+        for (int h = 0; h < Tile<BY>::NYH; ++h) yh[h] = L.yh_ok[h] ? p[L.yh_off[h]] : 0.f;
+        xh = L.xh_ok ? p[L.xh_off] : 0.f;
+    }
+    __device__ __forceinline__ void store(float *lds, const TileLoader<BY> &L) const
+    {
+        lds[L.main_lds] = m;
+#pragma unroll
+        for (int h = 0; h < Tile<BY>::NYH; ++h) lds[L.yh_lds[h]] = yh[h];
+        if (L.has_xh) lds[L.xh_lds] = xh;
+    }
+};
+
+// Second derivative along x and y from the LDS tile, each scaled as the
+// reference scales it (kernel3d.cu:54-61): tx*sx + ty*sy, returned as ty+tx.
+__device__ __forceinline__ float lap_xy_scaled(const float *T, float sx, float sy)
+{
+    constexpr int W = 72;
+    const float s0 = T[0];
+    float tx = (VTI_C0 * s0 + VTI_C1 * (T[-1] + T[1]) + VTI_C2 * (T[-2] + T[2]) + VTI_C3 * (T[-3] + T[3]) +
+                VTI_C4 * (T[-4] + T[4])) *
+               sx;
+    float ty = (VTI_C0 * s0 + VTI_C1 * (T[-W] + T[W]) + VTI_C2 * (T[-2 * W] + T[2 * W]) +
+                VTI_C3 * (T[-3 * W] + T[3 * W]) + VTI_C4 * (T[-4 * W] + T[4 * W])) *
+               sy;
+    return ty + tx;
+}
+
+// Same stencils divided by dx2/dy2 (gradientKernel, kernel3d.cu:274-281).
+__device__ __forceinline__ float lap_xy_div(const float *T, float dx2, float dy2)
+{
+    constexpr int W = 72;
+    const float s0 = T[0];
+    float tx = (VTI_C0 * s0 + VTI_C1 * (T[-1] + T[1]) + VTI_C2 * (T[-2] + T[2]) + VTI_C3 * (T[-3] + T[3]) +
+                VTI_C4 * (T[-4] + T[4])) /
+               dx2;
+    float ty = (VTI_C0 * s0 + VTI_C1 * (T[-W] + T[W]) + VTI_C2 * (T[-2 * W] + T[2 * W]) +
+                VTI_C3 * (T[-3 * W] + T[3 * W]) + VTI_C4 * (T[-4 * W] + T[4 * W])) /
+               dy2;
+    return ty + tx;
+}
+
+__device__ __forceinline__ float lap_z(const float *e)
+{
+    return VTI_C0 * e[4] + VTI_C1 * (e[3] + e[5]) + VTI_C2 * (e[2] + e[6]) + VTI_C3 * (e[1] + e[7]) +
+           VTI_C4 * (e[0] + e[8]);
+}
+
+// Effective sigma_z column for output plane 4b+R.  The reference damps each
+// z-block's current level right after that block's step, before the next
+// block runs (objFuncGradient3d_f.cu:660-662), so the 4-R planes of the block
+// above that this point's z-stencil reaches have been damped one extra time.
+template <int R, bool ABC>
+__device__ __forceinline__ void effective_column(float *e, const float *q, float dxy, const float *__restrict__ dzv,
+                                                 int z)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) e[k] = q[k];
+    if (ABC && z >= 4) {
+#pragma unroll
+        for (int k = 0; k < 4 - R; ++k) e[k] = q[k] * (dxy * dzv[z - 4 + k]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward step (forwardKernel semantics, kernel3d.cu:3-74, plus the fused
+// injectSource and abcXYZ/abcXY of objFuncGradient3d_f.cu:655-662).
+// prev_x/prev_z are read at each point and overwritten with the next level.
+// With ABC the stored levels are damped once; the second damping the
+// reference applies to the current level is applied on read, to prev.
+// ---------------------------------------------------------------------------
+template <bool ABC, int BY>
+__global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
+    float *__restrict__ nxt_x, const float *__restrict__ cur_x, float *__restrict__ nxt_z,
+    const float *__restrict__ cur_z, const float *__restrict__ c11, const float *__restrict__ c13,
+    const float *__restrict__ c33, const float *__restrict__ dxy_plane, const float *__restrict__ dzv, Geo3 g,
+    int zc, int ntx, int nty, float dt2dx2, float dt2dy2, float dt2dz2, long long src_idx, float src_val)
+{
+    constexpr int W = Tile<BY>::W;
+    constexpr int R = Tile<BY>::R;
+    __shared__ float tile[2][R * W];
+
+    const int lb = xcd_logical_block(gridDim.x * gridDim.y * gridDim.z);
+    const int bx = lb % ntx, by = (lb / ntx) % nty, bz = lb / (ntx * nty);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x0 = bx * 64, y0 = 4 + by * BY;
+    const int zs = bz * zc;
+    const int ze = min(zs + zc, g.zmax);
+    const int ix = x0 + tx, iy = y0 + ty;
+    const bool inrange = ix < g.nx && iy < g.ny;
+    const bool active = ix >= 4 && ix < g.nx - 4 && iy < g.ny - 4;
+
+    TileLoader<BY> L;
+    L.init(tx, ty, x0, y0, g);
+    const long long col = L.main_off;
+    const float dxy = (ABC && inrange) ? dxy_plane[col] : 1.f;
+
+    float q[9];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int zz = zs - 4 + k;
+        q[k] = (inrange && zz >= 0 && zz < g.zmax) ? cur_z[zz * g.plane + col] : 0.f;
+    }
+    {
+        PlaneRegs<BY> P;
+        P.load(cur_x + zs * g.plane, L);
+        P.store(tile[0], L);
+    }
+    __syncthreads();
+
+    for (int z0 = zs; z0 < ze; z0 += 4) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int z = z0 + r;
+            const long long gi = z * g.plane + col;
+            q[8] = (inrange && z + 4 < g.zmax) ? cur_z[gi + 4 * g.plane] : 0.f;
+            PlaneRegs<BY> P;
+            const bool more = z + 1 < ze;
+            if (more) P.load(cur_x + (z + 1) * g.plane, L);
+            float px = 0.f, pz = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f;
+            if (inrange) {
+                px = nxt_x[gi];
+                pz = nxt_z[gi];
+                a11 = c11[gi];
+                a13 = c13[gi];
+                a33 = c33[gi];
+            }
+            const float *T = &tile[r & 1][(ty + 4) * W + tx + 4];
+            const float s0 = T[0];
+            const float txy = lap_xy_scaled(T, dt2dx2, dt2dy2);
+            float e[9];
+            if (r == 0) effective_column<0, ABC>(e, q, dxy, dzv, z);
+            if (r == 1) effective_column<1, ABC>(e, q, dxy, dzv, z);
+            if (r == 2) effective_column<2, ABC>(e, q, dxy, dzv, z);
+            if (r == 3) effective_column<3, ABC>(e, q, dxy, dzv, z);
+            const float tz = lap_z(e) * dt2dz2;
+            float d = 1.f;
+            if (ABC) {
+                d = dxy * dzv[z];
+                px = px * d;
+                pz = pz * d;
+            }
+            float vx = ((a11 * txy + a13 * tz) + 2.f * s0) - px;
+            float vz = ((a13 * txy + a33 * tz) + 2.f * q[4]) - pz;
+            if (gi == src_idx) {
+                vx = vx + src_val;
+                vz = vz + src_val;
+            }
+            if (ABC) {
+                vx = vx * d;
+                vz = vz * d;
+            }
+            if (active) {
+                nxt_x[gi] = vx;
+                nxt_z[gi] = vz;
+            }
+            if (more) P.store(tile[(r + 1) & 1], L);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
+        }
+    }
+}
+
+}  // namespace vti

@@ -1,0 +1,66 @@
+// Error reporting and the execution context shared by all entry points.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "vti_ctx.h"
+#include "vti_internal.h"
+
+namespace vti {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+const char *get_error() { return g_err; }
+
+}  // namespace vti
+
+extern "C" {
+
+const char *vti_last_error(void) { return vti::get_error(); }
+
+int vti_ctx_create(vti_ctx **out, int device)
+{
+    VTI_CHECK(out != nullptr, "vti_ctx_create: null output pointer");
+    int ndev = 0;
+    VTI_HIP(hipGetDeviceCount(&ndev));
+    VTI_CHECK(device >= 0 && device < ndev, "vti_ctx_create: device %d not present (%d devices)", device, ndev);
+    VTI_HIP(hipSetDevice(device));
+    vti_ctx *c = new vti_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        vti::set_error("hipStreamCreate: %s", hipGetErrorString(e));
+        return vti::kErrHip;
+    }
+    *out = c;
+    return vti::kOk;
+}
+
+int vti_ctx_destroy(vti_ctx *c)
+{
+    if (!c) return vti::kOk;
+    VTI_HIP(hipSetDevice(c->device));
+    VTI_HIP(hipStreamSynchronize(c->stream));
+    VTI_HIP(hipStreamDestroy(c->stream));
+    delete c;
+    return vti::kOk;
+}
+
+int vti_ctx_synchronize(vti_ctx *c)
+{
+    VTI_CHECK(c != nullptr, "null context");
+    VTI_HIP(hipSetDevice(c->device));
+    VTI_HIP(hipStreamSynchronize(c->stream));
+    return vti::kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// Internal definitions shared by the HIP translation units of libvtiamd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+// Finite-difference weights: the reference's truncated decimals of -205/72,
+// 8/5, -1/5, 8/315, -1/560 (lib/vti-fwi-2d/kernels.h:4-8,
+// lib/vti-fwi-3d/gpu/wave3d.h:8-12), as float literals.
+#define VTI_C0 (-2.84722222222f)
+#define VTI_C1 (1.6f)
+#define VTI_C2 (-0.2f)
+#define VTI_C3 (0.02539682539f)
+#define VTI_C4 (-0.00178571428f)
+
+// wave3d.h:5-6 -- the receiver/source weights of the 3D path (doubles there)
+#define VTI_ONETHIRD_D 0.3333333333
+#define VTI_TWOTHIRD_D 0.6666666666
+#define VTI_ONETHIRD_F ((float)VTI_ONETHIRD_D)
+#define VTI_TWOTHIRD_F ((float)VTI_TWOTHIRD_D)
+
+namespace vti {
+
+// Status codes returned by every extern "C" entry point (0 = success).
+enum Status : int {
+    kOk = 0,
+    kErrArg = -1,       // invalid argument / unsupported configuration
+    kErrHip = -2,       // a HIP runtime call failed
+    kErrNoMem = -3,     // device allocation failed
+    kErrState = -4,     // call sequence error (e.g. no model uploaded)
+};
+
+void set_error(const char *fmt, ...);
+const char *get_error();
+
+}  // namespace vti
+
+#define VTI_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            vti::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+            return vti::kErrHip;                                                            \
+        }                                                                                   \
+    } while (0)
+
+#define VTI_CHECK(cond, ...)                 \
+    do {                                     \
+        if (!(cond)) {                       \
+            vti::set_error(__VA_ARGS__);     \
+            return vti::kErrArg;             \
+        }                                    \
+    } while (0)

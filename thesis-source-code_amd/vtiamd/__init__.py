@@ -1,0 +1,230 @@
+"""ctypes binding of libvtiamd.so (the C ABI declared in include/vti_amd.h).
+
+This is the host-side mirror used by tests, the benchmark and Python callers.
+It loads the in-tree library and raises if it is missing: there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_float, c_int, c_longlong, c_size_t, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libvtiamd.so")
+
+
+class VtiError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise VtiError(f"libvtiamd.so not built at {LIB_PATH}; run __graft_entry__.build() first")
+    return ctypes.CDLL(LIB_PATH)
+
+
+_lib = _load()
+P = c_void_p
+
+_SIGS = {
+    "vti_last_error": (c_char_p, []),
+    "vti_ctx_create": (c_int, [POINTER(c_void_p), c_int]),
+    "vti_ctx_destroy": (c_int, [P]),
+    "vti_ctx_synchronize": (c_int, [P]),
+    "vti_init_damping3d": (c_int, [P, c_int, c_int, c_int, c_int]),
+    "vti_init_taper2d": (c_int, [P, c_int, c_int, c_int]),
+    "vti_vepsdel_to_cij": (c_int, [P, P, P, P, P, P, c_float, c_float, c_float, c_longlong]),
+    "vti_dvepsdel_to_dcij": (c_int, [P] * 12 + [c_float, c_float, c_float, c_longlong]),
+    "vti_gradcij_to_gradvepsdel": (c_int, [P] * 9 + [c_float, c_float, c_float, c_longlong]),
+    "vti_boundary_size": (c_longlong, [c_int, c_int, c_int, c_int]),
+    "vti_get_boundary": (c_int, [P, P, c_int, c_int, c_int, c_int]),
+    "vti_put_boundary": (c_int, [P, P, c_int, c_int, c_int, c_int]),
+    "vti_zero_boundary": (c_int, [P, c_int, c_int, c_int, c_int]),
+    "vti_ricker": (c_int, [P, c_float, c_int, c_float, c_float, c_float, c_float]),
+    "vti_glibc_rand": (c_int, [c_int, c_int, P]),
+    "vti_random_boundary": (
+        c_longlong,
+        [P, c_float, c_float, c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_int],
+    ),
+    "vti3d_prop_create": (
+        c_int,
+        [P, POINTER(c_void_p), c_int, c_int, c_int, c_int, c_float, c_float, c_float, c_float],
+    ),
+    "vti3d_prop_destroy": (c_int, [P]),
+    "vti3d_prop_bytes": (c_size_t, [P]),
+    "vti3d_prop_set_zchunk": (c_int, [P, c_int]),
+    "vti3d_prop_set_model": (c_int, [P, P, P, P]),
+    "vti3d_prop_reset": (c_int, [P, c_longlong, c_float]),
+    "vti3d_prop_set_state": (c_int, [P, P, P, P, P]),
+    "vti3d_prop_get_state": (c_int, [P, P, P, P, P]),
+    "vti3d_prop_forward_steps": (c_int, [P, c_int, c_int, POINTER(c_float)]),
+    "vti3d_prop_forward_shot": (
+        c_int,
+        [P, c_int, P, c_longlong, c_int, P, c_int, c_int, c_int, P],
+    ),
+}
+
+EXPORTED = tuple(_SIGS)
+
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+lib = _lib
+
+
+def last_error() -> str:
+    msg = _lib.vti_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str = "") -> int:
+    if rc < 0:
+        raise VtiError(f"{what} failed ({rc}): {last_error()}")
+    return rc
+
+
+def ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data
+
+
+def f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# ---- host helpers ---------------------------------------------------------
+
+def init_damping3d(nx, ny, nz, npad) -> np.ndarray:
+    d = np.zeros(nx * ny + nz, np.float32)
+    check(_lib.vti_init_damping3d(ptr(d), nx, ny, nz, npad), "vti_init_damping3d")
+    return d
+
+
+def init_taper2d(nx, nz, npad) -> np.ndarray:
+    t = np.zeros((nz + 2 * npad) * (nx + 2 * npad), np.float32)
+    check(_lib.vti_init_taper2d(ptr(t), nx, nz, npad), "vti_init_taper2d")
+    return t
+
+
+def vepsdel_to_cij(v, eps, dl, v0=1.0, eps0=1.0, del0=1.0):
+    v, eps, dl = f32(v), f32(eps), f32(dl)
+    c11, c13, c33 = (np.empty_like(v) for _ in range(3))
+    check(_lib.vti_vepsdel_to_cij(ptr(c11), ptr(c13), ptr(c33), ptr(v), ptr(eps), ptr(dl), v0, eps0, del0, v.size))
+    return c11, c13, c33
+
+
+def glibc_rand(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.int32)
+    check(_lib.vti_glibc_rand(seed, n, ptr(out)), "vti_glibc_rand")
+    return out
+
+
+def random_boundary(field, bval, eval_, dims, thick, pctG, seed) -> np.ndarray:
+    """RandomMY on a copy of `field` (dims = (n1, n2, n3), n1 fastest)."""
+    f = f32(field).copy()
+    n1, n2, n3 = dims
+    t1, t2, t3 = thick
+    check(_lib.vti_random_boundary(ptr(f), bval, eval_, n1, n2, n3, t1, t2, t3, pctG, seed), "vti_random_boundary")
+    return f
+
+
+def ricker(freq, nt, dt, ot=0.0, tdelay=0.0, scalefactor=1.0) -> np.ndarray:
+    w = np.zeros(nt, np.float32)
+    check(_lib.vti_ricker(ptr(w), freq, nt, dt, ot, tdelay, scalefactor), "vti_ricker")
+    return w
+
+
+# ---- device objects ---------------------------------------------------------
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = c_void_p()
+        check(_lib.vti_ctx_create(ctypes.byref(h), device), "vti_ctx_create")
+        self.h = h
+
+    def sync(self):
+        check(_lib.vti_ctx_synchronize(self.h), "vti_ctx_synchronize")
+
+    def close(self):
+        if self.h:
+            check(_lib.vti_ctx_destroy(self.h), "vti_ctx_destroy")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Prop3D:
+    """Device-resident 3D VTI propagator (vti3d_prop_*)."""
+
+    def __init__(self, ctx: Context, nx, ny, nz, npad, dx, dy, dz, dt):
+        h = c_void_p()
+        check(_lib.vti3d_prop_create(ctx.h, ctypes.byref(h), nx, ny, nz, npad, dx, dy, dz, dt), "vti3d_prop_create")
+        self.h = h
+        self.shape = (nz, ny, nx)
+
+    def close(self):
+        if self.h:
+            check(_lib.vti3d_prop_destroy(self.h), "vti3d_prop_destroy")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def nbytes(self) -> int:
+        return int(_lib.vti3d_prop_bytes(self.h))
+
+    def set_zchunk(self, zc: int):
+        check(_lib.vti3d_prop_set_zchunk(self.h, zc), "vti3d_prop_set_zchunk")
+
+    def set_model(self, c11, c13, c33):
+        c11, c13, c33 = f32(c11), f32(c13), f32(c33)
+        check(_lib.vti3d_prop_set_model(self.h, ptr(c11), ptr(c13), ptr(c33)), "vti3d_prop_set_model")
+
+    def reset(self, sou_index: int = -1, value: float = 0.0):
+        check(_lib.vti3d_prop_reset(self.h, sou_index, value), "vti3d_prop_reset")
+
+    def set_state(self, cur_x=None, cur_z=None, prev_x=None, prev_z=None):
+        arrs = [None if a is None else f32(a) for a in (cur_x, cur_z, prev_x, prev_z)]
+        check(_lib.vti3d_prop_set_state(self.h, *[ptr(a) for a in arrs]), "vti3d_prop_set_state")
+
+    def get_state(self):
+        outs = [np.empty(self.shape, np.float32) for _ in range(4)]
+        check(_lib.vti3d_prop_get_state(self.h, *[ptr(a) for a in outs]), "vti3d_prop_get_state")
+        return outs
+
+    def forward_steps(self, nsteps: int, absorbing: bool = True, timed: bool = False):
+        ms = c_float(0.0)
+        check(
+            _lib.vti3d_prop_forward_steps(self.h, nsteps, int(absorbing), ctypes.byref(ms) if timed else None),
+            "vti3d_prop_forward_steps",
+        )
+        return ms.value if timed else None
+
+    def forward_shot(self, lmax, wavelet, sou_index, rec, sts, nnt, absorbing=True):
+        """Returns traces (nr, nnt), trace-major like the reference; sample 0 = 0."""
+        wavelet = f32(wavelet)
+        rec = np.ascontiguousarray(rec, dtype=np.int64)
+        tr = np.zeros((nnt, rec.size), np.float32)
+        check(
+            _lib.vti3d_prop_forward_shot(
+                self.h, lmax, ptr(wavelet), int(sou_index), rec.size, ptr(rec), sts, nnt, int(absorbing), ptr(tr)
+            ),
+            "vti3d_prop_forward_shot",
+        )
+        return np.ascontiguousarray(tr.T)
