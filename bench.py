@@ -11,7 +11,10 @@ forwardKernel + injectSource + abcXYZ/abcXY over all z-blocks).
 Prints one JSON line: Gpts/s over the padded grid, the roofline of the step
 kernel (HIP-event timing on the propagator's stream; algorithmic bytes 36 per
 point), and the CPU baseline (the oracle's C/OpenMP restatement of the same
-step, timed here on a bounded number of steps of the same grid).
+step, timed here on a bounded number of steps of the same grid).  The
+stencil arithmetic is the reference's own (double weights and sums over FP32
+storage, kernel3d.cu:54-69); "extra" also reports the FP32-arithmetic
+variant and the fused backward (gradient, RTM) steps on the same grid.
 """
 from __future__ import annotations
 
@@ -29,7 +32,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_FWD = 36  # sigma_x, sigma_z cur+prev, c11, c13, c33 read; sigma_x, sigma_z written
-BYTES_BWD = 84  # fused gradient step
+BYTES_BWD = 84  # fused gradient step: 14 reads + 7 writes of 4 bytes per point
+BYTES_RTM = 68  # fused RTM step: 12 reads + 5 writes
 
 
 def layered(nz, ny, nx, seed=2024):
@@ -72,12 +76,12 @@ def cpu_baseline(c11, c13, c33, npad, dx, dt, budget_s=15.0):
     st = [np.zeros((nz, ny, nx), np.float32) for _ in range(4)]
     st[0][:] = rng.standard_normal((nz, ny, nx), dtype=np.float32) * 1e-3
     t0 = time.perf_counter()
-    oracle.steps3d_f32(st, c11, c13, c33, npad, dx, dx, dx, dt, 1, absorbing=True)
+    oracle.steps3d(oracle.REF, st, c11, c13, c33, npad, dx, dx, dx, dt, 1, absorbing=True)
     t1 = time.perf_counter()
     one = t1 - t0
     n = max(1, min(30, int(budget_s / max(one, 1e-3))))
     t0 = time.perf_counter()
-    oracle.steps3d_f32(st, c11, c13, c33, npad, dx, dx, dx, dt, n, absorbing=True)
+    oracle.steps3d(oracle.REF, st, c11, c13, c33, npad, dx, dx, dx, dt, n, absorbing=True)
     el = time.perf_counter() - t0
     gpts = nz * ny * nx * n / el / 1e9
     return {
@@ -86,7 +90,7 @@ def cpu_baseline(c11, c13, c33, npad, dx, dt, budget_s=15.0):
         "cores": threads,
         "kind": "port",
         "sample": f"{n} forward steps (absorbing) of the full {nz}x{ny}x{nx} grid, "
-        f"oracle FP32 C/OpenMP restatement, {el:.1f} s",
+        f"oracle C/OpenMP restatement in the reference arithmetic, {el:.1f} s",
     }
 
 
@@ -165,6 +169,22 @@ def main():
             "bytes_per_point": BYTES_FWD,
         },
     }
+    # secondary kernels of the same path on the same grid (reported, not the headline)
+    extra = {}
+    bw = prop.backward_steps(max(1, args.steps // 2), mode=0, timed=True)
+    extra["bwd_gradient_step"] = {
+        "ms": round(bw, 4), "gpts_s": round(npts / (bw * 1e-3) / 1e9, 3), "bytes_per_point": BYTES_BWD,
+        "achieved_gbs": round(BYTES_BWD * npts / (bw * 1e-3) / 1e9, 1),
+    }
+    br = prop.backward_steps(max(1, args.steps // 2), mode=1, timed=True)
+    extra["bwd_rtm_step"] = {
+        "ms": round(br, 4), "gpts_s": round(npts / (br * 1e-3) / 1e9, 3), "bytes_per_point": BYTES_RTM,
+        "achieved_gbs": round(BYTES_RTM * npts / (br * 1e-3) / 1e9, 1),
+    }
+    prop.set_arith(0)
+    f32ms = prop.forward_steps(max(1, args.steps // 2), absorbing=True, timed=True)
+    extra["fwd_step_fp32_arith"] = {"ms": round(f32ms, 4), "gpts_s": round(npts / (f32ms * 1e-3) / 1e9, 3)}
+    out["extra"] = extra
     prop.close()
     if not args.no_cpu and not args.small:
         out["cpu_baseline"] = cpu_baseline(c11, c13, c33, npad, dx, dt)

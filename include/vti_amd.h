@@ -34,6 +34,11 @@ const char *vti_last_error(void);
 int vti_ctx_create(vti_ctx **ctx, int device);
 int vti_ctx_destroy(vti_ctx *ctx);
 int vti_ctx_synchronize(vti_ctx *ctx);
+/* Arithmetic of the 3D bulk stencil for propagators and drivers created on
+ * ctx afterwards: 1 (default) evaluates the reference's expressions exactly
+ * as lib/vti-fwi-3d/gpu/kernel3d.cu:54-69 does -- double weights, products
+ * and sums over float storage; 0 evaluates them in FP32. */
+int vti_ctx_set_arith(vti_ctx *ctx, int arith);
 
 /* ---- host helpers (bit-exact restatements of the reference's host code) - */
 
@@ -86,6 +91,7 @@ int vti3d_prop_create(vti_ctx *ctx, vti3d_prop **prop, int nx, int ny, int nz, i
 int vti3d_prop_destroy(vti3d_prop *prop);
 size_t vti3d_prop_bytes(const vti3d_prop *prop);
 int vti3d_prop_set_zchunk(vti3d_prop *prop, int zchunk);
+int vti3d_prop_set_arith(vti3d_prop *prop, int arith);
 int vti3d_prop_set_model(vti3d_prop *prop, const float *c11, const float *c13, const float *c33);
 int vti3d_prop_reset(vti3d_prop *prop, long long sou_index, float value);
 int vti3d_prop_set_state(vti3d_prop *prop, const float *cur_x, const float *cur_z, const float *prev_x,
@@ -94,6 +100,48 @@ int vti3d_prop_get_state(vti3d_prop *prop, float *cur_x, float *cur_z, float *pr
 int vti3d_prop_forward_steps(vti3d_prop *prop, int nsteps, int absorbing, float *ms_per_step);
 int vti3d_prop_forward_shot(vti3d_prop *prop, int lmax, const float *wavelet, long long sou_index, int nr,
                             const long long *rec, int sts, int nnt, int absorbing, float *traces_tm);
+/* Fused backward levels on the current state (mode 0: gradientKernel +
+ * forwardKernel on lambda + imaging, objFuncGradient3d_f.cu:1008-1028;
+ * mode 1: RTM, rtm3d_f.cu:585-610), no source/receivers; for timing. */
+int vti3d_prop_backward_steps(vti3d_prop *prop, int mode, int nsteps, float *ms_per_step);
+
+/* ---- 3D drivers with the reference's semantics ----------------------------
+ * Geometry arrays as the reference takes them: souloc = 5 floats per shot
+ * {x, y, z, nr, first receiver}, recloc = 4 floats per receiver
+ * {x, y, z, live} (3 for RTM), positions in the units of o/d.  shotid lists
+ * the shots to run.  ngpu is the number of GPUs the reference would have used:
+ * it only fixes how many time levels its pipeline advances
+ * (nround*ngpu*64, objFuncGradient3d_f.cu:539); all work runs on ctx's device.
+ * obs stands in for the reference's read("data") inside the drivers:
+ * trace-major, nnt = (nt-1)/round(samplingRate/dt)+1 samples per receiver. */
+
+/* modelDataCij3d_f, lib/vti-fwi-3d/gpu/modelData3d_f.cu:427 (wave3d.h:110) */
+int vti3d_model_data(vti_ctx *ctx, float *ddata, const float *souloc, int ns, const int *shotid, int nshot,
+                     const float *recloc, const float *wavelet, const float *c11, const float *c13, const float *c33,
+                     int nx, int ny, int nz, int nt, int npad, float ox, float oy, float oz, float ot, float dx,
+                     float dy, float dz, float dt, float samplingRate, int ngpu);
+
+/* objFuncCij3d_f, lib/vti-fwi-3d/gpu/objFuncGradient3d_f.cu:1272 (wave3d.h:100).
+ * f[0] = 0.5*sum r^2.  tpow: the tpow= gain (0 = none). */
+int vti3d_objfunc(vti_ctx *ctx, float *f, const float *souloc, int ns, const int *shotid, int nshot,
+                  const float *recloc, const float *wavelet, float *cij, const float *padboundaryCij, int nx, int ny,
+                  int nz, int nt, int npad, float ox, float oy, float oz, float ot, float dx, float dy, float dz,
+                  float dt, float samplingRate, int ngpu, const float *obs, float tpow);
+
+/* objFuncGradientCij3d_f, lib/vti-fwi-3d/gpu/objFuncGradient3d_f.cu:361 (wave3d.h:96).
+ * fgcij = {f, gc11[nxyz], gc13[nxyz], gc33[nxyz]}; cij = {c11, c13, c33}
+ * whose boundary shell is left holding randboundaryCij, as in the reference. */
+int vti3d_objfunc_gradient(vti_ctx *ctx, float *fgcij, const float *souloc, int ns, const int *shotid, int nshot,
+                           const float *recloc, const float *wavelet, float *cij, const float *padboundaryCij,
+                           const float *randboundaryCij, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
+                           float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate, int ngpu,
+                           const float *obs, float tpow);
+
+/* rtmCij3d_f, lib/vti-fwi-3d/gpu/rtm3d_f.cu:161 (wave3d.h:88).  recloc stride 3. */
+int vti3d_rtm(vti_ctx *ctx, float *image, const float *souloc, int ns, const int *shotid, int nshot,
+              const float *recloc, const float *wavelet, const float *c11, const float *c13, const float *c33, int nx,
+              int ny, int nz, int nt, int npad, float ox, float oy, float oz, float ot, float dx, float dy, float dz,
+              float dt, float samplingRate, int ngpu, const float *obs);
 
 #ifdef __cplusplus
 }

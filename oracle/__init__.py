@@ -49,7 +49,8 @@ def lib():
             c_int, P, P, P, c_int, c_int, c_int, c_int, c_float, c_float, c_float, c_float, c_int, P, c_long,
             c_int, P, c_int, c_int, c_int, P, P, P, P, P, P,
         ]
-        L.orc3d_steps_f32.argtypes = [P, P, P, P, P, P, P] + [c_int] * 4 + [c_float] * 4 + [c_int, c_int]
+        L.orc3d_steps.argtypes = [c_int, P, P, P, P, P, P, P] + [c_int] * 4 + [c_float] * 4 + [c_int, c_int]
+        L.orc3d_steps_f64.argtypes = [P] * 7 + [c_int] * 4 + [c_float] * 4 + [c_int, c_int, c_int]
         L.orc3d_gradient.restype = c_double
         L.orc3d_gradient.argtypes = (
             [c_int, P, P, P, P, c_int, P, P, P, P, P]
@@ -57,7 +58,7 @@ def lib():
             + [c_float] * 9
             + [c_int, P]
         )
-        L.orc3d_rtm.argtypes = [c_int, P, P, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 7 + [c_int, P]
+        L.orc3d_rtm.argtypes = [c_int, P, P, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 8 + [c_int, P]
         _lib = L
     return _lib
 
@@ -103,11 +104,60 @@ def forward3d(mode, c11, c13, c33, npad, dx, dy, dz, dt, lmax, wavelet, sou_inde
     return tr, tr64, st
 
 
-def steps3d_f32(state, c11, c13, c33, npad, dx, dy, dz, dt, nsteps, absorbing=True):
-    """Advance (cur_x, prev_x, cur_z, prev_z) float32 arrays in place."""
+def steps3d(mode, state, c11, c13, c33, npad, dx, dy, dz, dt, nsteps, absorbing=True):
+    """Advance (cur_x, prev_x, cur_z, prev_z) float32 arrays in place (mode F32 or REF)."""
     cx, px, cz, pz = state
     nz, ny, nx = c11.shape
-    rc = lib().orc3d_steps_f32(_p(cx), _p(px), _p(cz), _p(pz), _p(c11), _p(c13), _p(c33), nx, ny, nz, npad,
-                               dx, dy, dz, dt, nsteps, int(absorbing))
+    rc = lib().orc3d_steps(mode, _p(cx), _p(px), _p(cz), _p(pz), _p(c11), _p(c13), _p(c33), nx, ny, nz, npad,
+                           dx, dy, dz, dt, nsteps, int(absorbing))
     if rc != 0:
-        raise RuntimeError("orc3d_steps_f32 failed")
+        raise RuntimeError("orc3d_steps failed")
+
+
+def steps3d_f64(state, c11, c13, c33, npad, dx, dy, dz, dt, nsteps, absorbing=True, gradient_form=False):
+    """Advance float64 (cur_x, prev_x, cur_z, prev_z) in place with the FP64 oracle."""
+    cx, px, cz, pz = state
+    nz, ny, nx = c11.shape
+    c11, c13, c33 = (np.ascontiguousarray(a, np.float64) for a in (c11, c13, c33))
+    rc = lib().orc3d_steps_f64(_p(cx), _p(px), _p(cz), _p(pz), _p(c11), _p(c13), _p(c33), nx, ny, nz, npad,
+                               dx, dy, dz, dt, nsteps, int(absorbing), int(gradient_form))
+    if rc != 0:
+        raise RuntimeError("orc3d_steps_f64 failed")
+
+
+def gradient3d(mode, souloc, shotid, recloc, wavelet, cij, padb, randb, nt, npad, o, d, dt, rate, obs, ngpu=1):
+    """objFuncGradientCij3d_f restated.  cij (3, nz, ny, nx) float32 modified in place.
+    Returns (objective, gradient (3, nz, ny, nx)) -- float32 for F32/REF, float64 for F64."""
+    _, nz, ny, nx = cij.shape
+    souloc = np.ascontiguousarray(souloc, np.float32).ravel()
+    recloc = np.ascontiguousarray(recloc, np.float32).ravel()
+    shotid = np.ascontiguousarray(shotid, np.int32)
+    w = np.ascontiguousarray(wavelet, np.float32)
+    padb = np.ascontiguousarray(padb, np.float32)
+    randb = np.ascontiguousarray(randb, np.float32)
+    obs = np.ascontiguousarray(obs, np.float32)
+    if mode == F64:
+        out = np.zeros(1 + cij.size, np.float64)
+        f = lib().orc3d_gradient(mode, None, _p(out), _p(souloc), _p(shotid), shotid.size, _p(recloc), _p(w), _p(cij),
+                                 _p(padb), _p(randb), nx, ny, nz, nt, npad, *o, 0.0, *d, dt, rate, ngpu, _p(obs))
+    else:
+        out = np.zeros(1 + cij.size, np.float32)
+        f = lib().orc3d_gradient(mode, _p(out), None, _p(souloc), _p(shotid), shotid.size, _p(recloc), _p(w), _p(cij),
+                                 _p(padb), _p(randb), nx, ny, nz, nt, npad, *o, 0.0, *d, dt, rate, ngpu, _p(obs))
+    return out[0], out[1:].reshape(3, nz, ny, nx)
+
+
+def rtm3d(mode, souloc, shotid, recloc3, wavelet, c11, c13, c33, nt, npad, o, d, dt, rate, obs, ngpu=1):
+    nz, ny, nx = c11.shape
+    souloc = np.ascontiguousarray(souloc, np.float32).ravel()
+    recloc3 = np.ascontiguousarray(recloc3, np.float32).ravel()
+    shotid = np.ascontiguousarray(shotid, np.int32)
+    w = np.ascontiguousarray(wavelet, np.float32)
+    c11, c13, c33 = (np.ascontiguousarray(a, np.float32) for a in (c11, c13, c33))
+    obs = np.ascontiguousarray(obs, np.float32)
+    img = np.zeros((nz, ny, nx), np.float64 if mode == F64 else np.float32)
+    rc = lib().orc3d_rtm(mode, _p(img), _p(souloc), _p(shotid), shotid.size, _p(recloc3), _p(w), _p(c11), _p(c13),
+                         _p(c33), nx, ny, nz, nt, npad, *o, *d, dt, rate, ngpu, _p(obs))
+    if rc != 0:
+        raise RuntimeError("orc3d_rtm failed")
+    return img

@@ -282,18 +282,18 @@ int orc3d_forward(int mode, const float *c11, const float *c13, const float *c33
     return 0;
 }
 
-/* One step of the forward operator on caller-owned state (used by the
- * time-reversibility and CPU-baseline measurements).  Runs `nsteps` levels
- * with the block order and damping of the forward pass; no source.
- * State arrays are updated in place (cur/prev swapped after each level, so
- * after an odd number of steps the caller's cur array holds the old prev). */
-int orc3d_steps_f32(float *cur_x, float *prev_x, float *cur_z, float *prev_z, const float *c11, const float *c13,
-                    const float *c33, int nx, int ny, int nz, int npad, float dx, float dy, float dz, float dt,
-                    int nsteps, int absorbing)
+/* Forward levels on caller-owned float state (time-reversibility checks and
+ * the CPU baseline): `nsteps` levels with the block order and damping of the
+ * forward pass, no source.  mode 0 FP32, 1 the reference's arithmetic.
+ * State arrays are updated in place; after each level cur/prev swap roles,
+ * so after an odd number of levels the caller's cur array holds the old prev. */
+int orc3d_steps(int mode, float *cur_x, float *prev_x, float *cur_z, float *prev_z, const float *c11,
+                const float *c13, const float *c33, int nx, int ny, int nz, int npad, float dx, float dy, float dz,
+                float dt, int nsteps, int absorbing)
 {
     orc3d_geom g;
     geom_setup(&g, nx, ny, nz, npad, dx, dy, dz, dt);
-    if (g.nb < 2) return -1;
+    if (g.nb < 2 || (mode != 0 && mode != 1)) return -1;
     float *damping = NULL;
     if (absorbing) {
         damping = (float *)malloc(sizeof(float) * ((long)nx * ny + nz));
@@ -301,12 +301,50 @@ int orc3d_steps_f32(float *cur_x, float *prev_x, float *cur_z, float *prev_z, co
     }
     for (int s = 0; s < nsteps; ++s) {
         for (int ib = 0; ib < g.nb; ++ib) {
-            fwd_block_f32(&g, ib, prev_x, cur_x, prev_x, prev_z, cur_z, prev_z, c11, c13, c33);
-            if (damping) abc_block_f32(&g, ib, damping, prev_x, cur_x, prev_z, cur_z);
+            if (mode == 0) {
+                fwd_block_f32(&g, ib, prev_x, cur_x, prev_x, prev_z, cur_z, prev_z, c11, c13, c33);
+                if (damping) abc_block_f32(&g, ib, damping, prev_x, cur_x, prev_z, cur_z);
+            } else {
+                fwd_block_ref(&g, ib, prev_x, cur_x, prev_x, prev_z, cur_z, prev_z, c11, c13, c33);
+                if (damping) abc_block_ref(&g, ib, damping, prev_x, cur_x, prev_z, cur_z);
+            }
         }
         float *t = cur_x; cur_x = prev_x; prev_x = t;
         t = cur_z; cur_z = prev_z; prev_z = t;
     }
+    free(damping);
+    return 0;
+}
+
+/* FP64 counterpart of orc3d_steps on double state; `gradient_form` selects
+ * the source-reconstruction formula of gradientKernel (no damping). */
+int orc3d_steps_f64(double *cur_x, double *prev_x, double *cur_z, double *prev_z, const double *c11,
+                    const double *c13, const double *c33, int nx, int ny, int nz, int npad, float dx, float dy,
+                    float dz, float dt, int nsteps, int absorbing, int gradient_form)
+{
+    orc3d_geom g;
+    geom_setup(&g, nx, ny, nz, npad, dx, dy, dz, dt);
+    if (g.nb < 2) return -1;
+    long n = (long)nx * ny * nz;
+    float *damping = NULL;
+    if (absorbing) {
+        damping = (float *)malloc(sizeof(float) * ((long)nx * ny + nz));
+        orc3d_init_damping(damping, nx, ny, nz, npad);
+    }
+    double *dummy = gradient_form ? (double *)calloc(5 * n, sizeof(double)) : NULL;
+    for (int s = 0; s < nsteps; ++s) {
+        for (int ib = 0; ib < g.nb; ++ib) {
+            if (gradient_form)
+                grad_block_f64(&g, ib, dummy, dummy + n, dummy + 2 * n, prev_x, cur_x, prev_x, prev_z, cur_z, prev_z,
+                               dummy + 3 * n, dummy + 4 * n, c11, c13, c33);
+            else
+                fwd_block_f64(&g, ib, prev_x, cur_x, prev_x, prev_z, cur_z, prev_z, c11, c13, c33);
+            if (damping && !gradient_form) abc_block_f64(&g, ib, damping, prev_x, cur_x, prev_z, cur_z);
+        }
+        double *t = cur_x; cur_x = prev_x; prev_x = t;
+        t = cur_z; cur_z = prev_z; prev_z = t;
+    }
+    free(dummy);
     free(damping);
     return 0;
 }

@@ -19,9 +19,10 @@ def _problem(nz=40, ny=36, nx=44, seed=3):
     return cij_from(v, e, d)
 
 
+@pytest.mark.parametrize("arith", [1, 0], ids=["ref-arith", "fp32-arith"])
 @pytest.mark.parametrize("absorbing", [True, False])
 @pytest.mark.parametrize("dims", [(40, 36, 44), (48, 29, 70)])
-def test_forward_shot_bitexact(vti, orc, ctx, absorbing, dims):
+def test_forward_shot_bitexact(vti, orc, ctx, absorbing, dims, arith):
     nz, ny, nx = dims
     c11, c13, c33 = _problem(nz, ny, nx)
     npad, dx, dt = 8, 12.5, 1.0e-3
@@ -31,11 +32,13 @@ def test_forward_shot_bitexact(vti, orc, ctx, absorbing, dims):
     sou = 17 + 15 * nx + 13 * nx * ny
     rec = np.array([ix + (ny // 2) * nx + 9 * nx * ny for ix in range(4, nx - 4)], np.int64)
     sts, nnt = 4, (nt - 1) // 4 + 1
-    print(f"case dims={dims} absorbing={absorbing}", flush=True)
-    tr_ref, tr64, st = orc.forward3d(orc.F32, c11, c13, c33, npad, dx, dx, dx, dt, lmax, w, sou, rec, sts, nnt,
+    print(f"case dims={dims} absorbing={absorbing} arith={arith}", flush=True)
+    mode = orc.REF if arith == 1 else orc.F32
+    tr_ref, tr64, st = orc.forward3d(mode, c11, c13, c33, npad, dx, dx, dx, dt, lmax, w, sou, rec, sts, nnt,
                                      absorbing=absorbing, want_state=True)
     with vti.Prop3D(ctx, nx, ny, nz, npad, dx, dx, dx, dt) as p:
         p.set_zchunk(16)
+        p.set_arith(arith)
         p.set_model(c11, c13, c33)
         p.reset(sou, np.float32(np.float32(dt) * np.float32(dt)) * w[0])
         tr = p.forward_shot(lmax, w, sou, rec, sts, nnt, absorbing=absorbing)

@@ -13,9 +13,12 @@
 // plane).  sigma_z needs only the z derivative: each lane keeps a 9-deep
 // register queue of its own column.
 //
-// Arithmetic is FP32 with contraction off, in the operation order of the
-// reference's forwardKernel / gradientKernel expressions, so results are
-// reproducible bit for bit by the oracle's FP32 mode.
+// Arithmetic (template parameter ARITH): kArithRef evaluates the reference's
+// own expressions -- float pair sums, double weights, products and sums,
+// rounded to float once per stencil and once per update (kernel3d.cu:54-69)
+// -- and is the default; kArithF32 evaluates the same expressions in FP32.
+// Contraction is off in both, so each mode is reproducible bit for bit by the
+// oracle's matching mode.
 #pragma once
 
 #include "vti_internal.h"
@@ -151,6 +154,72 @@ __device__ __forceinline__ float lap_z(const float *e)
            VTI_C4 * (e[0] + e[8]);
 }
 
+// ---- the reference's own 3D arithmetic (kernel3d.cu:54-69, wave3d.h:8-12):
+// pair sums of float neighbours in float, products and accumulation in
+// double with double weights, rounded to float once per stencil.
+#define VTI_C0D (-2.84722222222)
+#define VTI_C1D (1.6)
+#define VTI_C2D (-0.2)
+#define VTI_C3D (0.02539682539)
+#define VTI_C4D (-0.00178571428)
+
+__device__ __forceinline__ double acc9(float m, float p1, float p2, float p3, float p4)
+{
+    return VTI_C0D * (double)m + VTI_C1D * (double)p1 + VTI_C2D * (double)p2 + VTI_C3D * (double)p3 +
+           VTI_C4D * (double)p4;
+}
+
+// Arithmetic mode of the bulk stencil: 0 = FP32 throughout, 1 = the
+// reference's mixed FP64 arithmetic on FP32 storage.
+enum { kArithF32 = 0, kArithRef = 1 };
+
+template <int ARITH>
+__device__ __forceinline__ float sx_scaled(const float *T, float sx, float sy)
+{
+    if (ARITH == kArithF32) return lap_xy_scaled(T, sx, sy);
+    constexpr int W = 72;
+    const float tx = (float)(acc9(T[0], T[-1] + T[1], T[-2] + T[2], T[-3] + T[3], T[-4] + T[4]) * (double)sx);
+    float ty = (float)(acc9(T[0], T[-W] + T[W], T[-2 * W] + T[2 * W], T[-3 * W] + T[3 * W], T[-4 * W] + T[4 * W]) *
+                       (double)sy);
+    ty += tx;
+    return ty;
+}
+
+template <int ARITH>
+__device__ __forceinline__ float sx_div(const float *T, float dx2, float dy2)
+{
+    if (ARITH == kArithF32) return lap_xy_div(T, dx2, dy2);
+    constexpr int W = 72;
+    const float tx = (float)(acc9(T[0], T[-1] + T[1], T[-2] + T[2], T[-3] + T[3], T[-4] + T[4]) / (double)dx2);
+    float ty = (float)(acc9(T[0], T[-W] + T[W], T[-2 * W] + T[2 * W], T[-3 * W] + T[3 * W], T[-4 * W] + T[4 * W]) /
+                       (double)dy2);
+    ty += tx;
+    return ty;
+}
+
+template <int ARITH>
+__device__ __forceinline__ float sz_scaled(const float *e, float s)
+{
+    if (ARITH == kArithF32) return lap_z(e) * s;
+    return (float)(acc9(e[4], e[3] + e[5], e[2] + e[6], e[1] + e[7], e[0] + e[8]) * (double)s);
+}
+
+template <int ARITH>
+__device__ __forceinline__ float sz_div(const float *e, float d2)
+{
+    if (ARITH == kArithF32) return lap_z(e) / d2;
+    return (float)(acc9(e[4], e[3] + e[5], e[2] + e[6], e[1] + e[7], e[0] + e[8]) / (double)d2);
+}
+
+// next = a + 2*cur - prev: float in FP32 mode, double then rounded in the
+// reference's arithmetic (kernel3d.cu:68-69)
+template <int ARITH>
+__device__ __forceinline__ float leap(float a, float cur, float prev)
+{
+    if (ARITH == kArithF32) return (a + 2.f * cur) - prev;
+    return (float)((double)a + 2. * (double)cur - (double)prev);
+}
+
 // Effective sigma_z column for output plane 4b+R.  The reference damps each
 // z-block's current level right after that block's step, before the next
 // block runs (objFuncGradient3d_f.cu:660-662), so the 4-R planes of the block
@@ -174,7 +243,7 @@ __device__ __forceinline__ void effective_column(float *e, const float *q, float
 // With ABC the stored levels are damped once; the second damping the
 // reference applies to the current level is applied on read, to prev.
 // ---------------------------------------------------------------------------
-template <bool ABC, int BY>
+template <int ARITH, bool ABC, int BY>
 __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
     float *__restrict__ nxt_x, const float *__restrict__ cur_x, float *__restrict__ nxt_z,
     const float *__restrict__ cur_z, const float *__restrict__ c11, const float *__restrict__ c13,
@@ -232,21 +301,21 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
             }
             const float *T = &tile[r & 1][(ty + 4) * W + tx + 4];
             const float s0 = T[0];
-            const float txy = lap_xy_scaled(T, dt2dx2, dt2dy2);
+            const float txy = sx_scaled<ARITH>(T, dt2dx2, dt2dy2);
             float e[9];
             if (r == 0) effective_column<0, ABC>(e, q, dxy, dzv, z);
             if (r == 1) effective_column<1, ABC>(e, q, dxy, dzv, z);
             if (r == 2) effective_column<2, ABC>(e, q, dxy, dzv, z);
             if (r == 3) effective_column<3, ABC>(e, q, dxy, dzv, z);
-            const float tz = lap_z(e) * dt2dz2;
+            const float tz = sz_scaled<ARITH>(e, dt2dz2);
             float d = 1.f;
             if (ABC) {
                 d = dxy * dzv[z];
                 px = px * d;
                 pz = pz * d;
             }
-            float vx = ((a11 * txy + a13 * tz) + 2.f * s0) - px;
-            float vz = ((a13 * txy + a33 * tz) + 2.f * q[4]) - pz;
+            float vx = leap<ARITH>(a11 * txy + a13 * tz, s0, px);
+            float vz = leap<ARITH>(a13 * txy + a33 * tz, q[4], pz);
             if (gi == src_idx) {
                 vx = vx + src_val;
                 vz = vz + src_val;
@@ -265,6 +334,213 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
             for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Fused backward step of the 3D gradient and RTM passes.
+//
+// MODE 0 (gradient, objFuncGradient3d_f.cu:1008-1028): per point
+//   source side: gradientKernel -- sigma_next = dt2*(C*lap sigma) + 2 sigma - sigma_prev
+//                (the undamped random-boundary reconstruction, run backward in
+//                time) with gc11 += lx*Lxy, gc33 += lz*Lz, gc13 += lx*Lz + lz*Lxy
+//   receiver side: forwardKernel on lambda with the damping of the absorbing pass
+// MODE 1 (RTM, rtm3d_f.cu:585-610): forwardKernel on both fields and
+//   image += (2/3 sx + 1/3 sz)(2/3 lx + 1/3 lz), with lambda at the current
+//   level after its damping.
+// The source injection into sigma_next is fused (own point).  In the four
+// receiver planes lambda_next is stored undamped: the residual injection that
+// the reference performs before damping is applied next by
+// inject_damp3d_kernel, which then damps those planes.
+// ---------------------------------------------------------------------------
+template <int MODE, int ARITH, int BY>
+__global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
+    float *__restrict__ s_nx, const float *__restrict__ s_cx, float *__restrict__ s_nz,
+    const float *__restrict__ s_cz, float *__restrict__ l_nx, const float *__restrict__ l_cx,
+    float *__restrict__ l_nz, const float *__restrict__ l_cz, const float *__restrict__ c11,
+    const float *__restrict__ c13, const float *__restrict__ c33, float *__restrict__ g11, float *__restrict__ g13,
+    float *__restrict__ g33, const float *__restrict__ dxy_plane, const float *__restrict__ dzv, Geo3 g, int zc,
+    int ntx, int nty, float dx2, float dy2, float dz2, float dt2, float dt2dx2, float dt2dy2, float dt2dz2,
+    long long src_idx, float src_val, int zrec0)
+{
+    constexpr int W = Tile<BY>::W;
+    constexpr int R = Tile<BY>::R;
+    __shared__ float ts[2][R * W];
+    __shared__ float tl[2][R * W];
+
+    const int lb = xcd_logical_block(gridDim.x * gridDim.y * gridDim.z);
+    const int bx = lb % ntx, by = (lb / ntx) % nty, bz = lb / (ntx * nty);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x0 = bx * 64, y0 = 4 + by * BY;
+    const int zs = bz * zc;
+    const int ze = min(zs + zc, g.zmax);
+    const int ix = x0 + tx, iy = y0 + ty;
+    const bool inrange = ix < g.nx && iy < g.ny;
+    const bool active = ix >= 4 && ix < g.nx - 4 && iy < g.ny - 4;
+
+    TileLoader<BY> L;
+    L.init(tx, ty, x0, y0, g);
+    const long long col = L.main_off;
+    const float dxy = inrange ? dxy_plane[col] : 1.f;
+
+    float qs[9], ql[9];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int zz = zs - 4 + k;
+        const bool ok = inrange && zz >= 0 && zz < g.zmax;
+        qs[k] = ok ? s_cz[zz * g.plane + col] : 0.f;
+        ql[k] = ok ? l_cz[zz * g.plane + col] : 0.f;
+    }
+    {
+        PlaneRegs<BY> P;
+        P.load(s_cx + zs * g.plane, L);
+        P.store(ts[0], L);
+        P.load(l_cx + zs * g.plane, L);
+        P.store(tl[0], L);
+    }
+    __syncthreads();
+
+    for (int z0 = zs; z0 < ze; z0 += 4) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int z = z0 + r;
+            const long long gi = z * g.plane + col;
+            const bool ok4 = inrange && z + 4 < g.zmax;
+            qs[8] = ok4 ? s_cz[gi + 4 * g.plane] : 0.f;
+            ql[8] = ok4 ? l_cz[gi + 4 * g.plane] : 0.f;
+            PlaneRegs<BY> PS, PL;
+            const bool more = z + 1 < ze;
+            if (more) {
+                PS.load(s_cx + (z + 1) * g.plane, L);
+                PL.load(l_cx + (z + 1) * g.plane, L);
+            }
+            float spx = 0.f, spz = 0.f, lpx = 0.f, lpz = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f;
+            float o11 = 0.f, o13 = 0.f, o33 = 0.f;
+            if (inrange) {
+                spx = s_nx[gi];
+                spz = s_nz[gi];
+                lpx = l_nx[gi];
+                lpz = l_nz[gi];
+                a11 = c11[gi];
+                a13 = c13[gi];
+                a33 = c33[gi];
+                o11 = g11[gi];
+                if (MODE == 0) {
+                    o13 = g13[gi];
+                    o33 = g33[gi];
+                }
+            }
+            const float *TS = &ts[r & 1][(ty + 4) * W + tx + 4];
+            const float *TL = &tl[r & 1][(ty + 4) * W + tx + 4];
+            const float s0 = TS[0], l0 = TL[0];
+
+            // ---- source side
+            float svx, svz, sTxy = 0.f, sTz = 0.f;
+            if (MODE == 0) {
+                sTxy = sx_div<ARITH>(TS, dx2, dy2);
+                sTz = sz_div<ARITH>(qs, dz2);
+                svx = leap<ARITH>(dt2 * (a11 * sTxy + a13 * sTz), s0, spx);
+                svz = leap<ARITH>(dt2 * (a13 * sTxy + a33 * sTz), qs[4], spz);
+            } else {
+                const float txy = sx_scaled<ARITH>(TS, dt2dx2, dt2dy2);
+                const float tz = sz_scaled<ARITH>(qs, dt2dz2);
+                svx = leap<ARITH>(a11 * txy + a13 * tz, s0, spx);
+                svz = leap<ARITH>(a13 * txy + a33 * tz, qs[4], spz);
+            }
+            if (gi == src_idx) {
+                svx = svx + src_val;
+                svz = svz + src_val;
+            }
+
+            // ---- receiver side (forwardKernel + damping)
+            const float lTxy = sx_scaled<ARITH>(TL, dt2dx2, dt2dy2);
+            float e[9];
+            if (r == 0) effective_column<0, true>(e, ql, dxy, dzv, z);
+            if (r == 1) effective_column<1, true>(e, ql, dxy, dzv, z);
+            if (r == 2) effective_column<2, true>(e, ql, dxy, dzv, z);
+            if (r == 3) effective_column<3, true>(e, ql, dxy, dzv, z);
+            const float lTz = sz_scaled<ARITH>(e, dt2dz2);
+            const float d = dxy * dzv[z];
+            float lvx = leap<ARITH>(a11 * lTxy + a13 * lTz, l0, lpx * d);
+            float lvz = leap<ARITH>(a13 * lTxy + a33 * lTz, ql[4], lpz * d);
+            if (z < zrec0 || z >= zrec0 + 4) {
+                lvx = lvx * d;
+                lvz = lvz * d;
+            }
+
+            if (MODE == 0) {
+                o11 = o11 + l0 * sTxy;
+                o33 = o33 + ql[4] * sTz;
+                o13 = o13 + (l0 * sTz + ql[4] * sTxy);
+            } else if (ARITH == kArithF32) {
+                const float ls = VTI_TWOTHIRD_F * (l0 * d) + VTI_ONETHIRD_F * (ql[4] * d);
+                o11 = o11 + (VTI_TWOTHIRD_F * s0 + VTI_ONETHIRD_F * qs[4]) * ls;
+            } else {
+                // imagingKernel (kernel3d.cu:465): double weights, float storage
+                const double ls = VTI_TWOTHIRD_D * (double)(l0 * d) + VTI_ONETHIRD_D * (double)(ql[4] * d);
+                o11 = (float)((double)o11 + (VTI_TWOTHIRD_D * (double)s0 + VTI_ONETHIRD_D * (double)qs[4]) * ls);
+            }
+            if (active) {
+                s_nx[gi] = svx;
+                s_nz[gi] = svz;
+                l_nx[gi] = lvx;
+                l_nz[gi] = lvz;
+                g11[gi] = o11;
+                if (MODE == 0) {
+                    g13[gi] = o13;
+                    g33[gi] = o33;
+                }
+            }
+            if (more) {
+                PS.store(ts[(r + 1) & 1], L);
+                PL.store(tl[(r + 1) & 1], L);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                qs[k] = qs[k + 1];
+                ql[k] = ql[k + 1];
+            }
+        }
+    }
+}
+
+// Residual (or observed-data) injection into lambda_next at the receivers
+// (injectResidual, injectRecord3d.cu:38-46) with the time interpolation of
+// interpolateResidual (injectRecord3d.cu:60-78) evaluated on the device:
+// the coarse traces are time-major rows [it][ir]; i1 < 0 selects row nnt-1.
+__global__ void inject_res3d_kernel(float *__restrict__ lx, float *__restrict__ lz,
+                                    const long long *__restrict__ rec, const float *__restrict__ coarse, int nr,
+                                    int i1, float f, float dt2)
+{
+    const int ir = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ir >= nr) return;
+    float fine;
+    if (i1 < 0) {
+        fine = coarse[ir];
+    } else {
+        fine = (float)((1. - (double)f) * (double)coarse[(size_t)i1 * nr + ir] +
+                       (double)(f * coarse[(size_t)(i1 + 1) * nr + ir]));
+    }
+    const float s = dt2 * fine;
+    const long long i = rec[ir];
+    lx[i] = (float)((double)lx[i] + VTI_TWOTHIRD_D * (double)s);
+    lz[i] = (float)((double)lz[i] + VTI_ONETHIRD_D * (double)s);
+}
+
+// Damping of the four receiver planes after the injection (abcXYZ/abcXY).
+__global__ void damp_planes3d_kernel(float *__restrict__ lx, float *__restrict__ lz,
+                                     const float *__restrict__ dxy_plane, const float *__restrict__ dzv, Geo3 g,
+                                     int z0)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y;
+    const int z = z0 + blockIdx.z;
+    if (ix < 4 || ix >= g.nx - 4 || iy < 4 || iy >= g.ny - 4 || z >= g.zmax) return;
+    const long long col = (long long)iy * g.pitch + ix;
+    const long long gi = z * g.plane + col;
+    const float d = dxy_plane[col] * dzv[z];
+    lx[gi] = lx[gi] * d;
+    lz[gi] = lz[gi] * d;
 }
 
 }  // namespace vti

@@ -55,6 +55,13 @@ int vti_ctx_destroy(vti_ctx *c)
     return vti::kOk;
 }
 
+int vti_ctx_set_arith(vti_ctx *c, int arith)
+{
+    VTI_CHECK(c != nullptr && (arith == 0 || arith == 1), "vti_ctx_set_arith: arith must be 0 (fp32) or 1 (reference)");
+    c->arith = arith;
+    return vti::kOk;
+}
+
 int vti_ctx_synchronize(vti_ctx *c)
 {
     VTI_CHECK(c != nullptr, "null context");

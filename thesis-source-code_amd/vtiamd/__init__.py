@@ -34,6 +34,7 @@ _SIGS = {
     "vti_ctx_create": (c_int, [POINTER(c_void_p), c_int]),
     "vti_ctx_destroy": (c_int, [P]),
     "vti_ctx_synchronize": (c_int, [P]),
+    "vti_ctx_set_arith": (c_int, [P, c_int]),
     "vti_init_damping3d": (c_int, [P, c_int, c_int, c_int, c_int]),
     "vti_init_taper2d": (c_int, [P, c_int, c_int, c_int]),
     "vti_vepsdel_to_cij": (c_int, [P, P, P, P, P, P, c_float, c_float, c_float, c_longlong]),
@@ -56,6 +57,7 @@ _SIGS = {
     "vti3d_prop_destroy": (c_int, [P]),
     "vti3d_prop_bytes": (c_size_t, [P]),
     "vti3d_prop_set_zchunk": (c_int, [P, c_int]),
+    "vti3d_prop_set_arith": (c_int, [P, c_int]),
     "vti3d_prop_set_model": (c_int, [P, P, P, P]),
     "vti3d_prop_reset": (c_int, [P, c_longlong, c_float]),
     "vti3d_prop_set_state": (c_int, [P, P, P, P, P]),
@@ -64,6 +66,23 @@ _SIGS = {
     "vti3d_prop_forward_shot": (
         c_int,
         [P, c_int, P, c_longlong, c_int, P, c_int, c_int, c_int, P],
+    ),
+    "vti3d_prop_backward_steps": (c_int, [P, c_int, c_int, POINTER(c_float)]),
+    "vti3d_model_data": (
+        c_int,
+        [P, P, P, c_int, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int],
+    ),
+    "vti3d_objfunc": (
+        c_int,
+        [P, P, P, c_int, P, c_int, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int, P, c_float],
+    ),
+    "vti3d_objfunc_gradient": (
+        c_int,
+        [P, P, P, c_int, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int, P, c_float],
+    ),
+    "vti3d_rtm": (
+        c_int,
+        [P, P, P, c_int, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int, P],
     ),
 }
 
@@ -153,6 +172,10 @@ class Context:
     def sync(self):
         check(_lib.vti_ctx_synchronize(self.h), "vti_ctx_synchronize")
 
+    def set_arith(self, arith: int):
+        """1: the reference's mixed FP64 stencil arithmetic (default); 0: FP32."""
+        check(_lib.vti_ctx_set_arith(self.h, arith), "vti_ctx_set_arith")
+
     def close(self):
         if self.h:
             check(_lib.vti_ctx_destroy(self.h), "vti_ctx_destroy")
@@ -192,6 +215,9 @@ class Prop3D:
     def set_zchunk(self, zc: int):
         check(_lib.vti3d_prop_set_zchunk(self.h, zc), "vti3d_prop_set_zchunk")
 
+    def set_arith(self, arith: int):
+        check(_lib.vti3d_prop_set_arith(self.h, arith), "vti3d_prop_set_arith")
+
     def set_model(self, c11, c13, c33):
         c11, c13, c33 = f32(c11), f32(c13), f32(c33)
         check(_lib.vti3d_prop_set_model(self.h, ptr(c11), ptr(c13), ptr(c33)), "vti3d_prop_set_model")
@@ -228,3 +254,71 @@ class Prop3D:
             "vti3d_prop_forward_shot",
         )
         return np.ascontiguousarray(tr.T)
+
+    def backward_steps(self, nsteps: int, mode: int = 0, timed: bool = False):
+        ms = c_float(0.0)
+        check(
+            _lib.vti3d_prop_backward_steps(self.h, mode, nsteps, ctypes.byref(ms) if timed else None),
+            "vti3d_prop_backward_steps",
+        )
+        return ms.value if timed else None
+
+
+# ---- 3D drivers (reference semantics) ----------------------------------------
+
+def _geom3(souloc, shotid, recloc, wavelet):
+    souloc = f32(souloc).ravel()
+    recloc = f32(recloc).ravel()
+    shotid = np.ascontiguousarray(shotid, dtype=np.int32)
+    wavelet = f32(wavelet)
+    return souloc, shotid, recloc, wavelet
+
+
+def model_data3d(ctx, souloc, shotid, recloc, wavelet, c11, c13, c33, nt, npad, o, d, dt, rate, ngpu=1):
+    """modelDataCij3d_f: returns the trace-major data array (nr_total * nnt)."""
+    souloc, shotid, recloc, wavelet = _geom3(souloc, shotid, recloc, wavelet)
+    c11, c13, c33 = f32(c11), f32(c13), f32(c33)
+    nz, ny, nx = c11.shape
+    sts = int(np.round(np.float32(rate) / np.float32(dt)))
+    nnt = (nt - 1) // sts + 1
+    nrt = recloc.size // 4
+    dd = np.zeros(nrt * nnt, np.float32)
+    check(_lib.vti3d_model_data(ctx.h, ptr(dd), ptr(souloc), souloc.size // 5, ptr(shotid), shotid.size, ptr(recloc),
+                                ptr(wavelet), ptr(c11), ptr(c13), ptr(c33), nx, ny, nz, nt, npad, *o, 0.0, *d, dt,
+                                rate, ngpu), "vti3d_model_data")
+    return dd
+
+
+def objfunc_gradient3d(ctx, souloc, shotid, recloc, wavelet, cij, padb, randb, nt, npad, o, d, dt, rate, obs,
+                       ngpu=1, tpow=0.0, gradient=True):
+    """objFuncGradientCij3d_f / objFuncCij3d_f.  cij (3, nz, ny, nx) is modified in place
+    like the reference (its boundary shell ends up holding randb / padb)."""
+    souloc, shotid, recloc, wavelet = _geom3(souloc, shotid, recloc, wavelet)
+    assert cij.dtype == np.float32 and cij.flags["C_CONTIGUOUS"]
+    _, nz, ny, nx = cij.shape
+    padb, obs = f32(padb), f32(obs)
+    if gradient:
+        randb = f32(randb)
+        out = np.zeros(1 + cij.size, np.float32)
+        check(_lib.vti3d_objfunc_gradient(ctx.h, ptr(out), ptr(souloc), souloc.size // 5, ptr(shotid), shotid.size,
+                                          ptr(recloc), ptr(wavelet), ptr(cij), ptr(padb), ptr(randb), nx, ny, nz, nt,
+                                          npad, *o, 0.0, *d, dt, rate, ngpu, ptr(obs), tpow),
+              "vti3d_objfunc_gradient")
+        return float(out[0]), out[1:].reshape(3, nz, ny, nx)
+    out = np.zeros(1, np.float32)
+    check(_lib.vti3d_objfunc(ctx.h, ptr(out), ptr(souloc), souloc.size // 5, ptr(shotid), shotid.size, ptr(recloc),
+                             ptr(wavelet), ptr(cij), ptr(padb), nx, ny, nz, nt, npad, *o, 0.0, *d, dt, rate, ngpu,
+                             ptr(obs), tpow), "vti3d_objfunc")
+    return float(out[0])
+
+
+def rtm3d(ctx, souloc, shotid, recloc3, wavelet, c11, c13, c33, nt, npad, o, d, dt, rate, obs, ngpu=1):
+    """rtmCij3d_f: returns the image (nz, ny, nx).  recloc3 has 3 floats per receiver."""
+    souloc, shotid, recloc3, wavelet = _geom3(souloc, shotid, recloc3, wavelet)
+    c11, c13, c33, obs = f32(c11), f32(c13), f32(c33), f32(obs)
+    nz, ny, nx = c11.shape
+    img = np.zeros((nz, ny, nx), np.float32)
+    check(_lib.vti3d_rtm(ctx.h, ptr(img), ptr(souloc), souloc.size // 5, ptr(shotid), shotid.size, ptr(recloc3),
+                         ptr(wavelet), ptr(c11), ptr(c13), ptr(c33), nx, ny, nz, nt, npad, *o, 0.0, *d, dt, rate,
+                         ngpu, ptr(obs)), "vti3d_rtm")
+    return img
