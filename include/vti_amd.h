@@ -143,6 +143,34 @@ int vti3d_rtm(vti_ctx *ctx, float *image, const float *souloc, int ns, const int
               int ny, int nz, int nt, int npad, float ox, float oy, float oz, float ot, float dx, float dy, float dz,
               float dt, float samplingRate, int ngpu, const float *obs);
 
+/* ---- 2D drivers with the reference's semantics ----------------------------
+ * Arguments exactly as lib/vti-fwi-2d/wave.h declares them (padded grid
+ * nnx = nx+2npad, nnz = nz+2npad; sloc = 4 ints per shot {ix, iz, nr_shot,
+ * first receiver}; rloc = 2 ints per receiver {ix, iz}; taper from
+ * vti_init_taper2d).  The reference reads the GPU list from the global argv
+ * ("gpu="); all work here runs on ctx's device. */
+
+/* synthetic_f, lib/vti-fwi-2d/synthetic_f.cu:13 (wave.h:4).  data: nr columns
+ * (every shot's receivers side by side) x ((nt-ntNeg-1)/ratio+1) rows. */
+int vti2d_synthetic(vti_ctx *ctx, float *data, const float *c11, const float *c13, const float *c33,
+                    const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
+                    int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot);
+
+/* objFuncGradient_f, lib/vti-fwi-2d/objFuncGradient_f.cu:15 (wave.h:12).
+ * *objfunc receives the value the reference returns (0.5*sum r^2).
+ * Requires rate/dt >= 2, (nt-1) % (rate/dt) == 0 and ot == 0, the conditions
+ * under which the reference's snapshot buffers line up. */
+int vti2d_objfunc_gradient(vti_ctx *ctx, float *objfunc, float *gc11, float *gc13, float *gc33, const float *data,
+                           const float *c11, const float *c13, const float *c33, const float *wavelet, const int *sloc,
+                           int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad,
+                           float dx, float dz, float dt, float rate, float ot);
+
+/* GNhessian_f, lib/vti-fwi-2d/hessian_f.cu:575 (wave.h:34). */
+int vti2d_gn_hessian(vti_ctx *ctx, float *gc11, float *gc13, float *gc33, const float *data, const float *c11,
+                     const float *c13, const float *c33, const float *dc11, const float *dc13, const float *dc33,
+                     const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
+                     int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot);
+
 #ifdef __cplusplus
 }
 #endif

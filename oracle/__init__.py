@@ -161,3 +161,51 @@ def rtm3d(mode, souloc, shotid, recloc3, wavelet, c11, c13, c33, nt, npad, o, d,
     if rc != 0:
         raise RuntimeError("orc3d_rtm failed")
     return img
+
+
+def _lib2():
+    L = lib()
+    if not getattr(L, "_has2", False):
+        P = c_void_p
+        L.orc2d_synthetic.argtypes = [c_int, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5
+        L.orc2d_fg.restype = c_double
+        L.orc2d_fg.argtypes = [c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + \
+            [c_float] * 5
+        L.orc2d_forwardCD_f64.argtypes = [P] * 7 + [c_double] * 3 + [c_int, c_int]
+        L.orc2d_backwardDC_f64.argtypes = [P] * 7 + [c_double] * 3 + [c_int, c_int]
+        L._has2 = True
+    return L
+
+
+def synthetic2d(mode, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0):
+    c11, c13, c33, wavelet, taper = (np.ascontiguousarray(a, np.float32).ravel() for a in (c11, c13, c33, wavelet, taper))
+    sloc = np.ascontiguousarray(sloc, np.int32).ravel()
+    rloc = np.ascontiguousarray(rloc, np.int32).ravel()
+    ratio = int(rate / dt + 0.5)
+    ntneg = int(round(abs(ot) / dt))
+    nnt_data = (nt - ntneg - 1) // ratio + 1
+    data = np.zeros((nnt_data, rloc.size // 2), np.float64 if mode == F64 else np.float32)
+    rc = _lib2().orc2d_synthetic(mode, _p(data), _p(c11), _p(c13), _p(c33), _p(wavelet), _p(sloc), sloc.size // 4,
+                                 _p(rloc), rloc.size // 2, _p(taper), nx, nz, nt, npad, dx, dz, dt, rate, ot)
+    if rc != 0:
+        raise RuntimeError("orc2d_synthetic failed")
+    return data
+
+
+def fg2d(mode, born, data, c11, c13, c33, dc, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0,
+         ngpu=1):
+    """objFuncGradient_f (born=False) / GNhessian_f (born=True) restated literally,
+    as the reference runs them on ngpu GPUs (shots dealt round robin; each GPU
+    carries its buffers from one of its shots to the next).
+    Returns (objective, gc (3, nnz, nnx))."""
+    a = [np.ascontiguousarray(x, np.float32).ravel() for x in (c11, c13, c33, wavelet, taper)]
+    dcs = [None, None, None] if dc is None else [np.ascontiguousarray(x, np.float32).ravel() for x in dc]
+    data = None if data is None else np.ascontiguousarray(data, np.float32).ravel()
+    sloc = np.ascontiguousarray(sloc, np.int32).ravel()
+    rloc = np.ascontiguousarray(rloc, np.int32).ravel()
+    nnx, nnz = nx + 2 * npad, nz + 2 * npad
+    gc = np.zeros((3, nnz, nnx), np.float64 if mode == F64 else np.float32)
+    f = _lib2().orc2d_fg(mode, int(born), int(ngpu), _p(gc), _p(data), _p(a[0]), _p(a[1]), _p(a[2]), *[_p(x) for x in dcs],
+                         _p(a[3]), _p(sloc), sloc.size // 4, _p(rloc), rloc.size // 2, _p(a[4]), nx, nz, nt, npad, dx,
+                         dz, dt, rate, ot)
+    return f, gc

@@ -1,0 +1,372 @@
+/*
+ * Body of the 2D oracle, instantiated for T = float (the reference's own
+ * FP32 arithmetic, float constants as in lib/vti-fwi-2d/kernels.h:4-8) and
+ * T = double (FP64).  Test infrastructure only.
+ *
+ * Grid: i = ix + iz*nnx (x fastest).  Every kernel of lib/vti-fwi-2d/
+ * kernels.cu updates the interior ix in [4, nnx-4), iz in [4, nnz-4) only.
+ */
+#define S_(a, b) a##_##b
+#define S(a, b) S_(a, b)
+#define FN(name) S(name, SFX)
+
+#define ST(a, i, s) (CK0 * a[i] + CK1 * (a[i - s] + a[i + s]) + CK2 * (a[i - 2 * s] + a[i + 2 * s]) + \
+                     CK3 * (a[i - 3 * s] + a[i + 3 * s]) + CK4 * (a[i - 4 * s] + a[i + 4 * s]))
+
+#define INTERIOR(body)                                            \
+    _Pragma("omp parallel for schedule(static)")                  \
+    for (int iz = 4; iz < nnz - 4; ++iz)                          \
+        for (int ix = 4; ix < nnx - 4; ++ix) {                    \
+            const long i = ix + (long)iz * nnx;                   \
+            body                                                  \
+        }
+
+/* D, kernels.cu:345-387 */
+static void FN(D)(T *tp, T *tq, const T *p, const T *q, T dx2, T dz2, int nnx, int nnz)
+{
+    INTERIOR(tp[i] = ST(p, i, 1) / dx2; tq[i] = ST(q, i, nnx) / dz2;)
+}
+
+/* forwardC, kernels.cu:546-556 */
+static void FN(forwardC)(T *p0, T *q0, const T *p1, const T *q1, const T *tp, const T *tq, const T *c11,
+                         const T *c13, const T *c33, T dt2, int nnx, int nnz)
+{
+    INTERIOR(p0[i] = (T)2 * p1[i] + dt2 * (c11[i] * tp[i] + c13[i] * tq[i]) - p0[i];
+             q0[i] = (T)2 * q1[i] + dt2 * (c13[i] * tp[i] + c33[i] * tq[i]) - q0[i];)
+}
+
+/* forwardCD, kernels.cu:452-498 (the stencils of p1, q1 inline) */
+static void FN(forwardCD)(T *p0, T *q0, const T *p1, const T *q1, const T *c11, const T *c13, const T *c33, T dx2,
+                          T dz2, T dt2, int nnx, int nnz)
+{
+    INTERIOR(const T tp = ST(p1, i, 1) / dx2; const T tq = ST(q1, i, nnx) / dz2;
+             p0[i] = (T)2 * p1[i] + dt2 * (c11[i] * tp + c13[i] * tq) - p0[i];
+             q0[i] = (T)2 * q1[i] + dt2 * (c13[i] * tp + c33[i] * tq) - q0[i];)
+}
+
+/* backwardDC, kernels.cu:104-148: the exact transpose, D applied to C*lambda */
+static void FN(backwardDC)(T *p1, T *q1, const T *p0, const T *q0, const T *c11, const T *c13, const T *c33, T dx2,
+                           T dz2, T dt2, int nnx, int nnz, T *sp, T *sq)
+{
+    const long n = (long)nnx * nnz;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        sp[i] = c11[i] * p0[i] + c13[i] * q0[i];
+        sq[i] = c13[i] * p0[i] + c33[i] * q0[i];
+    }
+    INTERIOR(const T tp0 = ST(sp, i, 1) / dx2; const T tq0 = ST(sq, i, nnx) / dz2;
+             p1[i] = (T)2 * p0[i] + dt2 * tp0 - p1[i];
+             q1[i] = (T)2 * q0[i] + dt2 * tq0 - q1[i];)
+}
+
+/* abc, kernels.cu:293-315 */
+static void FN(abc2)(T *p, T *q, const float *taper, int nnx, int nnz)
+{
+    INTERIOR(p[i] *= (T)taper[i]; q[i] *= (T)taper[i];)
+}
+
+static void FN(abc4)(T *p1, T *q1, T *p0, T *q0, const float *taper, int nnx, int nnz)
+{
+    INTERIOR(p1[i] *= (T)taper[i]; q1[i] *= (T)taper[i]; p0[i] *= (T)taper[i]; q0[i] *= (T)taper[i];)
+}
+
+/* scattering (7-argument form), kernels.cu:282-291 */
+static void FN(scattering)(T *dp, T *dq, const T *dc11, const T *dc13, const T *dc33, const T *Dp, const T *Dq, T dt2,
+                           int nnx, int nnz)
+{
+    INTERIOR(dp[i] += dt2 * (dc11[i] * Dp[i] + dc13[i] * Dq[i]); dq[i] += dt2 * (dc13[i] * Dp[i] + dc33[i] * Dq[i]);)
+}
+
+/* gradientCrossCor, kernels.cu:438-450 */
+static void FN(xcor)(T *gc11, T *gc13, T *gc33, const T *ap, const T *aq, const T *Dp0, const T *Dq0, const T *Dp1,
+                     const T *Dq1, float f, int nnx, int nnz)
+{
+    const T ff = (T)f;
+    INTERIOR(const T Dp = ((T)1 - ff) * Dp0[i] + ff * Dp1[i]; const T Dq = ((T)1 - ff) * Dq0[i] + ff * Dq1[i];
+             gc11[i] += ap[i] * Dp; gc13[i] += ap[i] * Dq + aq[i] * Dp; gc33[i] += aq[i] * Dq;)
+}
+
+/* recordData, kernels.cu:317-324 */
+static void FN(record)(T *d, const T *p, const T *q, const int *rloc, int nr, int nnx)
+{
+    for (int ir = 0; ir < nr; ++ir) {
+        const long i = rloc[2 * ir] + (long)rloc[2 * ir + 1] * nnx;
+        d[ir] = (T)0.5 * (p[i] + q[i]);
+    }
+}
+
+/* injectData, kernels.cu:208-217 */
+static void FN(inject_data)(T *p, T *q, const T *d0, const T *d1, float f, const int *rloc, int nr, int nnx, T dt2)
+{
+    const T ff = (T)f;
+    for (int ir = 0; ir < nr; ++ir) {
+        const long i = rloc[2 * ir] + (long)rloc[2 * ir + 1] * nnx;
+        const T s = (T)0.5 * (((T)1 - ff) * d0[ir] + ff * d1[ir]);
+        p[i] += dt2 * s;
+        q[i] += dt2 * s;
+    }
+}
+
+#define SWAP(a, b)    \
+    do {              \
+        T *t__ = (a); \
+        (a) = (b);    \
+        (b) = t__;    \
+    } while (0)
+
+/* synthetic_f, lib/vti-fwi-2d/synthetic_f.cu:13-152, one GPU */
+static void FN(synthetic)(T *data, const T *c11, const T *c13, const T *c33, const float *wavelet, const int *sloc,
+                          int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad,
+                          float dx, float dz, float dt, float rate, float ot)
+{
+    const int ratio = (int)(rate / dt + 0.5);
+    const int ntNeg = (int)round(fabs(ot) / dt);
+    const int nnx = nx + 2 * npad, nnz = nz + 2 * npad;
+    const long nnxz = (long)nnx * nnz;
+    const float dx2f = dx * dx, dz2f = dz * dz, dt2f = dt * dt;
+    const T dx2 = (T)dx2f, dz2 = (T)dz2f, dt2 = (T)dt2f;
+    T *p0 = (T *)calloc(nnxz, sizeof(T)), *q0 = (T *)calloc(nnxz, sizeof(T));
+    T *p1 = (T *)calloc(nnxz, sizeof(T)), *q1 = (T *)calloc(nnxz, sizeof(T));
+    T *d = (T *)malloc(sizeof(T) * (nr > 0 ? nr : 1));
+    for (int is = 0; is < ns; ++is) {
+        const long s = sloc[4 * is] + (long)sloc[4 * is + 1] * nnx;
+        const int nri = sloc[4 * is + 2], r0 = sloc[4 * is + 3];
+        const int *rl = rloc + 2 * r0;
+        memset(p0, 0, sizeof(T) * nnxz);
+        memset(q0, 0, sizeof(T) * nnxz);
+        memset(p1, 0, sizeof(T) * nnxz);
+        memset(q1, 0, sizeof(T) * nnxz);
+        const float v0 = dt2f * wavelet[0];
+        p1[s] += (T)v0;
+        q1[s] += (T)v0;
+        FN(abc2)(p1, q1, taper, nnx, nnz);
+        if (ratio == 1 && ot == 0.f) {
+            FN(record)(d, p1, q1, rl, nri, nnx);
+            for (int ir = 0; ir < nri; ++ir) data[nr + r0 + ir] = d[ir];
+        }
+        for (int it = 2; it < nt; ++it) {
+            const float t = it * dt + ot;
+            FN(forwardCD)(p0, q0, p1, q1, c11, c13, c33, dx2, dz2, dt2, nnx, nnz);
+            const float v = dt2f * wavelet[it - 1];
+            p0[s] += (T)v;
+            q0[s] += (T)v;
+            FN(abc4)(p1, q1, p0, q0, taper, nnx, nnz);
+            if (t >= 0. && (it - ntNeg) % ratio == 0) {
+                FN(record)(d, p0, q0, rl, nri, nnx);
+                for (int ir = 0; ir < nri; ++ir) data[(long)((it - ntNeg) / ratio) * nr + r0 + ir] = d[ir];
+            }
+            SWAP(p0, p1);
+            SWAP(q0, q1);
+        }
+    }
+    free(p0);
+    free(q0);
+    free(p1);
+    free(q1);
+    free(d);
+}
+
+/*
+ * objFuncGradient_f (objFuncGradient_f.cu:15-371) and GNhessian_f
+ * (hessian_f.cu:575-1010), one GPU, restated literally: the device buffers
+ * d_Dpq/d_Dpqa/d_Dpqb and the pinned host buffers Dpqa/Dpqb rotate exactly as
+ * the reference's pointer juggling does, the host snapshot arrays Dp/Dq and
+ * all of these persist from one shot to the next, and the asynchronous
+ * copies happen at the points the reference's stream synchronisations pin.
+ * born == 0: gradient (backwardDC adjoint of the residual).  born == 1: the
+ * Gauss-Newton Hessian-vector product (background + Born forward, forwardCD
+ * adjoint of the Born data).  born == 2 (a check only, no reference
+ * counterpart): the gradient path with the Hessian's forwardCD adjoint, so
+ * that GNhessian_f can be checked as that adjoint applied to Born data
+ * obtained independently.  Returns 0.5*sum r^2 (gradient) as the reference
+ * returns it.
+ */
+static double FN(fg_gpu)(int born, int is0, int stride, T *gc11, T *gc13, T *gc33, const float *data, const T *c11, const T *c13,
+                           const T *c33, const T *dc11, const T *dc13, const T *dc33, const float *wavelet,
+                           const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
+                           int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    const int ratio = (int)(rate / dt + 0.5f);
+    const int ntNeg = (int)round(fabs(ot) / dt);
+    const int nnt = (nt - 1) / ratio + 1;
+    const int nnt_data = (nt - ntNeg - 1) / ratio + 1;
+    const int nnx = nx + 2 * npad, nnz = nz + 2 * npad;
+    const long nnxz = (long)nnx * nnz;
+    const float dx2f = dx * dx, dz2f = dz * dz, dt2f = dt * dt;
+    const T dx2 = (T)dx2f, dz2 = (T)dz2f, dt2 = (T)dt2f;
+    double objfunc = 0;
+
+    T *p0 = (T *)calloc(nnxz, sizeof(T)), *q0 = (T *)calloc(nnxz, sizeof(T));
+    T *p1 = (T *)calloc(nnxz, sizeof(T)), *q1 = (T *)calloc(nnxz, sizeof(T));
+    T *dp0 = (T *)calloc(nnxz, sizeof(T)), *dq0 = (T *)calloc(nnxz, sizeof(T));
+    T *dp1 = (T *)calloc(nnxz, sizeof(T)), *dq1 = (T *)calloc(nnxz, sizeof(T));
+    T *sp = (T *)calloc(nnxz, sizeof(T)), *sq = (T *)calloc(nnxz, sizeof(T));
+    T *d_Dpq = (T *)calloc(2 * nnxz, sizeof(T));
+    T *d_Ddpq = (T *)calloc(2 * nnxz, sizeof(T));
+    T *d_Dpqa = (T *)calloc(2 * nnxz, sizeof(T));
+    T *d_Dpqb = (T *)calloc(2 * nnxz, sizeof(T));
+    T *Dp = (T *)calloc(nnxz * nnt, sizeof(T)), *Dq = (T *)calloc(nnxz * nnt, sizeof(T));
+    T *Dpqa = (T *)calloc(2 * nnxz, sizeof(T)), *Dpqb = (T *)calloc(2 * nnxz, sizeof(T));
+    T *gd11 = (T *)calloc(nnxz, sizeof(T)), *gd13 = (T *)calloc(nnxz, sizeof(T)), *gd33 = (T *)calloc(nnxz, sizeof(T));
+    T *d_Dp0 = d_Dpqa, *d_Dq0 = d_Dpqa + nnxz, *d_Dp1 = d_Dpq, *d_Dq1 = d_Dpq + nnxz;
+
+    for (int is = is0; is < ns; is += stride) {
+        const long slocxz = sloc[4 * is] + (long)sloc[4 * is + 1] * nnx;
+        const int nri = sloc[4 * is + 2];
+        const int *rl = rloc + 2 * sloc[4 * is + 3];
+        /* bgdata / borndata: pinned memory the reference never writes at sample 0 (zero here) */
+        T *bg = (T *)calloc((size_t)(nri > 0 ? nri : 1) * nnt_data, sizeof(T));
+        T *drec = (T *)calloc(nri > 0 ? nri : 1, sizeof(T));
+        memset(p0, 0, sizeof(T) * nnxz);
+        memset(q0, 0, sizeof(T) * nnxz);
+        memset(p1, 0, sizeof(T) * nnxz);
+        memset(q1, 0, sizeof(T) * nnxz);
+        if (born == 1) {
+            memset(dp0, 0, sizeof(T) * nnxz);
+            memset(dq0, 0, sizeof(T) * nnxz);
+            memset(dp1, 0, sizeof(T) * nnxz);
+            memset(dq1, 0, sizeof(T) * nnxz);
+        }
+        {
+            const float v0 = dt2f * wavelet[0];
+            p1[slocxz] += (T)v0;
+            q1[slocxz] += (T)v0;
+        }
+        FN(D)(d_Dpq, d_Dpq + nnxz, p1, q1, dx2, dz2, nnx, nnz);
+        for (int it = 2; it < nt; ++it) {
+            const float t = it * dt + ot;
+            FN(forwardC)(p0, q0, p1, q1, d_Dpq, d_Dpq + nnxz, c11, c13, c33, dt2, nnx, nnz);
+            if (born == 1) FN(forwardC)(dp0, dq0, dp1, dq1, d_Ddpq, d_Ddpq + nnxz, c11, c13, c33, dt2, nnx, nnz);
+            const float v = dt2f * wavelet[it - 1];
+            p0[slocxz] += (T)v;
+            q0[slocxz] += (T)v;
+            if (born == 1) FN(scattering)(dp0, dq0, dc11, dc13, dc33, d_Dpq, d_Dpq + nnxz, dt2, nnx, nnz);
+            FN(abc4)(p1, q1, p0, q0, taper, nnx, nnz);
+            if (born == 1) FN(abc4)(dp1, dq1, dp0, dq0, taper, nnx, nnz);
+            FN(D)(d_Dpq, d_Dpq + nnxz, p0, q0, dx2, dz2, nnx, nnz);
+            if (born == 1) FN(D)(d_Ddpq, d_Ddpq + nnxz, dp0, dq0, dx2, dz2, nnx, nnz);
+            if (t >= 0.f && (it - ntNeg) % ratio == 0) {
+                if (born == 1) FN(record)(drec, dp0, dq0, rl, nri, nnx);
+                else FN(record)(drec, p0, q0, rl, nri, nnx);
+                memcpy(bg + (long)((it - ntNeg) / ratio) * nri, drec, sizeof(T) * nri);
+            }
+            if (it % ratio == 0) {
+                if (it != nt - 1) memcpy(d_Dpqa, d_Dpq, sizeof(T) * 2 * nnxz);
+                if (it > ratio) {
+                    memcpy(Dp + (long)(it / ratio - 1) * nnxz, Dpqa, sizeof(T) * nnxz);
+                    memcpy(Dq + (long)(it / ratio - 1) * nnxz, Dpqa + nnxz, sizeof(T) * nnxz);
+                }
+            }
+            if (it % ratio == 1 && it != nt - ratio) memcpy(Dpqa, d_Dpqa, sizeof(T) * 2 * nnxz);
+            SWAP(p0, p1);
+            SWAP(q0, q1);
+            if (born == 1) {
+                SWAP(dp0, dp1);
+                SWAP(dq0, dq1);
+            }
+        }
+        if (born != 1) {
+            for (int it = 0; it < nnt_data; ++it)
+                for (int ir = 0; ir < nri; ++ir) {
+                    T r = bg[(long)it * nri + ir] - (T)data[(long)it * nr + sloc[4 * is + 3] + ir];
+                    bg[(long)it * nri + ir] = r;
+                    objfunc += (double)(r * r);
+                }
+        }
+        memset(p0, 0, sizeof(T) * nnxz);
+        memset(q0, 0, sizeof(T) * nnxz);
+        memset(p1, 0, sizeof(T) * nnxz);
+        memset(q1, 0, sizeof(T) * nnxz);
+        T *r0 = (T *)calloc(nri > 0 ? nri : 1, sizeof(T)), *r1 = (T *)calloc(nri > 0 ? nri : 1, sizeof(T));
+        memcpy(r0, bg + (long)(nnt_data - 1) * nri, sizeof(T) * nri);
+        FN(inject_data)(p0, q0, r0, r1, 0.f, rl, nri, nnx, dt2);
+        float f = (nt - 2.) / ratio;
+        int i1 = (int)f;
+        memcpy(Dpqb, Dp + (long)(nnt - 3) * nnxz, sizeof(T) * nnxz);
+        memcpy(Dpqb + nnxz, Dq + (long)(nnt - 3) * nnxz, sizeof(T) * nnxz);
+        f = f - i1;
+        FN(xcor)(gd11, gd13, gd33, p0, q0, d_Dp0, d_Dq0, d_Dp1, d_Dq1, f, nnx, nnz);
+        for (int it = nt - 3; it >= 0; --it) {
+            if (born) FN(forwardCD)(p1, q1, p0, q0, c11, c13, c33, dx2, dz2, dt2, nnx, nnz);
+            else FN(backwardDC)(p1, q1, p0, q0, c11, c13, c33, dx2, dz2, dt2, nnx, nnz, sp, sq);
+            if (it % ratio == 0) {
+                SWAP(Dpqa, Dpqb);
+                memcpy(d_Dpqb, Dpqa, sizeof(T) * 2 * nnxz);
+                if (it >= 2 * ratio) {
+                    memcpy(Dpqb, Dp + (long)(it / ratio - 2) * nnxz, sizeof(T) * nnxz);
+                    memcpy(Dpqb + nnxz, Dq + (long)(it / ratio - 2) * nnxz, sizeof(T) * nnxz);
+                }
+            }
+            if (it >= ntNeg) {
+                f = (it - ntNeg + 1.) / ratio;
+                i1 = (int)f;
+                if ((it - ntNeg + 2) % ratio == 0) {
+                    memcpy(r1, bg + (long)i1 * nri, sizeof(T) * nri);
+                    SWAP(r0, r1);
+                }
+                f = f - i1;
+                FN(inject_data)(p1, q1, r0, r1, f, rl, nri, nnx, dt2);
+            }
+            FN(abc4)(p1, q1, p0, q0, taper, nnx, nnz);
+            f = (float)it / ratio;
+            i1 = (int)f;
+            if ((it + 1) % ratio == 0) {
+                T *tt = d_Dpqb;
+                d_Dpqb = d_Dpq;
+                d_Dpq = d_Dpqa;
+                d_Dpqa = tt;
+                d_Dp1 = d_Dpq;
+                d_Dq1 = d_Dpq + nnxz;
+                d_Dp0 = d_Dpqa;
+                d_Dq0 = d_Dpqa + nnxz;
+            }
+            f = f - i1;
+            FN(xcor)(gd11, gd13, gd33, p1, q1, d_Dp0, d_Dq0, d_Dp1, d_Dq1, f, nnx, nnz);
+            SWAP(p0, p1);
+            SWAP(q0, q1);
+        }
+        free(r0);
+        free(r1);
+        free(bg);
+        free(drec);
+    }
+    /* the host sums the devices' gradients (objFuncGradient_f.cu:325-330) */
+    for (long i = 0; i < nnxz; ++i) {
+        gc11[i] += gd11[i];
+        gc13[i] += gd13[i];
+        gc33[i] += gd33[i];
+    }
+    free(p0); free(q0); free(p1); free(q1);
+    free(dp0); free(dq0); free(dp1); free(dq1);
+    free(sp); free(sq);
+    /* the rotation may have moved the original allocations between the names */
+    free(d_Dpq); free(d_Ddpq); free(d_Dpqa); free(d_Dpqb);
+    free(Dp); free(Dq); free(Dpqa); free(Dpqb);
+    free(gd11); free(gd13); free(gd33);
+    return objfunc;
+}
+
+/* The reference deals shot is = pass*nGPUs + i to GPU i (objFuncGradient_f.cu:
+ * 140-150); each GPU keeps its buffers from one of its shots to the next. */
+static double FN(fg_or_gn)(int born, int ngpu, T *gc11, T *gc13, T *gc33, const float *data, const T *c11,
+                           const T *c13, const T *c33, const T *dc11, const T *dc13, const T *dc33,
+                           const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper,
+                           int nx, int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    const long nnxz = (long)(nx + 2 * npad) * (nz + 2 * npad);
+    memset(gc11, 0, sizeof(T) * nnxz);
+    memset(gc13, 0, sizeof(T) * nnxz);
+    memset(gc33, 0, sizeof(T) * nnxz);
+    double obj = 0;
+    for (int g = 0; g < ngpu && g < ns; ++g)
+        obj += FN(fg_gpu)(born, g, ngpu, gc11, gc13, gc33, data, c11, c13, c33, dc11, dc13, dc33, wavelet, sloc, ns,
+                          rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot);
+    return obj;
+}
+
+#undef SWAP
+#undef INTERIOR
+#undef ST
+#undef FN
+#undef S
+#undef S_

@@ -1,0 +1,106 @@
+"""2D drivers on the GPU against the oracle (lib/vti-fwi-2d restated).
+
+synthetic_f, objFuncGradient_f and GNhessian_f are bit-exact with the
+oracle's FP32 restatement of the reference (one shot per GPU, so that the
+cross-shot sum is one rounding as in the HIP path's FP64 sum of two shots),
+and each result states its error against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from common2d import args2d, problem2d
+from common3d import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def P(vti):
+    return problem2d(vti)
+
+
+@pytest.fixture(scope="module")
+def obs(orc, P):
+    return orc.synthetic2d(orc.F64, *P["ctrue"], P["w"], P["sloc"], P["rloc"], P["taper"], *args2d(P)).astype(
+        np.float32)
+
+
+def test_synthetic_bitexact(vti, orc, ctx, P):
+    _say("2d synthetic", P["nnz"], P["nnx"], P["nt"])
+    d = vti.synthetic2d(ctx, *P["c"], P["w"], P["sloc"], P["rloc"], P["taper"], *args2d(P))
+    o = orc.synthetic2d(orc.F32, *P["c"], P["w"], P["sloc"], P["rloc"], P["taper"], *args2d(P))
+    o64 = orc.synthetic2d(orc.F64, *P["c"], P["w"], P["sloc"], P["rloc"], P["taper"], *args2d(P))
+    assert np.abs(o).max() > 0
+    assert np.array_equal(d, o)
+    assert rel(d, o64) < 4e-5  # the reference's FP32 itself: 2.0e-5
+
+
+def test_synthetic_ratio1_and_negative_time(vti, orc, ctx, P):
+    """rate == dt records level 1 too; ot < 0 shifts the recorded window."""
+    for rate, ot in ((1e-3, 0.0), (4e-3, -0.02)):
+        _say("2d synthetic rate", rate, "ot", ot)
+        a = list(args2d(P))
+        a[-1] = rate
+        d = vti.synthetic2d(ctx, *P["c"], P["w"], P["sloc"], P["rloc"], P["taper"], *a, ot=ot)
+        o = orc.synthetic2d(orc.F32, *P["c"], P["w"], P["sloc"], P["rloc"], P["taper"], *a, ot=ot)
+        assert np.abs(o).max() > 0
+        assert np.array_equal(d, o)
+
+
+@pytest.mark.parametrize("nshot", [1, 2])
+@pytest.mark.parametrize("kind", ["zero", "model"])
+def test_objfunc_gradient(vti, orc, ctx, P, obs, nshot, kind):
+    _say("2d gradient nshot", nshot, kind)
+    sloc = P["sloc"][:nshot]
+    data = obs if kind == "model" else np.zeros_like(obs)
+    f, g = vti.objfunc_gradient2d(ctx, data, *P["c"], P["w"], sloc, P["rloc"], P["taper"], *args2d(P))
+    fo, go = orc.fg2d(orc.F32, False, data, *P["c"], None, P["w"], sloc, P["rloc"], P["taper"], *args2d(P),
+                      ngpu=nshot)
+    f64, g64 = orc.fg2d(orc.F64, False, data, *P["c"], None, P["w"], sloc, P["rloc"], P["taper"], *args2d(P))
+    assert np.abs(go).max() > 0
+    assert np.array_equal(g, go)
+    assert np.float32(f) == np.float32(fo)
+    e = rel(g, g64)
+    _say("  rel vs FP64", e)
+    # zero data: well conditioned (1.1e-5).  Model data: the residual is 0.2% of
+    # the data, so FP32 trace rounding shows at 1e-2 of it (as in the reference)
+    assert e < (1e-4 if kind == "zero" else 3e-2)
+
+
+@pytest.mark.parametrize("nshot", [1, 2])
+def test_gn_hessian(vti, orc, ctx, P, nshot):
+    _say("2d GN Hv nshot", nshot)
+    rng = np.random.default_rng(7)
+    dc = [(rng.standard_normal((P["nnz"], P["nnx"])) * P["c"][k].mean() * 0.01).astype(np.float32) for k in range(3)]
+    sloc = P["sloc"][:nshot]
+    h = vti.gn_hessian2d(ctx, *P["c"], *dc, P["w"], sloc, P["rloc"], P["taper"], *args2d(P))
+    ho = orc.fg2d(orc.F32, True, None, *P["c"], dc, P["w"], sloc, P["rloc"], P["taper"], *args2d(P), ngpu=nshot)[1]
+    h64 = orc.fg2d(orc.F64, True, None, *P["c"], dc, P["w"], sloc, P["rloc"], P["taper"], *args2d(P), ngpu=nshot)[1]
+    assert np.abs(ho).max() > 0
+    assert np.array_equal(h, ho)
+    e = rel(h, h64)
+    _say("  rel vs FP64", e)
+    assert e < 1e-4
+
+
+def test_rejects_unaligned_schedule(vti, ctx, P, obs):
+    a = list(args2d(P))
+    a[2] = P["nt"] + 1  # (nt-1) % ratio != 0
+    with pytest.raises(vti.VtiError, match="snapshot"):
+        vti.objfunc_gradient2d(ctx, obs, *P["c"], P["w"], P["sloc"], P["rloc"], P["taper"], *a)
+
+
+def test_synthetic_cfg1_size(vti, orc, ctx):
+    """256 x 256 interior + 40-point boundary (336 x 336), 2000 steps, one
+    shot, 256 receivers at z=10: bit-exact with the oracle."""
+    P = problem2d(vti, nx=256, nz=256, npad=40, nt=2001, ns=1, rate=4e-3, rec_step=1, seed=9)
+    P["sloc"] = np.array([[40 + 128, 40 + 20, 256, 0]], np.int32)  # source at (z=20, x=128)
+    P["rloc"] = np.array([[40 + ix, 40 + 10] for ix in range(256)], np.int32)  # receivers at z=10
+    _say("2d synthetic 336x336x2000")
+    d = vti.synthetic2d(ctx, *P["c"], P["w"], P["sloc"], P["rloc"], P["taper"], *args2d(P))
+    o = orc.synthetic2d(orc.F32, *P["c"], P["w"], P["sloc"], P["rloc"], P["taper"], *args2d(P))
+    assert np.abs(o).max() > 0
+    assert np.array_equal(d, o)

@@ -1,0 +1,499 @@
+// 2D VTI drivers on one MI355X with the reference's signatures and results:
+// synthetic_f, objFuncGradient_f and GNhessian_f of lib/vti-fwi-2d.
+//
+// Where the reference streams D snapshots between device and pinned host
+// buffers through a three-buffer rotation (objFuncGradient_f.cu:104-300),
+// all snapshots of a shot stay in HBM (cfg2: 751 x 2 x 0.65 M floats =
+// 3.9 GB) and the backward pass reads the two that bracket each level.  For
+// ratio >= 2 and (nt-1) % ratio == 0 -- the conditions the reference's
+// rotation needs to line its buffers up -- this is the same schedule: level
+// it is imaged against (1-f) D(ratio*k) + f D(ratio*(k+1)), k = it/ratio.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "vti2d_kernels.h"
+#include "vti_ctx.h"
+#include "vti_internal.h"
+
+using vti::Geo2;
+using vti::kBY2;
+
+namespace {
+
+__global__ void set2_kernel(float *a, float *b, long long i, float v)
+{
+    a[i] = v;
+    b[i] = v;
+}
+
+// recordData (kernels.cu:317-324) into one time-major row
+__global__ void record2d_kernel(float *__restrict__ row, const float *__restrict__ p, const float *__restrict__ q,
+                                const long long *__restrict__ idx, int nr)
+{
+    const int ir = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ir < nr) row[ir] = 0.5f * (p[idx[ir]] + q[idx[ir]]);
+}
+
+// injectData (kernels.cu:208-217) at arbitrary receiver points (first adjoint level)
+__global__ void inject2d_kernel(float *__restrict__ p, float *__restrict__ q, const long long *__restrict__ idx,
+                                const float *__restrict__ r0, const float *__restrict__ r1, float fr, int nr,
+                                float dt2)
+{
+    const int ir = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ir < nr) {
+        const float s = 0.5f * ((1.f - fr) * r0[ir] + fr * r1[ir]);
+        p[idx[ir]] += dt2 * s;
+        q[idx[ir]] += dt2 * s;
+    }
+}
+
+// residual rows r = modeled - observed and partial sums of r*r
+// (objFuncGradient_f.cu:214-219); rows time-major [it][ir]
+__global__ void residual2d_kernel(float *__restrict__ bg, const float *__restrict__ obs, long long n,
+                                  double *__restrict__ partial)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+        const float r = bg[k] - obs[k];
+        bg[k] = r;
+        acc += (double)(r * r);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void acc64_2d_kernel(double *__restrict__ acc, const float *__restrict__ v, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        acc[i] += (double)v[i];
+}
+
+__global__ void narrow2d_kernel(float *__restrict__ out, const double *__restrict__ acc, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = (float)acc[i];
+}
+
+// RAII device buffer
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t count, hipStream_t s)
+    {
+        n = std::max<size_t>(count, 1);
+        hipError_t e = hipMalloc((void **)&p, n * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            vti::set_error("hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+            return vti::kErrNoMem;
+        }
+        e = hipMemsetAsync(p, 0, n * sizeof(T), s);
+        if (e != hipSuccess) {
+            vti::set_error("hipMemsetAsync: %s", hipGetErrorString(e));
+            return vti::kErrHip;
+        }
+        return vti::kOk;
+    }
+};
+
+struct Call2 {
+    vti_ctx *ctx;
+    Geo2 g;
+    int nx, nz, npad, nt, ratio, ntNeg, nnt, nnt_data;
+    long long nnxz;
+    float dt, ot, dx2, dz2, dt2;
+    dim3 grid, block;
+};
+
+int setup(Call2 &c, vti_ctx *ctx, int nx, int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot,
+          bool fwd_ratio_f)
+{
+    VTI_CHECK(ctx, "null context");
+    VTI_CHECK(nx > 0 && nz > 0 && npad >= 4 && nt >= 3, "bad grid/time sizes");
+    VTI_CHECK(dx > 0 && dz > 0 && dt > 0 && rate > 0, "spacings and rates must be positive");
+    c.ctx = ctx;
+    c.nx = nx;
+    c.nz = nz;
+    c.npad = npad;
+    c.nt = nt;
+    c.g.nnx = nx + 2 * npad;
+    c.g.nnz = nz + 2 * npad;
+    c.nnxz = (long long)c.g.nnx * c.g.nnz;
+    // synthetic_f rounds rate/dt+0.5 in double, objFuncGradient_f in float
+    c.ratio = fwd_ratio_f ? (int)(rate / dt + 0.5f) : (int)(rate / dt + 0.5);
+    c.ntNeg = (int)std::round(std::fabs(ot) / dt);
+    VTI_CHECK(c.ratio >= 1, "sampling rate below the time step");
+    c.nnt = (nt - 1) / c.ratio + 1;
+    c.nnt_data = (nt - c.ntNeg - 1) / c.ratio + 1;
+    c.dt = dt;
+    c.ot = ot;
+    c.dx2 = dx * dx;
+    c.dz2 = dz * dz;
+    c.dt2 = dt * dt;
+    c.block = dim3(64, kBY2);
+    c.grid = dim3((c.g.nnx - 8 + 63) / 64, (c.g.nnz - 8 + kBY2 - 1) / kBY2);
+    VTI_HIP(hipSetDevice(ctx->device));
+    (void)hipGetLastError();
+    return vti::kOk;
+}
+
+bool interior(const Call2 &c, int ix, int iz) { return ix >= 4 && ix < c.g.nnx - 4 && iz >= 4 && iz < c.g.nnz - 4; }
+
+int upload(const Call2 &c, void *dst, const void *src, size_t bytes)
+{
+    VTI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.ctx->stream));
+    return vti::kOk;
+}
+
+int launch_blocks(long long n) { return (int)std::min<long long>(2048, std::max<long long>(1, (n + 255) / 256)); }
+
+struct Shot2 {
+    long long src;
+    int ix, iz, nri, r0;
+    std::vector<long long> idx;
+};
+
+int shot_of(const Call2 &c, const int *sloc, int is, const int *rloc, int nr, Shot2 &s)
+{
+    s.ix = sloc[4 * is];
+    s.iz = sloc[4 * is + 1];
+    s.nri = sloc[4 * is + 2];
+    s.r0 = sloc[4 * is + 3];
+    VTI_CHECK(s.nri >= 0 && s.r0 >= 0 && s.r0 + s.nri <= nr, "shot %d: receiver range outside rloc", is);
+    VTI_CHECK(interior(c, s.ix, s.iz), "shot %d: source (%d,%d) outside the updated interior", is, s.ix, s.iz);
+    s.src = s.ix + (long long)s.iz * c.g.nnx;
+    s.idx.resize(s.nri);
+    for (int ir = 0; ir < s.nri; ++ir) {
+        const int x = rloc[2 * (s.r0 + ir)], z = rloc[2 * (s.r0 + ir) + 1];
+        VTI_CHECK(x >= 0 && x < c.g.nnx && z >= 0 && z < c.g.nnz, "receiver %d outside the grid", s.r0 + ir);
+        s.idx[ir] = x + (long long)z * c.g.nnx;
+    }
+    return vti::kOk;
+}
+
+struct Fields2 {
+    DBuf<float> c11, c13, c33, taper, p[2], q[2];
+};
+
+int load_model(const Call2 &c, Fields2 &F, const float *c11, const float *c13, const float *c33, const float *taper)
+{
+    const size_t n = (size_t)c.nnxz;
+    int rc = F.c11.alloc(n, c.ctx->stream);
+    if (!rc) rc = F.c13.alloc(n, c.ctx->stream);
+    if (!rc) rc = F.c33.alloc(n, c.ctx->stream);
+    if (!rc) rc = F.taper.alloc(n, c.ctx->stream);
+    for (int k = 0; k < 2 && !rc; ++k) {
+        rc = F.p[k].alloc(n, c.ctx->stream);
+        if (!rc) rc = F.q[k].alloc(n, c.ctx->stream);
+    }
+    if (rc) return rc;
+    rc = upload(c, F.c11.p, c11, n * 4);
+    if (!rc) rc = upload(c, F.c13.p, c13, n * 4);
+    if (!rc) rc = upload(c, F.c33.p, c33, n * 4);
+    if (!rc) rc = upload(c, F.taper.p, taper, n * 4);
+    return rc;
+}
+
+int zero2(const Call2 &c, DBuf<float> *a, DBuf<float> *b)
+{
+    for (int k = 0; k < 2; ++k) {
+        VTI_HIP(hipMemsetAsync(a[k].p, 0, a[k].n * 4, c.ctx->stream));
+        VTI_HIP(hipMemsetAsync(b[k].p, 0, b[k].n * 4, c.ctx->stream));
+    }
+    return vti::kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
+// synthetic_f, lib/vti-fwi-2d/synthetic_f.cu:13-152 (wave.h:4).  data: nr
+// columns (all shots' receivers) x nnt_data rows, time-major; rows a shot
+// never records keep their input values, as in the reference.
+int vti2d_synthetic(vti_ctx *ctx, float *data, const float *c11, const float *c13, const float *c33,
+                    const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
+                    int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    Call2 c;
+    int rc = setup(c, ctx, nx, nz, nt, npad, dx, dz, dt, rate, ot, false);
+    if (rc) return rc;
+    VTI_CHECK(data && c11 && c13 && c33 && wavelet && sloc && rloc && taper && ns >= 0, "vti2d_synthetic: null argument");
+    Fields2 F;
+    rc = load_model(c, F, c11, c13, c33, taper);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    for (int is = 0; is < ns; ++is) {
+        Shot2 s;
+        rc = shot_of(c, sloc, is, rloc, nr, s);
+        if (rc) return rc;
+        DBuf<long long> idx;
+        DBuf<float> tr;
+        rc = idx.alloc(s.nri, st);
+        if (!rc) rc = tr.alloc((size_t)s.nri * c.nnt_data, st);
+        if (!rc && s.nri) rc = upload(c, idx.p, s.idx.data(), sizeof(long long) * s.nri);
+        if (rc) return rc;
+        rc = zero2(c, F.p, F.q);
+        if (rc) return rc;
+        // level 1: dt2*w[0] at the source, then abc (synthetic_f.cu:86-89)
+        const float v0 = c.dt2 * wavelet[0];
+        hipLaunchKernelGGL(set2_kernel, dim3(1), dim3(1), 0, st, F.p[0].p, F.q[0].p, s.src, v0 * taper[s.src]);
+        std::vector<char> rec_row(c.nnt_data, 0);
+        auto record = [&](int row) {
+            hipLaunchKernelGGL(record2d_kernel, dim3((s.nri + 255) / 256), dim3(256), 0, st, tr.p + (size_t)row * s.nri,
+                               F.p[0].p, F.q[0].p, idx.p, s.nri);
+            rec_row[row] = 1;
+        };
+        if (c.ratio == 1 && ot == 0.f && s.nri && 1 < c.nnt_data) record(1);
+        for (int it = 2; it < nt; ++it) {
+            const float t = it * dt + ot;
+            hipLaunchKernelGGL((vti::fwd2d_kernel<false>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
+                               F.q[0].p, nullptr, nullptr, nullptr, nullptr, F.c11.p, F.c13.p, F.c33.p, nullptr,
+                               nullptr, nullptr, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, c.dt2 * wavelet[it - 1],
+                               nullptr, nullptr);
+            std::swap(F.p[0].p, F.p[1].p);
+            std::swap(F.q[0].p, F.q[1].p);
+            if (t >= 0. && (it - c.ntNeg) % c.ratio == 0 && s.nri) {
+                const int row = (it - c.ntNeg) / c.ratio;
+                if (row < c.nnt_data) record(row);
+            }
+        }
+        VTI_HIP(hipGetLastError());
+        std::vector<float> h((size_t)s.nri * c.nnt_data);
+        if (s.nri) VTI_HIP(hipMemcpyAsync(h.data(), tr.p, h.size() * 4, hipMemcpyDeviceToHost, st));
+        VTI_HIP(hipStreamSynchronize(st));
+        for (int row = 0; row < c.nnt_data; ++row)
+            if (rec_row[row])
+                for (int ir = 0; ir < s.nri; ++ir) data[(size_t)row * nr + s.r0 + ir] = h[(size_t)row * s.nri + ir];
+    }
+    return vti::kOk;
+}
+
+}  // extern "C"
+
+namespace {
+
+// objFuncGradient_f (born = false) and GNhessian_f (born = true).
+int fg2d(vti_ctx *ctx, bool born, float *objfunc, float *gc11, float *gc13, float *gc33, const float *data,
+         const float *c11, const float *c13, const float *c33, const float *dc11, const float *dc13, const float *dc33,
+         const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz,
+         int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    Call2 c;
+    int rc = setup(c, ctx, nx, nz, nt, npad, dx, dz, dt, rate, ot, true);
+    if (rc) return rc;
+    VTI_CHECK(gc11 && gc13 && gc33 && c11 && c13 && c33 && wavelet && sloc && rloc && taper && ns >= 0,
+              "null argument");
+    VTI_CHECK(born || data, "objective/gradient needs the observed data");
+    VTI_CHECK(!born || (dc11 && dc13 && dc33), "Hessian-vector product needs dc11, dc13, dc33");
+    VTI_CHECK(c.ratio >= 2 && (nt - 1) % c.ratio == 0 && c.ntNeg == 0,
+              "the reference's snapshot rotation lines up only for rate/dt >= 2, (nt-1) %% (rate/dt) == 0 and ot == 0 "
+              "(got ratio %d, nt %d, ot %g)",
+              c.ratio, nt, (double)ot);
+    hipStream_t st = ctx->stream;
+    const size_t n = (size_t)c.nnxz;
+    Fields2 F, B;
+    rc = load_model(c, F, c11, c13, c33, taper);
+    if (rc) return rc;
+    DBuf<float> dc[3], snap, acc[3];
+    DBuf<double> acc64[3];
+    DBuf<int> recmap;
+    if (born) {
+        for (int k = 0; k < 2 && !rc; ++k) {
+            rc = B.p[k].alloc(n, st);
+            if (!rc) rc = B.q[k].alloc(n, st);
+        }
+        const float *src[3] = {dc11, dc13, dc33};
+        for (int k = 0; k < 3 && !rc; ++k) {
+            rc = dc[k].alloc(n, st);
+            if (!rc) rc = upload(c, dc[k].p, src[k], n * 4);
+        }
+    }
+    if (!rc) rc = snap.alloc((size_t)c.nnt * 2 * n, st);  // [k][p | q], slot 0 stays zero
+    for (int k = 0; k < 3 && !rc; ++k) {
+        rc = acc[k].alloc(n, st);
+        if (!rc) rc = acc64[k].alloc(n, st);
+    }
+    if (!rc) rc = recmap.alloc(n, st);
+    if (rc) return rc;
+    auto snap_p = [&](int k) { return snap.p + (size_t)k * 2 * n; };
+    auto snap_q = [&](int k) { return snap.p + (size_t)k * 2 * n + n; };
+    double obj = 0.0;
+    std::vector<int> hmap(n, -1);
+    for (int is = 0; is < ns; ++is) {
+        Shot2 s;
+        rc = shot_of(c, sloc, is, rloc, nr, s);
+        if (rc) return rc;
+        for (int ir = 0; ir < s.nri; ++ir) {
+            const int x = (int)(s.idx[ir] % c.g.nnx), z = (int)(s.idx[ir] / c.g.nnx);
+            VTI_CHECK(interior(c, x, z), "receiver %d outside the updated interior", s.r0 + ir);
+            VTI_CHECK(hmap[s.idx[ir]] < 0, "receivers %d and %d share a grid point", hmap[s.idx[ir]] + s.r0, s.r0 + ir);
+            hmap[s.idx[ir]] = ir;
+        }
+        DBuf<long long> idx;
+        DBuf<float> bg, obs, zrow;
+        DBuf<double> part;
+        const int nblk = launch_blocks((long long)s.nri * c.nnt_data);
+        rc = idx.alloc(s.nri, st);
+        if (!rc) rc = bg.alloc((size_t)s.nri * c.nnt_data, st);
+        if (!rc) rc = zrow.alloc(s.nri, st);
+        if (!rc) rc = part.alloc(nblk, st);
+        if (!rc && s.nri) rc = upload(c, idx.p, s.idx.data(), sizeof(long long) * s.nri);
+        if (!rc) rc = upload(c, recmap.p, hmap.data(), n * sizeof(int));
+        if (rc) return rc;
+
+        // ---- forward (objFuncGradient_f.cu:160-210 / hessian_f.cu:735-806)
+        rc = zero2(c, F.p, F.q);
+        if (!rc && born) rc = zero2(c, B.p, B.q);
+        if (rc) return rc;
+        hipLaunchKernelGGL(set2_kernel, dim3(1), dim3(1), 0, st, F.p[0].p, F.q[0].p, s.src, c.dt2 * wavelet[0]);
+        for (int it = 2; it < nt; ++it) {
+            // the step computing level it reads level it-1: snapshot its D when due
+            const int lc = it - 1;
+            float *sp = nullptr, *sq = nullptr;
+            if (lc % c.ratio == 0) {
+                sp = snap_p(lc / c.ratio);
+                sq = snap_q(lc / c.ratio);
+            }
+            const float sv = c.dt2 * wavelet[it - 1];
+            if (born)
+                hipLaunchKernelGGL((vti::fwd2d_kernel<true>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
+                                   F.q[0].p, B.p[1].p, B.q[1].p, B.p[0].p, B.q[0].p, F.c11.p, F.c13.p, F.c33.p,
+                                   dc[0].p, dc[1].p, dc[2].p, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv, sp, sq);
+            else
+                hipLaunchKernelGGL((vti::fwd2d_kernel<false>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
+                                   F.q[0].p, nullptr, nullptr, nullptr, nullptr, F.c11.p, F.c13.p, F.c33.p, nullptr,
+                                   nullptr, nullptr, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv, sp, sq);
+            std::swap(F.p[0].p, F.p[1].p);
+            std::swap(F.q[0].p, F.q[1].p);
+            if (born) {
+                std::swap(B.p[0].p, B.p[1].p);
+                std::swap(B.q[0].p, B.q[1].p);
+            }
+            if (it % c.ratio == 0 && s.nri) {
+                const DBuf<float> *rp = born ? &B.p[0] : &F.p[0];
+                const DBuf<float> *rq = born ? &B.q[0] : &F.q[0];
+                hipLaunchKernelGGL(record2d_kernel, dim3((s.nri + 255) / 256), dim3(256), 0, st,
+                                   bg.p + (size_t)(it / c.ratio) * s.nri, rp->p, rq->p, idx.p, s.nri);
+            }
+        }
+        hipLaunchKernelGGL(vti::dsnap2d_kernel, c.grid, c.block, 0, st, F.p[0].p, F.q[0].p, c.g, c.dx2, c.dz2,
+                           snap_p(c.nnt - 1), snap_q(c.nnt - 1));
+        VTI_HIP(hipGetLastError());
+
+        // ---- residual and objective (objFuncGradient_f.cu:214-219)
+        if (!born) {
+            std::vector<float> o((size_t)s.nri * c.nnt_data);
+            for (int row = 0; row < c.nnt_data; ++row)
+                for (int ir = 0; ir < s.nri; ++ir) o[(size_t)row * s.nri + ir] = data[(size_t)row * nr + s.r0 + ir];
+            rc = obs.alloc(o.size(), st);
+            if (!rc && !o.empty()) rc = upload(c, obs.p, o.data(), o.size() * 4);
+            if (rc) return rc;
+            hipLaunchKernelGGL(residual2d_kernel, dim3(nblk), dim3(256), 0, st, bg.p, obs.p,
+                               (long long)s.nri * c.nnt_data, part.p);
+            std::vector<double> hp(nblk);
+            VTI_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * nblk, hipMemcpyDeviceToHost, st));
+            VTI_HIP(hipStreamSynchronize(st));
+            for (double v : hp) obj += v;
+        }
+
+        // ---- backward (objFuncGradient_f.cu:225-300 / hessian_f.cu:834-900)
+        rc = zero2(c, F.p, F.q);
+        if (rc) return rc;
+        if (s.nri)
+            hipLaunchKernelGGL(inject2d_kernel, dim3((s.nri + 255) / 256), dim3(256), 0, st, F.p[0].p, F.q[0].p,
+                               idx.p, bg.p + (size_t)(c.nnt_data - 1) * s.nri, zrow.p, 0.f, s.nri, c.dt2);
+        {
+            float f = (float)((nt - 2.) / c.ratio);
+            const int i1 = (int)f;
+            f = f - i1;
+            hipLaunchKernelGGL(vti::xcor2d_kernel, dim3((c.g.nnx - 8 + 255) / 256, c.g.nnz - 8), dim3(256), 0, st,
+                               F.p[0].p, F.q[0].p, snap_p(i1), snap_q(i1), snap_p(i1 + 1), snap_q(i1 + 1), f,
+                               acc[0].p, acc[1].p, acc[2].p, c.g);
+        }
+        for (int it = nt - 3; it >= 0; --it) {
+            float fr = (float)((it + 1.) / c.ratio);
+            const int ir1 = (int)fr;
+            fr = fr - ir1;
+            float f = (float)it / c.ratio;
+            const int i1 = (int)f;
+            f = f - i1;
+            const float *r0 = bg.p + (size_t)ir1 * s.nri, *r1 = bg.p + (size_t)(ir1 + 1) * s.nri;
+            if (born)
+                hipLaunchKernelGGL((vti::bwd2d_kernel<1>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
+                                   F.q[0].p, F.c11.p, F.c13.p, F.c33.p, F.taper.p, recmap.p, r0, r1, fr, snap_p(i1),
+                                   snap_q(i1), snap_p(i1 + 1), snap_q(i1 + 1), f, acc[0].p, acc[1].p, acc[2].p, c.g,
+                                   c.dx2, c.dz2, c.dt2);
+            else
+                hipLaunchKernelGGL((vti::bwd2d_kernel<0>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
+                                   F.q[0].p, F.c11.p, F.c13.p, F.c33.p, F.taper.p, recmap.p, r0, r1, fr, snap_p(i1),
+                                   snap_q(i1), snap_p(i1 + 1), snap_q(i1 + 1), f, acc[0].p, acc[1].p, acc[2].p, c.g,
+                                   c.dx2, c.dz2, c.dt2);
+            std::swap(F.p[0].p, F.p[1].p);
+            std::swap(F.q[0].p, F.q[1].p);
+        }
+        VTI_HIP(hipGetLastError());
+        // per-shot FP32 gradient into the FP64 cross-shot sum
+        for (int k = 0; k < 3; ++k) {
+            hipLaunchKernelGGL(acc64_2d_kernel, dim3(launch_blocks((long long)n)), dim3(256), 0, st, acc64[k].p,
+                               acc[k].p, (long long)n);
+            VTI_HIP(hipMemsetAsync(acc[k].p, 0, n * 4, st));
+        }
+        for (int ir = 0; ir < s.nri; ++ir) hmap[s.idx[ir]] = -1;
+        VTI_HIP(hipStreamSynchronize(st));
+    }
+    float *out[3] = {gc11, gc13, gc33};
+    for (int k = 0; k < 3; ++k) {
+        hipLaunchKernelGGL(narrow2d_kernel, dim3(launch_blocks((long long)n)), dim3(256), 0, st, acc[k].p, acc64[k].p,
+                           (long long)n);
+        VTI_HIP(hipMemcpyAsync(out[k], acc[k].p, n * 4, hipMemcpyDeviceToHost, st));
+    }
+    VTI_HIP(hipStreamSynchronize(st));
+    // objFuncGradient_f returns 0.5f*objfunc as float (objFuncGradient_f.cu:370)
+    if (objfunc) *objfunc = (float)(0.5f * obj);
+    return vti::kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
+// objFuncGradient_f, lib/vti-fwi-2d/objFuncGradient_f.cu:15-371 (wave.h:12).
+// gc11/gc13/gc33 (nnx*nnz) are overwritten; *objfunc = 0.5*sum r^2.
+int vti2d_objfunc_gradient(vti_ctx *ctx, float *objfunc, float *gc11, float *gc13, float *gc33, const float *data,
+                           const float *c11, const float *c13, const float *c33, const float *wavelet, const int *sloc,
+                           int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad,
+                           float dx, float dz, float dt, float rate, float ot)
+{
+    return fg2d(ctx, false, objfunc, gc11, gc13, gc33, data, c11, c13, c33, nullptr, nullptr, nullptr, wavelet, sloc,
+                ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot);
+}
+
+// GNhessian_f, lib/vti-fwi-2d/hessian_f.cu:575-1010 (wave.h:34).  The data
+// argument is unused there as well (the adjoint source is the Born data).
+int vti2d_gn_hessian(vti_ctx *ctx, float *gc11, float *gc13, float *gc33, const float *data, const float *c11,
+                     const float *c13, const float *c33, const float *dc11, const float *dc13, const float *dc33,
+                     const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
+                     int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    (void)data;
+    return fg2d(ctx, true, nullptr, gc11, gc13, gc33, nullptr, c11, c13, c33, dc11, dc13, dc33, wavelet, sloc, ns,
+                rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot);
+}
+
+}  // extern "C"

@@ -1,0 +1,226 @@
+// 2D VTI kernels for gfx950 (included by vti2d.hip).
+//
+// Grid: the reference's layout i = ix + iz*nnx (x fastest); every kernel
+// updates the interior ix in [4, nnx-4), iz in [4, nnz-4) only
+// (lib/vti-fwi-2d/kernels.cu).  p needs only its x derivative and q only its
+// z derivative, so a workgroup of 64 x BY threads keeps two LDS tiles: p rows
+// with a 4-point x halo and q columns with a 4-point z halo.
+//
+// Arithmetic is the reference's: FP32 with float weights, every expression
+// evaluated in source order with contraction off, so results reproduce the
+// oracle's FP32 restatement bit for bit.
+//
+// Each step kernel fuses what the reference runs as separate launches per
+// time step: the stencil update, the source or receiver injection, the
+// taper of the absorbing boundary (abc), the D snapshot of the current level
+// and the imaging condition.  The taper the reference applies in place to the
+// current level after each step is applied here when that level is read
+// back as the previous level.
+#pragma once
+
+#include "vti_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace vti {
+
+struct Geo2 {
+    int nnx, nnz;
+};
+
+constexpr int kBY2 = 8;
+
+// 9-point second difference of the reference (kernels.cu:486-493)
+__device__ __forceinline__ float st9(float c, float a1, float b1, float a2, float b2, float a3, float b3, float a4,
+                                     float b4)
+{
+    return VTI_C0 * c + VTI_C1 * (a1 + b1) + VTI_C2 * (a2 + b2) + VTI_C3 * (a3 + b3) + VTI_C4 * (a4 + b4);
+}
+
+struct Tiles2 {
+    float p[kBY2][72];      // x-halo tile
+    float q[kBY2 + 8][64];  // z-halo tile
+};
+
+// Load the tiles.  MODE 0: p and q themselves.  MODE 1: c11*p + c13*q and
+// c13*p + c33*q (the products the exact adjoint differentiates, kernels.cu:116-132).
+template <int MODE>
+__device__ __forceinline__ void load_tiles(Tiles2 &t, const float *__restrict__ P, const float *__restrict__ Q,
+                                           const float *__restrict__ c11, const float *__restrict__ c13,
+                                           const float *__restrict__ c33, const Geo2 &g, int x0, int z0)
+{
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    auto pv = [&](int x, int z) -> float {
+        if (x < 0 || x >= g.nnx || z < 0 || z >= g.nnz) return 0.f;
+        const long long i = x + (long long)z * g.nnx;
+        if (MODE == 0) return P[i];
+        return c11[i] * P[i] + c13[i] * Q[i];
+    };
+    auto qv = [&](int x, int z) -> float {
+        if (x < 0 || x >= g.nnx || z < 0 || z >= g.nnz) return 0.f;
+        const long long i = x + (long long)z * g.nnx;
+        if (MODE == 0) return Q[i];
+        return c13[i] * P[i] + c33[i] * Q[i];
+    };
+    const int ix = x0 + tx, iz = z0 + ty;
+    t.p[ty][tx + 4] = pv(ix, iz);
+    if (tx < 4) {
+        t.p[ty][tx] = pv(x0 - 4 + tx, iz);
+        t.p[ty][tx + 68] = pv(x0 + 64 + tx, iz);
+    }
+    t.q[ty + 4][tx] = qv(ix, iz);
+    if (ty < 4) {
+        t.q[ty][tx] = qv(ix, z0 - 4 + ty);
+        t.q[ty + kBY2 + 4][tx] = qv(ix, z0 + kBY2 + ty);
+    }
+}
+
+__device__ __forceinline__ float dxx(const Tiles2 &t, float dx2)
+{
+    const int c = threadIdx.x + 4, r = threadIdx.y;
+    const float *a = t.p[r];
+    return st9(a[c], a[c - 1], a[c + 1], a[c - 2], a[c + 2], a[c - 3], a[c + 3], a[c - 4], a[c + 4]) / dx2;
+}
+
+__device__ __forceinline__ float dzz(const Tiles2 &t, float dz2)
+{
+    const int c = threadIdx.x, r = threadIdx.y + 4;
+    return st9(t.q[r][c], t.q[r - 1][c], t.q[r + 1][c], t.q[r - 2][c], t.q[r + 2][c], t.q[r - 3][c], t.q[r + 3][c],
+               t.q[r - 4][c], t.q[r + 4][c]) /
+           dz2;
+}
+
+// Forward step: forwardCD / forwardC semantics (kernels.cu:452-498, 546-556)
+// + injectSource + abc; optionally writes D of the current level (the
+// snapshot the reference's D kernel takes, kernels.cu:345-387).
+// BORN: also steps the Born field dp (GNhessian_f, hessian_f.cu:752-806) with
+// the scattering source dt2*(dC * D p) (kernels.cu:282-291).
+template <bool BORN>
+__global__ __launch_bounds__(64 * kBY2) void fwd2d_kernel(
+    float *__restrict__ p_n, float *__restrict__ q_n, const float *__restrict__ p_c, const float *__restrict__ q_c,
+    float *__restrict__ dp_n, float *__restrict__ dq_n, const float *__restrict__ dp_c,
+    const float *__restrict__ dq_c, const float *__restrict__ c11, const float *__restrict__ c13,
+    const float *__restrict__ c33, const float *__restrict__ dc11, const float *__restrict__ dc13,
+    const float *__restrict__ dc33, const float *__restrict__ taper, Geo2 g, float dx2, float dz2, float dt2,
+    long long src, float src_val, float *__restrict__ snap_p, float *__restrict__ snap_q)
+{
+    __shared__ Tiles2 t;
+    __shared__ Tiles2 tb;
+    const int x0 = 4 + blockIdx.x * 64, z0 = 4 + blockIdx.y * kBY2;
+    const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
+    load_tiles<0>(t, p_c, q_c, nullptr, nullptr, nullptr, g, x0, z0);
+    if (BORN) load_tiles<0>(tb, dp_c, dq_c, nullptr, nullptr, nullptr, g, x0, z0);
+    __syncthreads();
+    if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
+    const long long i = ix + (long long)iz * g.nnx;
+    const float tp = dxx(t, dx2), tq = dzz(t, dz2);
+    if (snap_p) {
+        snap_p[i] = tp;
+        snap_q[i] = tq;
+    }
+    const float tap = taper[i];
+    const float a11 = c11[i], a13 = c13[i], a33 = c33[i];
+    float pn = 2.f * p_c[i] + dt2 * (a11 * tp + a13 * tq) - p_n[i] * tap;
+    float qn = 2.f * q_c[i] + dt2 * (a13 * tp + a33 * tq) - q_n[i] * tap;
+    if (i == src) {
+        pn += src_val;
+        qn += src_val;
+    }
+    p_n[i] = pn * tap;
+    q_n[i] = qn * tap;
+    if (BORN) {
+        const float btp = dxx(tb, dx2), btq = dzz(tb, dz2);
+        float dpn = 2.f * dp_c[i] + dt2 * (a11 * btp + a13 * btq) - dp_n[i] * tap;
+        float dqn = 2.f * dq_c[i] + dt2 * (a13 * btp + a33 * btq) - dq_n[i] * tap;
+        dpn += dt2 * (dc11[i] * tp + dc13[i] * tq);
+        dqn += dt2 * (dc13[i] * tp + dc33[i] * tq);
+        dp_n[i] = dpn * tap;
+        dq_n[i] = dqn * tap;
+    }
+}
+
+// D of a level into a snapshot slot (the last level of the forward pass).
+__global__ __launch_bounds__(64 * kBY2) void dsnap2d_kernel(const float *__restrict__ p, const float *__restrict__ q,
+                                                           Geo2 g, float dx2, float dz2, float *__restrict__ sp,
+                                                           float *__restrict__ sq)
+{
+    __shared__ Tiles2 t;
+    const int x0 = 4 + blockIdx.x * 64, z0 = 4 + blockIdx.y * kBY2;
+    const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
+    load_tiles<0>(t, p, q, nullptr, nullptr, nullptr, g, x0, z0);
+    __syncthreads();
+    if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
+    const long long i = ix + (long long)iz * g.nnx;
+    sp[i] = dxx(t, dx2);
+    sq[i] = dzz(t, dz2);
+}
+
+// Adjoint step fused with injectData, abc and gradientCrossCor
+// (objFuncGradient_f.cu:249-300 / hessian_f.cu:853-900).
+// ADJ 0: backwardDC, the exact transpose (kernels.cu:104-148).
+// ADJ 1: forwardCD, the forward operator (GNhessian_f's adjoint, hessian_f.cu:854).
+// recmap[i] is the receiver index injected at point i (or -1); the injected
+// value is 0.5*((1-fr)*r0 + fr*r1) of the two trace rows bracketing the time.
+// The image uses the snapshots D0, D1 of the levels bracketing this one.
+template <int ADJ>
+__global__ __launch_bounds__(64 * kBY2) void bwd2d_kernel(
+    float *__restrict__ p_n, float *__restrict__ q_n, const float *__restrict__ p_c, const float *__restrict__ q_c,
+    const float *__restrict__ c11, const float *__restrict__ c13, const float *__restrict__ c33,
+    const float *__restrict__ taper, const int *__restrict__ recmap, const float *__restrict__ r0,
+    const float *__restrict__ r1, float fr, const float *__restrict__ D0p, const float *__restrict__ D0q,
+    const float *__restrict__ D1p, const float *__restrict__ D1q, float f, float *__restrict__ g11,
+    float *__restrict__ g13, float *__restrict__ g33, Geo2 g, float dx2, float dz2, float dt2)
+{
+    __shared__ Tiles2 t;
+    const int x0 = 4 + blockIdx.x * 64, z0 = 4 + blockIdx.y * kBY2;
+    const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
+    load_tiles<ADJ == 0 ? 1 : 0>(t, p_c, q_c, c11, c13, c33, g, x0, z0);
+    __syncthreads();
+    if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
+    const long long i = ix + (long long)iz * g.nnx;
+    const float tap = taper[i];
+    float pn, qn;
+    if (ADJ == 0) {
+        const float tp0 = dxx(t, dx2), tq0 = dzz(t, dz2);
+        pn = 2.f * p_c[i] + dt2 * tp0 - p_n[i] * tap;
+        qn = 2.f * q_c[i] + dt2 * tq0 - q_n[i] * tap;
+    } else {
+        const float tp = dxx(t, dx2), tq = dzz(t, dz2);
+        pn = 2.f * p_c[i] + dt2 * (c11[i] * tp + c13[i] * tq) - p_n[i] * tap;
+        qn = 2.f * q_c[i] + dt2 * (c13[i] * tp + c33[i] * tq) - q_n[i] * tap;
+    }
+    const int ir = recmap[i];
+    if (ir >= 0) {
+        const float s = 0.5f * ((1.f - fr) * r0[ir] + fr * r1[ir]);
+        pn += dt2 * s;
+        qn += dt2 * s;
+    }
+    pn = pn * tap;
+    qn = qn * tap;
+    p_n[i] = pn;
+    q_n[i] = qn;
+    const float Dp = (1.f - f) * D0p[i] + f * D1p[i];
+    const float Dq = (1.f - f) * D0q[i] + f * D1q[i];
+    g11[i] += pn * Dp;
+    g13[i] += pn * Dq + qn * Dp;
+    g33[i] += qn * Dq;
+}
+
+// gradientCrossCor on a given level (the first adjoint level, which the
+// reference images before its backward loop, objFuncGradient_f.cu:247).
+__global__ void xcor2d_kernel(const float *__restrict__ ap, const float *__restrict__ aq,
+                              const float *__restrict__ D0p, const float *__restrict__ D0q,
+                              const float *__restrict__ D1p, const float *__restrict__ D1q, float f,
+                              float *__restrict__ g11, float *__restrict__ g13, float *__restrict__ g33, Geo2 g)
+{
+    const int ix = 4 + blockIdx.x * blockDim.x + threadIdx.x, iz = 4 + blockIdx.y;
+    if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
+    const long long i = ix + (long long)iz * g.nnx;
+    const float Dp = (1.f - f) * D0p[i] + f * D1p[i];
+    const float Dq = (1.f - f) * D0q[i] + f * D1q[i];
+    g11[i] += ap[i] * Dp;
+    g13[i] += ap[i] * Dq + aq[i] * Dp;
+    g33[i] += aq[i] * Dq;
+}
+
+}  // namespace vti

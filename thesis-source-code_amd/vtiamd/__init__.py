@@ -84,6 +84,15 @@ _SIGS = {
         c_int,
         [P, P, P, c_int, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int, P],
     ),
+    "vti2d_synthetic": (c_int, [P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
+    "vti2d_objfunc_gradient": (
+        c_int,
+        [P, P, P, P, P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5,
+    ),
+    "vti2d_gn_hessian": (
+        c_int,
+        [P] * 13 + [c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5,
+    ),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -322,3 +331,51 @@ def rtm3d(ctx, souloc, shotid, recloc3, wavelet, c11, c13, c33, nt, npad, o, d, 
                          ptr(wavelet), ptr(c11), ptr(c13), ptr(c33), nx, ny, nz, nt, npad, *o, 0.0, *d, dt, rate,
                          ngpu, ptr(obs)), "vti3d_rtm")
     return img
+
+
+# ---- 2D drivers (reference semantics) ---------------------------------------
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32).ravel()
+
+
+def synthetic2d(ctx, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0):
+    """synthetic_f: returns data (nnt_data, nr) time-major, zeros where never recorded."""
+    c11, c13, c33, wavelet, taper = (f32(a).ravel() for a in (c11, c13, c33, wavelet, taper))
+    sloc, rloc = _i32(sloc), _i32(rloc)
+    ns, nr = sloc.size // 4, rloc.size // 2
+    ratio = int(rate / dt + 0.5)
+    ntneg = int(round(abs(ot) / dt))
+    nnt_data = (nt - ntneg - 1) // ratio + 1
+    data = np.zeros((nnt_data, nr), np.float32)
+    check(_lib.vti2d_synthetic(ctx.h, ptr(data), ptr(c11), ptr(c13), ptr(c33), ptr(wavelet), ptr(sloc), ns, ptr(rloc),
+                               nr, ptr(taper), nx, nz, nt, npad, dx, dz, dt, rate, ot), "vti2d_synthetic")
+    return data
+
+
+def objfunc_gradient2d(ctx, data, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate,
+                       ot=0.0):
+    """objFuncGradient_f: returns (objective, gc (3, nnz, nnx))."""
+    c11, c13, c33, wavelet, taper, data = (f32(a).ravel() for a in (c11, c13, c33, wavelet, taper, data))
+    sloc, rloc = _i32(sloc), _i32(rloc)
+    nnx, nnz = nx + 2 * npad, nz + 2 * npad
+    gc = np.zeros((3, nnz, nnx), np.float32)
+    f = ctypes.c_float(0.0)
+    check(_lib.vti2d_objfunc_gradient(ctx.h, ctypes.byref(f), ptr(gc[0]), ptr(gc[1]), ptr(gc[2]), ptr(data), ptr(c11),
+                                      ptr(c13), ptr(c33), ptr(wavelet), ptr(sloc), sloc.size // 4, ptr(rloc),
+                                      rloc.size // 2, ptr(taper), nx, nz, nt, npad, dx, dz, dt, rate, ot),
+          "vti2d_objfunc_gradient")
+    return f.value, gc
+
+
+def gn_hessian2d(ctx, c11, c13, c33, dc11, dc13, dc33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate,
+                 ot=0.0):
+    """GNhessian_f: returns H*dc as (3, nnz, nnx)."""
+    arrs = [f32(a).ravel() for a in (c11, c13, c33, dc11, dc13, dc33, wavelet, taper)]
+    sloc, rloc = _i32(sloc), _i32(rloc)
+    nnx, nnz = nx + 2 * npad, nz + 2 * npad
+    gc = np.zeros((3, nnz, nnx), np.float32)
+    check(_lib.vti2d_gn_hessian(ctx.h, ptr(gc[0]), ptr(gc[1]), ptr(gc[2]), None, *[ptr(a) for a in arrs[:7]],
+                                ptr(sloc), sloc.size // 4, ptr(rloc), rloc.size // 2, ptr(arrs[7]), nx, nz, nt, npad,
+                                dx, dz, dt, rate, ot), "vti2d_gn_hessian")
+    return gc
