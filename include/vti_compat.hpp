@@ -1,0 +1,264 @@
+// vti_compat.hpp -- the reference's C++ driver signatures on top of the C ABI
+// of libvtiamd.so (vti_amd.h).  Header-only; link with -lvtiamd.
+//
+// Lets code written against lib/vti-fwi-2d/wave.h and
+// lib/vti-fwi-3d/gpu/wave3d.h (the L-BFGS drivers, HessianOp, the *-param
+// mains) call this implementation unchanged, in place of the CUDA objects
+// that define these functions: exactly one translation unit defines
+// VTI_COMPAT_IMPLEMENTATION before including this header and so holds the
+// (non-inline, external) definitions; every other one keeps including the
+// reference's wave.h, or this header for the declarations.
+//
+// Behaviour kept from the reference:
+//  - outputs are overwritten (zeroed by the callee), inputs are host memory;
+//  - objFuncGradientCij3d_f leaves cij's boundary shell holding
+//    randboundaryCij, as the reference's putBoundary does
+//    (objFuncGradient3d_f.cu:786);
+//  - on failure the reference prints to stderr and carries on; so does this
+//    shim (the message comes from vti_last_error()).
+//
+// What the reference reads from its global argv inside the drivers -- the
+// GPU list (gpu=), the observed data (data=) and the tpow= gain of the 3D
+// objective -- is supplied here in one of two ways:
+//  - define VTI_COMPAT_USE_MYIO and include the reference's lib/io/myio.h
+//    first: the shim then calls get_array("gpu"), read("data") and
+//    get_param("tpow") exactly where the reference does;
+//  - otherwise call vti_compat::set_observed_data() / set_tpow() /
+//    set_ngpu() before the 3D drivers.
+// All work runs on one device (vti_compat::set_device, default 0); the GPU
+// list only fixes the reference's pipeline length nround*NGPU*64
+// (objFuncGradient3d_f.cu:539).
+#ifndef VTI_COMPAT_HPP
+#define VTI_COMPAT_HPP
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "vti_amd.h"
+
+namespace vti_compat {
+
+struct State {
+    vti_ctx *ctx = nullptr;
+    int device = 0;
+    int ngpu = 1;
+    float tpow = 0.f;
+    const float *observed = nullptr;
+};
+
+inline State &state()
+{
+    static State s;
+    return s;
+}
+
+inline void set_device(int device) { state().device = device; }
+inline void set_ngpu(int ngpu) { state().ngpu = ngpu > 0 ? ngpu : 1; }
+inline void set_tpow(float tpow) { state().tpow = tpow; }
+// trace-major observed data of every receiver (the reference's "data" file)
+inline void set_observed_data(const float *data) { state().observed = data; }
+
+inline vti_ctx *context()
+{
+    State &s = state();
+    if (!s.ctx && vti_ctx_create(&s.ctx, s.device) != 0) {
+        std::fprintf(stderr, "vti_compat: %s\n", vti_last_error());
+        s.ctx = nullptr;
+    }
+    return s.ctx;
+}
+
+inline void report(int rc, const char *what)
+{
+    if (rc != 0) std::fprintf(stderr, "%s: %s\n", what, vti_last_error());
+}
+
+inline int ngpu()
+{
+#ifdef VTI_COMPAT_USE_MYIO
+    std::vector<int> gpus;
+    if (get_array("gpu", gpus) && !gpus.empty()) return (int)gpus.size();
+#endif
+    return state().ngpu;
+}
+
+inline float tpow()
+{
+#ifdef VTI_COMPAT_USE_MYIO
+    float t = 0.f;
+    get_param("tpow", t);
+    return t;
+#else
+    return state().tpow;
+#endif
+}
+
+// The receivers the listed shots use: one past the last receiver index.
+inline long long receivers_used(const float *souloc, const std::vector<int> &shotid)
+{
+    long long n = 0;
+    for (int is : shotid) {
+        const long long end = (long long)souloc[5 * is + 4] + (long long)souloc[5 * is + 3];
+        if (end > n) n = end;
+    }
+    return n;
+}
+
+inline int nnt3d(int nt, float dt, float samplingRate)
+{
+    return (nt - 1) / (int)std::round(samplingRate / dt) + 1;
+}
+
+// Observed data for the 3D objective: the reference's read("data") (or the
+// registered buffer).  buf keeps what was read alive for the call.
+inline const float *observed(std::vector<float> &buf, const float *souloc, const std::vector<int> &shotid, int nt,
+                             float dt, float samplingRate)
+{
+#ifdef VTI_COMPAT_USE_MYIO
+    buf.assign((size_t)nnt3d(nt, dt, samplingRate) * (size_t)receivers_used(souloc, shotid), 0.f);
+    read("data", buf.data(), buf.size());
+    return buf.data();
+#else
+    (void)buf;
+    (void)souloc;
+    (void)shotid;
+    (void)nt;
+    (void)dt;
+    (void)samplingRate;
+    return state().observed;
+#endif
+}
+
+}  // namespace vti_compat
+
+// ---- lib/vti-fwi-2d/wave.h ------------------------------------------------
+#ifndef WAVE_H
+void synthetic_f(float *d, const float *c11, const float *c13, const float *c33, const float *wavelet, const int *sloc,
+                 int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad, float dx,
+                 float dz, float dt, float rate, float ot = 0.);
+float objFuncGradient_f(float *gc11, float *gc13, float *gc33, const float *data, const float *c11, const float *c13,
+                        const float *c33, const float *wavelet, const int *sloc, int ns, const int *rloc, int nr,
+                        const float *taper, int nx, int nz, int nt, int npad, float dx, float dz, float dt, float rate,
+                        float ot = 0.);
+void GNhessian_f(float *gc11, float *gc13, float *gc33, const float *data, const float *c11, const float *c13,
+                 const float *c33, const float *dc11, const float *dc13, const float *dc33, const float *wavelet,
+                 const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
+                 int npad, float dx, float dz, float dt, float rate, float ot);
+#endif
+
+// 3D declarations (wave3d.h itself needs CUDA's stream type, so it is not
+// included here)
+void modelDataCij3d_f(float *ddata, float *souloc, int ns, std::vector<int> &shotid, float *recloc,
+                      const float *wavelet, float *c11, float *c13, float *c33, int nx, int ny, int nz, int nt,
+                      int npad, float ox, float oy, float oz, float ot, float dx, float dy, float dz, float dt,
+                      float samplingRate, std::vector<int> &GPUs);
+void objFuncCij3d_f(float *f, float *souloc, int ns, std::vector<int> &shotid, float *recloc, float *wavelet,
+                    float *cij, float *padboundaryCij, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
+                    float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate,
+                    std::vector<int> &GPUs);
+void objFuncGradientCij3d_f(float *fgcij, float *souloc, int ns, std::vector<int> &shotid, float *recloc,
+                            float *wavelet, float *cij, float *padboundaryCij, float *randboundaryCij, int nx, int ny,
+                            int nz, int nt, int npad, float ox, float oy, float oz, float ot, float dx, float dy,
+                            float dz, float dt, float samplingRate, std::vector<int> &GPUs);
+void rtmCij3d_f(float *image, float *souloc, int ns, std::vector<int> &shotid, float *recloc, float *wavelet,
+                float *c11, float *c13, float *c33, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
+                float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate);
+
+#ifdef VTI_COMPAT_IMPLEMENTATION
+
+// synthetic_f (wave.h:4, synthetic_f.cu:13)
+void synthetic_f(float *d, const float *c11, const float *c13, const float *c33, const float *wavelet,
+                        const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
+                        int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    vti_compat::report(vti2d_synthetic(vti_compat::context(), d, c11, c13, c33, wavelet, sloc, ns, rloc, nr, taper, nx,
+                                       nz, nt, npad, dx, dz, dt, rate, ot),
+                       "synthetic_f");
+}
+
+// objFuncGradient_f (wave.h:12, objFuncGradient_f.cu:15)
+float objFuncGradient_f(float *gc11, float *gc13, float *gc33, const float *data, const float *c11,
+                               const float *c13, const float *c33, const float *wavelet, const int *sloc, int ns,
+                               const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad, float dx,
+                               float dz, float dt, float rate, float ot)
+{
+    float f = 0.f;
+    vti_compat::report(vti2d_objfunc_gradient(vti_compat::context(), &f, gc11, gc13, gc33, data, c11, c13, c33,
+                                              wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate,
+                                              ot),
+                       "objFuncGradient_f");
+    return f;
+}
+
+// GNhessian_f (wave.h:34, hessian_f.cu:575)
+void GNhessian_f(float *gc11, float *gc13, float *gc33, const float *data, const float *c11, const float *c13,
+                        const float *c33, const float *dc11, const float *dc13, const float *dc33,
+                        const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper,
+                        int nx, int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    vti_compat::report(vti2d_gn_hessian(vti_compat::context(), gc11, gc13, gc33, data, c11, c13, c33, dc11, dc13, dc33,
+                                        wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot),
+                       "GNhessian_f");
+}
+
+// ---- lib/vti-fwi-3d/gpu/wave3d.h ------------------------------------------
+
+// modelDataCij3d_f (wave3d.h:110, modelData3d_f.cu:427)
+void modelDataCij3d_f(float *ddata, float *souloc, int ns, std::vector<int> &shotid, float *recloc,
+                             const float *wavelet, float *c11, float *c13, float *c33, int nx, int ny, int nz, int nt,
+                             int npad, float ox, float oy, float oz, float ot, float dx, float dy, float dz, float dt,
+                             float samplingRate, std::vector<int> &GPUs)
+{
+    vti_compat::report(vti3d_model_data(vti_compat::context(), ddata, souloc, ns, shotid.data(), (int)shotid.size(),
+                                        recloc, wavelet, c11, c13, c33, nx, ny, nz, nt, npad, ox, oy, oz, ot, dx, dy,
+                                        dz, dt, samplingRate, GPUs.empty() ? 1 : (int)GPUs.size()),
+                       "modelDataCij3d_f");
+}
+
+// objFuncCij3d_f (wave3d.h:100, objFuncGradient3d_f.cu:1272)
+void objFuncCij3d_f(float *f, float *souloc, int ns, std::vector<int> &shotid, float *recloc, float *wavelet,
+                           float *cij, float *padboundaryCij, int nx, int ny, int nz, int nt, int npad, float ox,
+                           float oy, float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate,
+                           std::vector<int> &GPUs)
+{
+    std::vector<float> buf;
+    const float *obs = vti_compat::observed(buf, souloc, shotid, nt, dt, samplingRate);
+    vti_compat::report(vti3d_objfunc(vti_compat::context(), f, souloc, ns, shotid.data(), (int)shotid.size(), recloc,
+                                     wavelet, cij, padboundaryCij, nx, ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz,
+                                     dt, samplingRate, GPUs.empty() ? 1 : (int)GPUs.size(), obs, vti_compat::tpow()),
+                       "objFuncCij3d_f");
+}
+
+// objFuncGradientCij3d_f (wave3d.h:96, objFuncGradient3d_f.cu:361)
+void objFuncGradientCij3d_f(float *fgcij, float *souloc, int ns, std::vector<int> &shotid, float *recloc,
+                                   float *wavelet, float *cij, float *padboundaryCij, float *randboundaryCij, int nx,
+                                   int ny, int nz, int nt, int npad, float ox, float oy, float oz, float ot, float dx,
+                                   float dy, float dz, float dt, float samplingRate, std::vector<int> &GPUs)
+{
+    std::vector<float> buf;
+    const float *obs = vti_compat::observed(buf, souloc, shotid, nt, dt, samplingRate);
+    vti_compat::report(vti3d_objfunc_gradient(vti_compat::context(), fgcij, souloc, ns, shotid.data(),
+                                              (int)shotid.size(), recloc, wavelet, cij, padboundaryCij,
+                                              randboundaryCij, nx, ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz, dt,
+                                              samplingRate, GPUs.empty() ? 1 : (int)GPUs.size(), obs,
+                                              vti_compat::tpow()),
+                       "objFuncGradientCij3d_f");
+}
+
+// rtmCij3d_f (wave3d.h:88, rtm3d_f.cu:161); the reference reads gpu= itself
+void rtmCij3d_f(float *image, float *souloc, int ns, std::vector<int> &shotid, float *recloc, float *wavelet,
+                       float *c11, float *c13, float *c33, int nx, int ny, int nz, int nt, int npad, float ox,
+                       float oy, float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate)
+{
+    std::vector<float> buf;
+    const float *obs = vti_compat::observed(buf, souloc, shotid, nt, dt, samplingRate);
+    vti_compat::report(vti3d_rtm(vti_compat::context(), image, souloc, ns, shotid.data(), (int)shotid.size(), recloc,
+                                 wavelet, c11, c13, c33, nx, ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz, dt,
+                                 samplingRate, vti_compat::ngpu(), obs),
+                       "rtmCij3d_f");
+}
+
+#endif  // VTI_COMPAT_IMPLEMENTATION
+
+#endif  // VTI_COMPAT_HPP
