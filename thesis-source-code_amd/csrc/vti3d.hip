@@ -129,6 +129,14 @@ namespace {
 
 long long vox(const Geo3 &g) { return g.plane * (long long)g.nz; }
 
+// Largest z-chunk whose planes (plus the 4-plane halo on each side) stay
+// within the 32-bit byte offsets of the kernels' buffer accesses.
+int max_zchunk(const Geo3 &g)
+{
+    const long long planes = 0xFFFFFFFFLL / (4 * g.plane);
+    return (int)std::min<long long>(1 << 20, (planes - 8) / 4 * 4);
+}
+
 void freep(void *p)
 {
     if (p) (void)hipFree(p);
@@ -446,6 +454,9 @@ int vti3d_prop_create(vti_ctx *ctx, vti3d_prop **out, int nx, int ny, int nz, in
     VTI_CHECK(nx >= 9 && ny >= 9 && nz >= 8, "vti3d_prop_create: grid %dx%dx%d too small", nx, ny, nz);
     VTI_CHECK(npad >= 1 && 2 * npad < nx && 2 * npad < ny && 2 * npad < nz, "vti3d_prop_create: bad npad %d", npad);
     VTI_CHECK(dx > 0 && dy > 0 && dz > 0 && dt > 0, "vti3d_prop_create: spacings must be positive");
+    // the kernels index within one x-y plane with 32-bit offsets
+    VTI_CHECK((long long)((nx + 63) / 64 * 64) * ny < (1LL << 31), "vti3d_prop_create: x-y plane %dx%d too large", nx,
+              ny);
     VTI_HIP(hipSetDevice(ctx->device));
     (void)hipGetLastError();  // drop a stale error left by an unrelated earlier call
     vti3d_prop *p = new vti3d_prop();
@@ -471,6 +482,11 @@ int vti3d_prop_create(vti_ctx *ctx, vti3d_prop **out, int nx, int ny, int nz, in
     p->dt2dy2 = p->dt2 / p->dy2;
     p->dt2dz2 = p->dt2 / p->dz2;
     if (const char *s = getenv("VTI_ZCHUNK")) p->zc = std::max(4, atoi(s) / 4 * 4);
+    if (max_zchunk(g) < 4) {
+        delete p;
+        VTI_CHECK(false, "vti3d_prop_create: x-y plane %dx%d too large for 32-bit chunk offsets", nx, ny);
+    }
+    p->zc = std::min(p->zc, max_zchunk(g));
     const size_t nv = (size_t)vox(g);
     int rc = vti::kOk;
     for (int k = 0; k < 3 && rc == vti::kOk; ++k) rc = dalloc(p, &p->cm[0][k], nv);
@@ -510,6 +526,8 @@ int vti3d_prop_set_arith(vti3d_prop *p, int arith)
 int vti3d_prop_set_zchunk(vti3d_prop *p, int zc)
 {
     VTI_CHECK(p && zc >= 4 && zc % 4 == 0, "vti3d_prop_set_zchunk: chunk must be a positive multiple of 4");
+    VTI_CHECK(zc <= max_zchunk(p->g), "vti3d_prop_set_zchunk: chunk %d above %d for this plane size", zc,
+              max_zchunk(p->g));
     p->zc = zc;
     return vti::kOk;
 }

@@ -45,24 +45,47 @@ __device__ __forceinline__ int xcd_logical_block(int nblocks)
     return (xcd < r) ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
 }
 
+// Raw buffer access (buffer_load/store_dword): a wave-uniform 128-bit
+// resource in SGPRs, a 32-bit per-lane byte offset and a wave-uniform byte
+// offset, so a plane's loads cost one VGPR of address per distinct lane
+// offset instead of a 64-bit address pair.  Every access is also guarded by
+// the kernels' own predicates; the resource's range is not relied on.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const float *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0, (int)0xFFFFFFFF, 0x00020000);
+}
+
+__device__ __forceinline__ float bload(rsrc_t r, unsigned lane_byte, unsigned uni_byte)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)lane_byte, (int)uni_byte, 0));
+}
+
+__device__ __forceinline__ void bstore(rsrc_t r, unsigned lane_byte, unsigned uni_byte, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)lane_byte, (int)uni_byte, 0);
+}
+
 template <int BY>
 struct Tile {
     // LDS row width: 64 columns + 4 halo on each side
     static constexpr int W = 72;
     static constexpr int R = BY + 8;
-    static constexpr int NYH = 512 / (64 * BY);  // y-halo elements per thread
+    static constexpr int NYH = (512 + 64 * BY - 1) / (64 * BY);  // y-halo elements per thread (8 rows x 64)
 };
 
 // Per-thread description of what it loads into the x/y tile each plane.
 template <int BY>
 struct TileLoader {
-    long long main_off;  // column offset of own point
+    unsigned main_off;  // offset of own point within a plane (unsigned: zero-extended address offsets)
     bool main_ok;
     int main_lds;
-    long long yh_off[Tile<BY>::NYH];
+    unsigned yh_off[Tile<BY>::NYH];
+    bool yh_has[Tile<BY>::NYH];
     bool yh_ok[Tile<BY>::NYH];
     int yh_lds[Tile<BY>::NYH];
-    long long xh_off;
+    unsigned xh_off;
     bool xh_ok;
     int xh_lds;
     bool has_xh;
@@ -73,16 +96,17 @@ struct TileLoader {
         const int tid = tx + 64 * ty;
         const int ix = x0 + tx, iy = y0 + ty;
         main_ok = ix < g.nx && iy < g.ny;
-        main_off = (long long)iy * g.pitch + ix;
+        main_off = (unsigned)(iy * g.pitch + ix);
         main_lds = (ty + 4) * W + tx + 4;
 #pragma unroll
         for (int h = 0; h < Tile<BY>::NYH; ++h) {
-            const int e = tid + h * 64 * BY;
-            const int r = e >> 6, c = e & 63;
+            const int e = tid + h * 64 * BY;  // e < 512: one of the 8 halo rows x 64 columns
+            const int r = (e >> 6) & 7, c = e & 63;
             const int hy = r < 4 ? y0 - 4 + r : y0 + BY + r - 4;
             const int hx = x0 + c;
-            yh_ok[h] = hy < g.ny && hx < g.nx;
-            yh_off[h] = (long long)hy * g.pitch + hx;
+            yh_has[h] = e < 512;
+            yh_ok[h] = e < 512 && hy < g.ny && hx < g.nx;
+            yh_off[h] = (unsigned)(hy * g.pitch + hx);
             yh_lds[h] = (r < 4 ? r : r + BY) * W + c + 4;
         }
         has_xh = tid < 8 * BY;
@@ -91,7 +115,7 @@ struct TileLoader {
             const int hx = c < 4 ? x0 - 4 + c : x0 + 64 + c - 4;
             const int hy = y0 + r;
             xh_ok = has_xh && hx >= 0 && hx < g.nx && hy < g.ny;
-            xh_off = (long long)hy * g.pitch + hx;
+            xh_off = (unsigned)(hy * g.pitch + hx);
             xh_lds = (r + 4) * W + (c < 4 ? c : 64 + c);
         }
     }
@@ -110,11 +134,19 @@ struct PlaneRegs {
         for (int h = 0; h < Tile<BY>::NYH; ++h) yh[h] = L.yh_ok[h] ? p[L.yh_off[h]] : 0.f;
         xh = L.xh_ok ? p[L.xh_off] : 0.f;
     }
+    __device__ __forceinline__ void loadb(rsrc_t r, unsigned uni_byte, const TileLoader<BY> &L)
+    {
+        m = L.main_ok ? bload(r, 4u * L.main_off, uni_byte) : 0.f;
+#pragma unroll
+        for (int h = 0; h < Tile<BY>::NYH; ++h) yh[h] = L.yh_ok[h] ? bload(r, 4u * L.yh_off[h], uni_byte) : 0.f;
+        xh = L.xh_ok ? bload(r, 4u * L.xh_off, uni_byte) : 0.f;
+    }
     __device__ __forceinline__ void store(float *lds, const TileLoader<BY> &L) const
     {
         lds[L.main_lds] = m;
 #pragma unroll
-        for (int h = 0; h < Tile<BY>::NYH; ++h) lds[L.yh_lds[h]] = yh[h];
+        for (int h = 0; h < Tile<BY>::NYH; ++h)
+            if (L.yh_has[h]) lds[L.yh_lds[h]] = yh[h];
         if (L.has_xh) lds[L.xh_lds] = xh;
     }
 };
@@ -242,6 +274,12 @@ __device__ __forceinline__ void effective_column(float *e, const float *q, float
 // prev_x/prev_z are read at each point and overwritten with the next level.
 // With ABC the stored levels are damped once; the second damping the
 // reference applies to the current level is applied on read, to prev.
+//
+// Software pipeline along z: while plane z is computed, the loads for plane
+// z+1 (prev, c11/c13/c33, the sigma_z plane entering the queue) and the
+// sigma_x tile of plane z+2 are in flight, so each wave keeps about two
+// planes of requests outstanding.  Plane bases are wave-uniform; per-lane
+// offsets within a plane are 32-bit.
 // ---------------------------------------------------------------------------
 template <int ARITH, bool ABC, int BY>
 __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
@@ -266,39 +304,59 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
 
     TileLoader<BY> L;
     L.init(tx, ty, x0, y0, g);
-    const long long col = L.main_off;
-    const float dxy = (ABC && inrange) ? dxy_plane[col] : 1.f;
+    const unsigned col4 = 4u * L.main_off;
+    const float dxy = (ABC && inrange) ? dxy_plane[L.main_off] : 1.f;
+    // resources based at the first plane this chunk touches; the host keeps
+    // (zc + 8) planes within 32-bit byte offsets
+    const int zb = max(zs - 4, 0);
+    const size_t base = (size_t)zb * (size_t)g.plane;
+    const rsrc_t Rcx = make_rsrc(cur_x + base), Rcz = make_rsrc(cur_z + base);
+    const rsrc_t Rnx = make_rsrc(nxt_x + base), Rnz = make_rsrc(nxt_z + base);
+    const rsrc_t R11 = make_rsrc(c11 + base), R13 = make_rsrc(c13 + base), R33 = make_rsrc(c33 + base);
+    const unsigned pb = (unsigned)(4 * g.plane);
+    auto so = [&](int z) { return (unsigned)(z - zb) * pb; };
 
+    // sigma_z queue: q[k] = sigma_z(z-4+k) at plane z; qn = sigma_z(z+5) in flight
     float q[9];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < 9; ++k) {
         const int zz = zs - 4 + k;
-        q[k] = (inrange && zz >= 0 && zz < g.zmax) ? cur_z[zz * g.plane + col] : 0.f;
+        q[k] = (inrange && zz >= 0 && zz < g.zmax) ? bload(Rcz, col4, so(zz)) : 0.f;
     }
-    {
-        PlaneRegs<BY> P;
-        P.load(cur_x + zs * g.plane, L);
-        P.store(tile[0], L);
+    // point data of plane z (prev and coefficients)
+    float px = 0.f, pz = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f;
+    if (inrange) {
+        px = bload(Rnx, col4, so(zs));
+        pz = bload(Rnz, col4, so(zs));
+        a11 = bload(R11, col4, so(zs));
+        a13 = bload(R13, col4, so(zs));
+        a33 = bload(R33, col4, so(zs));
     }
+    PlaneRegs<BY> P;
+    P.loadb(Rcx, so(zs), L);
+    P.store(tile[0], L);
+    if (zs + 1 < ze) P.loadb(Rcx, so(zs + 1), L);
     __syncthreads();
 
     for (int z0 = zs; z0 < ze; z0 += 4) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int z = z0 + r;
-            const long long gi = z * g.plane + col;
-            q[8] = (inrange && z + 4 < g.zmax) ? cur_z[gi + 4 * g.plane] : 0.f;
-            PlaneRegs<BY> P;
-            const bool more = z + 1 < ze;
-            if (more) P.load(cur_x + (z + 1) * g.plane, L);
-            float px = 0.f, pz = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f;
-            if (inrange) {
-                px = nxt_x[gi];
-                pz = nxt_z[gi];
-                a11 = c11[gi];
-                a13 = c13[gi];
-                a33 = c33[gi];
+            // the tile of plane z+1 goes to LDS; plane z+2's is requested
+            if (z + 1 < ze) P.store(tile[(r + 1) & 1], L);
+            if (z + 2 < ze) P.loadb(Rcx, so(z + 2), L);
+            const bool nx1 = inrange && z + 1 < ze;
+            float npx = 0.f, npz = 0.f, n11 = 0.f, n13 = 0.f, n33 = 0.f, qn = 0.f;
+            if (nx1) {
+                const unsigned o1 = so(z + 1);
+                npx = bload(Rnx, col4, o1);
+                npz = bload(Rnz, col4, o1);
+                n11 = bload(R11, col4, o1);
+                n13 = bload(R13, col4, o1);
+                n33 = bload(R33, col4, o1);
             }
+            if (inrange && z + 5 < g.zmax) qn = bload(Rcz, col4, so(z + 5));
+
             const float *T = &tile[r & 1][(ty + 4) * W + tx + 4];
             const float s0 = T[0];
             const float txy = sx_scaled<ARITH>(T, dt2dx2, dt2dy2);
@@ -316,6 +374,7 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
             }
             float vx = leap<ARITH>(a11 * txy + a13 * tz, s0, px);
             float vz = leap<ARITH>(a13 * txy + a33 * tz, q[4], pz);
+            const long long gi = (long long)z * g.plane + L.main_off;
             if (gi == src_idx) {
                 vx = vx + src_val;
                 vz = vz + src_val;
@@ -325,13 +384,18 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
                 vz = vz * d;
             }
             if (active) {
-                nxt_x[gi] = vx;
-                nxt_z[gi] = vz;
+                bstore(Rnx, col4, so(z), vx);
+                bstore(Rnz, col4, so(z), vz);
             }
-            if (more) P.store(tile[(r + 1) & 1], L);
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
+            q[8] = qn;
+            px = npx;
+            pz = npz;
+            a11 = n11;
+            a13 = n13;
+            a33 = n33;
         }
     }
 }
