@@ -443,23 +443,56 @@ __global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
 
     TileLoader<BY> L;
     L.init(tx, ty, x0, y0, g);
-    const long long col = L.main_off;
-    const float dxy = inrange ? dxy_plane[col] : 1.f;
+    const unsigned col4 = 4u * L.main_off;
+    const float dxy = inrange ? dxy_plane[L.main_off] : 1.f;
+    const int zb = max(zs - 4, 0);
+    const size_t base = (size_t)zb * (size_t)g.plane;
+    const rsrc_t Rscx = make_rsrc(s_cx + base), Rscz = make_rsrc(s_cz + base);
+    const rsrc_t Rlcx = make_rsrc(l_cx + base), Rlcz = make_rsrc(l_cz + base);
+    const rsrc_t Rsnx = make_rsrc(s_nx + base), Rsnz = make_rsrc(s_nz + base);
+    const rsrc_t Rlnx = make_rsrc(l_nx + base), Rlnz = make_rsrc(l_nz + base);
+    const rsrc_t R11 = make_rsrc(c11 + base), R13 = make_rsrc(c13 + base), R33 = make_rsrc(c33 + base);
+    const rsrc_t G11 = make_rsrc(g11 + base);
+    const rsrc_t G13 = make_rsrc(MODE == 0 ? g13 + base : g11 + base);
+    const rsrc_t G33 = make_rsrc(MODE == 0 ? g33 + base : g11 + base);
+    const unsigned pb = (unsigned)(4 * g.plane);
+    auto so = [&](int z) { return (unsigned)(z - zb) * pb; };
 
+    // queues: qs/ql[k] = sigma_z/lambda_z(z-4+k) at plane z; the (z+5) values in flight
     float qs[9], ql[9];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < 9; ++k) {
         const int zz = zs - 4 + k;
         const bool ok = inrange && zz >= 0 && zz < g.zmax;
-        qs[k] = ok ? s_cz[zz * g.plane + col] : 0.f;
-        ql[k] = ok ? l_cz[zz * g.plane + col] : 0.f;
+        qs[k] = ok ? bload(Rscz, col4, so(zz)) : 0.f;
+        ql[k] = ok ? bload(Rlcz, col4, so(zz)) : 0.f;
     }
-    {
-        PlaneRegs<BY> P;
-        P.load(s_cx + zs * g.plane, L);
-        P.store(ts[0], L);
-        P.load(l_cx + zs * g.plane, L);
-        P.store(tl[0], L);
+    // point data of plane z
+    float spx = 0.f, spz = 0.f, lpx = 0.f, lpz = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f;
+    float o11 = 0.f, o13 = 0.f, o33 = 0.f;
+    if (inrange) {
+        const unsigned o0 = so(zs);
+        spx = bload(Rsnx, col4, o0);
+        spz = bload(Rsnz, col4, o0);
+        lpx = bload(Rlnx, col4, o0);
+        lpz = bload(Rlnz, col4, o0);
+        a11 = bload(R11, col4, o0);
+        a13 = bload(R13, col4, o0);
+        a33 = bload(R33, col4, o0);
+        o11 = bload(G11, col4, o0);
+        if (MODE == 0) {
+            o13 = bload(G13, col4, o0);
+            o33 = bload(G33, col4, o0);
+        }
+    }
+    PlaneRegs<BY> PS, PL;
+    PS.loadb(Rscx, so(zs), L);
+    PL.loadb(Rlcx, so(zs), L);
+    PS.store(ts[0], L);
+    PL.store(tl[0], L);
+    if (zs + 1 < ze) {
+        PS.loadb(Rscx, so(zs + 1), L);
+        PL.loadb(Rlcx, so(zs + 1), L);
     }
     __syncthreads();
 
@@ -467,31 +500,34 @@ __global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int z = z0 + r;
-            const long long gi = z * g.plane + col;
-            const bool ok4 = inrange && z + 4 < g.zmax;
-            qs[8] = ok4 ? s_cz[gi + 4 * g.plane] : 0.f;
-            ql[8] = ok4 ? l_cz[gi + 4 * g.plane] : 0.f;
-            PlaneRegs<BY> PS, PL;
-            const bool more = z + 1 < ze;
-            if (more) {
-                PS.load(s_cx + (z + 1) * g.plane, L);
-                PL.load(l_cx + (z + 1) * g.plane, L);
+            if (z + 1 < ze) {
+                PS.store(ts[(r + 1) & 1], L);
+                PL.store(tl[(r + 1) & 1], L);
             }
-            float spx = 0.f, spz = 0.f, lpx = 0.f, lpz = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f;
-            float o11 = 0.f, o13 = 0.f, o33 = 0.f;
-            if (inrange) {
-                spx = s_nx[gi];
-                spz = s_nz[gi];
-                lpx = l_nx[gi];
-                lpz = l_nz[gi];
-                a11 = c11[gi];
-                a13 = c13[gi];
-                a33 = c33[gi];
-                o11 = g11[gi];
+            if (z + 2 < ze) {
+                PS.loadb(Rscx, so(z + 2), L);
+                PL.loadb(Rlcx, so(z + 2), L);
+            }
+            float n_spx = 0.f, n_spz = 0.f, n_lpx = 0.f, n_lpz = 0.f, n11 = 0.f, n13 = 0.f, n33 = 0.f;
+            float n_o11 = 0.f, n_o13 = 0.f, n_o33 = 0.f, qsn = 0.f, qln = 0.f;
+            if (inrange && z + 1 < ze) {
+                const unsigned o1 = so(z + 1);
+                n_spx = bload(Rsnx, col4, o1);
+                n_spz = bload(Rsnz, col4, o1);
+                n_lpx = bload(Rlnx, col4, o1);
+                n_lpz = bload(Rlnz, col4, o1);
+                n11 = bload(R11, col4, o1);
+                n13 = bload(R13, col4, o1);
+                n33 = bload(R33, col4, o1);
+                n_o11 = bload(G11, col4, o1);
                 if (MODE == 0) {
-                    o13 = g13[gi];
-                    o33 = g33[gi];
+                    n_o13 = bload(G13, col4, o1);
+                    n_o33 = bload(G33, col4, o1);
                 }
+            }
+            if (inrange && z + 5 < g.zmax) {
+                qsn = bload(Rscz, col4, so(z + 5));
+                qln = bload(Rlcz, col4, so(z + 5));
             }
             const float *TS = &ts[r & 1][(ty + 4) * W + tx + 4];
             const float *TL = &tl[r & 1][(ty + 4) * W + tx + 4];
@@ -510,6 +546,7 @@ __global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
                 svx = leap<ARITH>(a11 * txy + a13 * tz, s0, spx);
                 svz = leap<ARITH>(a13 * txy + a33 * tz, qs[4], spz);
             }
+            const long long gi = (long long)z * g.plane + L.main_off;
             if (gi == src_idx) {
                 svx = svx + src_val;
                 svz = svz + src_val;
@@ -544,19 +581,16 @@ __global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
                 o11 = (float)((double)o11 + (VTI_TWOTHIRD_D * (double)s0 + VTI_ONETHIRD_D * (double)qs[4]) * ls);
             }
             if (active) {
-                s_nx[gi] = svx;
-                s_nz[gi] = svz;
-                l_nx[gi] = lvx;
-                l_nz[gi] = lvz;
-                g11[gi] = o11;
+                const unsigned oz = so(z);
+                bstore(Rsnx, col4, oz, svx);
+                bstore(Rsnz, col4, oz, svz);
+                bstore(Rlnx, col4, oz, lvx);
+                bstore(Rlnz, col4, oz, lvz);
+                bstore(G11, col4, oz, o11);
                 if (MODE == 0) {
-                    g13[gi] = o13;
-                    g33[gi] = o33;
+                    bstore(G13, col4, oz, o13);
+                    bstore(G33, col4, oz, o33);
                 }
-            }
-            if (more) {
-                PS.store(ts[(r + 1) & 1], L);
-                PL.store(tl[(r + 1) & 1], L);
             }
             __syncthreads();
 #pragma unroll
@@ -564,6 +598,18 @@ __global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
                 qs[k] = qs[k + 1];
                 ql[k] = ql[k + 1];
             }
+            qs[8] = qsn;
+            ql[8] = qln;
+            spx = n_spx;
+            spz = n_spz;
+            lpx = n_lpx;
+            lpz = n_lpz;
+            a11 = n11;
+            a13 = n13;
+            a33 = n33;
+            o11 = n_o11;
+            o13 = n_o13;
+            o33 = n_o33;
         }
     }
 }
