@@ -171,6 +171,31 @@ int vti2d_gn_hessian(vti_ctx *ctx, float *gc11, float *gc13, float *gc33, const 
                      const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
                      int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot);
 
+/* ---- 2D v/eps/delta parameterisation ---------------------------------------- */
+
+/* checkEpsDel, lib/vti-fwi-2d/check.cpp:22-40: projects points with
+ * del0*del > eps0*eps onto del0*del = eps0*eps, in place; m[i] = 1 where they
+ * are equal, 2 where projected, 0 elsewhere.  Returns the number projected. */
+int vti_check_eps_del(float *eps, float *del, float eps0, float del0, long long n, float *m);
+
+/* objFuncGradientVEpsDel, lib/vti-fwi-2d/objFuncGradient.cu:28-54 (wave.h:10):
+ * checkEpsDel on eps/del (in place, mask in m), VEpsDel2Cij, objFuncGradient_f,
+ * the chain rule to gv/geps/gdel, and zero rows [0, npad + (int)(wbottom/dz+1)).
+ * v/eps/del are divided by v0/eps0/del0 as the reference takes them. */
+int vti2d_objfunc_gradient_vepsdel(vti_ctx *ctx, float *objfunc, float *gv, float *geps, float *gdel,
+                                   const float *data, float *v, float *eps, float *del, float v0, float eps0,
+                                   float del0, const float *wavelet, const int *sloc, int ns, const int *rloc, int nr,
+                                   const float *taper, int nx, int nz, int nt, int npad, float dx, float dz, float dt,
+                                   float rate, float ot, float wbottom, float *m);
+
+/* GNhessianVEpsDel, lib/vti-fwi-2d/hessians.cu:72-95 (wave.h:42): the
+ * Gauss-Newton product for a perturbation (dv, deps, ddel). */
+int vti2d_gn_hessian_vepsdel(vti_ctx *ctx, float *gv, float *geps, float *gdel, const float *data, float *v,
+                             float *eps, float *del, const float *dv, const float *deps, const float *ddel, float v0,
+                             float eps0, float del0, const float *wavelet, const int *sloc, int ns, const int *rloc,
+                             int nr, const float *taper, int nx, int nz, int nt, int npad, float dx, float dz, float dt,
+                             float rate, float ot, float wbottom, float *m);
+
 #ifdef __cplusplus
 }
 #endif

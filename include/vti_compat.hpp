@@ -145,6 +145,15 @@ void GNhessian_f(float *gc11, float *gc13, float *gc33, const float *data, const
                  const float *c33, const float *dc11, const float *dc13, const float *dc33, const float *wavelet,
                  const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
                  int npad, float dx, float dz, float dt, float rate, float ot);
+float objFuncGradientVEpsDel(float *gv, float *geps, float *gdel, const float *data, float *v, float *eps, float *del,
+                             float v0, float eps0, float del0, const float *wavelet, const int *sloc, int ns,
+                             const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad, float dx,
+                             float dz, float dt, float rate, float ot, float wbottom, float *m);
+void GNhessianVEpsDel(float *gv, float *geps, float *gdel, const float *d0, float *v, float *eps, float *del,
+                      const float *dv, const float *deps, const float *ddel, float v0, float eps0, float del0,
+                      const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
+                      int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot, float wbottom,
+                      float *m);
 #endif
 
 // 3D declarations (wave3d.h itself needs CUDA's stream type, so it is not
@@ -200,6 +209,33 @@ void GNhessian_f(float *gc11, float *gc13, float *gc33, const float *data, const
     vti_compat::report(vti2d_gn_hessian(vti_compat::context(), gc11, gc13, gc33, data, c11, c13, c33, dc11, dc13, dc33,
                                         wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot),
                        "GNhessian_f");
+}
+
+// objFuncGradientVEpsDel (wave.h:10, objFuncGradient.cu:28)
+float objFuncGradientVEpsDel(float *gv, float *geps, float *gdel, const float *data, float *v, float *eps, float *del,
+                             float v0, float eps0, float del0, const float *wavelet, const int *sloc, int ns,
+                             const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad, float dx,
+                             float dz, float dt, float rate, float ot, float wbottom, float *m)
+{
+    float f = 0.f;
+    vti_compat::report(vti2d_objfunc_gradient_vepsdel(vti_compat::context(), &f, gv, geps, gdel, data, v, eps, del, v0,
+                                                      eps0, del0, wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt, npad,
+                                                      dx, dz, dt, rate, ot, wbottom, m),
+                       "objFuncGradientVEpsDel");
+    return f;
+}
+
+// GNhessianVEpsDel (wave.h:42, hessians.cu:72)
+void GNhessianVEpsDel(float *gv, float *geps, float *gdel, const float *d0, float *v, float *eps, float *del,
+                      const float *dv, const float *deps, const float *ddel, float v0, float eps0, float del0,
+                      const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
+                      int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot, float wbottom,
+                      float *m)
+{
+    vti_compat::report(vti2d_gn_hessian_vepsdel(vti_compat::context(), gv, geps, gdel, d0, v, eps, del, dv, deps, ddel,
+                                                v0, eps0, del0, wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt, npad,
+                                                dx, dz, dt, rate, ot, wbottom, m),
+                       "GNhessianVEpsDel");
 }
 
 // ---- lib/vti-fwi-3d/gpu/wave3d.h ------------------------------------------

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "boundary.h"
+#include "check.h"
 #include "conversions.h"
 #include "init.h"
 
@@ -28,6 +29,10 @@ void ref_GradCij2GradVEpsDel(float *gv, float *geps, float *gdel, const float *g
                              float eps0, float del0, size_t n)
 {
     GradCij2GradVEpsDel(gv, geps, gdel, gc11, gc13, gc33, v, eps, del, v0, eps0, del0, n);
+}
+void ref_checkEpsDel(float *eps, float *del, float eps0, float del0, size_t n, float *m)
+{
+    checkEpsDel(eps, del, eps0, del0, n, m);
 }
 void ref_init_abc2d(float *taper, int nx, int nz, int npad) { init_abc(taper, nx, nz, npad); }
 void ref_getBoundary(float *b, const float *v, int nx, int ny, int nz, int npad) { getBoundary(b, v, nx, ny, nz, npad); }

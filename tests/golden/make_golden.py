@@ -7,7 +7,8 @@ It executes, as built from the reference's own sources under oracle/_ref/:
     the reference's random-boundary program, run in its own process on SEP
     files this script writes;
   - libref.so: VEpsDel2Cij / dVEpsDel2dCij / GradCij2GradVEpsDel
-    (lib/vti-fwi-2d/conversions.cpp), init_abc 2D (lib/vti-fwi-2d/init.cpp),
+    (lib/vti-fwi-2d/conversions.cpp), checkEpsDel (lib/vti-fwi-2d/check.cpp),
+    init_abc 2D (lib/vti-fwi-2d/init.cpp),
     getBoundary / putBoundary / zeroBoundary (lib/vti-fwi-3d/gpu/boundary.cpp),
     ricker (lib/vti-fwi-2d/ricker.cpp).
 Inputs are generated here from seeds; the fixture stores the seeds, shapes,
@@ -118,6 +119,18 @@ def main():
                               eps.ctypes.data, dl.ctypes.data, v0, e0, d0, n)
     fx["conversions"]["GradCij2GradVEpsDel"] = [sha(a) for a in gout]
     fx["conversions"]["inputs"] = {"seed": 11, "n": n, "v0": v0, "eps0": e0, "del0": d0}
+
+    # checkEpsDel (lib/vti-fwi-2d/check.cpp): eps/del pairs with del0*del above, equal to and below eps0*eps
+    rng2 = np.random.default_rng(12)
+    ce = rng2.uniform(0.0, 0.3, n).astype(np.float32)
+    cd = rng2.uniform(0.0, 0.3, n).astype(np.float32)
+    cd[::7] = ce[::7]
+    cm = np.zeros(n, np.float32)
+    L.ref_checkEpsDel.argtypes = [P, P, f, f, ctypes.c_size_t, P]
+    L.ref_checkEpsDel(ce.ctypes.data, cd.ctypes.data, 1.0, 1.0, n, cm.ctypes.data)
+    fx["conversions"]["checkEpsDel"] = {"seed": 12, "n": n, "eps0": 1.0, "del0": 1.0, "equal_every": 7,
+                                        "sha256": [sha(ce), sha(cd), sha(cm)],
+                                        "projected": int(np.count_nonzero(cm == 2))}
 
     for nx, nz, npad in ((256, 256, 40), (1024, 512, 40), (30, 20, 7)):
         t = np.zeros((nz + 2 * npad) * (nx + 2 * npad), np.float32)

@@ -36,6 +36,9 @@ CALLER = r"""
 float call(float *g, const float *d, const float *c, const float *w, const int *s, const int *r, const float *t) {
   synthetic_f(g, c, c, c, w, s, 1, r, 1, t, 10, 10, 10, 4, 1.f, 1.f, 1.f, 1.f);
   GNhessian_f(g, g, g, d, c, c, c, c, c, c, w, s, 1, r, 1, t, 10, 10, 10, 4, 1.f, 1.f, 1.f, 1.f, 0.f);
+  float m[1];
+  GNhessianVEpsDel(g, g, g, d, g, g, g, c, c, c, 1.f, 1.f, 1.f, w, s, 1, r, 1, t, 10, 10, 10, 4, 1.f, 1.f, 1.f, 1.f, 0.f, 0.f, m);
+  objFuncGradientVEpsDel(g, g, g, d, g, g, g, 1.f, 1.f, 1.f, w, s, 1, r, 1, t, 10, 10, 10, 4, 1.f, 1.f, 1.f, 1.f, 0.f, 0.f, m);
   return objFuncGradient_f(g, g, g, d, c, c, c, w, s, 1, r, 1, t, 10, 10, 10, 4, 1.f, 1.f, 1.f, 1.f);
 }
 """

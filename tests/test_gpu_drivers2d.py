@@ -104,3 +104,61 @@ def test_synthetic_cfg1_size(vti, orc, ctx):
     o = orc.synthetic2d(orc.F32, *P["c"], P["w"], P["sloc"], P["rloc"], P["taper"], *args2d(P))
     assert np.abs(o).max() > 0
     assert np.array_equal(d, o)
+
+
+def _vepsdel_problem(vti):
+    from common2d import layered2d
+
+    P = problem2d(vti)
+    v, e, dl = layered2d(P["nnz"], P["nnx"], seed=3)
+    v0, eps0, del0 = 2000.0, 0.5, 0.25
+    vn, en, dn = v / v0, e / eps0, dl / del0
+    dn = dn.copy()
+    dn[P["npad"] + 20: P["npad"] + 24, :] = en[P["npad"] + 20: P["npad"] + 24, :] * eps0 / del0 * 1.3  # del > eps
+    return P, vn.astype(np.float32), en.astype(np.float32), dn.astype(np.float32), (v0, eps0, del0)
+
+
+def test_vepsdel_gradient_wrapper(vti, ctx):
+    """objFuncGradientVEpsDel = checkEpsDel, VEpsDel2Cij, objFuncGradient_f,
+    GradCij2GradVEpsDel and zeroed rows (each pinned on its own): the wrapper
+    equals that composition bit for bit."""
+    P, v, e, d, (v0, eps0, del0) = _vepsdel_problem(vti)
+    a = args2d(P)
+    wb = 30.0
+    ct = vti.vepsdel_to_cij(v * 1.03, e, np.minimum(d, e * eps0 / del0), v0, eps0, del0)
+    data = vti.synthetic2d(ctx, *ct, P["w"], P["sloc"], P["rloc"], P["taper"], *a)
+    f, g, e2, d2, m = vti.objfunc_gradient2d_vepsdel(ctx, data, v, e, d, v0, eps0, del0, P["w"], P["sloc"], P["rloc"],
+                                                      P["taper"], *a, wbottom=wb)
+    ce, cd, cm, count = vti.check_eps_del(e, d, eps0, del0)
+    assert count > 0
+    c = vti.vepsdel_to_cij(v, ce, cd, v0, eps0, del0)
+    f1, gc = vti.objfunc_gradient2d(ctx, data, *c, P["w"], P["sloc"], P["rloc"], P["taper"], *a)
+    gg = vti.gradcij_to_gradvepsdel(gc[0], gc[1], gc[2], v, ce, cd, v0, eps0, del0)
+    rows = P["npad"] + int(wb / P["dz"] + 1)
+    for k in range(3):
+        gk = gg[k].reshape(P["nnz"], P["nnx"])
+        gk[:rows] = 0
+        assert np.array_equal(g[k], gk)
+    assert f == f1
+    assert np.array_equal(e2, ce.ravel()) and np.array_equal(d2, cd.ravel()) and np.array_equal(m, cm.ravel())
+    assert np.abs(g[0]).max() > 0
+
+
+def test_vepsdel_gn_wrapper(vti, ctx):
+    P, v, e, d, (v0, eps0, del0) = _vepsdel_problem(vti)
+    a = args2d(P)
+    wb = 20.0
+    rng = np.random.default_rng(4)
+    dv, de, dd = (rng.standard_normal(v.shape).astype(np.float32) * 0.01 for _ in range(3))
+    h, e2, d2, m = vti.gn_hessian2d_vepsdel(ctx, v, e, d, dv, de, dd, v0, eps0, del0, P["w"], P["sloc"], P["rloc"],
+                                             P["taper"], *a, wbottom=wb)
+    ce, cd, cm, _ = vti.check_eps_del(e, d, eps0, del0)
+    cc = vti.dvepsdel_to_dcij(v, ce, cd, dv, de, dd, v0, eps0, del0)
+    hc = vti.gn_hessian2d(ctx, *cc, P["w"], P["sloc"], P["rloc"], P["taper"], *a)
+    gg = vti.gradcij_to_gradvepsdel(hc[0], hc[1], hc[2], v, ce, cd, v0, eps0, del0)
+    rows = int(np.float32(P["npad"]) + np.float32(wb) / np.float32(P["dz"]) + np.float32(1))
+    for k in range(3):
+        gk = gg[k].reshape(P["nnz"], P["nnx"])
+        gk[:rows] = 0
+        assert np.array_equal(h[k], gk)
+    assert np.abs(h[0]).max() > 0

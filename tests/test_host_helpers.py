@@ -86,6 +86,21 @@ def test_conversions_bitexact(vti):
     assert [sha(a) for a in out] == g["GradCij2GradVEpsDel"]
 
 
+def test_check_eps_del_bitexact(vti):
+    g = GOLD["conversions"]["checkEpsDel"]
+    n = g["n"]
+    rng = np.random.default_rng(g["seed"])
+    e = rng.uniform(0.0, 0.3, n).astype(np.float32)
+    d = rng.uniform(0.0, 0.3, n).astype(np.float32)
+    d[:: g["equal_every"]] = e[:: g["equal_every"]]
+    e2, d2, m, count = vti.check_eps_del(e, d, g["eps0"], g["del0"])
+    assert [sha(e2), sha(d2), sha(m)] == g["sha256"]
+    assert count == g["projected"]
+    # what the projection fixes: del0*del <= eps0*eps everywhere afterwards, equal on the projected points
+    assert np.all(d2 <= e2 * (1 + 1e-6))
+    assert np.allclose(d2[m == 2], e2[m == 2], rtol=1e-6)
+
+
 @pytest.mark.parametrize("nx,nz,npad", [(256, 256, 40), (1024, 512, 40), (30, 20, 7)])
 def test_taper2d_bitexact(vti, nx, nz, npad):
     t = vti.init_taper2d(nx, nz, npad)

@@ -268,6 +268,21 @@ __device__ __forceinline__ void effective_column(float *e, const float *q, float
     }
 }
 
+// effective_column with the plane's position in its 4-plane block known only
+// at run time (r = z % 4).
+template <bool ABC>
+__device__ __forceinline__ void effective_column_rt(float *e, const float *q, float dxy,
+                                                    const float *__restrict__ dzv, int z, int r)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) e[k] = q[k];
+    if (ABC && z >= 4) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < 4 - r) e[k] = q[k] * (dxy * dzv[z - 4 + k]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Forward step (forwardKernel semantics, kernel3d.cu:3-74, plus the fused
 // injectSource and abcXYZ/abcXY of objFuncGradient3d_f.cu:655-662).
@@ -339,7 +354,7 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
     __syncthreads();
 
     for (int z0 = zs; z0 < ze; z0 += 4) {
-#pragma unroll
+#pragma unroll 1
         for (int r = 0; r < 4; ++r) {
             const int z = z0 + r;
             // the tile of plane z+1 goes to LDS; plane z+2's is requested
@@ -361,10 +376,7 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
             const float s0 = T[0];
             const float txy = sx_scaled<ARITH>(T, dt2dx2, dt2dy2);
             float e[9];
-            if (r == 0) effective_column<0, ABC>(e, q, dxy, dzv, z);
-            if (r == 1) effective_column<1, ABC>(e, q, dxy, dzv, z);
-            if (r == 2) effective_column<2, ABC>(e, q, dxy, dzv, z);
-            if (r == 3) effective_column<3, ABC>(e, q, dxy, dzv, z);
+            effective_column_rt<ABC>(e, q, dxy, dzv, z, r);
             const float tz = sz_scaled<ARITH>(e, dt2dz2);
             float d = 1.f;
             if (ABC) {
@@ -417,7 +429,7 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
 // inject_damp3d_kernel, which then damps those planes.
 // ---------------------------------------------------------------------------
 template <int MODE, int ARITH, int BY>
-__global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
+__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) void bwd3d_kernel(
     float *__restrict__ s_nx, const float *__restrict__ s_cx, float *__restrict__ s_nz,
     const float *__restrict__ s_cz, float *__restrict__ l_nx, const float *__restrict__ l_cx,
     float *__restrict__ l_nz, const float *__restrict__ l_cz, const float *__restrict__ c11,
@@ -469,7 +481,6 @@ __global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
     }
     // point data of plane z
     float spx = 0.f, spz = 0.f, lpx = 0.f, lpz = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f;
-    float o11 = 0.f, o13 = 0.f, o33 = 0.f;
     if (inrange) {
         const unsigned o0 = so(zs);
         spx = bload(Rsnx, col4, o0);
@@ -479,11 +490,6 @@ __global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
         a11 = bload(R11, col4, o0);
         a13 = bload(R13, col4, o0);
         a33 = bload(R33, col4, o0);
-        o11 = bload(G11, col4, o0);
-        if (MODE == 0) {
-            o13 = bload(G13, col4, o0);
-            o33 = bload(G33, col4, o0);
-        }
     }
     PlaneRegs<BY> PS, PL;
     PS.loadb(Rscx, so(zs), L);
@@ -497,7 +503,7 @@ __global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
     __syncthreads();
 
     for (int z0 = zs; z0 < ze; z0 += 4) {
-#pragma unroll
+#pragma unroll 1
         for (int r = 0; r < 4; ++r) {
             const int z = z0 + r;
             if (z + 1 < ze) {
@@ -508,8 +514,18 @@ __global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
                 PS.loadb(Rscx, so(z + 2), L);
                 PL.loadb(Rlcx, so(z + 2), L);
             }
+            // the image/gradient accumulators of this plane, needed last
+            float o11 = 0.f, o13 = 0.f, o33 = 0.f;
+            if (inrange) {
+                const unsigned oz = so(z);
+                o11 = bload(G11, col4, oz);
+                if (MODE == 0) {
+                    o13 = bload(G13, col4, oz);
+                    o33 = bload(G33, col4, oz);
+                }
+            }
             float n_spx = 0.f, n_spz = 0.f, n_lpx = 0.f, n_lpz = 0.f, n11 = 0.f, n13 = 0.f, n33 = 0.f;
-            float n_o11 = 0.f, n_o13 = 0.f, n_o33 = 0.f, qsn = 0.f, qln = 0.f;
+            float qsn = 0.f, qln = 0.f;
             if (inrange && z + 1 < ze) {
                 const unsigned o1 = so(z + 1);
                 n_spx = bload(Rsnx, col4, o1);
@@ -519,11 +535,6 @@ __global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
                 n11 = bload(R11, col4, o1);
                 n13 = bload(R13, col4, o1);
                 n33 = bload(R33, col4, o1);
-                n_o11 = bload(G11, col4, o1);
-                if (MODE == 0) {
-                    n_o13 = bload(G13, col4, o1);
-                    n_o33 = bload(G33, col4, o1);
-                }
             }
             if (inrange && z + 5 < g.zmax) {
                 qsn = bload(Rscz, col4, so(z + 5));
@@ -555,10 +566,7 @@ __global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
             // ---- receiver side (forwardKernel + damping)
             const float lTxy = sx_scaled<ARITH>(TL, dt2dx2, dt2dy2);
             float e[9];
-            if (r == 0) effective_column<0, true>(e, ql, dxy, dzv, z);
-            if (r == 1) effective_column<1, true>(e, ql, dxy, dzv, z);
-            if (r == 2) effective_column<2, true>(e, ql, dxy, dzv, z);
-            if (r == 3) effective_column<3, true>(e, ql, dxy, dzv, z);
+            effective_column_rt<true>(e, ql, dxy, dzv, z, r);
             const float lTz = sz_scaled<ARITH>(e, dt2dz2);
             const float d = dxy * dzv[z];
             float lvx = leap<ARITH>(a11 * lTxy + a13 * lTz, l0, lpx * d);
@@ -607,9 +615,6 @@ __global__ __launch_bounds__(64 * BY) void bwd3d_kernel(
             a11 = n11;
             a13 = n13;
             a33 = n33;
-            o11 = n_o11;
-            o13 = n_o13;
-            o33 = n_o33;
         }
     }
 }

@@ -442,4 +442,24 @@ long long random_boundary(float *field, float bval, float eval, int n1, int n2, 
     return cluster_id;
 }
 
+// checkEpsDel, check.cpp:22-40: FP32 throughout, as the reference computes it
+long long check_eps_del(float *eps, float *del, float eps0, float del0, size_t n, float *m)
+{
+    long long count = 0;
+    for (size_t i = 0; i < n; ++i) {
+        m[i] = 0.f;
+        const float d = del0 * del[i], e = eps0 * eps[i];
+        if (d == e) {
+            m[i] = 1.f;
+        } else if (d > e) {
+            const float t = (eps0 * del[i] + del0 * eps[i]) / (eps0 * eps0 + del0 * del0);
+            eps[i] = t * del0;
+            del[i] = t * eps0;
+            m[i] = 2.f;
+            ++count;
+        }
+    }
+    return count;
+}
+
 }  // namespace vti

@@ -27,6 +27,11 @@ void gradcij_to_gradvepsdel(float *gv, float *geps, float *gdel, const float *gc
                             float del0, size_t n);
 
 // getBoundary / putBoundary / zeroBoundary, lib/vti-fwi-3d/gpu/boundary.cpp:10-117
+// checkEpsDel (lib/vti-fwi-2d/check.cpp:22-40): where del0*del > eps0*eps,
+// project (eps0*eps, del0*del) onto the line eps0*eps = del0*del; m marks
+// equal points with 1 and projected ones with 2.  Returns the number projected.
+long long check_eps_del(float *eps, float *del, float eps0, float del0, size_t n, float *m);
+
 long long boundary_size(int nx, int ny, int nz, int npad);
 void get_boundary(float *packed, const float *v, int nx, int ny, int nz, int npad);
 void put_boundary(const float *packed, float *v, int nx, int ny, int nz, int npad);

@@ -93,6 +93,15 @@ _SIGS = {
         c_int,
         [P] * 13 + [c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5,
     ),
+    "vti_check_eps_del": (c_int, [P, P, c_float, c_float, c_longlong, P]),
+    "vti2d_objfunc_gradient_vepsdel": (
+        c_int,
+        [P] * 9 + [c_float] * 3 + [P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 6 + [P],
+    ),
+    "vti2d_gn_hessian_vepsdel": (
+        c_int,
+        [P] * 11 + [c_float] * 3 + [P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 6 + [P],
+    ),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -379,3 +388,62 @@ def gn_hessian2d(ctx, c11, c13, c33, dc11, dc13, dc33, wavelet, sloc, rloc, tape
                                 ptr(sloc), sloc.size // 4, ptr(rloc), rloc.size // 2, ptr(arrs[7]), nx, nz, nt, npad,
                                 dx, dz, dt, rate, ot), "vti2d_gn_hessian")
     return gc
+
+
+def check_eps_del(eps, dl, eps0=1.0, del0=1.0):
+    """checkEpsDel on copies: returns (eps, del, mask, count)."""
+    e, d = f32(eps).copy(), f32(dl).copy()
+    m = np.zeros_like(e)
+    n = check(_lib.vti_check_eps_del(ptr(e), ptr(d), eps0, del0, e.size, ptr(m)), "vti_check_eps_del")
+    return e, d, m, n
+
+
+def objfunc_gradient2d_vepsdel(ctx, data, v, eps, dl, v0, eps0, del0, wavelet, sloc, rloc, taper, nx, nz, nt, npad,
+                               dx, dz, dt, rate, ot=0.0, wbottom=0.0):
+    """objFuncGradientVEpsDel: returns (objective, (gv, geps, gdel), eps', del', mask); eps/del are
+    projected by checkEpsDel as the reference does (the caller's arrays are not modified)."""
+    v, e, d = f32(v).ravel().copy(), f32(eps).ravel().copy(), f32(dl).ravel().copy()
+    w, t, dd = f32(wavelet).ravel(), f32(taper).ravel(), f32(data).ravel()
+    sloc, rloc = _i32(sloc), _i32(rloc)
+    nnx, nnz = nx + 2 * npad, nz + 2 * npad
+    g = np.zeros((3, nnz, nnx), np.float32)
+    m = np.zeros(nnx * nnz, np.float32)
+    f = ctypes.c_float(0.0)
+    check(_lib.vti2d_objfunc_gradient_vepsdel(ctx.h, ctypes.byref(f), ptr(g[0]), ptr(g[1]), ptr(g[2]), ptr(dd), ptr(v),
+                                              ptr(e), ptr(d), v0, eps0, del0, ptr(w), ptr(sloc), sloc.size // 4,
+                                              ptr(rloc), rloc.size // 2, ptr(t), nx, nz, nt, npad, dx, dz, dt, rate,
+                                              ot, wbottom, ptr(m)), "vti2d_objfunc_gradient_vepsdel")
+    return f.value, g, e, d, m
+
+
+def gn_hessian2d_vepsdel(ctx, v, eps, dl, dv, deps, ddel, v0, eps0, del0, wavelet, sloc, rloc, taper, nx, nz, nt,
+                         npad, dx, dz, dt, rate, ot=0.0, wbottom=0.0):
+    """GNhessianVEpsDel: returns (H (3, nnz, nnx) for (dv, deps, ddel), eps', del', mask)."""
+    v, e, d = f32(v).ravel().copy(), f32(eps).ravel().copy(), f32(dl).ravel().copy()
+    pv, pe, pd = (f32(a).ravel() for a in (dv, deps, ddel))
+    w, t = f32(wavelet).ravel(), f32(taper).ravel()
+    sloc, rloc = _i32(sloc), _i32(rloc)
+    nnx, nnz = nx + 2 * npad, nz + 2 * npad
+    g = np.zeros((3, nnz, nnx), np.float32)
+    m = np.zeros(nnx * nnz, np.float32)
+    check(_lib.vti2d_gn_hessian_vepsdel(ctx.h, ptr(g[0]), ptr(g[1]), ptr(g[2]), None, ptr(v), ptr(e), ptr(d), ptr(pv),
+                                        ptr(pe), ptr(pd), v0, eps0, del0, ptr(w), ptr(sloc), sloc.size // 4, ptr(rloc),
+                                        rloc.size // 2, ptr(t), nx, nz, nt, npad, dx, dz, dt, rate, ot, wbottom,
+                                        ptr(m)), "vti2d_gn_hessian_vepsdel")
+    return g, e, d, m
+
+
+def dvepsdel_to_dcij(v, eps, dl, dv, deps, ddel, v0=1.0, eps0=1.0, del0=1.0):
+    a = [f32(x).ravel() for x in (v, eps, dl, dv, deps, ddel)]
+    out = [np.empty_like(a[0]) for _ in range(6)]
+    check(_lib.vti_dvepsdel_to_dcij(*[ptr(o) for o in out], *[ptr(x) for x in a], v0, eps0, del0, a[0].size),
+          "vti_dvepsdel_to_dcij")
+    return out
+
+
+def gradcij_to_gradvepsdel(gc11, gc13, gc33, v, eps, dl, v0=1.0, eps0=1.0, del0=1.0):
+    a = [f32(x).ravel() for x in (gc11, gc13, gc33, v, eps, dl)]
+    out = [np.empty_like(a[0]) for _ in range(3)]
+    check(_lib.vti_gradcij_to_gradvepsdel(*[ptr(o) for o in out], *[ptr(x) for x in a], v0, eps0, del0, a[0].size),
+          "vti_gradcij_to_gradvepsdel")
+    return out
