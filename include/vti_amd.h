@@ -196,6 +196,21 @@ int vti2d_gn_hessian_vepsdel(vti_ctx *ctx, float *gv, float *geps, float *gdel, 
                              int nr, const float *taper, int nx, int nz, int nt, int npad, float dx, float dz, float dt,
                              float rate, float ot, float wbottom, float *m);
 
+/* ---- L-BFGS (FP64) -----------------------------------------------------------
+ * lbfgs, lib/lbfgs/lbfgs_c.cpp:11 (lbfgs.h:32) with mcsrch/mcstep,
+ * lib/lbfgs/lsearch.cpp:9-606: reverse communication.  Start with *iflag = 0
+ * and f, g at x; while the call returns *iflag = 1, evaluate f and g at the
+ * new x and call again; *iflag = 0 on convergence (|g|/max(1,|x|) <= 1e-5),
+ * -1 when the line search fails (vti_last_error() says why).  The state
+ * (m corrections, line search) is FP64; the diagonal H0 is (y's)/(y'y).
+ * unit_step = 0 keeps the reference's choice of starting every line search
+ * after the first at the step the previous one accepted (lbfgs_c.cpp:433-436);
+ * 1 restarts at stp = 1 as in Liu & Nocedal's method. */
+typedef struct vti_lbfgs vti_lbfgs;
+int vti_lbfgs_create(vti_lbfgs **state, long long n, int m, int unit_step);
+int vti_lbfgs_destroy(vti_lbfgs *state);
+int vti_lbfgs_step(vti_lbfgs *state, float *x, double f, const float *g, int *iflag, int *iter, int *nfun);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,6 +33,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <map>
 #include <vector>
 
 #include "vti_amd.h"
@@ -174,6 +175,12 @@ void rtmCij3d_f(float *image, float *souloc, int ns, std::vector<int> &shotid, f
                 float *c11, float *c13, float *c33, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
                 float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate);
 
+// lib/lbfgs/lbfgs.h:32
+#ifndef LBFGS_H
+void lbfgs(size_t n, int &m, float *x, float &f, float *g, bool diagco, float *diag, float *w, int &iflag, int *isave,
+           float *dsave);
+#endif
+
 #ifdef VTI_COMPAT_IMPLEMENTATION
 
 // synthetic_f (wave.h:4, synthetic_f.cu:13)
@@ -236,6 +243,51 @@ void GNhessianVEpsDel(float *gv, float *geps, float *gdel, const float *d0, floa
                                                 v0, eps0, del0, wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt, npad,
                                                 dx, dz, dt, rate, ot, wbottom, m),
                        "GNhessianVEpsDel");
+}
+
+// lbfgs (lbfgs.h:32, lbfgs_c.cpp:11) on the FP64 method of vti_lbfgs_*.  The
+// caller's w/isave/dsave keep their sizes but not the state: that lives in an
+// FP64 object keyed by w, made on iflag == 0 and dropped when the run ends.
+// diag is not read (H0 is (y's)/(y'y)); diagco = true is refused.
+void lbfgs(size_t n, int &m, float *x, float &f, float *g, bool diagco, float *diag, float *w, int &iflag, int *isave,
+           float *dsave)
+{
+    (void)diag;
+    (void)isave;
+    (void)dsave;
+    static std::map<float *, vti_lbfgs *> states;
+    if (diagco) {
+        std::fprintf(stderr, "lbfgs: a user-supplied diagonal (diagco) is not supported\n");
+        iflag = -3;
+        return;
+    }
+    vti_lbfgs *&st = states[w];
+    if (iflag == 0) {
+        if (st) vti_lbfgs_destroy(st);
+        st = nullptr;
+        if (vti_lbfgs_create(&st, (long long)n, m, 0) != 0) {
+            vti_compat::report(-1, "lbfgs");
+            states.erase(w);
+            iflag = -3;
+            return;
+        }
+    } else if (!st) {
+        std::fprintf(stderr, "lbfgs: re-entry with iflag=%d but no run started on this workspace\n", iflag);
+        states.erase(w);
+        iflag = -3;
+        return;
+    }
+    int it = 0, nf = 0;
+    if (vti_lbfgs_step(st, x, (double)f, g, &iflag, &it, &nf) != 0) {
+        vti_compat::report(-1, "lbfgs");
+        iflag = -1;
+    } else if (iflag < 0) {
+        std::fprintf(stderr, "lbfgs: %s\n", vti_last_error());
+    }
+    if (iflag != 1) {
+        vti_lbfgs_destroy(st);
+        states.erase(w);
+    }
 }
 
 // ---- lib/vti-fwi-3d/gpu/wave3d.h ------------------------------------------

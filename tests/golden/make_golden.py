@@ -10,7 +10,10 @@ It executes, as built from the reference's own sources under oracle/_ref/:
     (lib/vti-fwi-2d/conversions.cpp), checkEpsDel (lib/vti-fwi-2d/check.cpp),
     init_abc 2D (lib/vti-fwi-2d/init.cpp),
     getBoundary / putBoundary / zeroBoundary (lib/vti-fwi-3d/gpu/boundary.cpp),
-    ricker (lib/vti-fwi-2d/ricker.cpp).
+    ricker (lib/vti-fwi-2d/ricker.cpp);
+  - libreflbfgs.so: lbfgs (lib/lbfgs/lbfgs_c.cpp, lsearch.cpp, la.cpp) on the
+    test functions of lib/lbfgs/functions.cpp, driven as lib/lbfgs/test1c.cpp
+    drives them.
 Inputs are generated here from seeds; the fixture stores the seeds, shapes,
 SHA-256 of every output array and a few sampled values.
 """
@@ -156,6 +159,25 @@ def main():
         L.ref_ricker.argtypes = [P, f, ctypes.c_int, f, f, f, f]
         L.ref_ricker(w.ctypes.data, freq, nt, dt, ot, tdelay, s)
         fx["misc"][f"ricker_{freq}_{nt}_{dt}_{ot}_{tdelay}_{s}"] = sha(w)
+
+    # the reference L-BFGS (lib/lbfgs, built without OpenMP) on its own test
+    # functions in the loop of lib/lbfgs/test1c.cpp (at most 201 calls)
+    LB = ctypes.CDLL(os.path.join(REF, "libreflbfgs.so"))
+    LB.ref_lbfgs_run.argtypes = [ctypes.c_int, ctypes.c_int, P, P, ctypes.POINTER(ctypes.c_int)]
+    fx["lbfgs"] = {}
+    for func, n in ((0, 2), (1, 2), (2, 100), (3, 100), (4, 50), (5, 25)):
+        xo = np.zeros(n, np.float32)
+        ft = np.zeros(201, np.float32)
+        fl = ctypes.c_int()
+        saved = os.dup(2)
+        os.dup2(os.open(os.devnull, os.O_WRONLY), 2)  # the reference reports every step on stderr
+        try:
+            calls = LB.ref_lbfgs_run(func, 201, xo.ctypes.data, ft.ctypes.data, ctypes.byref(fl))
+        finally:
+            os.dup2(saved, 2)
+        fx["lbfgs"][str(func)] = {"n": n, "calls": calls, "iflag": fl.value, "x": [float(v) for v in xo],
+                                  "f": [float(v) for v in ft[:calls]]}
+        print("lbfgs", func, "calls", calls, "iflag", fl.value)
 
     with open(OUT, "w") as fo:
         json.dump(fx, fo, indent=1)

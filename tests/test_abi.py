@@ -29,10 +29,11 @@ def kind(decl):
     if "*" in decl:
         return "ptr"
     t = decl.rsplit(" ", 1)[0].replace("const ", "").strip()
-    return {"int": "int", "float": "float", "long long": "i64", "size_t": "size"}[t]
+    return {"int": "int", "float": "float", "double": "f64", "long long": "i64", "size_t": "size"}[t]
 
 
-CT = {ctypes.c_int: "int", ctypes.c_float: "float", ctypes.c_longlong: "i64", ctypes.c_size_t: "size",
+CT = {ctypes.c_int: "int", ctypes.c_float: "float", ctypes.c_double: "f64", ctypes.c_longlong: "i64",
+      ctypes.c_size_t: "size",
       ctypes.c_void_p: "ptr", ctypes.c_char_p: "ptr"}
 
 

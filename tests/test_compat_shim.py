@@ -24,6 +24,7 @@ IMPL = r"""
 #include <type_traits>
 #include <vector>
 %(wave)s
+%(lbfgs)s
 #define VTI_COMPAT_IMPLEMENTATION
 #include "vti_compat.hpp"
 static_assert(std::is_same<decltype(&synthetic_f), void (*)(float *, const float *, const float *, const float *,
@@ -33,12 +34,16 @@ static_assert(std::is_same<decltype(&synthetic_f), void (*)(float *, const float
 
 CALLER = r"""
 #include "%(wave)s"
+#include "%(lbfgs)s"
 float call(float *g, const float *d, const float *c, const float *w, const int *s, const int *r, const float *t) {
   synthetic_f(g, c, c, c, w, s, 1, r, 1, t, 10, 10, 10, 4, 1.f, 1.f, 1.f, 1.f);
   GNhessian_f(g, g, g, d, c, c, c, c, c, c, w, s, 1, r, 1, t, 10, 10, 10, 4, 1.f, 1.f, 1.f, 1.f, 0.f);
   float m[1];
   GNhessianVEpsDel(g, g, g, d, g, g, g, c, c, c, 1.f, 1.f, 1.f, w, s, 1, r, 1, t, 10, 10, 10, 4, 1.f, 1.f, 1.f, 1.f, 0.f, 0.f, m);
   objFuncGradientVEpsDel(g, g, g, d, g, g, g, 1.f, 1.f, 1.f, w, s, 1, r, 1, t, 10, 10, 10, 4, 1.f, 1.f, 1.f, 1.f, 0.f, 0.f, m);
+  int m5 = 5, iflag = 0, isave[8];
+  float f = 0.f, dsave[14];
+  lbfgs(10, m5, g, f, g, false, g, g, iflag, isave, dsave);
   return objFuncGradient_f(g, g, g, d, c, c, c, w, s, 1, r, 1, t, 10, 10, 10, 4, 1.f, 1.f, 1.f, 1.f);
 }
 """
@@ -51,7 +56,7 @@ def _run(cmd, cwd):
 
 @needs_gxx
 def test_shim_compiles_standalone(tmp_path):
-    (tmp_path / "impl.cpp").write_text(IMPL % {"wave": ""})
+    (tmp_path / "impl.cpp").write_text(IMPL % {"wave": "", "lbfgs": ""})
     _run(["g++", "-std=c++17", "-Wall", "-Werror", "-fPIC", "-c", "impl.cpp", "-I", INC, "-o", "impl.o"], tmp_path)
 
 
@@ -59,10 +64,14 @@ def test_shim_compiles_standalone(tmp_path):
 @needs_ref
 def test_2d_callers_link_against_shim(tmp_path, vti):
     wave = os.path.join(REF, "vti-fwi-2d", "wave.h")
-    (tmp_path / "impl.cpp").write_text(IMPL % {"wave": f'#include "{wave}"'})
-    (tmp_path / "caller.cpp").write_text(CALLER % {"wave": wave})
-    _run(["g++", "-std=c++17", "-fPIC", "-c", "impl.cpp", "-I", INC, "-o", "impl.o"], tmp_path)
-    _run(["g++", "-std=c++17", "-fPIC", "-c", "caller.cpp", "-o", "caller.o"], tmp_path)
+    lb = os.path.join(REF, "lbfgs", "lbfgs.h")
+    inc = ["-I", os.path.join(REF, "lbfgs"), "-I", os.path.join(REF, "LinearSolver")]
+    # lbfgs.h #defines epsilon, which breaks <cmath> after it: the caller includes it, the
+    # implementation unit (as in INTEGRATION) does not, and the link checks the signature
+    (tmp_path / "impl.cpp").write_text(IMPL % {"wave": f'#include "{wave}"', "lbfgs": ""})
+    (tmp_path / "caller.cpp").write_text(CALLER % {"wave": wave, "lbfgs": lb})
+    _run(["g++", "-std=c++17", "-fPIC", "-c", "impl.cpp", "-I", INC, *inc, "-o", "impl.o"], tmp_path)
+    _run(["g++", "-std=c++17", "-fPIC", "-c", "caller.cpp", *inc, "-o", "caller.o"], tmp_path)
     _run(["g++", "-shared", "impl.o", "caller.o", "-L", LIBDIR, "-lvtiamd", "-Wl,--no-undefined",
           "-Wl,-rpath-link,/opt/rocm/lib", "-o", "libcaller.so"], tmp_path)
 

@@ -94,6 +94,9 @@ _SIGS = {
         [P] * 13 + [c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5,
     ),
     "vti_check_eps_del": (c_int, [P, P, c_float, c_float, c_longlong, P]),
+    "vti_lbfgs_create": (c_int, [POINTER(c_void_p), c_longlong, c_int, c_int]),
+    "vti_lbfgs_destroy": (c_int, [P]),
+    "vti_lbfgs_step": (c_int, [P, P, ctypes.c_double, P, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
     "vti2d_objfunc_gradient_vepsdel": (
         c_int,
         [P] * 9 + [c_float] * 3 + [P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 6 + [P],
@@ -447,3 +450,44 @@ def gradcij_to_gradvepsdel(gc11, gc13, gc33, v, eps, dl, v0=1.0, eps0=1.0, del0=
     check(_lib.vti_gradcij_to_gradvepsdel(*[ptr(o) for o in out], *[ptr(x) for x in a], v0, eps0, del0, a[0].size),
           "vti_gradcij_to_gradvepsdel")
     return out
+
+
+class LBFGS:
+    """lbfgs (lib/lbfgs/lbfgs_c.cpp) by reverse communication, FP64 state.
+
+    >>> opt = LBFGS(n, m=5); iflag = 0
+    >>> while True:
+    ...     f, g = evaluate(x)            # x: float32 array, updated in place
+    ...     iflag = opt.step(x, f, g, iflag)
+    ...     if iflag != 1: break
+    """
+
+    def __init__(self, n: int, m: int = 5, unit_step: bool = False):
+        h = c_void_p()
+        check(_lib.vti_lbfgs_create(ctypes.byref(h), n, m, int(unit_step)), "vti_lbfgs_create")
+        self.h = h
+        self.iter = 0
+        self.nfun = 0
+
+    def step(self, x: np.ndarray, f: float, g: np.ndarray, iflag: int) -> int:
+        if x.dtype != np.float32 or not x.flags["C_CONTIGUOUS"]:
+            raise ValueError("x must be a contiguous float32 array (updated in place)")
+        g = f32(g).ravel()
+        fl, it, nf = c_int(iflag), c_int(0), c_int(0)
+        check(_lib.vti_lbfgs_step(self.h, ptr(x), float(f), ptr(g), ctypes.byref(fl), ctypes.byref(it),
+                                  ctypes.byref(nf)), "vti_lbfgs_step")
+        self.iter, self.nfun = it.value, nf.value
+        if fl.value < 0:
+            raise VtiError(last_error())
+        return fl.value
+
+    def close(self):
+        if self.h:
+            _lib.vti_lbfgs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
