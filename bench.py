@@ -52,7 +52,7 @@ def layered(nz, ny, nx, seed=2024):
     c33 = v * v
     c11 = c33 * (1 + 2 * e)
     c13 = c33 * np.sqrt(1 + 2 * d)
-    return c11.astype(np.float32), c13.astype(np.float32), c33.astype(np.float32)
+    return c11.astype(np.float32), c13.astype(np.float32), c33.astype(np.float32), (v, e, d)
 
 
 def load_traffic(kernel: str, workload: str):
@@ -65,23 +65,26 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
-def cpu_baseline(c11, c13, c33, npad, dx, dt, budget_s=15.0):
-    """Time the oracle's C/OpenMP step on the same grid for a bounded number of steps."""
+def cpu_baseline(ved, npad, dx, dt, budget_s=15.0):
+    """The reference's CPU path (laplacian3d.ispc forward, c_ij formed per point
+    from v/eps/del, in the loop of vtifd3d8o-modeling.cpp with its abc),
+    restated in C/OpenMP in the oracle, timed on the same grid for a bounded
+    number of steps on the host's cores."""
     import oracle
 
-    nz, ny, nx = c11.shape
+    v, e, d = ved
+    nz, ny, nx = v.shape
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     rng = np.random.default_rng(0)
     st = [np.zeros((nz, ny, nx), np.float32) for _ in range(4)]
-    st[0][:] = rng.standard_normal((nz, ny, nx), dtype=np.float32) * 1e-3
+    st[1][:] = rng.standard_normal((nz, ny, nx), dtype=np.float32) * 1e-3
     t0 = time.perf_counter()
-    oracle.steps3d(oracle.REF, st, c11, c13, c33, npad, dx, dx, dx, dt, 1, absorbing=True)
-    t1 = time.perf_counter()
-    one = t1 - t0
-    n = max(1, min(30, int(budget_s / max(one, 1e-3))))
+    oracle.cpu3d_steps(0, st, v, e, d, npad, dx, dx, dx, dt, 1)
+    one = time.perf_counter() - t0
+    n = max(2, min(100, int(budget_s / max(one, 1e-3))))
     t0 = time.perf_counter()
-    oracle.steps3d(oracle.REF, st, c11, c13, c33, npad, dx, dx, dx, dt, n, absorbing=True)
+    oracle.cpu3d_steps(0, st, v, e, d, npad, dx, dx, dx, dt, n)
     el = time.perf_counter() - t0
     gpts = nz * ny * nx * n / el / 1e9
     return {
@@ -89,9 +92,36 @@ def cpu_baseline(c11, c13, c33, npad, dx, dt, budget_s=15.0):
         "unit": "Gpts/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n} forward steps (absorbing) of the full {nz}x{ny}x{nx} grid, "
-        f"oracle C/OpenMP restatement in the reference arithmetic, {el:.1f} s",
+        "sample": f"{n} absorbing forward steps of the full {nz}x{ny}x{nx} grid on {threads} threads, "
+        f"{el:.1f} s: the reference CPU path (laplacian3d.ispc forward + vtifd3d8o-modeling.cpp loop and abc) "
+        f"restated in C/OpenMP, FP32",
     }
+
+
+def shot_gradient_2d(vtiamd, ctx, nshots=2):
+    """Plain one-shot-per-call 2D gradients on the 512x1024 (+40 boundary)
+    grid, 3000 steps, 1024 receivers, rate 4 ms: shot-gradients/s."""
+    nx, nz, npad, nt, dx, dt, rate = 1024, 512, 40, 3001, 10.0, 1e-3, 4e-3
+    nnx, nnz = nx + 2 * npad, nz + 2 * npad
+    z = np.arange(nnz, dtype=np.float32)[:, None] * np.ones((1, nnx), np.float32)
+    v = 1500.0 + 3000.0 * z / nnz
+    c33 = (v * v).astype(np.float32)
+    c11 = (c33 * 1.2).astype(np.float32)
+    c13 = (c33 * 1.05).astype(np.float32)
+    taper = vtiamd.init_taper2d(nx, nz, npad)
+    t = np.arange(nt) * dt - 0.12
+    a = (np.pi * 10.0 * t) ** 2
+    w = ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
+    sloc = np.array([[npad + (k + 1) * nx // (nshots + 1), npad + 2, nx, 0] for k in range(nshots)], np.int32)
+    rloc = np.array([[npad + i, npad + 1] for i in range(nx)], np.int32)
+    nnt = (nt - 1) // int(rate / dt + 0.5) + 1
+    data = np.zeros((nnt, nx), np.float32)
+    vtiamd.objfunc_gradient2d(ctx, data, c11, c13, c33, w, sloc[:1], rloc, taper, nx, nz, nt, npad, dx, dx, dt, rate)
+    t0 = time.perf_counter()
+    vtiamd.objfunc_gradient2d(ctx, data, c11, c13, c33, w, sloc, rloc, taper, nx, nz, nt, npad, dx, dx, dt, rate)
+    el = time.perf_counter() - t0
+    return {"shots": nshots, "grid_padded": [nnz, nnx], "steps": nt - 1, "receivers": nx,
+            "ms_per_shot": round(el * 1e3 / nshots, 2), "shot_gradients_per_s": round(nshots / el, 3)}
 
 
 def main():
@@ -115,7 +145,7 @@ def main():
 
     nz, ny, nx, npad = (460, 572, 572, 30) if not args.small else (132, 140, 140, 10)
     dx, dt = 12.5, 1.0e-3
-    c11, c13, c33 = layered(nz, ny, nx)
+    c11, c13, c33, ved = layered(nz, ny, nx)
     npts = nz * ny * nx
     workload = f"3d_vti_forward_step_{nz}x{ny}x{nx}"
 
@@ -184,10 +214,12 @@ def main():
     prop.set_arith(0)
     f32ms = prop.forward_steps(max(1, args.steps // 2), absorbing=True, timed=True)
     extra["fwd_step_fp32_arith"] = {"ms": round(f32ms, 4), "gpts_s": round(npts / (f32ms * 1e-3) / 1e9, 3)}
-    out["extra"] = extra
     prop.close()
+    if not args.small:
+        extra["2d_shot_gradient_512x1024_3000steps"] = shot_gradient_2d(vtiamd, ctx)
+    out["extra"] = extra
     if not args.no_cpu and not args.small:
-        out["cpu_baseline"] = cpu_baseline(c11, c13, c33, npad, dx, dt)
+        out["cpu_baseline"] = cpu_baseline(ved, npad, dx, dt)
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -209,3 +209,33 @@ def fg2d(mode, born, data, c11, c13, c33, dc, wavelet, sloc, rloc, taper, nx, nz
                          _p(a[3]), _p(sloc), sloc.size // 4, _p(rloc), rloc.size // 2, _p(a[4]), nx, nz, nt, npad, dx,
                          dz, dt, rate, ot)
     return f, gc
+
+
+def cpu3d_steps(mode, state, a, b, c, npad, dx, dy, dz, dt, nsteps, src_index=-1, wavelet=None):
+    """The reference CPU path (laplacian3d.ispc forward/forwardCij + the
+    modeling loop of vtifd3d8o-modeling.cpp): advance state = [prev_x, cur_x,
+    prev_z, cur_z] (float32, (nz, ny, nx)) in place by nsteps levels.
+    mode 0: a, b, c = v, eps, del; mode 1: c11, c13, c33."""
+    L = lib()
+    if not getattr(L, "_has_cpu", False):
+        P = c_void_p
+        L.cpu3d_steps.argtypes = [c_int] + [P] * 7 + [c_int] * 4 + [c_float] * 4 + [c_int, c_long, P]
+        L.cpu3d_damping.argtypes = [P, c_int]
+        L._has_cpu = True
+    nz, ny, nx = a.shape
+    a, b, c = (np.ascontiguousarray(x, np.float32) for x in (a, b, c))
+    w = None if wavelet is None else np.ascontiguousarray(wavelet, np.float32)
+    rc = L.cpu3d_steps(mode, *[_p(s) for s in state], _p(a), _p(b), _p(c), nx, ny, nz, npad, dx, dy, dz, dt, nsteps,
+                       int(src_index), _p(w))
+    if rc != 0:
+        raise RuntimeError("cpu3d_steps failed")
+
+
+def cpu3d_damping(npad):
+    lib()
+    cpu3d_steps  # binds argtypes lazily
+    L = lib()
+    L.cpu3d_damping.argtypes = [c_void_p, c_int]
+    d = np.zeros(npad, np.float32)
+    L.cpu3d_damping(_p(d), npad)
+    return d
