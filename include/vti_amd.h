@@ -196,6 +196,27 @@ int vti2d_gn_hessian_vepsdel(vti_ctx *ctx, float *gv, float *geps, float *gdel, 
                              int nr, const float *taper, int nx, int nz, int nt, int npad, float dx, float dz, float dt,
                              float rate, float ot, float wbottom, float *m);
 
+/* ---- batched 2D gradient --------------------------------------------------------
+ * Many shots, each on its own aperture sub-grid of one global model
+ * (NNX = NX + 2*npad columns, nnz = nz + 2*npad rows, the reference layout),
+ * advanced in lockstep: one launch per time level for all shots of a wave
+ * (waves are sized to the free device memory, or at most max_wave shots when
+ * max_wave > 0).  Per shot the result is objFuncGradient_f
+ * (lib/vti-fwi-2d/objFuncGradient_f.cu:15) on the model cropped to the shot's
+ * columns [c0, c0 + nx + 2*npad), with the standard init_abc taper of that
+ * sub-grid; the shots' gradients are summed in FP64 into gc11/gc13/gc33
+ * (NNX*nnz doubles each), *objfunc = 0.5*sum r^2 over all shots.
+ * shots: 6 ints per shot {c0, nx, ix_src, iz_src, nr_shot, first receiver};
+ * rloc: 2 ints per receiver {ix, iz} in the shot's sub-grid coordinates;
+ * data: the shots' observed traces one after the other, each time-major
+ * ((nt-1)/(rate/dt)+1 rows of nr_shot).  Same schedule conditions as
+ * vti2d_objfunc_gradient (rate/dt >= 2, (nt-1) % (rate/dt) == 0, ot = 0).
+ * No counterpart in the reference, which runs one shot per call. */
+int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, double *gc13, double *gc33,
+                                 const float *data, const float *c11, const float *c13, const float *c33,
+                                 const float *wavelet, const int *shots, int ns, const int *rloc, int nr, int NX, int nz,
+                                 int nt, int npad, float dx, float dz, float dt, float rate, int max_wave);
+
 /* ---- L-BFGS (FP64) -----------------------------------------------------------
  * lbfgs, lib/lbfgs/lbfgs_c.cpp:11 (lbfgs.h:32) with mcsrch/mcstep,
  * lib/lbfgs/lsearch.cpp:9-606: reverse communication.  Start with *iflag = 0

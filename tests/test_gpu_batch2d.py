@@ -1,0 +1,94 @@
+"""Batched 2D gradient (many shots on aperture sub-grids of one model, one
+launch per time level) against the single-shot driver on the cropped models:
+per shot the same FP32 gradient, summed in FP64 in shot order -- so the
+global gradients are equal bit for bit."""
+import numpy as np
+import pytest
+
+from common2d import layered2d
+from common3d import cij_from, ricker
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def batch_problem(vti, NX=160, nz=40, npad=10, nt=161, ns=5, seed=8):
+    rng = np.random.default_rng(seed)
+    NNX, nnz = NX + 2 * npad, nz + 2 * npad
+    v, e, d = layered2d(nnz, NNX, seed=seed)
+    c = cij_from(v, e, d)
+    vt = v.copy()
+    vt[nnz // 2 - 4: nnz // 2 + 4, :] *= 1.04
+    ct = cij_from(vt, e, d)
+    shots, rloc = [], []
+    for s in range(ns):
+        nx = int(rng.integers(40, 90))
+        c0 = int(rng.integers(0, NX - nx + 1))
+        w = nx + 2 * npad
+        step = int(rng.integers(1, 4))
+        rx = list(range(npad, npad + nx, step))
+        shots.append((c0, nx, npad + int(rng.integers(4, nx - 4)), npad + 2, len(rx), len(rloc)))
+        rloc += [(x, npad + 1) for x in rx]
+    return dict(NX=NX, nz=nz, npad=npad, NNX=NNX, nnz=nnz, nt=nt, dx=10.0, dz=10.0, dt=1e-3, rate=4e-3, c=c, ct=ct,
+                w=ricker(nt, 1e-3, 20.0, 0.05), shots=np.array(shots, np.int32), rloc=np.array(rloc, np.int32))
+
+
+def crop(a, c0, w):
+    return np.ascontiguousarray(a[:, c0:c0 + w])
+
+
+def plain(vti, ctx, B, data):
+    """The single-shot driver on each shot's cropped model, summed in FP64 in shot order."""
+    g = np.zeros((3, B["nnz"], B["NNX"]), np.float64)
+    f = 0.0
+    off = 0
+    nnt = (B["nt"] - 1) // 4 + 1
+    for c0, nx, ix, iz, nr, r0 in B["shots"]:
+        w = nx + 2 * B["npad"]
+        taper = vti.init_taper2d(nx, B["nz"], B["npad"])
+        sl = np.array([[ix, iz, nr, 0]], np.int32)
+        rl = B["rloc"][r0:r0 + nr]
+        d = data[off:off + nnt * nr].reshape(nnt, nr)
+        off += nnt * nr
+        fs, gs = vti.objfunc_gradient2d(ctx, d, *[crop(c, c0, w) for c in B["c"]], B["w"], sl, rl, taper, nx, B["nz"],
+                                        B["nt"], B["npad"], B["dx"], B["dz"], B["dt"], B["rate"])
+        f += fs
+        g[:, :, c0:c0 + w] += gs.astype(np.float64)
+    return f, g
+
+
+def observed(vti, ctx, B):
+    out = []
+    for c0, nx, ix, iz, nr, r0 in B["shots"]:
+        w = nx + 2 * B["npad"]
+        taper = vti.init_taper2d(nx, B["nz"], B["npad"])
+        d = vti.synthetic2d(ctx, *[crop(c, c0, w) for c in B["ct"]], B["w"], np.array([[ix, iz, nr, 0]], np.int32),
+                            B["rloc"][r0:r0 + nr], taper, nx, B["nz"], B["nt"], B["npad"], B["dx"], B["dz"], B["dt"],
+                            B["rate"])
+        out.append(d.ravel())
+    return np.concatenate(out)
+
+
+@pytest.mark.parametrize("max_wave", [0, 2])
+def test_batch_equals_single_shot_driver(vti, ctx, max_wave):
+    B = batch_problem(vti)
+    _say("batch2d", len(B["shots"]), "shots, max_wave", max_wave)
+    data = observed(vti, ctx, B)
+    fb, gb = vti.objfunc_gradient2d_batch(ctx, data, *B["c"], B["w"], B["shots"], B["rloc"], B["NX"], B["nz"],
+                                          B["nt"], B["npad"], B["dx"], B["dz"], B["dt"], B["rate"], max_wave=max_wave)
+    fp, gp = plain(vti, ctx, B, data)
+    assert np.abs(gp).max() > 0
+    assert np.array_equal(gb, gp)
+    assert abs(fb - fp) <= 1e-6 * fp
+
+
+def test_batch_rejects_subgrid_outside_model(vti, ctx):
+    B = batch_problem(vti, ns=1)
+    sh = B["shots"].copy()
+    sh[0, 0] = B["NX"]  # starts past the last column that leaves room for it
+    with pytest.raises(vti.VtiError, match="outside the model"):
+        vti.objfunc_gradient2d_batch(ctx, np.zeros(10**5, np.float32), *B["c"], B["w"], sh, B["rloc"], B["NX"],
+                                     B["nz"], B["nt"], B["npad"], B["dx"], B["dz"], B["dt"], B["rate"])

@@ -14,9 +14,11 @@
 #include <vector>
 
 #include "vti2d_kernels.h"
+#include "vti_dbuf.h"
 #include "vti_ctx.h"
 #include "vti_internal.h"
 
+using vti::DBuf;
 using vti::Geo2;
 using vti::kBY2;
 
@@ -81,36 +83,6 @@ __global__ void narrow2d_kernel(float *__restrict__ out, const double *__restric
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         out[i] = (float)acc[i];
 }
-
-// RAII device buffer
-template <class T>
-struct DBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf &) = delete;
-    DBuf &operator=(const DBuf &) = delete;
-    ~DBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-    int alloc(size_t count, hipStream_t s)
-    {
-        n = std::max<size_t>(count, 1);
-        hipError_t e = hipMalloc((void **)&p, n * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            vti::set_error("hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
-            return vti::kErrNoMem;
-        }
-        e = hipMemsetAsync(p, 0, n * sizeof(T), s);
-        if (e != hipSuccess) {
-            vti::set_error("hipMemsetAsync: %s", hipGetErrorString(e));
-            return vti::kErrHip;
-        }
-        return vti::kOk;
-    }
-};
 
 struct Call2 {
     vti_ctx *ctx;

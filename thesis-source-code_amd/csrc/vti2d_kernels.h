@@ -140,7 +140,7 @@ __global__ __launch_bounds__(64 * kBY2) void fwd2d_kernel(
 }
 
 // D of a level into a snapshot slot (the last level of the forward pass).
-__global__ __launch_bounds__(64 * kBY2) void dsnap2d_kernel(const float *__restrict__ p, const float *__restrict__ q,
+static __global__ __attribute__((unused)) __launch_bounds__(64 * kBY2) void dsnap2d_kernel(const float *__restrict__ p, const float *__restrict__ q,
                                                            Geo2 g, float dx2, float dz2, float *__restrict__ sp,
                                                            float *__restrict__ sq)
 {
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(64 * kBY2) void bwd2d_kernel(
 
 // gradientCrossCor on a given level (the first adjoint level, which the
 // reference images before its backward loop, objFuncGradient_f.cu:247).
-__global__ void xcor2d_kernel(const float *__restrict__ ap, const float *__restrict__ aq,
+static __global__ __attribute__((unused)) void xcor2d_kernel(const float *__restrict__ ap, const float *__restrict__ aq,
                               const float *__restrict__ D0p, const float *__restrict__ D0q,
                               const float *__restrict__ D1p, const float *__restrict__ D1q, float f,
                               float *__restrict__ g11, float *__restrict__ g13, float *__restrict__ g33, Geo2 g)

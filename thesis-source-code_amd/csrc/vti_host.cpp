@@ -34,13 +34,18 @@ void init_damping3d(float *damping, int nx, int ny, int nz, int npad)
     for (int iz = 0; iz < nz; iz++) damping[nxy + iz] = profile(iz, nz);
 }
 
-void init_taper2d(float *taper, int nx, int nz, int npad)
+void taper_profile2d(float *t0, int npad)
 {
     // init.h:9 defines pi as 4.f*atan(1.f): the float value of pi, widened.
     const double pi = (double)(4.f * std::atan(1.f));
-    std::vector<float> t0(npad > 0 ? npad : 1);
     for (int i = 0; i < npad; ++i)
         t0[i] = (float)((double)kDamper2D + (1. - (double)kDamper2D) * std::cos(pi * (double)(float)(npad - 1 - i) / npad));
+}
+
+void init_taper2d(float *taper, int nx, int nz, int npad)
+{
+    std::vector<float> t0(npad > 0 ? npad : 1);
+    taper_profile2d(t0.data(), npad);
     const int nnx = nx + 2 * npad, nnz = nz + 2 * npad;
     for (size_t i = 0; i < (size_t)nnx * nnz; ++i) taper[i] = 1.f;
     for (int i = 0; i < nnz; ++i) {

@@ -15,6 +15,8 @@ void init_damping3d(float *damping, int nx, int ny, int nz, int npad);
 
 // init_abc(taper,nx,nz,npad), lib/vti-fwi-2d/init.cpp:70-89 (nx, nz unpadded)
 void init_taper2d(float *taper, int nx, int nz, int npad);
+// the 1D profile t0[npad] of init_abc: taper = t0(x) * t0(z) in the padding
+void taper_profile2d(float *t0, int npad);
 
 // VEpsDel2Cij / dVEpsDel2dCij / GradCij2GradVEpsDel, lib/vti-fwi-2d/conversions.cpp:42-81
 void vepsdel_to_cij(float *c11, float *c13, float *c33, const float *v, const float *eps, const float *del, float v0,
