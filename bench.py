@@ -98,6 +98,57 @@ def cpu_baseline(ved, npad, dx, dt, budget_s=15.0):
     }
 
 
+def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7):
+    """Shot-gradients/s of the batched 2D gradient against the plain
+    one-shot-per-call driver on the same shots: a 4096-wide seeded model
+    (nz 512, 40-point boundary), each shot on its own aperture of 300-900
+    columns, 2500 steps (rate 4 ms), a receiver on every column."""
+    rng = np.random.default_rng(seed)
+    NX, nz, npad, nt, dx, dt, rate = 4096, 512, 40, 2501, 10.0, 1e-3, 4e-3
+    NNX, nnz = NX + 2 * npad, nz + 2 * npad
+    z = np.arange(nnz, dtype=np.float32)[:, None] * np.ones((1, NNX), np.float32)
+    x = np.ones((nnz, 1), np.float32) * np.arange(NNX, dtype=np.float32)[None, :]
+    v = 1500.0 + 3000.0 * z / nnz + 100.0 * np.sin(x / 97.0)
+    c33 = (v * v).astype(np.float32)
+    c11 = (c33 * 1.2).astype(np.float32)
+    c13 = (c33 * 1.05).astype(np.float32)
+    t = np.arange(nt) * dt - 0.12
+    a = (np.pi * 10.0 * t) ** 2
+    w = ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
+    shots, rloc = [], []
+    for _ in range(ns):
+        nx = int(rng.integers(300, 901))
+        c0 = int(rng.integers(0, NX - nx + 1))
+        shots.append((c0, nx, npad + nx // 2, npad + 2, nx, len(rloc)))
+        rloc += [(npad + i, npad + 1) for i in range(nx)]
+    shots = np.array(shots, np.int32)
+    rloc = np.array(rloc, np.int32)
+    nnt = (nt - 1) // int(rate / dt + 0.5) + 1
+    data = np.zeros(int(sum(nnt * s[4] for s in shots)), np.float32)
+    t0 = time.perf_counter()
+    vtiamd.objfunc_gradient2d_batch(ctx, data, c11, c13, c33, w, shots, rloc, NX, nz, nt, npad, dx, dx, dt, rate)
+    tb = time.perf_counter() - t0
+    # the plain driver, one call per shot on the cropped model
+    tp = 0.0
+    for k in range(nplain):
+        c0, nx, ix, iz, nr, r0 = (int(q) for q in shots[k])
+        wd = nx + 2 * npad
+        taper = vtiamd.init_taper2d(nx, nz, npad)
+        crop = [np.ascontiguousarray(c[:, c0:c0 + wd]) for c in (c11, c13, c33)]
+        d = np.zeros((nnt, nr), np.float32)
+        sl = np.array([[ix, iz, nr, 0]], np.int32)
+        if k == 0:  # first call warms the driver up
+            vtiamd.objfunc_gradient2d(ctx, d, *crop, w, sl, rloc[r0:r0 + nr], taper, nx, nz, nt, npad, dx, dx, dt,
+                                      rate)
+        t0 = time.perf_counter()
+        vtiamd.objfunc_gradient2d(ctx, d, *crop, w, sl, rloc[r0:r0 + nr], taper, nx, nz, nt, npad, dx, dx, dt, rate)
+        tp += time.perf_counter() - t0
+    return {"shots": ns, "model_padded": [nnz, NNX], "aperture_cols": [300, 900], "steps": nt - 1,
+            "batched_shot_gradients_per_s": round(ns / tb, 3), "batched_s": round(tb, 3),
+            "plain_shot_gradients_per_s": round(nplain / tp, 3), "plain_shots_timed": nplain,
+            "speedup": round((ns / tb) / (nplain / tp), 2)}
+
+
 def shot_gradient_2d(vtiamd, ctx, nshots=2):
     """Plain one-shot-per-call 2D gradients on the 512x1024 (+40 boundary)
     grid, 3000 steps, 1024 receivers, rate 4 ms: shot-gradients/s."""
@@ -132,6 +183,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--small", action="store_true", help="small grid (profiling runs)")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline kernel (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -201,6 +253,11 @@ def main():
     }
     # secondary kernels of the same path on the same grid (reported, not the headline)
     extra = {}
+    if args.no_extra:
+        prop.close()
+        ctx.close()
+        print(json.dumps(out), flush=True)
+        return
     bw = prop.backward_steps(max(1, args.steps // 2), mode=0, timed=True)
     extra["bwd_gradient_step"] = {
         "ms": round(bw, 4), "gpts_s": round(npts / (bw * 1e-3) / 1e9, 3), "bytes_per_point": BYTES_BWD,
@@ -217,6 +274,7 @@ def main():
     prop.close()
     if not args.small:
         extra["2d_shot_gradient_512x1024_3000steps"] = shot_gradient_2d(vtiamd, ctx)
+        extra["2d_batched_vs_plain_gradient"] = batch_gradient_2d(vtiamd, ctx)
     out["extra"] = extra
     if not args.no_cpu and not args.small:
         out["cpu_baseline"] = cpu_baseline(ved, npad, dx, dt)
