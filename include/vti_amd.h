@@ -122,11 +122,14 @@ int vti3d_model_data(vti_ctx *ctx, float *ddata, const float *souloc, int ns, co
                      float dy, float dz, float dt, float samplingRate, int ngpu);
 
 /* objFuncCij3d_f, lib/vti-fwi-3d/gpu/objFuncGradient3d_f.cu:1272 (wave3d.h:100).
- * f[0] = 0.5*sum r^2.  tpow: the tpow= gain (0 = none). */
+ * f[0] = half the objective.  tpow: the tpow= gain (0 = none,
+ * agc.cpp:103-115); objtype/halfwidth: the objtype= and halfwidth= parameters
+ * (0 = least squares, 1 = AGC residual of agc.cpp:70-101 over a circular
+ * window of 2*halfwidth+1 samples; the reference's default half width is 20). */
 int vti3d_objfunc(vti_ctx *ctx, float *f, const float *souloc, int ns, const int *shotid, int nshot,
                   const float *recloc, const float *wavelet, float *cij, const float *padboundaryCij, int nx, int ny,
                   int nz, int nt, int npad, float ox, float oy, float oz, float ot, float dx, float dy, float dz,
-                  float dt, float samplingRate, int ngpu, const float *obs, float tpow);
+                  float dt, float samplingRate, int ngpu, const float *obs, float tpow, int objtype, int halfwidth);
 
 /* objFuncGradientCij3d_f, lib/vti-fwi-3d/gpu/objFuncGradient3d_f.cu:361 (wave3d.h:96).
  * fgcij = {f, gc11[nxyz], gc13[nxyz], gc33[nxyz]}; cij = {c11, c13, c33}
@@ -135,7 +138,7 @@ int vti3d_objfunc_gradient(vti_ctx *ctx, float *fgcij, const float *souloc, int 
                            const float *recloc, const float *wavelet, float *cij, const float *padboundaryCij,
                            const float *randboundaryCij, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
                            float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate, int ngpu,
-                           const float *obs, float tpow);
+                           const float *obs, float tpow, int objtype, int halfwidth);
 
 /* rtmCij3d_f, lib/vti-fwi-3d/gpu/rtm3d_f.cu:161 (wave3d.h:88).  recloc stride 3. */
 int vti3d_rtm(vti_ctx *ctx, float *image, const float *souloc, int ns, const int *shotid, int nshot,

@@ -18,13 +18,14 @@
 //    shim (the message comes from vti_last_error()).
 //
 // What the reference reads from its global argv inside the drivers -- the
-// GPU list (gpu=), the observed data (data=) and the tpow= gain of the 3D
-// objective -- is supplied here in one of two ways:
+// GPU list (gpu=), the observed data (data=), the tpow= gain and the
+// objtype=/halfwidth= choice of the 3D objective -- is supplied here in one
+// of two ways:
 //  - define VTI_COMPAT_USE_MYIO and include the reference's lib/io/myio.h
 //    first: the shim then calls get_array("gpu"), read("data") and
 //    get_param("tpow") exactly where the reference does;
 //  - otherwise call vti_compat::set_observed_data() / set_tpow() /
-//    set_ngpu() before the 3D drivers.
+//    set_objective() / set_ngpu() before the 3D drivers.
 // All work runs on one device (vti_compat::set_device, default 0); the GPU
 // list only fixes the reference's pipeline length nround*NGPU*64
 // (objFuncGradient3d_f.cu:539).
@@ -45,6 +46,7 @@ struct State {
     int device = 0;
     int ngpu = 1;
     float tpow = 0.f;
+    int objtype = 0, halfwidth = 20;
     const float *observed = nullptr;
 };
 
@@ -57,6 +59,12 @@ inline State &state()
 inline void set_device(int device) { state().device = device; }
 inline void set_ngpu(int ngpu) { state().ngpu = ngpu > 0 ? ngpu : 1; }
 inline void set_tpow(float tpow) { state().tpow = tpow; }
+// objtype= / halfwidth= of the 3D objective (0: least squares, 1: AGC)
+inline void set_objective(int objtype, int halfwidth)
+{
+    state().objtype = objtype;
+    state().halfwidth = halfwidth;
+}
 // trace-major observed data of every receiver (the reference's "data" file)
 inline void set_observed_data(const float *data) { state().observed = data; }
 
@@ -92,6 +100,28 @@ inline float tpow()
     return t;
 #else
     return state().tpow;
+#endif
+}
+
+inline int objtype()
+{
+#ifdef VTI_COMPAT_USE_MYIO
+    int t = 0;
+    get_param("objtype", t);
+    return t;
+#else
+    return state().objtype;
+#endif
+}
+
+inline int halfwidth()
+{
+#ifdef VTI_COMPAT_USE_MYIO
+    int h = 20;
+    get_param("halfwidth", h);
+    return h;
+#else
+    return state().halfwidth;
 #endif
 }
 
@@ -314,7 +344,8 @@ void objFuncCij3d_f(float *f, float *souloc, int ns, std::vector<int> &shotid, f
     const float *obs = vti_compat::observed(buf, souloc, shotid, nt, dt, samplingRate);
     vti_compat::report(vti3d_objfunc(vti_compat::context(), f, souloc, ns, shotid.data(), (int)shotid.size(), recloc,
                                      wavelet, cij, padboundaryCij, nx, ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz,
-                                     dt, samplingRate, GPUs.empty() ? 1 : (int)GPUs.size(), obs, vti_compat::tpow()),
+                                     dt, samplingRate, GPUs.empty() ? 1 : (int)GPUs.size(), obs, vti_compat::tpow(),
+                                     vti_compat::objtype(), vti_compat::halfwidth()),
                        "objFuncCij3d_f");
 }
 
@@ -330,7 +361,7 @@ void objFuncGradientCij3d_f(float *fgcij, float *souloc, int ns, std::vector<int
                                               (int)shotid.size(), recloc, wavelet, cij, padboundaryCij,
                                               randboundaryCij, nx, ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz, dt,
                                               samplingRate, GPUs.empty() ? 1 : (int)GPUs.size(), obs,
-                                              vti_compat::tpow()),
+                                              vti_compat::tpow(), vti_compat::objtype(), vti_compat::halfwidth()),
                        "objFuncGradientCij3d_f");
 }
 

@@ -56,8 +56,10 @@ def lib():
             [c_int, P, P, P, P, c_int, P, P, P, P, P]
             + [c_int] * 5
             + [c_float] * 9
-            + [c_int, P]
+            + [c_int, P, c_float, c_int, c_int]
         )
+        L.orc3d_agc_residual.restype = c_double
+        L.orc3d_agc_residual.argtypes = [c_int, P, P, c_int, c_int, c_int]
         L.orc3d_rtm.argtypes = [c_int, P, P, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 8 + [c_int, P]
         _lib = L
     return _lib
@@ -125,7 +127,8 @@ def steps3d_f64(state, c11, c13, c33, npad, dx, dy, dz, dt, nsteps, absorbing=Tr
         raise RuntimeError("orc3d_steps_f64 failed")
 
 
-def gradient3d(mode, souloc, shotid, recloc, wavelet, cij, padb, randb, nt, npad, o, d, dt, rate, obs, ngpu=1):
+def gradient3d(mode, souloc, shotid, recloc, wavelet, cij, padb, randb, nt, npad, o, d, dt, rate, obs, ngpu=1,
+               tpow=0.0, objtype=0, halfwidth=20, ot=0.0):
     """objFuncGradientCij3d_f restated.  cij (3, nz, ny, nx) float32 modified in place.
     Returns (objective, gradient (3, nz, ny, nx)) -- float32 for F32/REF, float64 for F64."""
     _, nz, ny, nx = cij.shape
@@ -139,12 +142,24 @@ def gradient3d(mode, souloc, shotid, recloc, wavelet, cij, padb, randb, nt, npad
     if mode == F64:
         out = np.zeros(1 + cij.size, np.float64)
         f = lib().orc3d_gradient(mode, None, _p(out), _p(souloc), _p(shotid), shotid.size, _p(recloc), _p(w), _p(cij),
-                                 _p(padb), _p(randb), nx, ny, nz, nt, npad, *o, 0.0, *d, dt, rate, ngpu, _p(obs))
+                                 _p(padb), _p(randb), nx, ny, nz, nt, npad, *o, ot, *d, dt, rate, ngpu, _p(obs),
+                                 tpow, objtype, halfwidth)
     else:
         out = np.zeros(1 + cij.size, np.float32)
         f = lib().orc3d_gradient(mode, _p(out), None, _p(souloc), _p(shotid), shotid.size, _p(recloc), _p(w), _p(cij),
-                                 _p(padb), _p(randb), nx, ny, nz, nt, npad, *o, 0.0, *d, dt, rate, ngpu, _p(obs))
+                                 _p(padb), _p(randb), nx, ny, nz, nt, npad, *o, ot, *d, dt, rate, ngpu, _p(obs),
+                                 tpow, objtype, halfwidth)
     return out[0], out[1:].reshape(3, nz, ny, nx)
+
+
+def agc_residual(mode, d, d0, halfwidth):
+    """residualAGC (agc.cpp:70-101) on trace-major (ntr, nt) traces: returns
+    (sum of squared AGC residuals, adjoint source)."""
+    ntr, nt = d.shape
+    a = np.ascontiguousarray(d, np.float64 if mode == F64 else np.float32).copy()
+    b = np.ascontiguousarray(d0, np.float32)
+    fi = lib().orc3d_agc_residual(mode, _p(a), _p(b), nt, ntr, halfwidth)
+    return fi, a
 
 
 def rtm3d(mode, souloc, shotid, recloc3, wavelet, c11, c13, c33, nt, npad, o, d, dt, rate, obs, ngpu=1):

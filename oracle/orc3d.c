@@ -349,6 +349,87 @@ int orc3d_steps_f64(double *cur_x, double *prev_x, double *cur_z, double *prev_z
     return 0;
 }
 
+/* ---- objective variants of the 3D drivers (lib/vti-fwi-3d/gpu/agc.cpp) ---- */
+#define AGC_EPS 1e-6f
+
+/* runsum, agc.cpp:14-26: circular running sum over [i-h, i+h], summed in
+ * ascending j in the element type */
+#define DEF_AGC(T, SFX, SQRT, MAX)                                                                       \
+    static void runsum_##SFX(int nt, int h, const T *w, T *s)                                           \
+    {                                                                                                   \
+        for (int i = 0; i < nt; i++) {                                                                  \
+            s[i] = 0;                                                                                   \
+            for (int j = i - h; j <= i + h; j++) s[i] += w[j < 0 ? j + nt : (j >= nt ? j - nt : j)];    \
+        }                                                                                               \
+    }                                                                                                   \
+    /* agc, agc.cpp:28-35 */                                                                            \
+    static void agc_##SFX(int nt, int h, const T *d, T *hd, T *sd, T *d2)                               \
+    {                                                                                                   \
+        for (int i = 0; i < nt; i++) d2[i] = d[i] * d[i];                                               \
+        runsum_##SFX(nt, h, d2, sd);                                                                    \
+        for (int i = 0; i < nt; i++) hd[i] = d[i] / SQRT(MAX((T)AGC_EPS, sd[i]));                      \
+    }                                                                                                   \
+    /* residualAGC (in place), agc.cpp:70-101: d becomes the adjoint source */                          \
+    static double residual_agc_##SFX(int nt, int ntr, int h, T *d, const float *d0f)                    \
+    {                                                                                                   \
+        T *hd = (T *)malloc(sizeof(T) * nt), *sd = (T *)malloc(sizeof(T) * nt);                         \
+        T *hd0 = (T *)malloc(sizeof(T) * nt), *sd0 = (T *)malloc(sizeof(T) * nt);                       \
+        T *d0 = (T *)malloc(sizeof(T) * nt), *d2 = (T *)malloc(sizeof(T) * nt);                         \
+        double fi = 0.;                                                                                 \
+        for (int j = 0; j < ntr; j++) {                                                                 \
+            T *dj = d + (long)j * nt;                                                                   \
+            for (int i = 0; i < nt; i++) d0[i] = (T)d0f[(long)j * nt + i];                             \
+            agc_##SFX(nt, h, dj, hd, sd, d2);                                                           \
+            agc_##SFX(nt, h, d0, hd0, sd0, d2);                                                         \
+            for (int i = 0; i < nt; i++) hd[i] = hd[i] - hd0[i];                                        \
+            for (int i = 0; i < nt; i++) fi += (double)(hd[i] * hd[i]);                                 \
+            T *rs = sd, *drss = sd0;                                                                    \
+            for (int i = 0; i < nt; i++) {                                                              \
+                const T t1 = MAX((T)AGC_EPS, sd[i]);                                                    \
+                const T t2 = hd[i] / SQRT(t1);                                                          \
+                drss[i] = dj[i] * t2 / t1;                                                              \
+                rs[i] = t2;                                                                             \
+            }                                                                                           \
+            runsum_##SFX(nt, h, drss, hd);                                                              \
+            for (int i = 0; i < nt; i++) dj[i] = rs[i] - dj[i] * hd[i];                                 \
+        }                                                                                               \
+        free(hd); free(sd); free(hd0); free(sd0); free(d0); free(d2);                                   \
+        return fi;                                                                                      \
+    }
+
+static inline float fmax_ref(float a, float b) { return a < b ? b : a; }
+static inline double dmax_ref(double a, double b) { return a < b ? b : a; }
+DEF_AGC(float, f32, sqrtf, fmax_ref)
+DEF_AGC(double, f64, sqrt, dmax_ref)
+
+/* tpower, agc.cpp:103-115: data *= (ot + it*dt)^p in float */
+static void tpower_f32(float *data, int nt, float ot, float dt, int nr, float p)
+{
+    for (int ir = 0; ir < nr; ir++)
+        for (int it = 0; it < nt; it++) {
+            float t = ot + it * dt;
+            t = powf(t, p);
+            data[it + (long)ir * nt] *= t;
+        }
+}
+
+static void tpower_f64(double *data, int nt, float ot, float dt, int nr, float p)
+{
+    for (int ir = 0; ir < nr; ir++)
+        for (int it = 0; it < nt; it++) {
+            const double t = pow((double)(ot + it * dt), (double)p);
+            data[it + (long)ir * nt] *= t;
+        }
+}
+
+/* AGC objective and adjoint source of traces (trace-major, F32 or F64):
+ * test hook for the objective variants. */
+double orc3d_agc_residual(int mode, void *d, const float *d0, int nt, int ntr, int halfwidth)
+{
+    if (mode == 2) return residual_agc_f64(nt, ntr, halfwidth, (double *)d, d0);
+    return residual_agc_f32(nt, ntr, halfwidth, (float *)d, d0);
+}
+
 /* objFuncGradientCij3d_f (objFuncGradient3d_f.cu:361-1231) for every shot in
  * shotid.  obs is the observed data file contents (trace-major, nnt samples
  * per receiver, receivers in recloc order), which the reference reads from
@@ -359,9 +440,8 @@ double orc3d_gradient(int mode, float *fgcij, double *fgcij64, const float *soul
                       const float *recloc, const float *wavelet, float *cij, const float *padboundaryCij,
                       const float *randboundaryCij, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
                       float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate, int ngpu,
-                      const float *obs)
+                      const float *obs, float tpow, int objtype, int halfwidth)
 {
-    (void)ot;
     orc3d_geom g;
     geom_setup(&g, nx, ny, nz, npad, dx, dy, dz, dt);
     if (g.nb < 2) return -1;
@@ -416,7 +496,7 @@ double orc3d_gradient(int mode, float *fgcij, double *fgcij64, const float *soul
                 forward_pass_ref(&g, &sh, c11, c13, c33, damping, wavelet, lmax, &cx, &px, &cz, &pz, data, NULL,
                                  sts, nnt);
         }
-        /* mute dead traces, residual, objective (:736-779, objtype 0) */
+        /* mute dead traces, t^p gain, residual, objective (:736-779) */
         for (int ir = 0; ir < sh.nr; ir++) {
             int ir1 = ir + irbegin;
             if (recloc[4 * ir1 + 3] == 0.f) {
@@ -424,14 +504,23 @@ double orc3d_gradient(int mode, float *fgcij, double *fgcij64, const float *soul
                 if (data64) memset(data64 + (long)ir * nnt, 0, sizeof(double) * nnt);
             }
         }
+        if (tpow != 0.f) {
+            if (mode == 2) tpower_f64(data64, nnt, ot, samplingRate, sh.nr, tpow);
+            else tpower_f32(data, nnt, ot, samplingRate, sh.nr, tpow);
+        }
         const float *ob = obs + (long)nnt * irbegin;
-        for (long i = 0; i < (long)nnt * sh.nr; i++) {
-            if (mode == 2) {
-                data64[i] = data64[i] - (double)ob[i];
-                objfunc += data64[i] * data64[i];
-            } else {
-                data[i] = data[i] - ob[i];
-                objfunc += (double)(data[i] * data[i]);
+        if (objtype != 0) {
+            objfunc += mode == 2 ? residual_agc_f64(nnt, sh.nr, halfwidth, data64, ob)
+                                 : residual_agc_f32(nnt, sh.nr, halfwidth, data, ob);
+        } else {
+            for (long i = 0; i < (long)nnt * sh.nr; i++) {
+                if (mode == 2) {
+                    data64[i] = data64[i] - (double)ob[i];
+                    objfunc += data64[i] * data64[i];
+                } else {
+                    data[i] = data[i] - ob[i];
+                    objfunc += (double)(data[i] * data[i]);
+                }
             }
         }
 

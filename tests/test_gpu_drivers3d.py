@@ -99,6 +99,35 @@ def test_objfunc_gradient_vs_oracle(vti, orc, ctx, nshot, obs_kind):
     assert f_only == f_gpu
 
 
+@pytest.mark.parametrize("variant", ["agc", "tpow"])
+def test_objective_variants_vs_oracle(vti, orc, ctx, variant):
+    """objtype=1 (AGC residual, agc.cpp:70-101, half width 6) and the tpow=
+    gain (agc.cpp:103-115): the GPU path against the oracle's restatement in
+    the reference arithmetic, one shot.  AGC sums, divisions and square roots
+    run in the reference's order and rounding, so the gradient is bit-exact;
+    the t^p gain calls the device powf, whose last bit may differ from the
+    host's, so that case is compared to 1e-6."""
+    P = problem()
+    obs = _obs(vti, ctx, P)
+    padb, randb = _boundaries(vti, P)
+    args = (P["souloc"], np.arange(1), P["recloc"], P["w"])
+    rest = (P["nt"], P["npad"], P["o"], P["d"], P["dt"], P["rate"], obs)
+    kw = dict(objtype=1, halfwidth=6) if variant == "agc" else dict(tpow=1.5, ot=0.001)
+    print("3d objective variant", variant, flush=True)
+    f_gpu, g_gpu = vti.objfunc_gradient3d(ctx, *args, P["cij"].copy(), padb, randb, *rest, **kw)
+    f_orc, g_orc = orc.gradient3d(orc.REF, *args, P["cij"].copy(), padb, randb, *rest, **kw)
+    f64, g64 = orc.gradient3d(orc.F64, *args, P["cij"].copy(), padb, randb, *rest, **kw)
+    print(f"  f_gpu={f_gpu:.9e} f_orc={f_orc:.9e} f64={f64:.9e} grad rel vs FP64 {rel(g_gpu, g64):.2e}",
+          flush=True)
+    assert np.abs(g_orc).max() > 0
+    assert abs(f_gpu - f_orc) <= 2e-6 * abs(f_orc)
+    if variant == "agc":
+        assert np.array_equal(g_gpu, g_orc)
+    else:
+        assert rel(g_gpu, g_orc) < 1e-6
+    assert rel(g_gpu, g64) < 3e-2
+
+
 def test_rtm_vs_oracle(vti, orc, ctx):
     P = problem()
     obs = _obs(vti, ctx, P)

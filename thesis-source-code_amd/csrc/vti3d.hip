@@ -63,7 +63,7 @@ __global__ void lambda_init3d_kernel(float *__restrict__ lx, float *__restrict__
 // time-major [it][ir]; obs is the reference's trace-major data file slice.
 __global__ void residual3d_kernel(float *__restrict__ tr, const float *__restrict__ obs,
                                   const unsigned char *__restrict__ live, int nr, int nnt, float ot, float rate,
-                                  float tpow, double *__restrict__ partial)
+                                  float tpow, int subtract, double *__restrict__ partial)
 {
     __shared__ double red[256];
     const long long n = (long long)nr * nnt;
@@ -75,7 +75,7 @@ __global__ void residual3d_kernel(float *__restrict__ tr, const float *__restric
             const float t = ot + it * rate;
             m *= powf(t, tpow);
         }
-        const float r = m - obs[(long long)ir * nnt + it];
+        const float r = subtract ? m - obs[(long long)ir * nnt + it] : m;
         tr[k] = r;
         acc += (double)(r * r);
     }
@@ -86,6 +86,48 @@ __global__ void residual3d_kernel(float *__restrict__ tr, const float *__restric
         __syncthreads();
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// residualAGC (lib/vti-fwi-3d/gpu/agc.cpp:70-101) on one receiver per
+// thread: the automatic-gain-controlled residual h(d) - h(d0) with
+// h(d)_i = d_i / sqrt(max(1e-6, sum of d_j^2 over the circular window
+// |i-j| <= hw)), its sum of squares (fi_out[ir]) and, in place of the traces,
+// the adjoint source.  Sums run in the reference's order; divisions and
+// square roots are correctly rounded, as on the host.
+__device__ __forceinline__ float agc_max(float a, float b) { return a < b ? b : a; }
+
+__global__ void agc3d_kernel(float *__restrict__ tr, const float *__restrict__ obs, int nr, int nnt, int hw,
+                             float *__restrict__ rsb, float *__restrict__ drb, double *__restrict__ fi_out)
+{
+    const int ir = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ir >= nr) return;
+    const float eps = 1e-6f;
+    auto wrap = [&](int j) { return j < 0 ? j + nnt : (j >= nnt ? j - nnt : j); };
+    auto D = [&](int i) { return tr[(long long)i * nr + ir]; };
+    auto O = [&](int i) { return obs[(long long)ir * nnt + i]; };
+    double fi = 0.0;
+    for (int i = 0; i < nnt; ++i) {
+        float s = 0.f, s0 = 0.f;
+        for (int j = i - hw; j <= i + hw; ++j) {
+            const float v = D(wrap(j)), v0 = O(wrap(j));
+            s += v * v;
+            s0 += v0 * v0;
+        }
+        const float hd = __fdiv_rn(D(i), __fsqrt_rn(agc_max(eps, s)));
+        const float hd0 = __fdiv_rn(O(i), __fsqrt_rn(agc_max(eps, s0)));
+        const float res = hd - hd0;
+        fi += (double)(res * res);
+        const float t1 = agc_max(eps, s);
+        const float t2 = __fdiv_rn(res, __fsqrt_rn(t1));
+        drb[(long long)i * nr + ir] = __fdiv_rn(D(i) * t2, t1);
+        rsb[(long long)i * nr + ir] = t2;
+    }
+    for (int i = 0; i < nnt; ++i) {
+        float s = 0.f;
+        for (int j = i - hw; j <= i + hw; ++j) s += drb[(long long)wrap(j) * nr + ir];
+        tr[(long long)i * nr + ir] = rsb[(long long)i * nr + ir] - D(i) * s;
+    }
+    fi_out[ir] = fi;
 }
 
 __global__ void accumulate64_kernel(double *__restrict__ acc, const float *__restrict__ v, long long n)
@@ -731,7 +773,8 @@ int check_common(const DriverArgs &a)
 // Per-shot forward with absorbing boundary and recording, then mute, gain and
 // (when obs != nullptr) the residual in sd.tr; returns sum(r*r) in *ss.
 int modeled_shot(vti3d_prop *p, const DriverArgs &a, const Shot3 &sh, ShotDev &sd, int lmax, int sts, int nnt,
-                 const float *obs, float tpow, double *ss, std::vector<float> *traces_out)
+                 const float *obs, float tpow, int objtype, int halfwidth, double *ss,
+                 std::vector<float> *traces_out)
 {
     int rc = zero_fields(p, true, false);
     if (rc) return rc;
@@ -762,10 +805,27 @@ int modeled_shot(vti3d_prop *p, const DriverArgs &a, const Shot3 &sh, ShotDev &s
         VTI_HIP(hipMemsetAsync(sd.obs, 0, sizeof(float) * nn, p->ctx->stream));
     }
     hipLaunchKernelGGL(residual3d_kernel, dim3(nblk), dim3(256), 0, p->ctx->stream, sd.tr, sd.obs, sd.live, sh.nr,
-                       nnt, a.ot, a.rate, tpow, sd.partial);
+                       nnt, a.ot, a.rate, tpow, objtype == 0 ? 1 : 0, sd.partial);
     VTI_HIP(hipGetLastError());
-    std::vector<double> part(nblk);
-    VTI_HIP(hipMemcpyAsync(part.data(), sd.partial, sizeof(double) * nblk, hipMemcpyDeviceToHost, p->ctx->stream));
+    std::vector<double> part(objtype == 0 ? nblk : std::max(1, sh.nr));
+    if (objtype == 0) {
+        VTI_HIP(hipMemcpyAsync(part.data(), sd.partial, sizeof(double) * nblk, hipMemcpyDeviceToHost, p->ctx->stream));
+    } else if (sh.nr > 0) {
+        // the AGC objective (objFuncGradient3d_f.cu:770-773): traces become the adjoint source
+        float *rsb = nullptr, *drb = nullptr;
+        double *fi = nullptr;
+        VTI_HIP(hipMalloc((void **)&rsb, sizeof(float) * nn));
+        VTI_HIP(hipMalloc((void **)&drb, sizeof(float) * nn));
+        VTI_HIP(hipMalloc((void **)&fi, sizeof(double) * sh.nr));
+        hipLaunchKernelGGL(agc3d_kernel, dim3((sh.nr + 63) / 64), dim3(64), 0, p->ctx->stream, sd.tr, sd.obs, sh.nr,
+                           nnt, halfwidth, rsb, drb, fi);
+        VTI_HIP(hipGetLastError());
+        VTI_HIP(hipMemcpyAsync(part.data(), fi, sizeof(double) * sh.nr, hipMemcpyDeviceToHost, p->ctx->stream));
+        VTI_HIP(hipStreamSynchronize(p->ctx->stream));
+        (void)hipFree(rsb);
+        (void)hipFree(drb);
+        (void)hipFree(fi);
+    }
     if (traces_out) {
         traces_out->resize((size_t)sh.nr * nnt);
         VTI_HIP(hipMemcpyAsync(traces_out->data(), sd.tr, sizeof(float) * (size_t)sh.nr * nnt, hipMemcpyDeviceToHost,
@@ -852,7 +912,7 @@ int vti3d_model_data(vti_ctx *ctx, float *ddata, const float *souloc, int ns, co
         ShotDev sd;
         std::vector<float> tr;
         double ss = 0;
-        rc = modeled_shot(G.p, a, sh, sd, lmax, sts, nnt, nullptr, 0.f, &ss, &tr);
+        rc = modeled_shot(G.p, a, sh, sd, lmax, sts, nnt, nullptr, 0.f, 0, 0, &ss, &tr);
         if (rc) return rc;
         float *dst = ddata + (size_t)nnt * sh.irbegin;
         for (int ir = 0; ir < sh.nr; ++ir)
@@ -873,7 +933,7 @@ static int objfunc_impl(vti_ctx *ctx, float *fgcij, bool grad, const float *soul
                         int nshot, const float *recloc, const float *wavelet, float *cij, const float *padboundaryCij,
                         const float *randboundaryCij, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
                         float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate, int ngpu,
-                        const float *obs, float tpow)
+                        const float *obs, float tpow, int objtype, int halfwidth)
 {
     DriverArgs a{souloc, shotid, nshot, recloc, wavelet, nx, ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz, dt,
                  samplingRate, ngpu};
@@ -881,6 +941,7 @@ static int objfunc_impl(vti_ctx *ctx, float *fgcij, bool grad, const float *soul
     if (rc) return rc;
     VTI_CHECK(ctx && fgcij && cij && padboundaryCij && obs && (!grad || randboundaryCij),
               "objective/gradient: null argument");
+    VTI_CHECK(objtype == 0 || objtype == 1, "objtype must be 0 (least squares) or 1 (AGC), got %d", objtype);
     for (int i = 0; i < nshot; ++i) VTI_CHECK(shotid[i] >= 0 && shotid[i] < ns, "shot id %d out of range", shotid[i]);
     PropGuard G;
     rc = vti3d_prop_create(ctx, &G.p, nx, ny, nz, npad, dx, dy, dz, dt);
@@ -895,6 +956,8 @@ static int objfunc_impl(vti_ctx *ctx, float *fgcij, bool grad, const float *soul
     rc = put_shell_and_upload(p, p->cm[0], cij, padboundaryCij, nx, ny, nz, npad);
     if (rc) return rc;
     const int sts = (int)std::round(samplingRate / dt), nnt = (nt - 1) / sts + 1, lmax = levels(nt, ngpu);
+    VTI_CHECK(objtype == 0 || (halfwidth >= 0 && 2 * halfwidth < nnt),
+              "AGC half width %d needs 0 <= 2*halfwidth < %d samples", halfwidth, nnt);
     double objfunc = 0.0;
     for (int k = 0; k < nshot; ++k) {
         Shot3 sh;
@@ -903,7 +966,7 @@ static int objfunc_impl(vti_ctx *ctx, float *fgcij, bool grad, const float *soul
         ShotDev sd;
         double ss = 0;
         p->active = 0;
-        rc = modeled_shot(p, a, sh, sd, lmax, sts, nnt, obs, tpow, &ss, nullptr);
+        rc = modeled_shot(p, a, sh, sd, lmax, sts, nnt, obs, tpow, objtype, halfwidth, &ss, nullptr);
         if (rc) return rc;
         objfunc += ss;
         if (!grad) continue;
@@ -949,20 +1012,21 @@ int vti3d_objfunc_gradient(vti_ctx *ctx, float *fgcij, const float *souloc, int 
                            const float *recloc, const float *wavelet, float *cij, const float *padboundaryCij,
                            const float *randboundaryCij, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
                            float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate, int ngpu,
-                           const float *obs, float tpow)
+                           const float *obs, float tpow, int objtype, int halfwidth)
 {
     return objfunc_impl(ctx, fgcij, true, souloc, ns, shotid, nshot, recloc, wavelet, cij, padboundaryCij,
                         randboundaryCij, nx, ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz, dt, samplingRate, ngpu, obs,
-                        tpow);
+                        tpow, objtype, halfwidth);
 }
 
 int vti3d_objfunc(vti_ctx *ctx, float *f, const float *souloc, int ns, const int *shotid, int nshot,
                   const float *recloc, const float *wavelet, float *cij, const float *padboundaryCij, int nx, int ny,
                   int nz, int nt, int npad, float ox, float oy, float oz, float ot, float dx, float dy, float dz,
-                  float dt, float samplingRate, int ngpu, const float *obs, float tpow)
+                  float dt, float samplingRate, int ngpu, const float *obs, float tpow, int objtype, int halfwidth)
 {
     return objfunc_impl(ctx, f, false, souloc, ns, shotid, nshot, recloc, wavelet, cij, padboundaryCij, nullptr, nx,
-                        ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz, dt, samplingRate, ngpu, obs, tpow);
+                        ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz, dt, samplingRate, ngpu, obs, tpow, objtype,
+                        halfwidth);
 }
 
 // rtmCij3d_f (lib/vti-fwi-3d/gpu/rtm3d_f.cu:161-830): per shot a forward pass

@@ -74,11 +74,11 @@ _SIGS = {
     ),
     "vti3d_objfunc": (
         c_int,
-        [P, P, P, c_int, P, c_int, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int, P, c_float],
+        [P, P, P, c_int, P, c_int, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int, P, c_float, c_int, c_int],
     ),
     "vti3d_objfunc_gradient": (
         c_int,
-        [P, P, P, c_int, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int, P, c_float],
+        [P, P, P, c_int, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int, P, c_float, c_int, c_int],
     ),
     "vti3d_rtm": (
         c_int,
@@ -315,7 +315,7 @@ def model_data3d(ctx, souloc, shotid, recloc, wavelet, c11, c13, c33, nt, npad, 
 
 
 def objfunc_gradient3d(ctx, souloc, shotid, recloc, wavelet, cij, padb, randb, nt, npad, o, d, dt, rate, obs,
-                       ngpu=1, tpow=0.0, gradient=True):
+                       ngpu=1, tpow=0.0, gradient=True, objtype=0, halfwidth=20, ot=0.0):
     """objFuncGradientCij3d_f / objFuncCij3d_f.  cij (3, nz, ny, nx) is modified in place
     like the reference (its boundary shell ends up holding randb / padb)."""
     souloc, shotid, recloc, wavelet = _geom3(souloc, shotid, recloc, wavelet)
@@ -327,13 +327,13 @@ def objfunc_gradient3d(ctx, souloc, shotid, recloc, wavelet, cij, padb, randb, n
         out = np.zeros(1 + cij.size, np.float32)
         check(_lib.vti3d_objfunc_gradient(ctx.h, ptr(out), ptr(souloc), souloc.size // 5, ptr(shotid), shotid.size,
                                           ptr(recloc), ptr(wavelet), ptr(cij), ptr(padb), ptr(randb), nx, ny, nz, nt,
-                                          npad, *o, 0.0, *d, dt, rate, ngpu, ptr(obs), tpow),
+                                          npad, *o, ot, *d, dt, rate, ngpu, ptr(obs), tpow, objtype, halfwidth),
               "vti3d_objfunc_gradient")
         return float(out[0]), out[1:].reshape(3, nz, ny, nx)
     out = np.zeros(1, np.float32)
     check(_lib.vti3d_objfunc(ctx.h, ptr(out), ptr(souloc), souloc.size // 5, ptr(shotid), shotid.size, ptr(recloc),
-                             ptr(wavelet), ptr(cij), ptr(padb), nx, ny, nz, nt, npad, *o, 0.0, *d, dt, rate, ngpu,
-                             ptr(obs), tpow), "vti3d_objfunc")
+                             ptr(wavelet), ptr(cij), ptr(padb), nx, ny, nz, nt, npad, *o, ot, *d, dt, rate, ngpu,
+                             ptr(obs), tpow, objtype, halfwidth), "vti3d_objfunc")
     return float(out[0])
 
 
