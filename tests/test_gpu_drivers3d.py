@@ -104,9 +104,8 @@ def test_objective_variants_vs_oracle(vti, orc, ctx, variant):
     """objtype=1 (AGC residual, agc.cpp:70-101, half width 6) and the tpow=
     gain (agc.cpp:103-115): the GPU path against the oracle's restatement in
     the reference arithmetic, one shot.  AGC sums, divisions and square roots
-    run in the reference's order and rounding, so the gradient is bit-exact;
-    the t^p gain calls the device powf, whose last bit may differ from the
-    host's, so that case is compared to 1e-6."""
+    run in the reference's order and rounding, and the gain is formed with the
+    host's powf as the reference forms it, so both gradients are bit-exact."""
     P = problem()
     obs = _obs(vti, ctx, P)
     padb, randb = _boundaries(vti, P)
@@ -121,10 +120,7 @@ def test_objective_variants_vs_oracle(vti, orc, ctx, variant):
           flush=True)
     assert np.abs(g_orc).max() > 0
     assert abs(f_gpu - f_orc) <= 2e-6 * abs(f_orc)
-    if variant == "agc":
-        assert np.array_equal(g_gpu, g_orc)
-    else:
-        assert rel(g_gpu, g_orc) < 1e-6
+    assert np.array_equal(g_gpu, g_orc)
     assert rel(g_gpu, g64) < 3e-2
 
 

@@ -58,12 +58,14 @@ __global__ void lambda_init3d_kernel(float *__restrict__ lx, float *__restrict__
     }
 }
 
-// Mute, optional t^p gain, residual against the observed traces and the
-// per-block partial sums of r*r (objFuncGradient3d_f.cu:736-779).  Traces are
+// Mute, optional t^p gain (gain[it] = (ot + it*rate)^tpow, formed on the host
+// with the host's powf as tpower does, agc.cpp:103-115), residual against the
+// observed traces and the per-block partial sums of r*r
+// (objFuncGradient3d_f.cu:736-779).  Traces are
 // time-major [it][ir]; obs is the reference's trace-major data file slice.
 __global__ void residual3d_kernel(float *__restrict__ tr, const float *__restrict__ obs,
-                                  const unsigned char *__restrict__ live, int nr, int nnt, float ot, float rate,
-                                  float tpow, int subtract, double *__restrict__ partial)
+                                  const unsigned char *__restrict__ live, int nr, int nnt,
+                                  const float *__restrict__ gain, int subtract, double *__restrict__ partial)
 {
     __shared__ double red[256];
     const long long n = (long long)nr * nnt;
@@ -71,10 +73,7 @@ __global__ void residual3d_kernel(float *__restrict__ tr, const float *__restric
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
         const int it = (int)(k / nr), ir = (int)(k % nr);
         float m = live[ir] ? tr[k] : 0.f;
-        if (tpow != 0.f) {
-            const float t = ot + it * rate;
-            m *= powf(t, tpow);
-        }
+        if (gain) m *= gain[it];
         const float r = subtract ? m - obs[(long long)ir * nnt + it] : m;
         tr[k] = r;
         acc += (double)(r * r);
@@ -113,13 +112,13 @@ __global__ void agc3d_kernel(float *__restrict__ tr, const float *__restrict__ o
             s += v * v;
             s0 += v0 * v0;
         }
-        const float hd = __fdiv_rn(D(i), __fsqrt_rn(agc_max(eps, s)));
-        const float hd0 = __fdiv_rn(O(i), __fsqrt_rn(agc_max(eps, s0)));
+        const float hd = D(i) / sqrtf(agc_max(eps, s));
+        const float hd0 = O(i) / sqrtf(agc_max(eps, s0));
         const float res = hd - hd0;
         fi += (double)(res * res);
         const float t1 = agc_max(eps, s);
-        const float t2 = __fdiv_rn(res, __fsqrt_rn(t1));
-        drb[(long long)i * nr + ir] = __fdiv_rn(D(i) * t2, t1);
+        const float t2 = res / sqrtf(t1);
+        drb[(long long)i * nr + ir] = D(i) * t2 / t1;
         rsb[(long long)i * nr + ir] = t2;
     }
     for (int i = 0; i < nnt; ++i) {
@@ -804,8 +803,18 @@ int modeled_shot(vti3d_prop *p, const DriverArgs &a, const Shot3 &sh, ShotDev &s
         VTI_HIP(hipMalloc((void **)&sd.obs, sizeof(float) * nn));
         VTI_HIP(hipMemsetAsync(sd.obs, 0, sizeof(float) * nn, p->ctx->stream));
     }
+    float *gain = nullptr;
+    if (tpow != 0.f) {
+        std::vector<float> g(nnt);
+        for (int it = 0; it < nnt; ++it) {
+            const float t = a.ot + it * a.rate;
+            g[it] = std::pow(t, tpow);
+        }
+        VTI_HIP(hipMalloc((void **)&gain, sizeof(float) * nnt));
+        VTI_HIP(hipMemcpy(gain, g.data(), sizeof(float) * nnt, hipMemcpyHostToDevice));
+    }
     hipLaunchKernelGGL(residual3d_kernel, dim3(nblk), dim3(256), 0, p->ctx->stream, sd.tr, sd.obs, sd.live, sh.nr,
-                       nnt, a.ot, a.rate, tpow, objtype == 0 ? 1 : 0, sd.partial);
+                       nnt, gain, objtype == 0 ? 1 : 0, sd.partial);
     VTI_HIP(hipGetLastError());
     std::vector<double> part(objtype == 0 ? nblk : std::max(1, sh.nr));
     if (objtype == 0) {
@@ -832,6 +841,7 @@ int modeled_shot(vti3d_prop *p, const DriverArgs &a, const Shot3 &sh, ShotDev &s
                                p->ctx->stream));
     }
     VTI_HIP(hipStreamSynchronize(p->ctx->stream));
+    if (gain) (void)hipFree(gain);
     double s = 0;
     for (double v : part) s += v;
     *ss = s;
