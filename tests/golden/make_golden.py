@@ -179,6 +179,24 @@ def main():
                                   "f": [float(v) for v in ft[:calls]]}
         print("lbfgs", func, "calls", calls, "iflag", fl.value)
 
+    # a small dataset written by the reference's own lib/io (through
+    # randomBound.x): header + binary, read back by vtiamd.sep in the tests
+    sepdir = os.path.join(HERE, "sep")
+    os.makedirs(sepdir, exist_ok=True)
+    for f in os.listdir(sepdir):
+        os.remove(os.path.join(sepdir, f))
+    vin = rb_input((48, 40, 1), 5)
+    vin.tofile(os.path.join(sepdir, "velin.bin"))
+    hin = os.path.join(sepdir, "velin.H")
+    with open(hin, "w") as f:
+        f.write("n1=48 o1=0 d1=10\nn2=40 o2=5 d2=12.5\nn3=1 o3=0 d3=1\n")
+        f.write(f"in={os.path.join(sepdir, 'velin.bin')}\ndata_format=native_float\n")
+    subprocess.run([os.path.join(REF, "randomBound.x"), f"modelIn={hin}", f"modelOut={sepdir}/velout.H", "seed=9",
+                    "pad=8", "pctG=0.9", f"datapath={sepdir}/"], check=True, capture_output=True, timeout=120)
+    fx["sep"] = {"header": "sep/velout.H", "dims": [48, 40, 1], "pad": 8, "pctG": 0.9, "seed": 9, "input_seed": 5,
+                 "sha256": sha(np.fromfile(os.path.join(sepdir, "velout.H@"), np.float32))}
+    print("sep dataset", fx["sep"]["sha256"][:12])
+
     with open(OUT, "w") as fo:
         json.dump(fx, fo, indent=1)
     print("wrote", OUT)
