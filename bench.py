@@ -98,7 +98,7 @@ def cpu_baseline(ved, npad, dx, dt, budget_s=15.0):
     }
 
 
-def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7):
+def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7, variants=None):
     """Shot-gradients/s of the batched 2D gradient against the plain
     one-shot-per-call driver on the same shots: a 4096-wide seeded model
     (nz 512, 40-point boundary), each shot on its own aperture of 300-900
@@ -125,9 +125,13 @@ def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7):
     rloc = np.array(rloc, np.int32)
     nnt = (nt - 1) // int(rate / dt + 0.5) + 1
     data = np.zeros(int(sum(nnt * s[4] for s in shots)), np.float32)
-    t0 = time.perf_counter()
-    vtiamd.objfunc_gradient2d_batch(ctx, data, c11, c13, c33, w, shots, rloc, NX, nz, nt, npad, dx, dx, dt, rate)
-    tb = time.perf_counter() - t0
+    timed = {}
+    for name, opts in (variants or {"default": {}}).items():
+        t0 = time.perf_counter()
+        vtiamd.objfunc_gradient2d_batch(ctx, data, c11, c13, c33, w, shots, rloc, NX, nz, nt, npad, dx, dx, dt, rate,
+                                        **opts)
+        timed[name] = time.perf_counter() - t0
+    tb = timed[next(iter(timed))]
     # the plain driver, one call per shot on the cropped model
     tp = 0.0
     for k in range(nplain):
@@ -146,7 +150,9 @@ def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7):
     return {"shots": ns, "model_padded": [nnz, NNX], "aperture_cols": [300, 900], "steps": nt - 1,
             "batched_shot_gradients_per_s": round(ns / tb, 3), "batched_s": round(tb, 3),
             "plain_shot_gradients_per_s": round(nplain / tp, 3), "plain_shots_timed": nplain,
-            "speedup": round((ns / tb) / (nplain / tp), 2)}
+            "speedup": round((ns / tb) / (nplain / tp), 2),
+            **({"variants_shot_gradients_per_s": {k: round(ns / v, 3) for k, v in timed.items()}}
+               if variants else {})}
 
 
 def shot_gradient_2d(vtiamd, ctx, nshots=2):

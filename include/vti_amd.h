@@ -214,11 +214,18 @@ int vti2d_gn_hessian_vepsdel(vti_ctx *ctx, float *gv, float *geps, float *gdel, 
  * data: the shots' observed traces one after the other, each time-major
  * ((nt-1)/(rate/dt)+1 rows of nr_shot).  Same schedule conditions as
  * vti2d_objfunc_gradient (rate/dt >= 2, (nt-1) % (rate/dt) == 0, ot = 0).
+ * levels: time levels advanced per launch, 1 or 2 (0: 2); tile_rows: rows
+ * of a two-level workgroup tile, 4, 8 or 16 (0: 16).  Neither changes a bit of
+ * the result.  snap_fmt: 0 keeps the imaging snapshots in FP32 (the result
+ * then equals the one-shot driver's bit for bit); 1 (FP16) and 2 (BF16)
+ * store each snapshot slice scaled by a power of two, halving their memory;
+ * the gradient then differs at the level the format's rounding sets.
  * No counterpart in the reference, which runs one shot per call. */
 int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, double *gc13, double *gc33,
                                  const float *data, const float *c11, const float *c13, const float *c33,
                                  const float *wavelet, const int *shots, int ns, const int *rloc, int nr, int NX, int nz,
-                                 int nt, int npad, float dx, float dz, float dt, float rate, int max_wave);
+                                 int nt, int npad, float dx, float dz, float dt, float rate, int max_wave, int levels,
+                                 int tile_rows, int snap_fmt);
 
 /* ---- L-BFGS (FP64) -----------------------------------------------------------
  * lbfgs, lib/lbfgs/lbfgs_c.cpp:11 (lbfgs.h:32) with mcsrch/mcstep,

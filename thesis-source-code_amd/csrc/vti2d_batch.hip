@@ -106,12 +106,19 @@ extern "C" {
 int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, double *gc13, double *gc33,
                                  const float *data, const float *c11, const float *c13, const float *c33,
                                  const float *wavelet, const int *shots, int ns, const int *rloc, int nr, int NX, int nz,
-                                 int nt, int npad, float dx, float dz, float dt, float rate, int max_wave)
+                                 int nt, int npad, float dx, float dz, float dt, float rate, int max_wave, int levels,
+                                 int tile_rows, int snap_fmt)
 {
     VTI_CHECK(ctx && objfunc && gc11 && gc13 && gc33 && data && c11 && c13 && c33 && wavelet && shots && rloc,
               "vti2d_objfunc_gradient_batch: null argument");
     VTI_CHECK(ns >= 1 && nr >= 0 && NX > 0 && nz > 0 && npad >= 0 && nt >= 3 && dx > 0 && dz > 0 && dt > 0,
               "vti2d_objfunc_gradient_batch: bad sizes");
+    if (levels == 0) levels = 2;
+    if (tile_rows == 0) tile_rows = 16;
+    VTI_CHECK(levels == 1 || levels == 2, "levels per launch must be 1 or 2, got %d", levels);
+    VTI_CHECK(tile_rows == 4 || tile_rows == 8 || tile_rows == 16, "tile rows must be 4, 8 or 16, got %d", tile_rows);
+    VTI_CHECK(snap_fmt >= 0 && snap_fmt <= 2, "snapshot format must be 0 (FP32), 1 (FP16) or 2 (BF16), got %d",
+              snap_fmt);
     VTI_HIP(hipSetDevice(ctx->device));
     (void)hipGetLastError();
     hipStream_t st = ctx->stream;
@@ -122,6 +129,7 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
               "(got ratio %d, nt %d)", ratio, nt);
     const int nnt = (nt - 1) / ratio + 1, nnt_data = nnt;
     const float dx2 = dx * dx, dz2 = dz * dz, dt2 = dt * dt;
+    const bool half = snap_fmt != 0;
 
     // ---- shots: check and lay out
     std::vector<ShotIn> in(ns);
@@ -173,10 +181,13 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
     if (!rc) rc = part.alloc(nblk_r, st);
     if (rc) return rc;
 
-    // ---- waves within the memory budget
+    // ---- waves within the memory budget: per shot 8 wavefield planes, 3
+    // gradient planes, the receiver map, the snapshots (and for 16-bit ones
+    // two staging planes and the slice scales/maxima) and two trace copies
     auto shot_bytes = [&](int s) {
         const long long nxz = (long long)(in[s].nx + 2 * npad) * nnz;
-        return (long long)4 * nxz * (4 + 3 + 1 + 2LL * nnt) + 8LL * nnt_data * in[s].nr + 64;
+        const long long snapb = half ? 2LL * nxz * 2 * nnt + 4LL * nxz * 2 + 16LL * nnt : 4LL * nxz * 2 * nnt;
+        return 4LL * nxz * (8 + 3 + 1) + snapb + 8LL * nnt_data * in[s].nr + 64;
     };
     size_t mfree = 0, mtotal = 0;
     VTI_HIP(hipMemGetInfo(&mfree, &mtotal));
@@ -192,17 +203,18 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
         if (s1 == s0) s1 = s0 + 1;
         const int nw = s1 - s0;
 
-        // layout of the wave
-        std::vector<ShotB> sb(nw);
-        std::vector<int> tile_shot;
-        long long f_total = 0, snap_total = 0, tr_total = 0;
+        // layout of the wave: sb/tile_shot for the one-level kernels (64 x 8
+        // tiles), sbt/tile_shot_t for the two-level ones (64 x tile_rows)
+        std::vector<ShotB> sb(nw), sbt(nw);
+        std::vector<int> tile_shot, tile_shot_t;
+        const int BYT = tile_rows, ntz = (nnz - 8 + kBY2 - 1) / kBY2, ntz_t = (nnz - 8 + BYT - 1) / BYT;
+        long long f_total = 0, snap_total = 0, tr_total = 0, max_plane = 0;
         int nrec = 0;
         for (int j = 0; j < nw; ++j) {
             const ShotIn &a = in[s0 + j];
             ShotB &b = sb[j];
             b.nnx = a.nx + 2 * npad;
             b.ntx = (b.nnx - 8 + 63) / 64;
-            const int ntz = (nnz - 8 + kBY2 - 1) / kBY2;
             b.tile0 = (int)tile_shot.size();
             for (int t = 0; t < b.ntx * ntz; ++t) tile_shot.push_back(j);
             b.nr = a.nr;
@@ -210,10 +222,23 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
             b.c_off = a.c0;
             b.snap_off = snap_total;
             b.r_off = tr_total;
+            b.stage_off = 2 * f_total;
             b.src = a.ix + a.iz * b.nnx;
-            b.pad_ = 0;
-            f_total += (long long)b.nnx * nnz;
-            snap_total += (long long)nnt * 2 * b.nnx * nnz;
+            b.rz0 = nnz;
+            b.rz1 = -1;
+            for (int r = 0; r < a.nr; ++r) {
+                const int z = rloc[2 * (a.r0 + r) + 1];
+                b.rz0 = std::min(b.rz0, z);
+                b.rz1 = std::max(b.rz1, z);
+            }
+            b.scale_off = 2 * nnt * j;
+            sbt[j] = b;
+            sbt[j].tile0 = (int)tile_shot_t.size();
+            for (int t = 0; t < b.ntx * ntz_t; ++t) tile_shot_t.push_back(j);
+            const long long plane = (long long)b.nnx * nnz;
+            max_plane = std::max(max_plane, plane);
+            f_total += plane;
+            snap_total += (long long)nnt * 2 * plane;
             tr_total += (long long)nnt_data * a.nr;
             nrec += a.nr;
         }
@@ -239,27 +264,41 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
                 std::memcpy(obs_h.data() + sb[j].r_off, data + data_off[s0 + j],
                             sizeof(float) * (size_t)nnt_data * in[s0 + j].nr);
 
-        DBuf<ShotB> sbd;
-        DBuf<int> tsh, recmap, tsdd;
+        DBuf<ShotB> sbd, sbtd;
+        DBuf<int> tsh, tsht, recmap, tsdd;
         DBuf<long long> fidxd, tbd;
-        DBuf<float> P[2], Q[2], G[3], snap, tr, obs;
+        DBuf<float> P[4], Q[4], G[3], snapf, stage, scale, tr, obs;
+        DBuf<unsigned short> snaph;
+        DBuf<unsigned> amax;
         rc = sbd.alloc(nw, st);
+        if (!rc) rc = sbtd.alloc(nw, st);
         if (!rc) rc = tsh.alloc(tile_shot.size(), st);
+        if (!rc) rc = tsht.alloc(tile_shot_t.size(), st);
         if (!rc) rc = recmap.alloc((size_t)f_total, st);
         if (!rc) rc = tsdd.alloc(nrec, st);
         if (!rc) rc = fidxd.alloc(nrec, st);
         if (!rc) rc = tbd.alloc(nrec, st);
-        for (int k = 0; k < 2 && !rc; ++k) {
+        for (int k = 0; k < 4 && !rc; ++k) {
             rc = P[k].alloc((size_t)f_total, st);
             if (!rc) rc = Q[k].alloc((size_t)f_total, st);
         }
         for (int k = 0; k < 3 && !rc; ++k) rc = G[k].alloc((size_t)f_total, st);
-        if (!rc) rc = snap.alloc((size_t)snap_total, st);
+        if (half) {
+            if (!rc) rc = snaph.alloc((size_t)snap_total, st);
+            if (!rc) rc = stage.alloc((size_t)(2 * f_total), st);
+            if (!rc) rc = scale.alloc((size_t)(2 * nnt * nw), st);
+            if (!rc) rc = amax.alloc((size_t)(2 * nnt * nw), st);
+        } else {
+            if (!rc) rc = snapf.alloc((size_t)snap_total, st);
+        }
         if (!rc) rc = tr.alloc((size_t)std::max<long long>(tr_total, 1), st);
         if (!rc) rc = obs.alloc((size_t)std::max<long long>(tr_total, 1), st);
         if (rc) return rc;
         VTI_HIP(hipMemcpyAsync(sbd.p, sb.data(), sizeof(ShotB) * nw, hipMemcpyHostToDevice, st));
+        VTI_HIP(hipMemcpyAsync(sbtd.p, sbt.data(), sizeof(ShotB) * nw, hipMemcpyHostToDevice, st));
         VTI_HIP(hipMemcpyAsync(tsh.p, tile_shot.data(), sizeof(int) * tile_shot.size(), hipMemcpyHostToDevice, st));
+        VTI_HIP(hipMemcpyAsync(tsht.p, tile_shot_t.data(), sizeof(int) * tile_shot_t.size(), hipMemcpyHostToDevice,
+                               st));
         VTI_HIP(hipMemcpyAsync(recmap.p, hmap.data(), sizeof(int) * hmap.size(), hipMemcpyHostToDevice, st));
         if (nrec) {
             VTI_HIP(hipMemcpyAsync(tsdd.p, tsd.data(), sizeof(int) * nrec, hipMemcpyHostToDevice, st));
@@ -275,24 +314,60 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
         bg.cpitch = NNX;
         bg.nnxz_total = f_total;
         const dim3 grid((unsigned)tile_shot.size()), block(64, kBY2);
+        const dim3 gridt((unsigned)tile_shot_t.size()), blockt(64, BYT);
         const int rb = (nrec + 255) / 256;
-
-        // ---- forward (objFuncGradient_f.cu:160-210 for every shot)
-        hipLaunchKernelGGL(set_src_b_kernel, dim3((nw + 63) / 64), dim3(64), 0, st, P[0].p, Q[0].p, sbd.p, nw,
-                           dt2 * wavelet[0]);
-        for (int it = 2; it < nt; ++it) {
-            const int lc = it - 1;
-            const int slot = lc % ratio == 0 ? lc / ratio : -1;
-            hipLaunchKernelGGL(vti::fwd2db_kernel, grid, block, 0, st, P[1].p, Q[1].p, P[0].p, Q[0].p, C[0].p, C[1].p,
-                               C[2].p, t0d.p, sbd.p, tsh.p, bg, dx2, dz2, dt2, dt2 * wavelet[it - 1], snap.p, slot);
-            std::swap(P[0].p, P[1].p);
-            std::swap(Q[0].p, Q[1].p);
+        const vti::SnapW sw{half ? stage.p : snapf.p, half ? amax.p : nullptr};
+        const vti::SnapR sr{snapf.p, snaph.p, scale.p};
+        const dim3 gpack((unsigned)std::min<long long>((2 * max_plane + 255) / 256, 1024), (unsigned)nw);
+        auto pack = [&](int slot) {
+            if (half && slot >= 0)
+                hipLaunchKernelGGL(vti::pack_snap_kernel, gpack, dim3(256), 0, st, snaph.p, scale.p, stage.p, amax.p,
+                                   sbd.p, nnz, slot, snap_fmt == 2 ? 1 : 0);
+        };
+        auto record = [&](int it, int buf) {
             if (it % ratio == 0 && nrec)
-                hipLaunchKernelGGL(record_b_kernel, dim3(rb), dim3(256), 0, st, tr.p, P[0].p, Q[0].p, fidxd.p, tbd.p,
-                                   tsdd.p, nrec, it / ratio);
+                hipLaunchKernelGGL(record_b_kernel, dim3(rb), dim3(256), 0, st, tr.p, P[buf].p, Q[buf].p, fidxd.p,
+                                   tbd.p, tsdd.p, nrec, it / ratio);
+        };
+        auto snap_slot = [&](int lc) { return lc % ratio == 0 ? lc / ratio : -1; };
+
+        // ---- forward (objFuncGradient_f.cu:160-210 for every shot).  Buffers
+        // ip/ic hold the previous and current levels; s0b/s1b are free.
+        int ip = 1, ic = 0, s0b = 2, s1b = 3;
+        hipLaunchKernelGGL(set_src_b_kernel, dim3((nw + 63) / 64), dim3(64), 0, st, P[ic].p, Q[ic].p, sbd.p, nw,
+                           dt2 * wavelet[0]);
+        for (int it = 2; it < nt;) {
+            if (levels == 2 && it + 1 < nt) {
+                // levels it and it+1, reading levels it-1 and it
+                const int sl1 = snap_slot(it - 1), sl2 = snap_slot(it);
+#define FWD_T2(BYV)                                                                                                 \
+    hipLaunchKernelGGL(vti::fwd2db_t2_kernel<BYV>, gridt, blockt, 0, st, P[s0b].p, Q[s0b].p, P[s1b].p, Q[s1b].p,     \
+                       P[ip].p, Q[ip].p, P[ic].p, Q[ic].p, C[0].p, C[1].p, C[2].p, t0d.p, sbtd.p, tsht.p, bg, dx2, \
+                       dz2, dt2, dt2 * wavelet[it - 1], dt2 * wavelet[it], sw, sl1, sl2)
+                if (BYT == 4) FWD_T2(4);
+                else if (BYT == 8) FWD_T2(8);
+                else FWD_T2(16);
+#undef FWD_T2
+                pack(sl1 >= 0 ? sl1 : sl2);
+                record(it, s0b);
+                record(it + 1, s1b);
+                std::swap(ip, s0b);
+                std::swap(ic, s1b);
+                it += 2;
+            } else {
+                const int sl = snap_slot(it - 1);
+                hipLaunchKernelGGL(vti::fwd2db_kernel, grid, block, 0, st, P[ip].p, Q[ip].p, P[ic].p, Q[ic].p,
+                                   C[0].p, C[1].p, C[2].p, t0d.p, sbd.p, tsh.p, bg, dx2, dz2, dt2,
+                                   dt2 * wavelet[it - 1], sw, sl);
+                pack(sl);
+                std::swap(ip, ic);
+                record(it, ic);
+                it += 1;
+            }
         }
-        hipLaunchKernelGGL(vti::dsnap2db_kernel, grid, block, 0, st, P[0].p, Q[0].p, sbd.p, tsh.p, bg, dx2, dz2,
-                           snap.p, nnt - 1);
+        hipLaunchKernelGGL(vti::dsnap2db_kernel, grid, block, 0, st, P[ic].p, Q[ic].p, sbd.p, tsh.p, bg, dx2, dz2, sw,
+                           nnt - 1);
+        pack(nnt - 1);
         VTI_HIP(hipGetLastError());
 
         // ---- residual and objective (objFuncGradient_f.cu:214-219)
@@ -305,32 +380,76 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
         }
 
         // ---- backward (objFuncGradient_f.cu:225-300 for every shot)
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < 4; ++k) {
             VTI_HIP(hipMemsetAsync(P[k].p, 0, (size_t)f_total * 4, st));
             VTI_HIP(hipMemsetAsync(Q[k].p, 0, (size_t)f_total * 4, st));
         }
+        ic = 0;
+        ip = 1;
+        s0b = 2;
+        s1b = 3;
         if (nrec)
-            hipLaunchKernelGGL(inject_first_b_kernel, dim3(rb), dim3(256), 0, st, P[0].p, Q[0].p, tr.p, fidxd.p,
+            hipLaunchKernelGGL(inject_first_b_kernel, dim3(rb), dim3(256), 0, st, P[ic].p, Q[ic].p, tr.p, fidxd.p,
                                tbd.p, tsdd.p, nrec, nnt_data - 1, dt2);
         {
             float f = (float)((nt - 2.) / ratio);
             const int i1 = (int)f;
             f = f - i1;
-            hipLaunchKernelGGL(vti::xcor2db_kernel, grid, block, 0, st, P[0].p, Q[0].p, sbd.p, tsh.p, bg, snap.p, i1,
-                               f, G[0].p, G[1].p, G[2].p);
+#define XCOR(SFV)                                                                                                  \
+    hipLaunchKernelGGL(vti::xcor2db_kernel<SFV>, grid, block, 0, st, P[ic].p, Q[ic].p, sbd.p, tsh.p, bg, sr, i1, f, \
+                       G[0].p, G[1].p, G[2].p)
+            if (snap_fmt == 0) XCOR(0);
+            else if (snap_fmt == 1) XCOR(1);
+            else XCOR(2);
+#undef XCOR
         }
-        for (int it = nt - 3; it >= 0; --it) {
+        // trace rows and snapshot slots bracketing adjoint level it
+        auto level = [&](int it) {
+            vti::LevelB l;
             float fr = (float)((it + 1.) / ratio);
-            const int ir1 = (int)fr;
-            fr = fr - ir1;
+            l.row = (int)fr;
+            l.fr = fr - l.row;
             float f = (float)it / ratio;
-            const int i1 = (int)f;
-            f = f - i1;
-            hipLaunchKernelGGL(vti::bwd2db_kernel, grid, block, 0, st, P[1].p, Q[1].p, P[0].p, Q[0].p, C[0].p, C[1].p,
-                               C[2].p, t0d.p, sbd.p, tsh.p, bg, recmap.p, tr.p, ir1, fr, snap.p, i1, f, G[0].p,
-                               G[1].p, G[2].p, dx2, dz2, dt2);
-            std::swap(P[0].p, P[1].p);
-            std::swap(Q[0].p, Q[1].p);
+            l.slot = (int)f;
+            l.f = f - l.slot;
+            return l;
+        };
+        for (int it = nt - 3; it >= 0;) {
+            if (levels == 2 && it >= 1) {
+                const vti::LevelB l1 = level(it), l2 = level(it - 1);
+#define BWD_T2(BYV, SFV)                                                                                             \
+    vti::bwd2db_t2_kernel<BYV, SFV><<<gridt, blockt, 0, st>>>(P[s0b].p, Q[s0b].p, P[s1b].p,                         \
+                       Q[s1b].p, P[ip].p, Q[ip].p, P[ic].p, Q[ic].p, C[0].p, C[1].p, C[2].p, t0d.p, sbtd.p, tsht.p, \
+                       bg, recmap.p, tr.p, l1, l2, sr, G[0].p, G[1].p, G[2].p, dx2, dz2, dt2)
+#define BWD_T2_SF(BYV)                  \
+    if (snap_fmt == 0) BWD_T2(BYV, 0);  \
+    else if (snap_fmt == 1) BWD_T2(BYV, 1); \
+    else BWD_T2(BYV, 2)
+                if (BYT == 4) {
+                    BWD_T2_SF(4);
+                } else if (BYT == 8) {
+                    BWD_T2_SF(8);
+                } else {
+                    BWD_T2_SF(16);
+                }
+#undef BWD_T2_SF
+#undef BWD_T2
+                std::swap(ip, s0b);
+                std::swap(ic, s1b);
+                it -= 2;
+            } else {
+                const vti::LevelB l = level(it);
+#define BWD1(SFV)                                                                                                    \
+    hipLaunchKernelGGL(vti::bwd2db_kernel<SFV>, grid, block, 0, st, P[ip].p, Q[ip].p, P[ic].p, Q[ic].p, C[0].p,      \
+                       C[1].p, C[2].p, t0d.p, sbd.p, tsh.p, bg, recmap.p, tr.p, l.row, l.fr, sr, l.slot, l.f, G[0].p, \
+                       G[1].p, G[2].p, dx2, dz2, dt2)
+                if (snap_fmt == 0) BWD1(0);
+                else if (snap_fmt == 1) BWD1(1);
+                else BWD1(2);
+#undef BWD1
+                std::swap(ip, ic);
+                it -= 1;
+            }
         }
         VTI_HIP(hipGetLastError());
         // ---- per-shot FP32 gradients into the FP64 global sum, in shot order

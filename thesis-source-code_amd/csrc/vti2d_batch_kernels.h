@@ -1,18 +1,32 @@
 // Batched 2D kernels: many shots, each on its own aperture sub-grid of one
-// global model, advanced one time level per launch.
+// global model, advanced one or two time levels per launch.
 //
-// A shot's fields (p, q at two levels, its gradient and its receiver map)
-// live in its own nnz x nnx block of the batch buffers; its coefficients are
-// the columns [c_off, c_off + nnx) of the global c11/c13/c33 (row pitch
-// cpitch).  Workgroups are numbered over all shots' 64 x 8 tiles;
-// tile_shot[] maps a workgroup to its shot.  The taper is the product of the
-// 1D profile t0 of init_abc (lib/vti-fwi-2d/init.cpp:70-89) along x and z,
-// formed in float as the reference forms its taper array.
+// A shot's fields (p, q, its gradient and its receiver map) live in its own
+// nnz x nnx block of the batch buffers; its coefficients are the columns
+// [c_off, c_off + nnx) of the global c11/c13/c33 (row pitch cpitch).
+// Workgroups are numbered over all shots' tiles; tile_shot[] maps a
+// workgroup to its shot.  The taper is the product of the 1D profile t0 of
+// init_abc (lib/vti-fwi-2d/init.cpp:70-89) along x and z, formed in float as
+// the reference forms its taper array.
 //
 // Per point the arithmetic is exactly that of vti2d_kernels.h (FP32, source
 // order, contraction off), so a shot in a batch reproduces the single-shot
-// driver on its cropped model bit for bit.
+// driver on its cropped model bit for bit when the snapshots are kept in
+// FP32.
+//
+// Snapshot formats (SF): 0 = FP32; 1 = FP16 and 2 = BF16, each slice (one
+// shot, one slot, one of the two components) scaled by a power of two s so
+// that its largest magnitude lies in [2^14, 2^15).  The forward kernels then
+// write the FP32 slice into a staging area and fold |D| into a per-slice
+// maximum (one atomic per block); pack_snap_kernel rounds the staged slice to
+// 16 bits after the launch.  The adjoint reads h * s.
+//
+// Every step kernel issues the loads of its points' own operands together
+// with the LDS tile loads, before the barrier, so that a workgroup waits for
+// one round trip to memory rather than two.
 #pragma once
+
+#include <hip/hip_fp16.h>
 
 #include "vti2d_kernels.h"
 
@@ -21,21 +35,36 @@
 namespace vti {
 
 struct ShotB {
-    int nnx;            // padded width of the shot's sub-grid
-    int ntx;            // 64-wide tiles across it
-    int tile0;          // first workgroup of this shot
-    int nr;             // receivers
-    long long f_off;    // offset of the shot's nnz x nnx block in the field buffers
-    long long c_off;    // first column of the sub-grid in the global model
-    long long snap_off; // offset of the shot's snapshots (nnt slots of 2 nnz x nnx planes)
-    long long r_off;    // offset of the shot's traces (time-major, nr per row)
-    int src;            // source point within the sub-grid (ix + iz * nnx)
-    int pad_;
+    int nnx;              // padded width of the shot's sub-grid
+    int ntx;              // 64-wide tiles across it
+    int tile0;            // first workgroup of this shot
+    int nr;               // receivers
+    long long f_off;      // offset of the shot's nnz x nnx block in the field buffers
+    long long c_off;      // first column of the sub-grid in the global model
+    long long snap_off;   // offset of the shot's snapshots (nnt slots of 2 nnz x nnx planes)
+    long long r_off;      // offset of the shot's traces (time-major, nr per row)
+    long long stage_off;  // offset of the shot's staging planes (16-bit snapshots)
+    int src;              // source point within the sub-grid (ix + iz * nnx)
+    int rz0, rz1;         // rows holding receivers (rz0 > rz1 when there are none)
+    int scale_off;        // first of the shot's 2*nnt slice scales / maxima
 };
 
 struct BatchGeo {
     int nnz, npad, cpitch;
     long long nnxz_total;
+};
+
+// Where a forward kernel puts the D snapshot of the level it reads.
+struct SnapW {
+    float *d;        // FP32 slots (amax == nullptr) or the staging planes
+    unsigned *amax;  // per-slice maxima of |D| as float bits, or nullptr
+};
+
+// Where the adjoint kernels read them.
+struct SnapR {
+    const float *f;           // FP32 slots
+    const unsigned short *h;  // 16-bit slots
+    const float *scale;       // per-slice scale of the 16-bit slots
 };
 
 // taper of the point (ix, iz) of a sub-grid nnx wide: t0 along x, then along z
@@ -49,14 +78,101 @@ __device__ __forceinline__ float taper_at(const float *__restrict__ t0, int npad
     return t;
 }
 
+// write D of point i (shot-relative) into its slot or the staging planes
+__device__ __forceinline__ void snap_put(const SnapW &w, const ShotB &sh, long long plane, int slot, long long i,
+                                         float tp, float tq)
+{
+    float *sp = w.amax ? w.d + sh.stage_off : w.d + sh.snap_off + (long long)slot * 2 * plane;
+    sp[i] = tp;
+    sp[i + plane] = tq;
+}
+
+// fold the block's largest |D| of both components into the slice maxima (one
+// atomic per block and component); every thread of the block must reach this
+// call, and red must hold one float2 per wave
+__device__ __forceinline__ void snap_amax(const SnapW &w, const ShotB &sh, int slot, float mp, float mq,
+                                          float2 *red)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        mp = fmaxf(mp, __shfl_xor(mp, o, 64));
+        mq = fmaxf(mq, __shfl_xor(mq, o, 64));
+    }
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y, nw = (blockDim.x * blockDim.y) / 64;
+    __syncthreads();
+    if (tid % 64 == 0) red[tid / 64] = make_float2(mp, mq);
+    __syncthreads();
+    if (tid == 0) {
+        for (int k = 1; k < nw; ++k) {
+            mp = fmaxf(mp, red[k].x);
+            mq = fmaxf(mq, red[k].y);
+        }
+        unsigned *a = w.amax + sh.scale_off + 2 * slot;
+        if (mp > 0.f) atomicMax(a, __float_as_uint(mp));
+        if (mq > 0.f) atomicMax(a + 1, __float_as_uint(mq));
+    }
+}
+
+// D component c of point i at a slot
+template <int SF>
+__device__ __forceinline__ float snap_get(const SnapR &r, const ShotB &sh, long long plane, int slot, int c,
+                                          long long i)
+{
+    const long long k = sh.snap_off + (long long)slot * 2 * plane + c * plane + i;
+    if (SF == 0) return r.f[k];
+    const float s = r.scale[sh.scale_off + 2 * slot + c];
+    if (SF == 1) return __half2float(__ushort_as_half(r.h[k])) * s;
+    return __uint_as_float((unsigned)r.h[k] << 16) * s;
+}
+
+// exponent e of the scale 2^e of a slice whose largest magnitude has float
+// bits mbits: the slice / 2^e lies in [2^14, 2^15); 0 for an empty slice
+__device__ __forceinline__ int slice_exp(unsigned mbits)
+{
+    const float m = __uint_as_float(mbits);
+    if (!(m > 0.f)) return 0;
+    int e;
+    (void)frexpf(m, &e);  // m in [2^(e-1), 2^e)
+    return max(e - 15, -110);
+}
+
+// Round the staged slices of slot `slot` of every shot (blockIdx.y) to 16
+// bits (FP16 when bf16 == 0, else BF16; round to nearest even) and record
+// their scales.
+static __global__ void pack_snap_kernel(unsigned short *__restrict__ snap, float *__restrict__ scale,
+                                        const float *__restrict__ stage, const unsigned *__restrict__ amax,
+                                        const ShotB *__restrict__ shots, int nnz, int slot, int bf16)
+{
+    const ShotB sh = shots[blockIdx.y];
+    const long long plane = (long long)sh.nnx * nnz;
+    const int e0 = slice_exp(amax[sh.scale_off + 2 * slot]), e1 = slice_exp(amax[sh.scale_off + 2 * slot + 1]);
+    if (blockIdx.x == 0 && threadIdx.x < 2)
+        scale[sh.scale_off + 2 * slot + threadIdx.x] = ldexpf(1.f, threadIdx.x ? e1 : e0);
+    const float inv0 = ldexpf(1.f, -e0), inv1 = ldexpf(1.f, -e1);
+    unsigned short *out = snap + sh.snap_off + (long long)slot * 2 * plane;
+    const float *in = stage + sh.stage_off;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < 2 * plane;
+         k += (long long)gridDim.x * blockDim.x) {
+        const float v = in[k] * (k < plane ? inv0 : inv1);
+        if (bf16) {
+            const unsigned u = __float_as_uint(v);
+            out[k] = (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+        } else {
+            out[k] = __half_as_ushort(__float2half_rn(v));
+        }
+    }
+}
+
 // LDS tiles of a sub-grid whose coefficients sit at another pitch/offset
+// (pc, qc: the raw p and q of the thread's own point, 0 outside the sub-grid)
 template <int MODE>
 __device__ __forceinline__ void load_tiles_b(Tiles2 &t, const float *__restrict__ P, const float *__restrict__ Q,
                                              const float *__restrict__ c11, const float *__restrict__ c13,
                                              const float *__restrict__ c33, int nnx, int nnz, int cpitch, int x0,
-                                             int z0)
+                                             int z0, float &pc, float &qc)
 {
     const int tx = threadIdx.x, ty = threadIdx.y;
+    pc = 0.f;
+    qc = 0.f;
     auto pv = [&](int x, int z) -> float {
         if (x < 0 || x >= nnx || z < 0 || z >= nnz) return 0.f;
         const long long i = x + (long long)z * nnx;
@@ -72,12 +188,27 @@ __device__ __forceinline__ void load_tiles_b(Tiles2 &t, const float *__restrict_
         return c13[c] * P[i] + c33[c] * Q[i];
     };
     const int ix = x0 + tx, iz = z0 + ty;
-    t.p[ty][tx + 4] = pv(ix, iz);
+    if (ix < nnx && iz < nnz) {
+        const long long i = ix + (long long)iz * nnx;
+        pc = P[i];
+        qc = Q[i];
+        if (MODE == 0) {
+            t.p[ty][tx + 4] = pc;
+            t.q[ty + 4][tx] = qc;
+        } else {
+            const long long c = ix + (long long)iz * cpitch;
+            const float a11 = c11[c], a13 = c13[c], a33 = c33[c];
+            t.p[ty][tx + 4] = a11 * pc + a13 * qc;
+            t.q[ty + 4][tx] = a13 * pc + a33 * qc;
+        }
+    } else {
+        t.p[ty][tx + 4] = 0.f;
+        t.q[ty + 4][tx] = 0.f;
+    }
     if (tx < 4) {
         t.p[ty][tx] = pv(x0 - 4 + tx, iz);
         t.p[ty][tx + 68] = pv(x0 + 64 + tx, iz);
     }
-    t.q[ty + 4][tx] = qv(ix, iz);
     if (ty < 4) {
         t.q[ty][tx] = qv(ix, z0 - 4 + ty);
         t.q[ty + kBY2 + 4][tx] = qv(ix, z0 + kBY2 + ty);
@@ -85,118 +216,156 @@ __device__ __forceinline__ void load_tiles_b(Tiles2 &t, const float *__restrict_
 }
 
 // forwardC step of every shot with injectSource, abc and (slot >= 0) the D
-// snapshot of the level read (objFuncGradient_f.cu:160-210 per shot).
+// snapshot of the level read (objFuncGradient_f.cu:160-210 per shot).  The
+// point's own operands are loaded with the tiles, before the barrier.
 static __global__ __launch_bounds__(64 * kBY2) void fwd2db_kernel(
     float *__restrict__ P_n, float *__restrict__ Q_n, const float *__restrict__ P_c, const float *__restrict__ Q_c,
     const float *__restrict__ c11, const float *__restrict__ c13, const float *__restrict__ c33,
     const float *__restrict__ t0, const ShotB *__restrict__ shots, const int *__restrict__ tile_shot, BatchGeo bg,
-    float dx2, float dz2, float dt2, float src_val, float *__restrict__ snap, int slot)
+    float dx2, float dz2, float dt2, float src_val, SnapW sw, int slot)
 {
     __shared__ Tiles2 t;
+    __shared__ float2 red[kBY2];
     const ShotB sh = shots[tile_shot[blockIdx.x]];
     const int tl = blockIdx.x - sh.tile0;
     const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * kBY2;
     const int nnx = sh.nnx, nnz = bg.nnz;
-    const float *p_c = P_c + sh.f_off, *q_c = Q_c + sh.f_off;
-    load_tiles_b<0>(t, p_c, q_c, nullptr, nullptr, nullptr, nnx, nnz, 0, x0, z0);
-    __syncthreads();
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
-    if (ix >= nnx - 4 || iz >= nnz - 4) return;
+    const bool act = ix < nnx - 4 && iz < nnz - 4;
     const long long i = ix + (long long)iz * nnx;
-    const long long c = sh.c_off + ix + (long long)iz * bg.cpitch;
-    const float tp = dxx(t, dx2), tq = dzz(t, dz2);
-    if (slot >= 0) {
-        float *sp = snap + sh.snap_off + (long long)slot * 2 * nnx * nnz;
-        sp[i] = tp;
-        sp[i + (long long)nnx * nnz] = tq;
-    }
-    const float tap = taper_at(t0, bg.npad, nnx, nnz, ix, iz);
     float *p_n = P_n + sh.f_off, *q_n = Q_n + sh.f_off;
-    const float a11 = c11[c], a13 = c13[c], a33 = c33[c];
-    float pn = 2.f * p_c[i] + dt2 * (a11 * tp + a13 * tq) - p_n[i] * tap;
-    float qn = 2.f * q_c[i] + dt2 * (a13 * tp + a33 * tq) - q_n[i] * tap;
-    if (i == sh.src) {
-        pn += src_val;
-        qn += src_val;
+    float pp = 0.f, qp = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f, tap = 0.f, pc, qc;
+    if (act) {
+        const long long c = sh.c_off + ix + (long long)iz * bg.cpitch;
+        pp = p_n[i];
+        qp = q_n[i];
+        a11 = c11[c];
+        a13 = c13[c];
+        a33 = c33[c];
+        tap = taper_at(t0, bg.npad, nnx, nnz, ix, iz);
     }
-    p_n[i] = pn * tap;
-    q_n[i] = qn * tap;
+    load_tiles_b<0>(t, P_c + sh.f_off, Q_c + sh.f_off, nullptr, nullptr, nullptr, nnx, nnz, 0, x0, z0, pc, qc);
+    __syncthreads();
+    float mp = 0.f, mq = 0.f;
+    if (act) {
+        const float tp = dxx(t, dx2), tq = dzz(t, dz2);
+        if (slot >= 0) {
+            snap_put(sw, sh, (long long)nnx * nnz, slot, i, tp, tq);
+            mp = fabsf(tp);
+            mq = fabsf(tq);
+        }
+        float pn = 2.f * pc + dt2 * (a11 * tp + a13 * tq) - pp * tap;
+        float qn = 2.f * qc + dt2 * (a13 * tp + a33 * tq) - qp * tap;
+        if (i == sh.src) {
+            pn += src_val;
+            qn += src_val;
+        }
+        p_n[i] = pn * tap;
+        q_n[i] = qn * tap;
+    }
+    if (sw.amax && slot >= 0) snap_amax(sw, sh, slot, mp, mq, red);
 }
 
 // D of the last forward level of every shot into its snapshot slot
-static __global__ __launch_bounds__(64 * kBY2) void dsnap2db_kernel(const float *__restrict__ P, const float *__restrict__ Q,
-                                                             const ShotB *__restrict__ shots,
-                                                             const int *__restrict__ tile_shot, BatchGeo bg, float dx2,
-                                                             float dz2, float *__restrict__ snap, int slot)
+static __global__ __launch_bounds__(64 * kBY2) void dsnap2db_kernel(const float *__restrict__ P,
+                                                                    const float *__restrict__ Q,
+                                                                    const ShotB *__restrict__ shots,
+                                                                    const int *__restrict__ tile_shot, BatchGeo bg,
+                                                                    float dx2, float dz2, SnapW sw, int slot)
 {
     __shared__ Tiles2 t;
+    __shared__ float2 red[kBY2];
     const ShotB sh = shots[tile_shot[blockIdx.x]];
     const int tl = blockIdx.x - sh.tile0;
     const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * kBY2;
     const int nnx = sh.nnx, nnz = bg.nnz;
-    load_tiles_b<0>(t, P + sh.f_off, Q + sh.f_off, nullptr, nullptr, nullptr, nnx, nnz, 0, x0, z0);
+    float pc, qc;
+    load_tiles_b<0>(t, P + sh.f_off, Q + sh.f_off, nullptr, nullptr, nullptr, nnx, nnz, 0, x0, z0, pc, qc);
     __syncthreads();
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
-    if (ix >= nnx - 4 || iz >= nnz - 4) return;
-    const long long i = ix + (long long)iz * nnx;
-    float *sp = snap + sh.snap_off + (long long)slot * 2 * nnx * nnz;
-    sp[i] = dxx(t, dx2);
-    sp[i + (long long)nnx * nnz] = dzz(t, dz2);
+    float mp = 0.f, mq = 0.f;
+    if (ix < nnx - 4 && iz < nnz - 4) {
+        const long long i = ix + (long long)iz * nnx;
+        const float tp = dxx(t, dx2), tq = dzz(t, dz2);
+        snap_put(sw, sh, (long long)nnx * nnz, slot, i, tp, tq);
+        mp = fabsf(tp);
+        mq = fabsf(tq);
+    }
+    if (sw.amax) snap_amax(sw, sh, slot, mp, mq, red);
 }
 
 // backwardDC step of every shot with its residual injection (receiver map),
 // abc and gradientCrossCor against the interpolated snapshots
-// (objFuncGradient_f.cu:249-300 per shot).
+// (objFuncGradient_f.cu:249-300 per shot).  The point's own operands (the
+// previous level, the injected residual, the snapshots, the gradient) are
+// loaded with the tiles, before the barrier.
+template <int SF>
 static __global__ __launch_bounds__(64 * kBY2) void bwd2db_kernel(
     float *__restrict__ P_n, float *__restrict__ Q_n, const float *__restrict__ P_c, const float *__restrict__ Q_c,
     const float *__restrict__ c11, const float *__restrict__ c13, const float *__restrict__ c33,
     const float *__restrict__ t0, const ShotB *__restrict__ shots, const int *__restrict__ tile_shot, BatchGeo bg,
-    const int *__restrict__ recmap, const float *__restrict__ traces, int row0, float fr,
-    const float *__restrict__ snap, int slot0, float f, float *__restrict__ G11, float *__restrict__ G13,
-    float *__restrict__ G33, float dx2, float dz2, float dt2)
+    const int *__restrict__ recmap, const float *__restrict__ traces, int row0, float fr, SnapR sr, int slot0,
+    float f, float *__restrict__ G11, float *__restrict__ G13, float *__restrict__ G33, float dx2, float dz2,
+    float dt2)
 {
     __shared__ Tiles2 t;
     const ShotB sh = shots[tile_shot[blockIdx.x]];
     const int tl = blockIdx.x - sh.tile0;
     const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * kBY2;
     const int nnx = sh.nnx, nnz = bg.nnz;
-    const float *p_c = P_c + sh.f_off, *q_c = Q_c + sh.f_off;
-    const float *cc11 = c11 + sh.c_off, *cc13 = c13 + sh.c_off, *cc33 = c33 + sh.c_off;
-    load_tiles_b<1>(t, p_c, q_c, cc11, cc13, cc33, nnx, nnz, bg.cpitch, x0, z0);
-    __syncthreads();
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
-    if (ix >= nnx - 4 || iz >= nnz - 4) return;
-    const long long i = ix + (long long)iz * nnx;
-    const float tap = taper_at(t0, bg.npad, nnx, nnz, ix, iz);
+    const bool act = ix < nnx - 4 && iz < nnz - 4;
+    const long long i = ix + (long long)iz * nnx, k = sh.f_off + i;
+    const long long plane = (long long)nnx * nnz;
     float *p_n = P_n + sh.f_off, *q_n = Q_n + sh.f_off;
+    float pp = 0.f, qp = 0.f, tap = 0.f, inj = 0.f, g11 = 0.f, g13 = 0.f, g33 = 0.f;
+    float d0p = 0.f, d1p = 0.f, d0q = 0.f, d1q = 0.f, pc, qc;
+    int ir = -1;
+    if (act) {
+        pp = p_n[i];
+        qp = q_n[i];
+        tap = taper_at(t0, bg.npad, nnx, nnz, ix, iz);
+        ir = (iz >= sh.rz0 && iz <= sh.rz1) ? recmap[k] : -1;
+        if (ir >= 0) {
+            const float *r0 = traces + sh.r_off + (long long)row0 * sh.nr;
+            inj = 0.5f * ((1.f - fr) * r0[ir] + fr * r0[ir + sh.nr]);
+        }
+        d0p = snap_get<SF>(sr, sh, plane, slot0, 0, i);
+        d1p = snap_get<SF>(sr, sh, plane, slot0 + 1, 0, i);
+        d0q = snap_get<SF>(sr, sh, plane, slot0, 1, i);
+        d1q = snap_get<SF>(sr, sh, plane, slot0 + 1, 1, i);
+        g11 = G11[k];
+        g13 = G13[k];
+        g33 = G33[k];
+    }
+    load_tiles_b<1>(t, P_c + sh.f_off, Q_c + sh.f_off, c11 + sh.c_off, c13 + sh.c_off, c33 + sh.c_off, nnx, nnz,
+                    bg.cpitch, x0, z0, pc, qc);
+    __syncthreads();
+    if (!act) return;
     const float tp0 = dxx(t, dx2), tq0 = dzz(t, dz2);
-    float pn = 2.f * p_c[i] + dt2 * tp0 - p_n[i] * tap;
-    float qn = 2.f * q_c[i] + dt2 * tq0 - q_n[i] * tap;
-    const int ir = recmap[sh.f_off + i];
+    float pn = 2.f * pc + dt2 * tp0 - pp * tap;
+    float qn = 2.f * qc + dt2 * tq0 - qp * tap;
     if (ir >= 0) {
-        const float *r0 = traces + sh.r_off + (long long)row0 * sh.nr;
-        const float s = 0.5f * ((1.f - fr) * r0[ir] + fr * r0[ir + sh.nr]);
-        pn += dt2 * s;
-        qn += dt2 * s;
+        pn += dt2 * inj;
+        qn += dt2 * inj;
     }
     pn = pn * tap;
     qn = qn * tap;
     p_n[i] = pn;
     q_n[i] = qn;
-    const long long plane = (long long)nnx * nnz;
-    const float *D0 = snap + sh.snap_off + (long long)slot0 * 2 * plane, *D1 = D0 + 2 * plane;
-    const float Dp = (1.f - f) * D0[i] + f * D1[i];
-    const float Dq = (1.f - f) * D0[i + plane] + f * D1[i + plane];
-    G11[sh.f_off + i] += pn * Dp;
-    G13[sh.f_off + i] += pn * Dq + qn * Dp;
-    G33[sh.f_off + i] += qn * Dq;
+    const float Dp = (1.f - f) * d0p + f * d1p;
+    const float Dq = (1.f - f) * d0q + f * d1q;
+    G11[k] = g11 + pn * Dp;
+    G13[k] = g13 + (pn * Dq + qn * Dp);
+    G33[k] = g33 + qn * Dq;
 }
 
 // gradientCrossCor of the first adjoint level of every shot
+template <int SF>
 static __global__ void xcor2db_kernel(const float *__restrict__ P, const float *__restrict__ Q,
-                               const ShotB *__restrict__ shots, const int *__restrict__ tile_shot, BatchGeo bg,
-                               const float *__restrict__ snap, int slot0, float f, float *__restrict__ G11,
-                               float *__restrict__ G13, float *__restrict__ G33)
+                                      const ShotB *__restrict__ shots, const int *__restrict__ tile_shot, BatchGeo bg,
+                                      SnapR sr, int slot0, float f, float *__restrict__ G11, float *__restrict__ G13,
+                                      float *__restrict__ G33)
 {
     const ShotB sh = shots[tile_shot[blockIdx.x]];
     const int tl = blockIdx.x - sh.tile0;
@@ -205,12 +374,360 @@ static __global__ void xcor2db_kernel(const float *__restrict__ P, const float *
     if (ix >= nnx - 4 || iz >= nnz - 4) return;
     const long long i = ix + (long long)iz * nnx, k = sh.f_off + i;
     const long long plane = (long long)nnx * nnz;
-    const float *D0 = snap + sh.snap_off + (long long)slot0 * 2 * plane, *D1 = D0 + 2 * plane;
-    const float Dp = (1.f - f) * D0[i] + f * D1[i];
-    const float Dq = (1.f - f) * D0[i + plane] + f * D1[i + plane];
+    const float Dp = (1.f - f) * snap_get<SF>(sr, sh, plane, slot0, 0, i) +
+                     f * snap_get<SF>(sr, sh, plane, slot0 + 1, 0, i);
+    const float Dq = (1.f - f) * snap_get<SF>(sr, sh, plane, slot0, 1, i) +
+                     f * snap_get<SF>(sr, sh, plane, slot0 + 1, 1, i);
     G11[k] += P[k] * Dp;
     G13[k] += P[k] * Dq + Q[k] * Dp;
     G33[k] += Q[k] * Dq;
+}
+
+// ---- Two levels per launch (temporal blocking) -----------------------------
+//
+// A workgroup of 64 x BY threads owns the tile [x0, x0+64) x [z0, z0+BY) of
+// one shot.  It first advances level L+1 on the rectangle
+// R1 = [x0-4, x0+68) x [z0-4, z0+BY+4), which holds every point its own step
+// to L+2 reads, then takes that step on the tile.  Points of R1 outside the
+// shot's interior hold 0 at every level, as in the one-level kernels, whose
+// per-point arithmetic this repeats operation for operation: one two-level
+// launch gives the same bits as two one-level launches.  Neighbouring
+// workgroups each compute the overlap of their rectangles for themselves.
+//
+// Level L-1 is read from (Pa, Qa) and level L from (Pb, Qb); L+1 goes to
+// (Pc, Qc) and L+2 to (Pd, Qd) -- four distinct buffers per field, since
+// neighbours still read the halos of L-1 and L during the launch.
+//
+// HBM per point and two levels: forward 7 reads (two levels, three
+// coefficients) + 4 writes; adjoint the same + the snapshot slots + the
+// gradient once (3 reads, 3 writes) instead of twice.
+
+template <int BY, bool ADJ>
+struct TB2 {
+    float a[BY + 8][80];              // level L: p (forward) or c11 p + c13 q (adjoint); x0-8.., z0-4..
+    float b[BY + 16][72];             // level L: q or c13 p + c33 q; x0-4.., z0-8..
+    float p1[BY + 8][72];             // level L+1 on R1
+    float q1[BY + 8][72];
+    float pp1[ADJ ? BY + 8 : 1][72];  // adjoint: c11 p + c13 q of level L+1 on R1
+    float qq1[ADJ ? BY + 8 : 1][72];  //          c13 p + c33 q
+};
+
+// trace rows and snapshot slots bracketing one adjoint level
+struct LevelB {
+    int row;   // first of the two trace rows
+    float fr;  // weight of the second row
+    int slot;  // first of the two snapshot slots
+    float f;   // weight of the second slot
+};
+
+template <int BY, bool ADJ>
+__device__ __forceinline__ void load_tiles_t2(TB2<BY, ADJ> &t, const float *__restrict__ P,
+                                              const float *__restrict__ Q, const float *__restrict__ c11,
+                                              const float *__restrict__ c13, const float *__restrict__ c33, int nnx,
+                                              int nnz, int cpitch, int x0, int z0, int tid)
+{
+    constexpr int NT = 64 * BY;
+    for (int k = tid; k < (BY + 8) * 80; k += NT) {
+        const int c = k % 80, r = k / 80;
+        const int x = x0 - 8 + c, z = z0 - 4 + r;
+        float v = 0.f;
+        if (x >= 0 && x < nnx && z >= 0 && z < nnz) {
+            const long long i = x + (long long)z * nnx;
+            if (ADJ) {
+                const long long cc = x + (long long)z * cpitch;
+                v = c11[cc] * P[i] + c13[cc] * Q[i];
+            } else {
+                v = P[i];
+            }
+        }
+        t.a[r][c] = v;
+    }
+    for (int k = tid; k < (BY + 16) * 72; k += NT) {
+        const int c = k % 72, r = k / 72;
+        const int x = x0 - 4 + c, z = z0 - 8 + r;
+        float v = 0.f;
+        if (x >= 0 && x < nnx && z >= 0 && z < nnz) {
+            const long long i = x + (long long)z * nnx;
+            if (ADJ) {
+                const long long cc = x + (long long)z * cpitch;
+                v = c13[cc] * P[i] + c33[cc] * Q[i];
+            } else {
+                v = Q[i];
+            }
+        }
+        t.b[r][c] = v;
+    }
+}
+
+// the reference's 9-point second difference along an LDS row (x) and down an
+// LDS column (z), operands in the order of dxx/dzz
+__device__ __forceinline__ float st9_row(const float *a, int c)
+{
+    return st9(a[c], a[c - 1], a[c + 1], a[c - 2], a[c + 2], a[c - 3], a[c + 3], a[c - 4], a[c + 4]);
+}
+template <int W>
+__device__ __forceinline__ float st9_col(const float (*a)[W], int r, int c)
+{
+    return st9(a[r][c], a[r - 1][c], a[r + 1][c], a[r - 2][c], a[r + 2][c], a[r - 3][c], a[r + 3][c], a[r - 4][c],
+               a[r + 4][c]);
+}
+
+// Forward: two forwardC steps with injectSource, abc and the D snapshots of
+// the two levels read (slot1 for level L, slot2 for level L+1; < 0: none).
+template <int BY>
+static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
+    float *__restrict__ Pc, float *__restrict__ Qc, float *__restrict__ Pd, float *__restrict__ Qd,
+    const float *__restrict__ Pa, const float *__restrict__ Qa, const float *__restrict__ Pb,
+    const float *__restrict__ Qb, const float *__restrict__ c11, const float *__restrict__ c13,
+    const float *__restrict__ c33, const float *__restrict__ t0, const ShotB *__restrict__ shots,
+    const int *__restrict__ tile_shot, BatchGeo bg, float dx2, float dz2, float dt2, float src1, float src2,
+    SnapW sw, int slot1, int slot2)
+{
+    constexpr int NT = 64 * BY, N1 = (BY + 8) * 72, NK = (N1 + NT - 1) / NT;
+    __shared__ TB2<BY, false> t;
+    __shared__ float2 red[BY];
+    const ShotB sh = shots[tile_shot[blockIdx.x]];
+    const int tl = blockIdx.x - sh.tile0;
+    const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * BY;
+    const int nnx = sh.nnx, nnz = bg.nnz, tid = threadIdx.x + 64 * threadIdx.y;
+    const long long plane = (long long)nnx * nnz;
+    const float *cc11 = c11 + sh.c_off, *cc13 = c13 + sh.c_off, *cc33 = c33 + sh.c_off;
+    Pa += sh.f_off;
+    Qa += sh.f_off;
+    Pb += sh.f_off;
+    Qb += sh.f_off;
+    // operands of this thread's points of R1 (level L-1, coefficients, taper)
+    float pa[NK], qa[NK], k11[NK], k13[NK], k33[NK], tk[NK];
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+        const int k = tid + j * NT;
+        pa[j] = qa[j] = k11[j] = k13[j] = k33[j] = tk[j] = 0.f;
+        const int x = x0 - 4 + k % 72, z = z0 - 4 + k / 72;
+        if (k < N1 && x >= 4 && x < nnx - 4 && z >= 4 && z < nnz - 4) {
+            const long long i = x + (long long)z * nnx, c = x + (long long)z * bg.cpitch;
+            pa[j] = Pa[i];
+            qa[j] = Qa[i];
+            k11[j] = cc11[c];
+            k13[j] = cc13[c];
+            k33[j] = cc33[c];
+            tk[j] = taper_at(t0, bg.npad, nnx, nnz, x, z);
+        }
+    }
+    // and of its own point of the tile
+    const int x = x0 + threadIdx.x, z = z0 + threadIdx.y;
+    const bool act = x < nnx - 4 && z < nnz - 4;
+    float a11 = 0.f, a13 = 0.f, a33 = 0.f, tap = 0.f;
+    if (act) {
+        const long long c = x + (long long)z * bg.cpitch;
+        a11 = cc11[c];
+        a13 = cc13[c];
+        a33 = cc33[c];
+        tap = taper_at(t0, bg.npad, nnx, nnz, x, z);
+    }
+    load_tiles_t2<BY, false>(t, Pb, Qb, nullptr, nullptr, nullptr, nnx, nnz, 0, x0, z0, tid);
+    __syncthreads();
+    float mp = 0.f, mq = 0.f;
+    // level L+1 on R1
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+        const int k = tid + j * NT;
+        if (k < N1) {
+            const int rx = k % 72, rz = k / 72;
+            const int xx = x0 - 4 + rx, zz = z0 - 4 + rz;
+            float p1 = 0.f, q1 = 0.f;
+            if (xx >= 4 && xx < nnx - 4 && zz >= 4 && zz < nnz - 4) {
+                const long long i = xx + (long long)zz * nnx;
+                const float tp = st9_row(t.a[rz], rx + 4) / dx2;
+                const float tq = st9_col<72>(t.b, rz + 4, rx) / dz2;
+                if (slot1 >= 0 && rx >= 4 && rx < 68 && rz >= 4 && rz < BY + 4) {
+                    snap_put(sw, sh, plane, slot1, i, tp, tq);
+                    mp = fmaxf(mp, fabsf(tp));
+                    mq = fmaxf(mq, fabsf(tq));
+                }
+                float pn = 2.f * t.a[rz][rx + 4] + dt2 * (k11[j] * tp + k13[j] * tq) - pa[j] * tk[j];
+                float qn = 2.f * t.b[rz + 4][rx] + dt2 * (k13[j] * tp + k33[j] * tq) - qa[j] * tk[j];
+                if (i == sh.src) {
+                    pn += src1;
+                    qn += src1;
+                }
+                p1 = pn * tk[j];
+                q1 = qn * tk[j];
+            }
+            t.p1[rz][rx] = p1;
+            t.q1[rz][rx] = q1;
+        }
+    }
+    if (sw.amax && slot1 >= 0) snap_amax(sw, sh, slot1, mp, mq, red);
+    __syncthreads();
+    // level L+2 on the tile
+    mp = 0.f;
+    mq = 0.f;
+    if (act) {
+        const int rx = threadIdx.x + 4, rz = threadIdx.y + 4;
+        const long long i = x + (long long)z * nnx;
+        const float tp = st9_row(t.p1[rz], rx) / dx2;
+        const float tq = st9_col<72>(t.q1, rz, rx) / dz2;
+        if (slot2 >= 0) {
+            snap_put(sw, sh, plane, slot2, i, tp, tq);
+            mp = fabsf(tp);
+            mq = fabsf(tq);
+        }
+        const float p1 = t.p1[rz][rx], q1 = t.q1[rz][rx];
+        float pn = 2.f * p1 + dt2 * (a11 * tp + a13 * tq) - t.a[rz][rx + 4] * tap;
+        float qn = 2.f * q1 + dt2 * (a13 * tp + a33 * tq) - t.b[rz + 4][rx] * tap;
+        if (i == sh.src) {
+            pn += src2;
+            qn += src2;
+        }
+        Pc[sh.f_off + i] = p1;
+        Qc[sh.f_off + i] = q1;
+        Pd[sh.f_off + i] = pn * tap;
+        Qd[sh.f_off + i] = qn * tap;
+    }
+    if (sw.amax && slot2 >= 0) snap_amax(sw, sh, slot2, mp, mq, red);
+}
+
+// Adjoint: two backwardDC steps with the residual injection, abc and
+// gradientCrossCor of both levels, the gradient read and written once.
+template <int BY, int SF>
+static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
+    float *__restrict__ Pc, float *__restrict__ Qc, float *__restrict__ Pd, float *__restrict__ Qd,
+    const float *__restrict__ Pa, const float *__restrict__ Qa, const float *__restrict__ Pb,
+    const float *__restrict__ Qb, const float *__restrict__ c11, const float *__restrict__ c13,
+    const float *__restrict__ c33, const float *__restrict__ t0, const ShotB *__restrict__ shots,
+    const int *__restrict__ tile_shot, BatchGeo bg, const int *__restrict__ recmap, const float *__restrict__ traces,
+    LevelB l1, LevelB l2, SnapR sr, float *__restrict__ G11, float *__restrict__ G13, float *__restrict__ G33,
+    float dx2, float dz2, float dt2)
+{
+    constexpr int NT = 64 * BY, N1 = (BY + 8) * 72, NK = (N1 + NT - 1) / NT;
+    __shared__ TB2<BY, true> t;
+    const ShotB sh = shots[tile_shot[blockIdx.x]];
+    const int tl = blockIdx.x - sh.tile0;
+    const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * BY;
+    const int nnx = sh.nnx, nnz = bg.nnz, tid = threadIdx.x + 64 * threadIdx.y;
+    const long long plane = (long long)nnx * nnz;
+    const float *cc11 = c11 + sh.c_off, *cc13 = c13 + sh.c_off, *cc33 = c33 + sh.c_off;
+    Pa += sh.f_off;
+    Qa += sh.f_off;
+    Pb += sh.f_off;
+    Qb += sh.f_off;
+    // operands of this thread's points of R1: levels L and L-1, coefficients,
+    // taper, injected residual
+    float pb[NK], qb[NK], pa[NK], qa[NK], k11[NK], k13[NK], k33[NK], tk[NK], inj[NK];
+    bool rec[NK];
+    const float *tr1 = traces + sh.r_off + (long long)l1.row * sh.nr;
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+        const int k = tid + j * NT;
+        pb[j] = qb[j] = pa[j] = qa[j] = k11[j] = k13[j] = k33[j] = tk[j] = inj[j] = 0.f;
+        rec[j] = false;
+        const int x = x0 - 4 + k % 72, z = z0 - 4 + k / 72;
+        if (k < N1 && x >= 4 && x < nnx - 4 && z >= 4 && z < nnz - 4) {
+            const long long i = x + (long long)z * nnx, c = x + (long long)z * bg.cpitch;
+            pb[j] = Pb[i];
+            qb[j] = Qb[i];
+            pa[j] = Pa[i];
+            qa[j] = Qa[i];
+            k11[j] = cc11[c];
+            k13[j] = cc13[c];
+            k33[j] = cc33[c];
+            tk[j] = taper_at(t0, bg.npad, nnx, nnz, x, z);
+            const int ir = (z >= sh.rz0 && z <= sh.rz1) ? recmap[sh.f_off + i] : -1;
+            if (ir >= 0) {
+                rec[j] = true;
+                inj[j] = 0.5f * ((1.f - l1.fr) * tr1[ir] + l1.fr * tr1[ir + sh.nr]);
+            }
+        }
+    }
+    // and of its own point of the tile: level L, taper, residual, the
+    // snapshots of both levels, the gradient
+    const int x = x0 + threadIdx.x, z = z0 + threadIdx.y;
+    const bool act = x < nnx - 4 && z < nnz - 4;
+    const long long i = x + (long long)z * nnx, kk = sh.f_off + i;
+    float pbc = 0.f, qbc = 0.f, tap = 0.f, inj2 = 0.f, g11 = 0.f, g13 = 0.f, g33 = 0.f;
+    float a0p = 0.f, a1p = 0.f, a0q = 0.f, a1q = 0.f, b0p = 0.f, b1p = 0.f, b0q = 0.f, b1q = 0.f;
+    int ir2 = -1;
+    if (act) {
+        pbc = Pb[i];
+        qbc = Qb[i];
+        tap = taper_at(t0, bg.npad, nnx, nnz, x, z);
+        ir2 = (z >= sh.rz0 && z <= sh.rz1) ? recmap[kk] : -1;
+        if (ir2 >= 0) {
+            const float *tr2 = traces + sh.r_off + (long long)l2.row * sh.nr;
+            inj2 = 0.5f * ((1.f - l2.fr) * tr2[ir2] + l2.fr * tr2[ir2 + sh.nr]);
+        }
+        a0p = snap_get<SF>(sr, sh, plane, l1.slot, 0, i);
+        a1p = snap_get<SF>(sr, sh, plane, l1.slot + 1, 0, i);
+        a0q = snap_get<SF>(sr, sh, plane, l1.slot, 1, i);
+        a1q = snap_get<SF>(sr, sh, plane, l1.slot + 1, 1, i);
+        b0p = snap_get<SF>(sr, sh, plane, l2.slot, 0, i);
+        b1p = snap_get<SF>(sr, sh, plane, l2.slot + 1, 0, i);
+        b0q = snap_get<SF>(sr, sh, plane, l2.slot, 1, i);
+        b1q = snap_get<SF>(sr, sh, plane, l2.slot + 1, 1, i);
+        g11 = G11[kk];
+        g13 = G13[kk];
+        g33 = G33[kk];
+    }
+    load_tiles_t2<BY, true>(t, Pb, Qb, cc11, cc13, cc33, nnx, nnz, bg.cpitch, x0, z0, tid);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+        const int k = tid + j * NT;
+        if (k < N1) {
+            const int rx = k % 72, rz = k / 72;
+            const int xx = x0 - 4 + rx, zz = z0 - 4 + rz;
+            float p1 = 0.f, q1 = 0.f, pp = 0.f, qq = 0.f;
+            if (xx >= 4 && xx < nnx - 4 && zz >= 4 && zz < nnz - 4) {
+                const float tp0 = st9_row(t.a[rz], rx + 4) / dx2;
+                const float tq0 = st9_col<72>(t.b, rz + 4, rx) / dz2;
+                float pn = 2.f * pb[j] + dt2 * tp0 - pa[j] * tk[j];
+                float qn = 2.f * qb[j] + dt2 * tq0 - qa[j] * tk[j];
+                if (rec[j]) {
+                    pn += dt2 * inj[j];
+                    qn += dt2 * inj[j];
+                }
+                p1 = pn * tk[j];
+                q1 = qn * tk[j];
+                pp = k11[j] * p1 + k13[j] * q1;
+                qq = k13[j] * p1 + k33[j] * q1;
+            }
+            t.p1[rz][rx] = p1;
+            t.q1[rz][rx] = q1;
+            t.pp1[rz][rx] = pp;
+            t.qq1[rz][rx] = qq;
+        }
+    }
+    __syncthreads();
+    if (!act) return;
+    const int rx = threadIdx.x + 4, rz = threadIdx.y + 4;
+    const float tp0 = st9_row(t.pp1[rz], rx) / dx2;
+    const float tq0 = st9_col<72>(t.qq1, rz, rx) / dz2;
+    const float p1 = t.p1[rz][rx], q1 = t.q1[rz][rx];
+    float pn = 2.f * p1 + dt2 * tp0 - pbc * tap;
+    float qn = 2.f * q1 + dt2 * tq0 - qbc * tap;
+    if (ir2 >= 0) {
+        pn += dt2 * inj2;
+        qn += dt2 * inj2;
+    }
+    pn = pn * tap;
+    qn = qn * tap;
+    Pc[kk] = p1;
+    Qc[kk] = q1;
+    Pd[kk] = pn;
+    Qd[kk] = qn;
+    const float Dp1 = (1.f - l1.f) * a0p + l1.f * a1p;
+    const float Dq1 = (1.f - l1.f) * a0q + l1.f * a1q;
+    const float Dp2 = (1.f - l2.f) * b0p + l2.f * b1p;
+    const float Dq2 = (1.f - l2.f) * b0q + l2.f * b1q;
+    g11 += p1 * Dp1;
+    g13 += p1 * Dq1 + q1 * Dp1;
+    g33 += q1 * Dq1;
+    g11 += pn * Dp2;
+    g13 += pn * Dq2 + qn * Dp2;
+    g33 += qn * Dq2;
+    G11[kk] = g11;
+    G13[kk] = g13;
+    G33[kk] = g33;
 }
 
 }  // namespace vti

@@ -96,7 +96,7 @@ _SIGS = {
     "vti_check_eps_del": (c_int, [P, P, c_float, c_float, c_longlong, P]),
     "vti2d_objfunc_gradient_batch": (
         c_int,
-        [P] * 11 + [c_int, P] + [c_int] * 5 + [c_float] * 4 + [c_int],
+        [P] * 11 + [c_int, P] + [c_int] * 5 + [c_float] * 4 + [c_int] * 4,
     ),
     "vti_lbfgs_create": (c_int, [POINTER(c_void_p), c_longlong, c_int, c_int]),
     "vti_lbfgs_destroy": (c_int, [P]),
@@ -497,10 +497,16 @@ class LBFGS:
             pass
 
 
+SNAP_FORMATS = {"fp32": 0, "fp16": 1, "bf16": 2}
+
+
 def objfunc_gradient2d_batch(ctx, data, c11, c13, c33, wavelet, shots, rloc, NX, nz, nt, npad, dx, dz, dt, rate,
-                             max_wave=0):
+                             max_wave=0, levels=0, tile_rows=0, snap="fp32"):
     """Batched objective + gradient: shots = (ns, 6) ints {c0, nx, ix, iz, nr, r0};
-    data = the shots' traces concatenated.  Returns (objective, gc (3, nnz, NNX) float64)."""
+    data = the shots' traces concatenated.  levels (1 or 2 per launch) and
+    tile_rows (4, 8, 16) change only speed; snap = "fp32" (bit-exact with the
+    one-shot driver), "fp16" or "bf16" (per-slice scaled 16-bit snapshots).
+    Returns (objective, gc (3, nnz, NNX) float64)."""
     c11, c13, c33, wavelet, data = (f32(a).ravel() for a in (c11, c13, c33, wavelet, data))
     shots, rloc = _i32(shots), _i32(rloc)
     NNX, nnz = NX + 2 * npad, nz + 2 * npad
@@ -508,6 +514,7 @@ def objfunc_gradient2d_batch(ctx, data, c11, c13, c33, wavelet, shots, rloc, NX,
     f = ctypes.c_double(0.0)
     check(_lib.vti2d_objfunc_gradient_batch(ctx.h, ctypes.byref(f), ptr(gc[0]), ptr(gc[1]), ptr(gc[2]), ptr(data),
                                             ptr(c11), ptr(c13), ptr(c33), ptr(wavelet), ptr(shots), shots.size // 6,
-                                            ptr(rloc), rloc.size // 2, NX, nz, nt, npad, dx, dz, dt, rate, max_wave),
+                                            ptr(rloc), rloc.size // 2, NX, nz, nt, npad, dx, dz, dt, rate, max_wave,
+                                            levels, tile_rows, SNAP_FORMATS[snap]),
           "vti2d_objfunc_gradient_batch")
     return f.value, gc
