@@ -129,9 +129,11 @@ def oracle_f64(orc, vti, B, data):
     return g
 
 
-# Measured on MI355X (zero data, 5 shots): FP32 snapshots 1.1e-5 from the FP64
-# oracle (the reference's FP32 stencil itself), FP16 TBD, BF16 TBD.
-@pytest.mark.parametrize("snap,tol", [("fp16", 1e-3), ("bf16", 1e-2)])
+# Measured on MI355X (zero data, 5 shots, relative L2 of the summed gradient
+# against the FP64 oracle): FP32 snapshots 1.4e-5 (the FP32 stencil itself),
+# FP16 1.1e-4, BF16 9.8e-4 -- the formats' unit roundoffs 2^-11 and 2^-8
+# scaled down by averaging over the imaging sum.
+@pytest.mark.parametrize("snap,tol", [("fp16", 3e-4), ("bf16", 3e-3)])
 def test_16bit_snapshots_vs_fp64_oracle(vti, orc, ctx, snap, tol):
     B = batch_problem(vti, seed=12)
     nnt = (B["nt"] - 1) // 4 + 1
