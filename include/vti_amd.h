@@ -242,6 +242,44 @@ int vti_lbfgs_create(vti_lbfgs **state, long long n, int m, int unit_step);
 int vti_lbfgs_destroy(vti_lbfgs *state);
 int vti_lbfgs_step(vti_lbfgs *state, float *x, double f, const float *g, int *iflag, int *iter, int *nfun);
 
+/* ---- request server ------------------------------------------------------------
+ * No counterpart in the reference, whose drivers run one call at a time.
+ * A server on one device runs 2D gradient and 3D modeling requests of any
+ * grid size as they arrive: `slots` worker threads (1..16), each with its own
+ * context and stream, take queued requests in arrival order as soon as a slot
+ * is free and the request's device-memory estimate fits in what the budget
+ * (bytes; <= 0: the free memory less 2 GiB) leaves after the resident models
+ * and the requests in flight; a request that would not fit even alone is
+ * refused when submitted.  Requests on different slots run concurrently on
+ * the device.  Each request's result equals that of the same driver call
+ * made alone.  Models are uploaded once (put_model: c11, c13, c33 of n
+ * points, the layout the drivers take) and stay resident until dropped.
+ * Every host buffer a request names must stay valid until vti_serve_wait
+ * returns for it.  Latency = completion - submission; vti_serve_stats gives
+ * the p50/p99 over completed requests (nearest rank), the completion rate
+ * (completed / (last completion - first submission)) and the peak of resident
+ * plus in-flight bytes. */
+typedef struct vti_server vti_server;
+int vti_serve_create(vti_server **out, int device, int slots, double budget_bytes);
+int vti_serve_destroy(vti_server *s); /* drains the queue first */
+int vti_serve_put_model(vti_server *s, const float *c11, const float *c13, const float *c33, long long n, int *handle);
+int vti_serve_drop_model(vti_server *s, int handle);
+/* vti2d_objfunc_gradient's arguments (ot = 0) on a resident model */
+int vti_serve_grad2d(vti_server *s, long long *id, int model, float *objfunc, float *gc11, float *gc13, float *gc33,
+                     const float *data, const float *wavelet, const int *sloc, int ns, const int *rloc, int nr,
+                     const float *taper, int nx, int nz, int nt, int npad, float dx, float dz, float dt, float rate);
+/* vti3d_model_data's arguments (ngpu = 1) on a resident model; nr = the
+ * number of receivers in recloc */
+int vti_serve_model3d(vti_server *s, long long *id, int model, float *ddata, const float *souloc, int ns,
+                      const int *shotid, int nshot, const float *recloc, int nr, const float *wavelet, int nx, int ny,
+                      int nz, int nt, int npad, float ox, float oy, float oz, float ot, float dx, float dy, float dz,
+                      float dt, float samplingRate);
+/* blocks until request `id` is done; returns its status (vti_last_error()
+ * then holds its message) and its latency and service time in seconds */
+int vti_serve_wait(vti_server *s, long long id, double *latency_s, double *service_s);
+int vti_serve_stats(vti_server *s, double *p50, double *p99, double *req_per_s, long long *completed,
+                    double *peak_bytes);
+
 #ifdef __cplusplus
 }
 #endif

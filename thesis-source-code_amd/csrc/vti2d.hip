@@ -127,9 +127,11 @@ int setup(Call2 &c, vti_ctx *ctx, int nx, int nz, int nt, int npad, float dx, fl
 
 bool interior(const Call2 &c, int ix, int iz) { return ix >= 4 && ix < c.g.nnx - 4 && iz >= 4 && iz < c.g.nnz - 4; }
 
+// copy to the device from host memory or, for a model kept resident by the
+// caller, from device memory (the direction is taken from the pointers)
 int upload(const Call2 &c, void *dst, const void *src, size_t bytes)
 {
-    VTI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.ctx->stream));
+    VTI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c.ctx->stream));
     return vti::kOk;
 }
 

@@ -178,30 +178,36 @@ int max_zchunk(const Geo3 &g)
     return (int)std::min<long long>(1 << 20, (planes - 8) / 4 * 4);
 }
 
-void freep(void *p)
-{
-    if (p) (void)hipFree(p);
-}
+void freep(void *p, hipStream_t s) { vti::dfree(p, s); }
+
+// stream-ordered allocation on the propagator's stream, returning on failure
+#define DMALLOC(ptr, bytes)                                                      \
+    do {                                                                         \
+        const int rc_ = vti::dmalloc((void **)(ptr), (bytes), p->ctx->stream); \
+        if (rc_) return rc_;                                                     \
+    } while (0)
 
 int prop_free(vti3d_prop *p)
 {
     if (!p) return vti::kOk;
     (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
+    hipStream_t s = p->ctx->stream;
     for (int k = 0; k < 3; ++k) {
-        freep(p->cm[0][k]);
-        freep(p->cm[1][k]);
-        freep(p->acc[k]);
-        freep(p->acc64[k]);
+        freep(p->cm[0][k], s);
+        freep(p->cm[1][k], s);
+        freep(p->acc[k], s);
+        freep(p->acc64[k], s);
     }
-    freep(p->dxy);
-    freep(p->dzv);
+    freep(p->dxy, s);
+    freep(p->dzv, s);
     for (int k = 0; k < 2; ++k) {
-        freep(p->sx[k]);
-        freep(p->sz[k]);
-        freep(p->lx[k]);
-        freep(p->lz[k]);
+        freep(p->sx[k], s);
+        freep(p->sz[k], s);
+        freep(p->lx[k], s);
+        freep(p->lz[k], s);
     }
+    (void)hipStreamSynchronize(s);
     delete p;
     return vti::kOk;
 }
@@ -210,12 +216,8 @@ template <class T>
 int dalloc(vti3d_prop *p, T **ptr, size_t n)
 {
     if (*ptr) return vti::kOk;
-    hipError_t e = hipMalloc((void **)ptr, n * sizeof(T));
-    if (e != hipSuccess) {
-        *ptr = nullptr;
-        vti::set_error("hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
-        return vti::kErrNoMem;
-    }
+    const int rc = vti::dmalloc((void **)ptr, n * sizeof(T), p->ctx->stream);
+    if (rc) return rc;
     p->bytes += n * sizeof(T);
     VTI_HIP(hipMemsetAsync(*ptr, 0, n * sizeof(T), p->ctx->stream));
     return vti::kOk;
@@ -239,11 +241,13 @@ int ensure_backward(vti3d_prop *p, bool gradient, bool random_model)
 }
 
 // host (reference layout, pitch nx) -> device (pitched)
+// src in host memory or, for a model kept resident by the caller, in device
+// memory (the direction is taken from the pointers)
 int upload_volume(vti3d_prop *p, float *dst, const float *src)
 {
     const Geo3 &g = p->g;
     VTI_HIP(hipMemcpy2DAsync(dst, (size_t)g.pitch * 4, src, (size_t)g.nx * 4, (size_t)g.nx * 4,
-                             (size_t)g.ny * g.nz, hipMemcpyHostToDevice, p->ctx->stream));
+                             (size_t)g.ny * g.nz, hipMemcpyDefault, p->ctx->stream));
     return vti::kOk;
 }
 
@@ -413,15 +417,17 @@ struct ShotDev {
     float *obs = nullptr;      // trace-major [nr][nnt]
     unsigned char *live = nullptr;
     double *partial = nullptr;
+    hipStream_t s;  // the stream that uses (and releases) them
+    explicit ShotDev(hipStream_t st) : s(st) {}
     ~ShotDev()
     {
-        freep(rec);
-        freep(rec_true);
-        freep(rec_eff);
-        freep(tr);
-        freep(obs);
-        freep(live);
-        freep(partial);
+        freep(rec, s);
+        freep(rec_true, s);
+        freep(rec_eff, s);
+        freep(tr, s);
+        freep(obs, s);
+        freep(live, s);
+        freep(partial, s);
     }
 };
 
@@ -429,7 +435,8 @@ int to_dev_idx(vti3d_prop *p, long long **dst, const std::vector<long long> &ref
 {
     std::vector<long long> d(ref.size());
     for (size_t i = 0; i < ref.size(); ++i) d[i] = dev_index(p->g, ref[i]);
-    VTI_HIP(hipMalloc((void **)dst, sizeof(long long) * std::max<size_t>(1, d.size())));
+    int rc = vti::dmalloc((void **)dst, sizeof(long long) * std::max<size_t>(1, d.size()), p->ctx->stream);
+    if (rc) return rc;
     if (!d.empty())
         VTI_HIP(hipMemcpyAsync(*dst, d.data(), sizeof(long long) * d.size(), hipMemcpyHostToDevice, p->ctx->stream));
     return vti::kOk;
@@ -721,14 +728,14 @@ int vti3d_prop_forward_shot(vti3d_prop *p, int lmax, const float *wavelet, long 
     (void)hipGetLastError();  // drop a stale error left by an unrelated earlier call
     const Geo3 &g = p->g;
     VTI_CHECK(sou_index >= 0 && sou_index < (long long)g.nx * g.ny * g.nz, "source index out of range");
-    ShotDev sd;
+    ShotDev sd(p->ctx->stream);
     if (nr > 0 && traces_tm) {
         std::vector<long long> r(rec, rec + nr);
         for (int i = 0; i < nr; ++i)
             VTI_CHECK(r[i] >= 0 && r[i] < (long long)g.nx * g.ny * g.nz, "receiver %d index out of range", i);
         int rc = to_dev_idx(p, &sd.rec_eff, r);
         if (rc) return rc;
-        VTI_HIP(hipMalloc((void **)&sd.tr, sizeof(float) * (size_t)nr * nnt));
+        DMALLOC(&sd.tr, sizeof(float) * (size_t)nr * nnt);
         VTI_HIP(hipMemcpyAsync(sd.tr, traces_tm, sizeof(float) * (size_t)nr * nnt, hipMemcpyHostToDevice,
                                p->ctx->stream));
     }
@@ -785,33 +792,34 @@ int modeled_shot(vti3d_prop *p, const DriverArgs &a, const Shot3 &sh, ShotDev &s
     rc = to_dev_idx(p, &sd.rec_true, sh.rec_true);
     if (rc) return rc;
     const size_t nn = (size_t)std::max(1, sh.nr) * nnt;
-    VTI_HIP(hipMalloc((void **)&sd.tr, sizeof(float) * nn));
+    DMALLOC(&sd.tr, sizeof(float) * nn);
     VTI_HIP(hipMemsetAsync(sd.tr, 0, sizeof(float) * nn, p->ctx->stream));
     rc = forward_pass(p, true, lmax, a.wavelet, sh.sou_ref, &sd, sh.nr, sts, nnt);
     if (rc) return rc;
     std::vector<unsigned char> live(std::max(1, sh.nr));
     for (int ir = 0; ir < sh.nr; ++ir) live[ir] = a.recloc[4 * (ir + sh.irbegin) + 3] != 0.f;
-    VTI_HIP(hipMalloc((void **)&sd.live, live.size()));
+    DMALLOC(&sd.live, live.size());
     VTI_HIP(hipMemcpyAsync(sd.live, live.data(), live.size(), hipMemcpyHostToDevice, p->ctx->stream));
     const int nblk = launch_blocks((long long)sh.nr * nnt);
-    VTI_HIP(hipMalloc((void **)&sd.partial, sizeof(double) * nblk));
+    DMALLOC(&sd.partial, sizeof(double) * nblk);
     if (obs) {
-        VTI_HIP(hipMalloc((void **)&sd.obs, sizeof(float) * nn));
+        DMALLOC(&sd.obs, sizeof(float) * nn);
         VTI_HIP(hipMemcpyAsync(sd.obs, obs + (size_t)nnt * sh.irbegin, sizeof(float) * (size_t)sh.nr * nnt,
                                hipMemcpyHostToDevice, p->ctx->stream));
     } else {
-        VTI_HIP(hipMalloc((void **)&sd.obs, sizeof(float) * nn));
+        DMALLOC(&sd.obs, sizeof(float) * nn);
         VTI_HIP(hipMemsetAsync(sd.obs, 0, sizeof(float) * nn, p->ctx->stream));
     }
     float *gain = nullptr;
+    std::vector<float> g;
     if (tpow != 0.f) {
-        std::vector<float> g(nnt);
+        g.resize(nnt);
         for (int it = 0; it < nnt; ++it) {
             const float t = a.ot + it * a.rate;
             g[it] = std::pow(t, tpow);
         }
-        VTI_HIP(hipMalloc((void **)&gain, sizeof(float) * nnt));
-        VTI_HIP(hipMemcpy(gain, g.data(), sizeof(float) * nnt, hipMemcpyHostToDevice));
+        DMALLOC(&gain, sizeof(float) * nnt);
+        VTI_HIP(hipMemcpyAsync(gain, g.data(), sizeof(float) * nnt, hipMemcpyHostToDevice, p->ctx->stream));
     }
     hipLaunchKernelGGL(residual3d_kernel, dim3(nblk), dim3(256), 0, p->ctx->stream, sd.tr, sd.obs, sd.live, sh.nr,
                        nnt, gain, objtype == 0 ? 1 : 0, sd.partial);
@@ -823,17 +831,17 @@ int modeled_shot(vti3d_prop *p, const DriverArgs &a, const Shot3 &sh, ShotDev &s
         // the AGC objective (objFuncGradient3d_f.cu:770-773): traces become the adjoint source
         float *rsb = nullptr, *drb = nullptr;
         double *fi = nullptr;
-        VTI_HIP(hipMalloc((void **)&rsb, sizeof(float) * nn));
-        VTI_HIP(hipMalloc((void **)&drb, sizeof(float) * nn));
-        VTI_HIP(hipMalloc((void **)&fi, sizeof(double) * sh.nr));
+        DMALLOC(&rsb, sizeof(float) * nn);
+        DMALLOC(&drb, sizeof(float) * nn);
+        DMALLOC(&fi, sizeof(double) * sh.nr);
         hipLaunchKernelGGL(agc3d_kernel, dim3((sh.nr + 63) / 64), dim3(64), 0, p->ctx->stream, sd.tr, sd.obs, sh.nr,
                            nnt, halfwidth, rsb, drb, fi);
         VTI_HIP(hipGetLastError());
         VTI_HIP(hipMemcpyAsync(part.data(), fi, sizeof(double) * sh.nr, hipMemcpyDeviceToHost, p->ctx->stream));
         VTI_HIP(hipStreamSynchronize(p->ctx->stream));
-        (void)hipFree(rsb);
-        (void)hipFree(drb);
-        (void)hipFree(fi);
+        freep(rsb, p->ctx->stream);
+        freep(drb, p->ctx->stream);
+        freep(fi, p->ctx->stream);
     }
     if (traces_out) {
         traces_out->resize((size_t)sh.nr * nnt);
@@ -841,7 +849,7 @@ int modeled_shot(vti3d_prop *p, const DriverArgs &a, const Shot3 &sh, ShotDev &s
                                p->ctx->stream));
     }
     VTI_HIP(hipStreamSynchronize(p->ctx->stream));
-    if (gain) (void)hipFree(gain);
+    freep(gain, p->ctx->stream);
     double s = 0;
     for (double v : part) s += v;
     *ss = s;
@@ -919,7 +927,7 @@ int vti3d_model_data(vti_ctx *ctx, float *ddata, const float *souloc, int ns, co
         Shot3 sh;
         rc = shot_setup(sh, souloc, shotid[k], recloc, 4, nx, ny, nz, ox, oy, oz, dx, dy, dz);
         if (rc) return rc;
-        ShotDev sd;
+        ShotDev sd(ctx->stream);
         std::vector<float> tr;
         double ss = 0;
         rc = modeled_shot(G.p, a, sh, sd, lmax, sts, nnt, nullptr, 0.f, 0, 0, &ss, &tr);
@@ -973,7 +981,7 @@ static int objfunc_impl(vti_ctx *ctx, float *fgcij, bool grad, const float *soul
         Shot3 sh;
         rc = shot_setup(sh, souloc, shotid[k], recloc, 4, nx, ny, nz, ox, oy, oz, dx, dy, dz);
         if (rc) return rc;
-        ShotDev sd;
+        ShotDev sd(p->ctx->stream);
         double ss = 0;
         p->active = 0;
         rc = modeled_shot(p, a, sh, sd, lmax, sts, nnt, obs, tpow, objtype, halfwidth, &ss, nullptr);
@@ -1068,7 +1076,7 @@ int vti3d_rtm(vti_ctx *ctx, float *image, const float *souloc, int ns, const int
         Shot3 sh;
         rc = shot_setup(sh, souloc, shotid[k], recloc, 3, nx, ny, nz, ox, oy, oz, dx, dy, dz);
         if (rc) return rc;
-        ShotDev sd;
+        ShotDev sd(p->ctx->stream);
         rc = to_dev_idx(p, &sd.rec_eff, sh.rec_eff);
         if (rc) return rc;
         rc = to_dev_idx(p, &sd.rec_true, sh.rec_true);
@@ -1079,7 +1087,7 @@ int vti3d_rtm(vti_ctx *ctx, float *image, const float *souloc, int ns, const int
         const float *ob = obs + (size_t)nnt * sh.irbegin;
         for (int ir = 0; ir < sh.nr; ++ir)
             for (int it = 0; it < nnt; ++it) tm[(size_t)it * sh.nr + ir] = ob[(size_t)ir * nnt + it];
-        VTI_HIP(hipMalloc((void **)&sd.tr, sizeof(float) * nn));
+        DMALLOC(&sd.tr, sizeof(float) * nn);
         VTI_HIP(hipMemcpyAsync(sd.tr, tm.data(), sizeof(float) * nn, hipMemcpyHostToDevice, p->ctx->stream));
         rc = zero_fields(p, true, true);
         if (rc) return rc;

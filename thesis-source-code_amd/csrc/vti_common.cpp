@@ -2,6 +2,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "vti_ctx.h"
 #include "vti_internal.h"
@@ -19,6 +20,41 @@ void set_error(const char *fmt, ...)
 }
 
 const char *get_error() { return g_err; }
+
+namespace {
+std::mutex g_pool_mu;
+bool g_pool_ready[256];
+}  // namespace
+
+int dmalloc(void **p, size_t bytes, hipStream_t s)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev >= 0 && dev < 256) {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        if (!g_pool_ready[dev]) {
+            hipMemPool_t pool;
+            if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+                uint64_t keep = UINT64_MAX;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            }
+            g_pool_ready[dev] = true;
+        }
+    }
+    const hipError_t e = hipMallocAsync(p, bytes ? bytes : 1, s);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        set_error("hipMallocAsync(%zu bytes): %s", bytes, hipGetErrorString(e));
+        return kErrNoMem;
+    }
+    return kOk;
+}
+
+void dfree(void *p, hipStream_t s)
+{
+    if (p) (void)hipFreeAsync(p, s);
+}
 
 }  // namespace vti
 

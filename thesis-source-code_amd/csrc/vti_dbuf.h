@@ -1,4 +1,5 @@
-// RAII device buffer, zero-filled on allocation (host code of the drivers).
+// RAII device buffer, zero-filled on allocation (host code of the drivers);
+// stream-ordered: allocated and released on the stream that uses it.
 #pragma once
 
 #include <algorithm>
@@ -14,23 +15,18 @@ template <class T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    hipStream_t s = nullptr;
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    ~DBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-    int alloc(size_t count, hipStream_t s)
+    ~DBuf() { dfree(p, s); }
+    int alloc(size_t count, hipStream_t st)
     {
         n = std::max<size_t>(count, 1);
-        hipError_t e = hipMalloc((void **)&p, n * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            vti::set_error("hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
-            return vti::kErrNoMem;
-        }
-        e = hipMemsetAsync(p, 0, n * sizeof(T), s);
+        s = st;
+        int rc = dmalloc((void **)&p, n * sizeof(T), st);
+        if (rc) return rc;
+        const hipError_t e = hipMemsetAsync(p, 0, n * sizeof(T), st);
         if (e != hipSuccess) {
             vti::set_error("hipMemsetAsync: %s", hipGetErrorString(e));
             return vti::kErrHip;

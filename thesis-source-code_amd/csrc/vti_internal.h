@@ -36,6 +36,15 @@ enum Status : int {
 void set_error(const char *fmt, ...);
 const char *get_error();
 
+// Device memory of the drivers, stream-ordered: allocated and released on the
+// stream that uses it (hipMallocAsync / hipFreeAsync) from the device's
+// default pool, whose release threshold is raised so that freed blocks stay
+// cached for the next call.  Unlike hipMalloc / hipFree neither waits for the
+// work of other streams, so calls on different streams overlap.  dmalloc
+// returns kOk, or kErrNoMem with the error message set.
+int dmalloc(void **p, size_t bytes, hipStream_t s);
+void dfree(void *p, hipStream_t s);
+
 }  // namespace vti
 
 #define VTI_HIP(call)                                                                       \

@@ -101,6 +101,16 @@ _SIGS = {
     "vti_lbfgs_create": (c_int, [POINTER(c_void_p), c_longlong, c_int, c_int]),
     "vti_lbfgs_destroy": (c_int, [P]),
     "vti_lbfgs_step": (c_int, [P, P, ctypes.c_double, P, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+    "vti_serve_create": (c_int, [P, c_int, c_int, ctypes.c_double]),
+    "vti_serve_destroy": (c_int, [P]),
+    "vti_serve_put_model": (c_int, [P, P, P, P, c_longlong, POINTER(c_int)]),
+    "vti_serve_drop_model": (c_int, [P, c_int]),
+    "vti_serve_grad2d": (c_int, [P, POINTER(c_longlong), c_int] + [P] * 7 + [c_int, P, c_int, P] + [c_int] * 4
+                         + [c_float] * 4),
+    "vti_serve_model3d": (c_int, [P, POINTER(c_longlong), c_int, P, P, c_int, P, c_int, P, c_int, P] + [c_int] * 5
+                          + [c_float] * 9),
+    "vti_serve_wait": (c_int, [P, c_longlong, POINTER(ctypes.c_double), POINTER(ctypes.c_double)]),
+    "vti_serve_stats": (c_int, [P] + [POINTER(ctypes.c_double)] * 3 + [POINTER(c_longlong), POINTER(ctypes.c_double)]),
     "vti2d_objfunc_gradient_vepsdel": (
         c_int,
         [P] * 9 + [c_float] * 3 + [P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 6 + [P],
@@ -518,3 +528,95 @@ def objfunc_gradient2d_batch(ctx, data, c11, c13, c33, wavelet, shots, rloc, NX,
                                             levels, tile_rows, SNAP_FORMATS[snap]),
           "vti2d_objfunc_gradient_batch")
     return f.value, gc
+
+
+class Ticket:
+    """A submitted server request: its id, its output arrays and the inputs
+    the server reads until it is done."""
+
+    def __init__(self, rid, kind, out, keep):
+        self.id, self.kind, self.out, self._keep = rid, kind, out, keep
+
+
+class Server:
+    """Request server on one device (vti_serve_*): `slots` concurrent
+    workers, admission by device-memory budget, resident models.  Submit
+    with grad2d / model3d, collect with wait."""
+
+    def __init__(self, device=0, slots=4, budget_bytes=0.0):
+        h = ctypes.c_void_p()
+        check(_lib.vti_serve_create(ctypes.byref(h), device, slots, float(budget_bytes)), "vti_serve_create")
+        self.h = h
+        self._models = {}
+
+    def close(self):
+        if self.h:
+            check(_lib.vti_serve_destroy(self.h), "vti_serve_destroy")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def put_model(self, c11, c13, c33):
+        """Upload c11, c13, c33 once; returns the model handle."""
+        c = [f32(a).ravel() for a in (c11, c13, c33)]
+        assert c[0].size == c[1].size == c[2].size
+        hd = c_int()
+        check(_lib.vti_serve_put_model(self.h, ptr(c[0]), ptr(c[1]), ptr(c[2]), c[0].size, ctypes.byref(hd)),
+              "vti_serve_put_model")
+        self._models[hd.value] = c[0].size
+        return hd.value
+
+    def drop_model(self, handle):
+        check(_lib.vti_serve_drop_model(self.h, handle), "vti_serve_drop_model")
+        self._models.pop(handle, None)
+
+    def grad2d(self, model, data, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate):
+        """objFuncGradient_f on a resident model; wait() returns (objective, gc (3, nnz, nnx))."""
+        data, wavelet, taper = (f32(a).ravel() for a in (data, wavelet, taper))
+        sloc, rloc = _i32(sloc), _i32(rloc)
+        gc = np.zeros((3, nz + 2 * npad, nx + 2 * npad), np.float32)
+        f = np.zeros(1, np.float32)
+        rid = c_longlong()
+        check(_lib.vti_serve_grad2d(self.h, ctypes.byref(rid), model, ptr(f), ptr(gc[0]), ptr(gc[1]), ptr(gc[2]),
+                                    ptr(data), ptr(wavelet), ptr(sloc), sloc.size // 4, ptr(rloc), rloc.size // 2,
+                                    ptr(taper), nx, nz, nt, npad, dx, dz, dt, rate), "vti_serve_grad2d")
+        return Ticket(rid.value, "grad2d", (f, gc), (data, wavelet, taper, sloc, rloc))
+
+    def model3d(self, model, souloc, shotid, recloc, wavelet, shape, nt, npad, o, d, dt, rate):
+        """modelDataCij3d_f on a resident model of shape (nz, ny, nx); wait()
+        returns the trace-major data (nr_total * nnt)."""
+        souloc, shotid, recloc, wavelet = _geom3(souloc, shotid, recloc, wavelet)
+        nz, ny, nx = shape
+        sts = int(np.round(np.float32(rate) / np.float32(dt)))
+        nnt = (nt - 1) // sts + 1
+        nrt = recloc.size // 4
+        dd = np.zeros(nrt * nnt, np.float32)
+        rid = c_longlong()
+        check(_lib.vti_serve_model3d(self.h, ctypes.byref(rid), model, ptr(dd), ptr(souloc), souloc.size // 5,
+                                     ptr(shotid), shotid.size, ptr(recloc), nrt, ptr(wavelet), nx, ny, nz, nt, npad,
+                                     *o, 0.0, *d, dt, rate), "vti_serve_model3d")
+        return Ticket(rid.value, "model3d", dd, (souloc, shotid, recloc, wavelet))
+
+    def wait(self, ticket):
+        """Block until the request is done: (result, latency_s, service_s)."""
+        lat, svc = ctypes.c_double(), ctypes.c_double()
+        check(_lib.vti_serve_wait(self.h, ticket.id, ctypes.byref(lat), ctypes.byref(svc)), "vti_serve_wait")
+        if ticket.kind == "grad2d":
+            f, gc = ticket.out
+            res = (float(f[0]), gc)
+        else:
+            res = ticket.out
+        ticket._keep = None
+        return res, lat.value, svc.value
+
+    def stats(self):
+        p50, p99, rate, peak = (ctypes.c_double() for _ in range(4))
+        n = c_longlong()
+        check(_lib.vti_serve_stats(self.h, ctypes.byref(p50), ctypes.byref(p99), ctypes.byref(rate), ctypes.byref(n),
+                                   ctypes.byref(peak)), "vti_serve_stats")
+        return {"p50_s": p50.value, "p99_s": p99.value, "req_per_s": rate.value, "completed": n.value,
+                "peak_bytes": peak.value}
