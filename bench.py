@@ -181,16 +181,14 @@ def shot_gradient_2d(vtiamd, ctx, nshots=2):
             "ms_per_shot": round(el * 1e3 / nshots, 2), "shot_gradients_per_s": round(nshots / el, 3)}
 
 
-def serve_stream(vtiamd, nreq=40, rate=5.0, seed=5):
-    """The served modeling/gradient stream: Poisson arrivals at `rate` req/s,
-    70% 2D gradient requests (nz 256-512, nx 256-1024 + 40-point boundary,
-    1000-3000 steps) and 30% 3D modeling requests (each dimension 128-400,
-    1000-3000 steps), on seeded layered models kept resident on the device.
-    The same request trace runs through the server with 4 slots and with one
-    slot (each request alone, in arrival order: the plain per-request
-    launch); then all requests at once (saturation) through both."""
+def serve_trace(vtiamd, nreq=40, rate=5.0, frac3d=0.3, seed=5):
+    """A seeded request trace: Poisson arrivals at `rate` req/s, a fraction
+    frac3d of 3D modeling requests (each dimension 128-400, 1000-3000
+    steps), the rest 2D gradient requests (nz 256-512, nx 256-1024 + a
+    40-point boundary, 1000-3000 steps), on seeded layered models (four 2D
+    and three 3D grids) that the server keeps resident."""
     rng = np.random.default_rng(seed)
-    npad2, dx2, dt, rate_s = 40, 10.0, 1e-3, 4e-3
+    npad2 = 40
     grids2 = sorted({(int(rng.integers(256, 513)), int(rng.integers(256, 1025))) for _ in range(4)})
     grids3 = [tuple(int(v) for v in rng.integers(128, 401, 3)) for _ in range(3)]
     models2 = {}
@@ -200,17 +198,24 @@ def serve_stream(vtiamd, nreq=40, rate=5.0, seed=5):
         v = 1500.0 + 3000.0 * z / nnz + 80.0 * np.sin(np.arange(nnx, dtype=np.float32)[None, :] / 53.0)
         c33 = (v * v).astype(np.float32)
         models2[(nz, nx)] = (c33 * np.float32(1.2), c33 * np.float32(1.05), c33, vtiamd.init_taper2d(nx, nz, npad2))
-    models3 = {g: layered(*g, seed=sum(g))[:3] for g in grids3}
+    models3 = {g: layered(*g, seed=sum(g))[:3] for g in grids3} if frac3d > 0 else {}
     reqs, t = [], 0.0
-    for k in range(nreq):
+    for _ in range(nreq):
         t += float(rng.exponential(1.0 / rate))
-        if rng.random() < 0.7:
+        if rng.random() >= frac3d:
             g = grids2[int(rng.integers(len(grids2)))]
-            nt = 4 * int(rng.integers(250, 751)) + 1
-            reqs.append({"t": t, "kind": "2d", "grid": g, "nt": nt})
+            reqs.append({"t": t, "kind": "2d", "grid": g, "nt": 4 * int(rng.integers(250, 751)) + 1})
         else:
             g = grids3[int(rng.integers(len(grids3)))]
             reqs.append({"t": t, "kind": "3d", "grid": g, "nt": int(rng.integers(1000, 3001))})
+    return {"reqs": reqs, "grids2": grids2, "grids3": grids3, "models2": models2, "models3": models3,
+            "npad2": npad2, "rate": rate, "frac3d": frac3d}
+
+
+def serve_run(vtiamd, tr, slots, realtime):
+    """Run a trace through a server with `slots` workers; realtime: submit at
+    the arrival times, else all at once.  Returns latency/throughput stats."""
+    npad2, dx2, dt, rate_s = tr["npad2"], 10.0, 1e-3, 4e-3
 
     def wavelet(nt):
         tt = np.arange(nt) * dt - 0.1
@@ -225,42 +230,57 @@ def serve_stream(vtiamd, nreq=40, rate=5.0, seed=5):
         sou = np.array([[nx // 2 * d + 0.1, ny // 2 * d + 0.1, 34 * d + 0.1, len(rec), 0]], np.float32)
         return sou, np.array(rec, np.float32)
 
-    def run(slots, realtime):
-        with vtiamd.Server(0, slots=slots) as srv:
-            h2 = {g: srv.put_model(*models2[g][:3]) for g in grids2}
-            h3 = {g: srv.put_model(*models3[g]) for g in grids3}
-            tickets = []
-            t0 = time.perf_counter()
-            for r in reqs:
-                if realtime:
-                    time.sleep(max(0.0, t0 + r["t"] - time.perf_counter()))
-                if r["kind"] == "2d":
-                    nz, nx = r["grid"]
-                    nnt = (r["nt"] - 1) // 4 + 1
-                    sl = np.array([[npad2 + nx // 2, npad2 + 2, nx, 0]], np.int32)
-                    rl = np.array([[npad2 + i, npad2 + 1] for i in range(nx)], np.int32)
-                    tickets.append(srv.grad2d(h2[r["grid"]], np.zeros((nnt, nx), np.float32), wavelet(r["nt"]), sl,
-                                              rl, models2[r["grid"]][3], nx, nz, r["nt"], npad2, dx2, dx2, dt, rate_s))
-                else:
-                    sou, rec = geom3(r["grid"])
-                    tickets.append(srv.model3d(h3[r["grid"]], sou, np.arange(1), rec, wavelet(r["nt"]), r["grid"],
-                                               r["nt"], 30, (0.0, 0.0, 0.0), (12.5, 12.5, 12.5), dt, rate_s))
-            for tk in tickets:
-                srv.wait(tk)
-            st = srv.stats()
-        return {"p50_s": round(st["p50_s"], 4), "p99_s": round(st["p99_s"], 4),
-                "req_per_s": round(st["req_per_s"], 3), "peak_gb": round(st["peak_bytes"] / 2**30, 2)}
+    with vtiamd.Server(0, slots=slots) as srv:
+        h2 = {g: srv.put_model(*tr["models2"][g][:3]) for g in tr["grids2"]}
+        h3 = {g: srv.put_model(*tr["models3"][g]) for g in tr["models3"]}
+        tickets = []
+        t0 = time.perf_counter()
+        for r in tr["reqs"]:
+            if realtime:
+                time.sleep(max(0.0, t0 + r["t"] - time.perf_counter()))
+            if r["kind"] == "2d":
+                nz, nx = r["grid"]
+                nnt = (r["nt"] - 1) // 4 + 1
+                sl = np.array([[npad2 + nx // 2, npad2 + 2, nx, 0]], np.int32)
+                rl = np.array([[npad2 + i, npad2 + 1] for i in range(nx)], np.int32)
+                tickets.append(srv.grad2d(h2[r["grid"]], np.zeros((nnt, nx), np.float32), wavelet(r["nt"]), sl, rl,
+                                          tr["models2"][r["grid"]][3], nx, nz, r["nt"], npad2, dx2, dx2, dt, rate_s))
+            else:
+                sou, rec = geom3(r["grid"])
+                tickets.append(srv.model3d(h3[r["grid"]], sou, np.arange(1), rec, wavelet(r["nt"]), r["grid"],
+                                           r["nt"], 30, (0.0, 0.0, 0.0), (12.5, 12.5, 12.5), dt, rate_s))
+        for tk in tickets:
+            srv.wait(tk)
+        st = srv.stats()
+    return {"p50_s": round(st["p50_s"], 4), "p99_s": round(st["p99_s"], 4), "req_per_s": round(st["req_per_s"], 3),
+            "peak_gb": round(st["peak_bytes"] / 2**30, 2)}
 
-    served = run(4, True)
-    plain = run(1, True)
-    sat_served = run(4, False)
-    sat_plain = run(1, False)
-    return {"requests": nreq, "arrival_rate_req_s": rate, "mix": "70% 2D gradient, 30% 3D modeling",
-            "grids_2d": [list(g) for g in grids2], "grids_3d": [list(g) for g in grids3],
-            "poisson_served_4slots": served, "poisson_plain_1slot": plain,
-            "saturation_served_req_s": sat_served["req_per_s"], "saturation_plain_req_s": sat_plain["req_per_s"],
-            "saturation_ratio": round(sat_served["req_per_s"] / max(sat_plain["req_per_s"], 1e-9), 2),
-            "p99_ratio_plain_over_served": round(plain["p99_s"] / max(served["p99_s"], 1e-9), 2)}
+
+def serve_stream(vtiamd, nreq=40):
+    """The served stream (70% 2D gradients, 30% 3D modeling) at 5 req/s and
+    at 12 req/s through a 4-slot server and through one slot (each request
+    alone, in arrival order: the plain per-request launch), and the
+    saturation throughput (all requests at once) of both, also for a stream
+    of 2D gradient requests only."""
+    out = {"requests": nreq, "mix": "70% 2D gradient, 30% 3D modeling"}
+    for rate in (5.0, 12.0):
+        tr = serve_trace(vtiamd, nreq, rate)
+        out.setdefault("grids_2d", [list(g) for g in tr["grids2"]])
+        out.setdefault("grids_3d", [list(g) for g in tr["grids3"]])
+        served, plain = serve_run(vtiamd, tr, 4, True), serve_run(vtiamd, tr, 1, True)
+        out[f"poisson_{rate:g}rps"] = {"served_4slots": served, "plain_1slot": plain,
+                                       "p99_ratio_plain_over_served": round(plain["p99_s"] / max(served["p99_s"],
+                                                                                                 1e-9), 2)}
+    sat = {"served_4slots": serve_run(vtiamd, tr, 4, False)["req_per_s"],
+           "plain_1slot": serve_run(vtiamd, tr, 1, False)["req_per_s"]}
+    sat["ratio"] = round(sat["served_4slots"] / max(sat["plain_1slot"], 1e-9), 2)
+    out["saturation_req_s"] = sat
+    tr2 = serve_trace(vtiamd, nreq, 5.0, frac3d=0.0, seed=6)
+    sat2 = {"served_4slots": serve_run(vtiamd, tr2, 4, False)["req_per_s"],
+            "plain_1slot": serve_run(vtiamd, tr2, 1, False)["req_per_s"]}
+    sat2["ratio"] = round(sat2["served_4slots"] / max(sat2["plain_1slot"], 1e-9), 2)
+    out["saturation_2d_only_shot_gradients_per_s"] = sat2
+    return out
 
 
 def main():
