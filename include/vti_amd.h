@@ -92,6 +92,9 @@ int vti3d_prop_destroy(vti3d_prop *prop);
 size_t vti3d_prop_bytes(const vti3d_prop *prop);
 int vti3d_prop_set_zchunk(vti3d_prop *prop, int zchunk);
 int vti3d_prop_set_arith(vti3d_prop *prop, int arith);
+/* Forward levels advanced per launch by vti3d_prop_forward_steps: 1 or 2
+ * (two levels per launch, a temporally blocked kernel; the same bits). */
+int vti3d_prop_set_levels(vti3d_prop *p, int levels);
 int vti3d_prop_set_model(vti3d_prop *prop, const float *c11, const float *c13, const float *c33);
 int vti3d_prop_reset(vti3d_prop *prop, long long sou_index, float value);
 int vti3d_prop_set_state(vti3d_prop *prop, const float *cur_x, const float *cur_z, const float *prev_x,

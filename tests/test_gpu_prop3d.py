@@ -19,10 +19,11 @@ def _problem(nz=40, ny=36, nx=44, seed=3):
     return cij_from(v, e, d)
 
 
+@pytest.mark.parametrize("levels", [1, 2], ids=["1-level", "2-level"])
 @pytest.mark.parametrize("arith", [1, 0], ids=["ref-arith", "fp32-arith"])
 @pytest.mark.parametrize("absorbing", [True, False])
-@pytest.mark.parametrize("dims", [(40, 36, 44), (48, 29, 70)])
-def test_forward_shot_bitexact(vti, orc, ctx, absorbing, dims, arith):
+@pytest.mark.parametrize("dims", [(40, 36, 44), (48, 29, 70), (43, 30, 40)])
+def test_forward_shot_bitexact(vti, orc, ctx, absorbing, dims, arith, levels):
     nz, ny, nx = dims
     c11, c13, c33 = _problem(nz, ny, nx)
     npad, dx, dt = 8, 12.5, 1.0e-3
@@ -32,13 +33,14 @@ def test_forward_shot_bitexact(vti, orc, ctx, absorbing, dims, arith):
     sou = 17 + 15 * nx + 13 * nx * ny
     rec = np.array([ix + (ny // 2) * nx + 9 * nx * ny for ix in range(4, nx - 4)], np.int64)
     sts, nnt = 4, (nt - 1) // 4 + 1
-    print(f"case dims={dims} absorbing={absorbing} arith={arith}", flush=True)
+    print(f"case dims={dims} absorbing={absorbing} arith={arith} levels={levels}", flush=True)
     mode = orc.REF if arith == 1 else orc.F32
     tr_ref, tr64, st = orc.forward3d(mode, c11, c13, c33, npad, dx, dx, dx, dt, lmax, w, sou, rec, sts, nnt,
                                      absorbing=absorbing, want_state=True)
     with vti.Prop3D(ctx, nx, ny, nz, npad, dx, dx, dx, dt) as p:
         p.set_zchunk(16)
         p.set_arith(arith)
+        p.set_levels(levels)
         p.set_model(c11, c13, c33)
         p.reset(sou, np.float32(np.float32(dt) * np.float32(dt)) * w[0])
         tr = p.forward_shot(lmax, w, sou, rec, sts, nnt, absorbing=absorbing)
@@ -73,3 +75,33 @@ def test_zchunk_invariance(vti, ctx):
     for o in outs[1:]:
         for a, b in zip(o, outs[0]):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("arith", [1, 0], ids=["ref-arith", "fp32-arith"])
+@pytest.mark.parametrize("absorbing", [True, False])
+def test_two_levels_equal_one_level(vti, ctx, arith, absorbing):
+    """Two levels per launch give the bits of one level per launch from an
+    arbitrary state -- nonzero values outside the updated interior included
+    -- for odd and even step counts and z-chunks down to one 4-plane block."""
+    nz, ny, nx = 47, 30, 71
+    c11, c13, c33 = _problem(nz, ny, nx, seed=9)
+    rng = np.random.default_rng(2)
+    st0 = [rng.standard_normal((nz, ny, nx)).astype(np.float32) for _ in range(4)]
+    print(f"two-level arith={arith} absorbing={absorbing}", flush=True)
+    for nsteps in (6, 7):
+        ref = None
+        for levels, zc in ((1, 16), (2, 4), (2, 12), (2, 44)):
+            with vti.Prop3D(ctx, nx, ny, nz, 6, 10.0, 10.0, 10.0, 1e-3) as p:
+                p.set_zchunk(zc)
+                p.set_arith(arith)
+                p.set_levels(levels)
+                p.set_model(c11, c13, c33)
+                p.set_state(*st0)
+                p.forward_steps(nsteps, absorbing=absorbing)
+                out = p.get_state()
+            if ref is None:
+                ref = out
+                assert np.abs(out[0]).max() > 0
+            else:
+                for a, b in zip(out, ref):
+                    assert np.array_equal(a, b), (nsteps, levels, zc)

@@ -164,6 +164,12 @@ struct vti3d_prop {
     int zc = 64;
     int arith = 1;  // kArithRef
     size_t bytes = 0;
+    // two forward levels per launch: the second buffer pair it writes, and
+    // whether their values outside the updated interior match (sx[1], sz[1])
+    // and (sx[0], sz[0]) -- any other write to the fields clears it
+    int levels = 1;
+    float *sx2[2] = {nullptr, nullptr}, *sz2[2] = {nullptr, nullptr};
+    bool t2_synced = false;
 };
 
 namespace {
@@ -206,6 +212,8 @@ int prop_free(vti3d_prop *p)
         freep(p->sz[k], s);
         freep(p->lx[k], s);
         freep(p->lz[k], s);
+        freep(p->sx2[k], s);
+        freep(p->sz2[k], s);
     }
     (void)hipStreamSynchronize(s);
     delete p;
@@ -294,6 +302,7 @@ void fwd_kernel(vti3d_prop *p, const Launch &L, long long src_dev, float src_val
 
 int fwd_launch(vti3d_prop *p, bool absorbing, long long src_dev, float src_val)
 {
+    p->t2_synced = false;
     const Launch L = launch_shape(p);
     if (p->arith == vti::kArithRef) {
         if (absorbing) fwd_kernel<vti::kArithRef, true>(p, L, src_dev, src_val);
@@ -305,6 +314,56 @@ int fwd_launch(vti3d_prop *p, bool absorbing, long long src_dev, float src_val)
     VTI_HIP(hipGetLastError());
     std::swap(p->sx[0], p->sx[1]);
     std::swap(p->sz[0], p->sz[1]);
+    return vti::kOk;
+}
+
+template <int ARITH, bool ABC>
+void fwd_t2_kernel(vti3d_prop *p, const Launch &L, long long src_dev, float v1, float v2)
+{
+    hipLaunchKernelGGL((vti::fwd3d_t2_kernel<ARITH, ABC, kBY>), L.grid, L.block, 0, p->ctx->stream, p->sx2[0],
+                       p->sz2[0], p->sx2[1], p->sz2[1], p->sx[1], p->sz[1], p->sx[0], p->sz[0], p->cm[p->active][0],
+                       p->cm[p->active][1], p->cm[p->active][2], p->dxy, p->dzv, p->g, p->zc, L.ntx, L.nty,
+                       p->dt2dx2, p->dt2dy2, p->dt2dz2, src_dev, v1, v2);
+}
+
+// Two forward levels in one launch (sources v1 at the first, v2 at the
+// second).  Allocates the second buffer pair on first use and, when the
+// fields were written otherwise since, copies the current pair into it so
+// that the values outside the updated interior agree.
+int fwd2_launch(vti3d_prop *p, bool absorbing, long long src_dev, float v1, float v2)
+{
+    const size_t nv = (size_t)vox(p->g);
+    for (int k = 0; k < 2; ++k) {
+        int rc = dalloc(p, &p->sx2[k], nv);
+        if (!rc) rc = dalloc(p, &p->sz2[k], nv);
+        if (rc) return rc;
+    }
+    if (!p->t2_synced) {
+        VTI_HIP(hipMemcpyAsync(p->sx2[0], p->sx[1], nv * 4, hipMemcpyDeviceToDevice, p->ctx->stream));
+        VTI_HIP(hipMemcpyAsync(p->sz2[0], p->sz[1], nv * 4, hipMemcpyDeviceToDevice, p->ctx->stream));
+        VTI_HIP(hipMemcpyAsync(p->sx2[1], p->sx[0], nv * 4, hipMemcpyDeviceToDevice, p->ctx->stream));
+        VTI_HIP(hipMemcpyAsync(p->sz2[1], p->sz[0], nv * 4, hipMemcpyDeviceToDevice, p->ctx->stream));
+    }
+    const Launch L = launch_shape(p);
+    if (p->arith == vti::kArithRef) {
+        if (absorbing) fwd_t2_kernel<vti::kArithRef, true>(p, L, src_dev, v1, v2);
+        else fwd_t2_kernel<vti::kArithRef, false>(p, L, src_dev, v1, v2);
+    } else {
+        if (absorbing) fwd_t2_kernel<vti::kArithF32, true>(p, L, src_dev, v1, v2);
+        else fwd_t2_kernel<vti::kArithF32, false>(p, L, src_dev, v1, v2);
+    }
+    VTI_HIP(hipGetLastError());
+    // (prev, cur) <- (L+1, L+2); the old pair becomes the spare pair
+    float *ax = p->sx[1], *az = p->sz[1], *bx = p->sx[0], *bz = p->sz[0];
+    p->sx[1] = p->sx2[0];
+    p->sz[1] = p->sz2[0];
+    p->sx[0] = p->sx2[1];
+    p->sz[0] = p->sz2[1];
+    p->sx2[0] = ax;
+    p->sz2[0] = az;
+    p->sx2[1] = bx;
+    p->sz2[1] = bz;
+    p->t2_synced = true;
     return vti::kOk;
 }
 
@@ -323,6 +382,7 @@ void bwd_kernel(vti3d_prop *p, const Launch &L, long long src_dev, float src_val
 int bwd_launch(vti3d_prop *p, int mode, long long src_dev, float src_val, int zrec0, const long long *d_rec_eff,
                const float *coarse, int nr, int i1, float f)
 {
+    p->t2_synced = false;
     const Geo3 &g = p->g;
     const Launch L = launch_shape(p);
     if (p->arith == vti::kArithRef) {
@@ -352,6 +412,7 @@ int bwd_launch(vti3d_prop *p, int mode, long long src_dev, float src_val, int zr
 
 int zero_fields(vti3d_prop *p, bool sigma, bool lambda)
 {
+    p->t2_synced = false;
     const size_t nb = (size_t)vox(p->g) * sizeof(float);
     for (int k = 0; k < 2; ++k) {
         if (sigma) {
@@ -448,14 +509,28 @@ int forward_pass(vti3d_prop *p, bool absorbing, int lmax, const float *wavelet, 
                  int nr, int sts, int nnt)
 {
     const long long src = dev_index(p->g, sou_ref);
-    for (int it = 2; it <= lmax; ++it) {
-        int rc = fwd_launch(p, absorbing, src, p->dt2 * wavelet[it - 1]);
-        if (rc) return rc;
+    // record level it, held in (sx[k], sz[k])
+    auto record = [&](int it, int k) -> int {
         if (sd && sd->tr && nr > 0 && it % sts == 0 && it / sts < nnt) {
             hipLaunchKernelGGL(record3d_kernel, dim3((nr + 255) / 256), dim3(256), 0, p->ctx->stream,
-                               sd->tr + (size_t)(it / sts) * nr, p->sx[0], p->sz[0], sd->rec_eff, nr);
+                               sd->tr + (size_t)(it / sts) * nr, p->sx[k], p->sz[k], sd->rec_eff, nr);
             VTI_HIP(hipGetLastError());
         }
+        return vti::kOk;
+    };
+    for (int it = 2; it <= lmax;) {
+        int rc;
+        if (p->levels == 2 && it + 1 <= lmax) {
+            rc = fwd2_launch(p, absorbing, src, p->dt2 * wavelet[it - 1], p->dt2 * wavelet[it]);
+            if (!rc) rc = record(it, 1);
+            if (!rc) rc = record(it + 1, 0);
+            it += 2;
+        } else {
+            rc = fwd_launch(p, absorbing, src, p->dt2 * wavelet[it - 1]);
+            if (!rc) rc = record(it, 0);
+            it += 1;
+        }
+        if (rc) return rc;
     }
     return vti::kOk;
 }
@@ -580,7 +655,17 @@ int vti3d_prop_set_zchunk(vti3d_prop *p, int zc)
     return vti::kOk;
 }
 
-// Upload c11, c13, c33 (host arrays in the reference layout).
+// Forward levels per launch of vti3d_prop_forward_steps / forward_shot: 1
+// or 2 (the same bits either way).
+int vti3d_prop_set_levels(vti3d_prop *p, int levels)
+{
+    VTI_CHECK(p && (levels == 1 || levels == 2), "vti3d_prop_set_levels: levels must be 1 or 2");
+    p->levels = levels;
+    return vti::kOk;
+}
+
+// Upload c11, c13, c33 (host arrays in the reference layout, or a model
+// the caller keeps in device memory).
 int vti3d_prop_set_model(vti3d_prop *p, const float *c11, const float *c13, const float *c33)
 {
     VTI_CHECK(p && c11 && c13 && c33, "vti3d_prop_set_model: null argument");
@@ -621,6 +706,7 @@ int vti3d_prop_set_state(vti3d_prop *p, const float *cur_x, const float *cur_z, 
     VTI_CHECK(p, "null propagator");
     VTI_HIP(hipSetDevice(p->ctx->device));
     (void)hipGetLastError();  // drop a stale error left by an unrelated earlier call
+    p->t2_synced = false;
     const float *src[4] = {cur_x, cur_z, prev_x, prev_z};
     float *dst[4] = {p->sx[0], p->sz[0], p->sx[1], p->sz[1]};
     for (int k = 0; k < 4; ++k)
@@ -662,8 +748,15 @@ int vti3d_prop_forward_steps(vti3d_prop *p, int nsteps, int absorbing, float *el
         VTI_HIP(hipEventCreate(&e1));
         VTI_HIP(hipEventRecord(e0, p->ctx->stream));
     }
-    for (int s = 0; s < nsteps; ++s) {
-        int rc = fwd_launch(p, absorbing != 0, -1, 0.f);
+    for (int s = 0; s < nsteps;) {
+        int rc;
+        if (p->levels == 2 && s + 1 < nsteps) {
+            rc = fwd2_launch(p, absorbing != 0, -1, 0.f, 0.f);
+            s += 2;
+        } else {
+            rc = fwd_launch(p, absorbing != 0, -1, 0.f);
+            s += 1;
+        }
         if (rc) return rc;
     }
     if (elapsed_ms) {

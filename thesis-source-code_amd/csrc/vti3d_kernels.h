@@ -413,6 +413,263 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Two forward levels per launch (temporal blocking).
+//
+// Level L-1 is read from (ax, az), level L from (bx, bz); level L+1 goes to
+// (cx, cz) and L+2 to (dx_, dz_) -- four distinct buffer pairs, since the
+// neighbouring workgroups still read the halos of L-1 and L.
+//
+// A workgroup of 64 x BY threads owns a 64 x BY column tile and a z-chunk.
+// Marching down z, it advances level L+1 at plane z on the tile plus a
+// 4-point x/y halo E = [x0-4, x0+68) x [y0-4, y0+BY+4), and level L+2 at
+// plane z-4 on the tile.  Per point the arithmetic is that of fwd3d_kernel,
+// operation for operation (same stencils, the same effective sigma_z column,
+// damping on read, source, damping on write), so one launch gives the bits of
+// two fwd3d_kernel launches.  Points of E outside the updated interior keep
+// the value of level L-1 there (a one-level step leaves them untouched in the
+// buffer it writes), out-of-range points read as 0.  The caller keeps the
+// values outside the interior of (cx, cz) equal to those of (ax, az), and of
+// (dx_, dz_) equal to (bx, bz).
+//
+// Thread t owns the tile point (t % 64, t / 64) of E's centre and up to
+// NE - 1 points of E's 4-wide rim; each owned point keeps a 9-deep register
+// queue of its sigma_z(L) column.  sigma_x(L) planes sit in an 80-wide LDS
+// tile (double buffered); sigma_x(L+1) planes of E in a 5-slot LDS ring, so
+// that level L+2 at plane z-4 finds its plane there; sigma_z(L+1) of the
+// centre column stays in a 9-deep register queue.
+// ---------------------------------------------------------------------------
+
+template <int ARITH, int W>
+__device__ __forceinline__ float sx_scaled_w(const float *T, float sx, float sy)
+{
+    if (ARITH == kArithF32) {
+        const float s0 = T[0];
+        float tx = (VTI_C0 * s0 + VTI_C1 * (T[-1] + T[1]) + VTI_C2 * (T[-2] + T[2]) + VTI_C3 * (T[-3] + T[3]) +
+                    VTI_C4 * (T[-4] + T[4])) *
+                   sx;
+        float ty = (VTI_C0 * s0 + VTI_C1 * (T[-W] + T[W]) + VTI_C2 * (T[-2 * W] + T[2 * W]) +
+                    VTI_C3 * (T[-3 * W] + T[3 * W]) + VTI_C4 * (T[-4 * W] + T[4 * W])) *
+                   sy;
+        return ty + tx;
+    }
+    const float tx = (float)(acc9(T[0], T[-1] + T[1], T[-2] + T[2], T[-3] + T[3], T[-4] + T[4]) * (double)sx);
+    float ty = (float)(acc9(T[0], T[-W] + T[W], T[-2 * W] + T[2 * W], T[-3 * W] + T[3 * W], T[-4 * W] + T[4 * W]) *
+                       (double)sy);
+    ty += tx;
+    return ty;
+}
+
+template <int BY>
+struct T2Shape {
+    static constexpr int NT = 64 * BY;            // threads
+    static constexpr int EW = 72, EH = BY + 8;    // E: x0-4 .. x0+68, y0-4 .. y0+BY+4
+    static constexpr int NRIM = EW * EH - 64 * BY;
+    static constexpr int NE = 1 + (NRIM + NT - 1) / NT;  // owned points of E per thread
+    static constexpr int TW = 80, TH = BY + 16;   // sigma_x(L) tile: x0-8 .., y0-8 ..
+};
+
+// E coordinates of the points a thread owns: slot 0 the tile point, then rim
+// points (top 4 rows, bottom 4 rows, left 4 columns, right 4 columns)
+template <int BY>
+__device__ __forceinline__ bool t2_point(int tid, int j, int &ex, int &ey)
+{
+    using S = T2Shape<BY>;
+    if (j == 0) {
+        ex = (tid & 63) + 4;
+        ey = (tid >> 6) + 4;
+        return true;
+    }
+    const int h = tid + (j - 1) * S::NT;
+    if (h >= S::NRIM) return false;
+    if (h < 288) {
+        ey = h / 72;
+        ex = h % 72;
+    } else if (h < 576) {
+        ey = BY + 4 + (h - 288) / 72;
+        ex = (h - 288) % 72;
+    } else if (h < 576 + 4 * BY) {
+        ey = 4 + (h - 576) / 4;
+        ex = (h - 576) % 4;
+    } else {
+        ey = 4 + (h - 576 - 4 * BY) / 4;
+        ex = 68 + (h - 576 - 4 * BY) % 4;
+    }
+    return true;
+}
+
+template <int ARITH, bool ABC, int BY>
+__global__ __launch_bounds__(64 * BY) void fwd3d_t2_kernel(
+    float *__restrict__ cx, float *__restrict__ cz, float *__restrict__ dx_, float *__restrict__ dz_,
+    const float *__restrict__ ax, const float *__restrict__ az, const float *__restrict__ bx,
+    const float *__restrict__ bz, const float *__restrict__ c11, const float *__restrict__ c13,
+    const float *__restrict__ c33, const float *__restrict__ dxy_plane, const float *__restrict__ dzv, Geo3 g,
+    int zc, int ntx, int nty, float dt2dx2, float dt2dy2, float dt2dz2, long long src_idx, float src1, float src2)
+{
+    using S = T2Shape<BY>;
+    constexpr int NE = S::NE, TW = S::TW, TH = S::TH, EW = S::EW, EH = S::EH;
+    __shared__ float tile[2][TH * TW];
+    __shared__ float ring[5][EH * EW];
+
+    const int lb = xcd_logical_block(gridDim.x * gridDim.y * gridDim.z);
+    const int bxi = lb % ntx, byi = (lb / ntx) % nty, bzi = lb / (ntx * nty);
+    const int tid = threadIdx.x + 64 * threadIdx.y;
+    const int x0 = bxi * 64, y0 = 4 + byi * BY;
+    const int zs = bzi * zc, ze = min(zs + zc, g.zmax);
+    const int z1s = max(zs - 4, 0), z1e = min(ze + 4, g.zmax);  // planes of level L+1
+
+    // owned points of E: position, class, offset within a plane, damping
+    int ex[NE], ey[NE];
+    bool own[NE], inr[NE], act[NE];
+    long long off[NE];
+    float dxy[NE];
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+        own[j] = t2_point<BY>(tid, j, ex[j], ey[j]);
+        const int x = x0 - 4 + ex[j], y = y0 - 4 + ey[j];
+        inr[j] = own[j] && x >= 0 && x < g.nx && y >= 0 && y < g.ny;
+        act[j] = inr[j] && x >= 4 && x < g.nx - 4 && y >= 4 && y < g.ny - 4;
+        off[j] = inr[j] ? (long long)y * g.pitch + x : 0;
+        dxy[j] = (ABC && inr[j]) ? dxy_plane[off[j]] : 1.f;
+    }
+    // sigma_z(L) queues of the owned columns: q[j][k] = sigma_z(L)(z-4+k)
+    float q[NE][9];
+#pragma unroll
+    for (int j = 0; j < NE; ++j)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int zz = z1s - 4 + k;
+            q[j][k] = (inr[j] && zz >= 0 && zz < g.zmax) ? bz[(long long)zz * g.plane + off[j]] : 0.f;
+        }
+    // sigma_z(L+1) of the centre column, planes z-8 .. z; sigma_x(L) of the
+    // centre, planes z-4 .. z
+    float q1[9], hx[5];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) q1[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) hx[k] = 0.f;
+
+    // sigma_x(L) tile loader: element e of the TW x TH tile
+    auto tile_load = [&](float *dst, int z) {
+        const long long pz = (long long)z * g.plane;
+        for (int e = tid; e < TW * TH; e += S::NT) {
+            const int x = x0 - 8 + e % TW, y = y0 - 8 + e / TW;
+            dst[e] = (x >= 0 && x < g.nx && y >= 0 && y < g.ny) ? bx[pz + (long long)y * g.pitch + x] : 0.f;
+        }
+    };
+    if (z1s < z1e) tile_load(tile[z1s & 1], z1s);
+    __syncthreads();
+
+    const int cxl = (tid & 63), cyl = (tid >> 6);  // centre point in tile coordinates
+    for (int z = z1s; z < ze + 4; ++z) {
+        const bool step1 = z < z1e;
+        // ---- level L+1 at plane z on E
+        if (step1) {
+            const float *Tp = tile[z & 1];
+            const float d1 = ABC ? dzv[z] : 1.f;
+            const long long pz = (long long)z * g.plane;
+#pragma unroll
+            for (int j = 0; j < NE; ++j) {
+                if (!own[j]) continue;
+                float vx = 0.f, vz = 0.f;
+                if (act[j]) {
+                    const float *T = &Tp[(ey[j] + 4) * TW + ex[j] + 4];
+                    const float s0 = T[0];
+                    const float txy = sx_scaled_w<ARITH, TW>(T, dt2dx2, dt2dy2);
+                    float e9[9];
+                    effective_column_rt<ABC>(e9, q[j], dxy[j], dzv, z, z & 3);
+                    const float tz = sz_scaled<ARITH>(e9, dt2dz2);
+                    const long long gi = pz + off[j];
+                    float px = ax[gi], pzv = az[gi];
+                    const float a11 = c11[gi], a13 = c13[gi], a33 = c33[gi];
+                    float d = 1.f;
+                    if (ABC) {
+                        d = dxy[j] * d1;
+                        px = px * d;
+                        pzv = pzv * d;
+                    }
+                    vx = leap<ARITH>(a11 * txy + a13 * tz, s0, px);
+                    vz = leap<ARITH>(a13 * txy + a33 * tz, q[j][4], pzv);
+                    if (gi == src_idx) {
+                        vx = vx + src1;
+                        vz = vz + src1;
+                    }
+                    if (ABC) {
+                        vx = vx * d;
+                        vz = vz * d;
+                    }
+                } else if (inr[j]) {
+                    const long long gi = pz + off[j];
+                    vx = ax[gi];
+                    vz = az[gi];
+                }
+                ring[z % 5][ey[j] * EW + ex[j]] = vx;
+                if (j == 0) {
+                    if (act[0] && z >= zs && z < ze) {
+                        cx[pz + off[0]] = vx;
+                        cz[pz + off[0]] = vz;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) q1[k] = q1[k + 1];
+                    q1[8] = vz;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) hx[k] = hx[k + 1];
+                    hx[4] = Tp[(cyl + 8) * TW + cxl + 8];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q1[k] = q1[k + 1];
+            q1[8] = 0.f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) hx[k] = hx[k + 1];
+            hx[4] = 0.f;
+        }
+        // the next sigma_x(L) plane into the other tile buffer
+        if (z + 1 < z1e) tile_load(tile[(z + 1) & 1], z + 1);
+        // ---- level L+2 at plane z-4 on the tile
+        const int zp = z - 4;
+        if (zp >= zs && zp < ze && act[0]) {
+            const float *T = &ring[zp % 5][ey[0] * EW + ex[0]];
+            const float s0 = T[0];
+            const float txy = sx_scaled_w<ARITH, EW>(T, dt2dx2, dt2dy2);
+            float e9[9];
+            effective_column_rt<ABC>(e9, q1, dxy[0], dzv, zp, zp & 3);
+            const float tz = sz_scaled<ARITH>(e9, dt2dz2);
+            const long long gi = (long long)zp * g.plane + off[0];
+            float px = hx[0], pzv = q[0][0];
+            const float a11 = c11[gi], a13 = c13[gi], a33 = c33[gi];
+            float d = 1.f;
+            if (ABC) {
+                d = dxy[0] * dzv[zp];
+                px = px * d;
+                pzv = pzv * d;
+            }
+            float vx = leap<ARITH>(a11 * txy + a13 * tz, s0, px);
+            float vz = leap<ARITH>(a13 * txy + a33 * tz, q1[4], pzv);
+            if (gi == src_idx) {
+                vx = vx + src2;
+                vz = vz + src2;
+            }
+            if (ABC) {
+                vx = vx * d;
+                vz = vz * d;
+            }
+            dx_[gi] = vx;
+            dz_[gi] = vz;
+        }
+        __syncthreads();
+        // shift the sigma_z(L) queues to plane z+1
+#pragma unroll
+        for (int j = 0; j < NE; ++j) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[j][k] = q[j][k + 1];
+            const int zz = z + 5;
+            q[j][8] = (inr[j] && zz < g.zmax) ? bz[(long long)zz * g.plane + off[j]] : 0.f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Fused backward step of the 3D gradient and RTM passes.
 //
 // MODE 0 (gradient, objFuncGradient3d_f.cu:1008-1028): per point

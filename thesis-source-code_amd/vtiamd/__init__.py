@@ -57,6 +57,7 @@ _SIGS = {
     "vti3d_prop_destroy": (c_int, [P]),
     "vti3d_prop_bytes": (c_size_t, [P]),
     "vti3d_prop_set_zchunk": (c_int, [P, c_int]),
+    "vti3d_prop_set_levels": (c_int, [P, c_int]),
     "vti3d_prop_set_arith": (c_int, [P, c_int]),
     "vti3d_prop_set_model": (c_int, [P, P, P, P]),
     "vti3d_prop_reset": (c_int, [P, c_longlong, c_float]),
@@ -249,6 +250,10 @@ class Prop3D:
 
     def set_zchunk(self, zc: int):
         check(_lib.vti3d_prop_set_zchunk(self.h, zc), "vti3d_prop_set_zchunk")
+
+    def set_levels(self, levels):
+        """Forward levels per launch of forward_steps: 1 or 2 (same bits)."""
+        check(_lib.vti3d_prop_set_levels(self.h, levels), "vti3d_prop_set_levels")
 
     def set_arith(self, arith: int):
         check(_lib.vti3d_prop_set_arith(self.h, arith), "vti3d_prop_set_arith")
