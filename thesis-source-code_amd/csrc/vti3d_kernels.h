@@ -507,6 +507,7 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_t2_kernel(
 {
     using S = T2Shape<BY>;
     constexpr int NE = S::NE, TW = S::TW, TH = S::TH, EW = S::EW, EH = S::EH;
+    constexpr int TL = (TW * TH + S::NT - 1) / S::NT;  // tile elements loaded per thread
     __shared__ float tile[2][TH * TW];
     __shared__ float ring[5][EH * EW];
 
@@ -520,7 +521,7 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_t2_kernel(
     // owned points of E: position, class, offset within a plane, damping
     int ex[NE], ey[NE];
     bool own[NE], inr[NE], act[NE];
-    long long off[NE];
+    unsigned off[NE];
     float dxy[NE];
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
@@ -528,9 +529,22 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_t2_kernel(
         const int x = x0 - 4 + ex[j], y = y0 - 4 + ey[j];
         inr[j] = own[j] && x >= 0 && x < g.nx && y >= 0 && y < g.ny;
         act[j] = inr[j] && x >= 4 && x < g.nx - 4 && y >= 4 && y < g.ny - 4;
-        off[j] = inr[j] ? (long long)y * g.pitch + x : 0;
+        off[j] = inr[j] ? (unsigned)(y * g.pitch + x) : 0u;
         dxy[j] = (ABC && inr[j]) ? dxy_plane[off[j]] : 1.f;
     }
+    // sigma_x(L) tile elements this thread loads: offsets within a plane
+    unsigned toff[TL];
+    bool tok[TL], thas[TL];
+#pragma unroll
+    for (int k = 0; k < TL; ++k) {
+        const int e = tid + k * S::NT;
+        const int x = x0 - 8 + e % TW, y = y0 - 8 + e / TW;
+        thas[k] = e < TW * TH;
+        tok[k] = thas[k] && x >= 0 && x < g.nx && y >= 0 && y < g.ny;
+        toff[k] = tok[k] ? (unsigned)(y * g.pitch + x) : 0u;
+    }
+    auto P = [&](const float *base, int z) { return base + (size_t)z * (size_t)g.plane; };
+
     // sigma_z(L) queues of the owned columns: q[j][k] = sigma_z(L)(z-4+k)
     float q[NE][9];
 #pragma unroll
@@ -538,30 +552,75 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_t2_kernel(
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             const int zz = z1s - 4 + k;
-            q[j][k] = (inr[j] && zz >= 0 && zz < g.zmax) ? bz[(long long)zz * g.plane + off[j]] : 0.f;
+            q[j][k] = (inr[j] && zz >= 0 && zz < g.zmax) ? P(bz, zz)[off[j]] : 0.f;
         }
-    // sigma_z(L+1) of the centre column, planes z-8 .. z; sigma_x(L) of the
-    // centre, planes z-4 .. z
+    // sigma_z(L+1) of the centre column (planes z-8 .. z) and sigma_x(L) of the
+    // centre (planes z-4 .. z)
     float q1[9], hx[5];
 #pragma unroll
     for (int k = 0; k < 9; ++k) q1[k] = 0.f;
 #pragma unroll
     for (int k = 0; k < 5; ++k) hx[k] = 0.f;
 
-    // sigma_x(L) tile loader: element e of the TW x TH tile
-    auto tile_load = [&](float *dst, int z) {
-        const long long pz = (long long)z * g.plane;
-        for (int e = tid; e < TW * TH; e += S::NT) {
-            const int x = x0 - 8 + e % TW, y = y0 - 8 + e / TW;
-            dst[e] = (x >= 0 && x < g.nx && y >= 0 && y < g.ny) ? bx[pz + (long long)y * g.pitch + x] : 0.f;
+    // per-point operands of level L+1 at one plane: the previous level and the
+    // coefficients (rim points only need what sigma_x takes)
+    float pax[NE], paz, k11[NE], k13[NE], k33;
+    auto load_pts = [&](int z, float *oax, float &oaz, float *o11, float *o13, float &o33) {
+        const float *Ax = P(ax, z), *Az = P(az, z), *C11 = P(c11, z), *C13 = P(c13, z), *C33 = P(c33, z);
+#pragma unroll
+        for (int j = 0; j < NE; ++j) {
+            const bool ld = (z < z1e) && (j == 0 ? inr[0] : act[j] || inr[j]);
+            oax[j] = ld ? Ax[off[j]] : 0.f;
+            o11[j] = ld && act[j] ? C11[off[j]] : 0.f;
+            o13[j] = ld && act[j] ? C13[off[j]] : 0.f;
         }
+        const bool l0 = (z < z1e) && inr[0];
+        oaz = l0 ? Az[off[0]] : 0.f;
+        o33 = l0 && act[0] ? C33[off[0]] : 0.f;
     };
-    if (z1s < z1e) tile_load(tile[z1s & 1], z1s);
+    // the centre's coefficients for level L+2 at one plane
+    float s11 = 0.f, s13 = 0.f, s33 = 0.f;
+    auto load_s2 = [&](int zp, float &o11, float &o13, float &o33) {
+        const bool ld = act[0] && zp >= zs && zp < ze;
+        o11 = ld ? P(c11, zp)[off[0]] : 0.f;
+        o13 = ld ? P(c13, zp)[off[0]] : 0.f;
+        o33 = ld ? P(c33, zp)[off[0]] : 0.f;
+    };
+    float treg[TL];
+    auto tile_regs = [&](int z) {
+        const float *B = P(bx, z);
+#pragma unroll
+        for (int k = 0; k < TL; ++k) treg[k] = tok[k] ? B[toff[k]] : 0.f;
+    };
+    auto tile_store = [&](float *dst) {
+#pragma unroll
+        for (int k = 0; k < TL; ++k)
+            if (thas[k]) dst[tid + k * S::NT] = treg[k];
+    };
+
+    if (z1s < z1e) {
+        tile_regs(z1s);
+        tile_store(tile[z1s & 1]);
+    }
+    if (z1s + 1 < z1e) tile_regs(z1s + 1);
+    load_pts(z1s, pax, paz, k11, k13, k33);
+    load_s2(z1s - 4, s11, s13, s33);
     __syncthreads();
 
     const int cxl = (tid & 63), cyl = (tid >> 6);  // centre point in tile coordinates
     for (int z = z1s; z < ze + 4; ++z) {
         const bool step1 = z < z1e;
+        // requests for the next plane, in flight while this one is computed
+        if (z + 1 < z1e) tile_store(tile[(z + 1) & 1]);
+        if (z + 2 < z1e) tile_regs(z + 2);
+        float nax[NE], naz, n11[NE], n13[NE], n33;
+        load_pts(z + 1, nax, naz, n11, n13, n33);
+        float t11, t13, t33;
+        load_s2(z - 3, t11, t13, t33);
+        float qn[NE];
+#pragma unroll
+        for (int j = 0; j < NE; ++j) qn[j] = (inr[j] && z + 5 < g.zmax) ? P(bz, z + 5)[off[j]] : 0.f;
+
         // ---- level L+1 at plane z on E
         if (step1) {
             const float *Tp = tile[z & 1];
@@ -578,29 +637,26 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_t2_kernel(
                     float e9[9];
                     effective_column_rt<ABC>(e9, q[j], dxy[j], dzv, z, z & 3);
                     const float tz = sz_scaled<ARITH>(e9, dt2dz2);
-                    const long long gi = pz + off[j];
-                    float px = ax[gi], pzv = az[gi];
-                    const float a11 = c11[gi], a13 = c13[gi], a33 = c33[gi];
+                    float px = pax[j];
                     float d = 1.f;
                     if (ABC) {
                         d = dxy[j] * d1;
                         px = px * d;
-                        pzv = pzv * d;
                     }
-                    vx = leap<ARITH>(a11 * txy + a13 * tz, s0, px);
-                    vz = leap<ARITH>(a13 * txy + a33 * tz, q[j][4], pzv);
-                    if (gi == src_idx) {
-                        vx = vx + src1;
-                        vz = vz + src1;
-                    }
-                    if (ABC) {
-                        vx = vx * d;
-                        vz = vz * d;
+                    vx = leap<ARITH>(k11[j] * txy + k13[j] * tz, s0, px);
+                    const long long gi = pz + off[j];
+                    if (gi == src_idx) vx = vx + src1;
+                    if (ABC) vx = vx * d;
+                    if (j == 0) {
+                        float pzv = paz;
+                        if (ABC) pzv = pzv * d;
+                        vz = leap<ARITH>(k13[0] * txy + k33 * tz, q[0][4], pzv);
+                        if (gi == src_idx) vz = vz + src1;
+                        if (ABC) vz = vz * d;
                     }
                 } else if (inr[j]) {
-                    const long long gi = pz + off[j];
-                    vx = ax[gi];
-                    vz = az[gi];
+                    vx = pax[j];
+                    if (j == 0) vz = paz;
                 }
                 ring[z % 5][ey[j] * EW + ex[j]] = vx;
                 if (j == 0) {
@@ -624,8 +680,6 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_t2_kernel(
             for (int k = 0; k < 4; ++k) hx[k] = hx[k + 1];
             hx[4] = 0.f;
         }
-        // the next sigma_x(L) plane into the other tile buffer
-        if (z + 1 < z1e) tile_load(tile[(z + 1) & 1], z + 1);
         // ---- level L+2 at plane z-4 on the tile
         const int zp = z - 4;
         if (zp >= zs && zp < ze && act[0]) {
@@ -637,15 +691,14 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_t2_kernel(
             const float tz = sz_scaled<ARITH>(e9, dt2dz2);
             const long long gi = (long long)zp * g.plane + off[0];
             float px = hx[0], pzv = q[0][0];
-            const float a11 = c11[gi], a13 = c13[gi], a33 = c33[gi];
             float d = 1.f;
             if (ABC) {
                 d = dxy[0] * dzv[zp];
                 px = px * d;
                 pzv = pzv * d;
             }
-            float vx = leap<ARITH>(a11 * txy + a13 * tz, s0, px);
-            float vz = leap<ARITH>(a13 * txy + a33 * tz, q1[4], pzv);
+            float vx = leap<ARITH>(s11 * txy + s13 * tz, s0, px);
+            float vz = leap<ARITH>(s13 * txy + s33 * tz, q1[4], pzv);
             if (gi == src_idx) {
                 vx = vx + src2;
                 vz = vz + src2;
@@ -658,14 +711,21 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_t2_kernel(
             dz_[gi] = vz;
         }
         __syncthreads();
-        // shift the sigma_z(L) queues to plane z+1
+        // advance to plane z+1
 #pragma unroll
         for (int j = 0; j < NE; ++j) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) q[j][k] = q[j][k + 1];
-            const int zz = z + 5;
-            q[j][8] = (inr[j] && zz < g.zmax) ? bz[(long long)zz * g.plane + off[j]] : 0.f;
+            q[j][8] = qn[j];
+            pax[j] = nax[j];
+            k11[j] = n11[j];
+            k13[j] = n13[j];
         }
+        paz = naz;
+        k33 = n33;
+        s11 = t11;
+        s13 = t13;
+        s33 = t33;
     }
 }
 
