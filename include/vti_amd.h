@@ -171,6 +171,26 @@ int vti2d_objfunc_gradient(vti_ctx *ctx, float *objfunc, float *gc11, float *gc1
                            int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad,
                            float dx, float dz, float dt, float rate, float ot);
 
+/* rtm_f, lib/vti-fwi-2d/rtm_f.cu:13 (wave.h:6): dipole source and receivers
+ * (the pole one row below each point subtracted), the receiver side
+ * propagated with the forward operator, image += 0.5 (p+q) x the source
+ * wavefield slice interpolated in time.  image: nnx*nnz.  Needs
+ * (nt-1) % (rate/dt) == 0 (otherwise the reference reads a slice past its
+ * array), the source, every receiver and the points below them inside the
+ * updated interior, and no receiver right below another. */
+int vti2d_rtm(vti_ctx *ctx, float *image, const float *data, const float *c11, const float *c13, const float *c33,
+              const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz,
+              int nt, int npad, float dx, float dz, float dt, float rate, float ot);
+
+/* odcig_f, lib/vti-fwi-2d/odcig_f.cu:12 (wave.h:16): rtm_f with the
+ * subsurface-offset imaging condition, image[h][i] += 0.5 (p+q)(i-h) x
+ * slice(i+h), h = -nhx..nhx; image: (2*nhx+1) planes of nnx*nnz.  Keeps the
+ * reference's source scaling: dt2*wavelet[0] at the first level, wavelet[it-1]
+ * (no dt2) after it (odcig_f.cu:111,123). */
+int vti2d_odcig(vti_ctx *ctx, float *image, const float *data, const float *c11, const float *c13, const float *c33,
+                const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
+                int nz, int nhx, int nt, int npad, float dx, float dz, float dt, float rate, float ot);
+
 /* GNhessian_f, lib/vti-fwi-2d/hessian_f.cu:575 (wave.h:34). */
 int vti2d_gn_hessian(vti_ctx *ctx, float *gc11, float *gc13, float *gc33, const float *data, const float *c11,
                      const float *c13, const float *c33, const float *dc11, const float *dc13, const float *dc33,

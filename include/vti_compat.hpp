@@ -176,6 +176,12 @@ void GNhessian_f(float *gc11, float *gc13, float *gc33, const float *data, const
                  const float *c33, const float *dc11, const float *dc13, const float *dc33, const float *wavelet,
                  const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
                  int npad, float dx, float dz, float dt, float rate, float ot);
+void rtm_f(float *image, const float *data, const float *c11, const float *c13, const float *c33, const float *wavelet,
+           const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad,
+           float dx, float dz, float dt, float rate, float ot = 0.);
+void odcig_f(float *image, const float *data, const float *c11, const float *c13, const float *c33,
+             const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz,
+             int nhx, int nt, int npad, float dx, float dz, float dt, float rate, float ot = 0.);
 float objFuncGradientVEpsDel(float *gv, float *geps, float *gdel, const float *data, float *v, float *eps, float *del,
                              float v0, float eps0, float del0, const float *wavelet, const int *sloc, int ns,
                              const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad, float dx,
@@ -246,6 +252,26 @@ void GNhessian_f(float *gc11, float *gc13, float *gc33, const float *data, const
     vti_compat::report(vti2d_gn_hessian(vti_compat::context(), gc11, gc13, gc33, data, c11, c13, c33, dc11, dc13, dc33,
                                         wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot),
                        "GNhessian_f");
+}
+
+// rtm_f (wave.h:6, rtm_f.cu:13)
+void rtm_f(float *image, const float *data, const float *c11, const float *c13, const float *c33, const float *wavelet,
+           const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad,
+           float dx, float dz, float dt, float rate, float ot)
+{
+    vti_compat::report(vti2d_rtm(vti_compat::context(), image, data, c11, c13, c33, wavelet, sloc, ns, rloc, nr, taper,
+                                 nx, nz, nt, npad, dx, dz, dt, rate, ot),
+                       "rtm_f");
+}
+
+// odcig_f (wave.h:16, odcig_f.cu:12)
+void odcig_f(float *image, const float *data, const float *c11, const float *c13, const float *c33,
+             const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz,
+             int nhx, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    vti_compat::report(vti2d_odcig(vti_compat::context(), image, data, c11, c13, c33, wavelet, sloc, ns, rloc, nr,
+                                   taper, nx, nz, nhx, nt, npad, dx, dz, dt, rate, ot),
+                       "odcig_f");
 }
 
 // objFuncGradientVEpsDel (wave.h:10, objFuncGradient.cu:28)

@@ -186,6 +186,8 @@ def _lib2():
         L.orc2d_fg.restype = c_double
         L.orc2d_fg.argtypes = [c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + \
             [c_float] * 5
+        L.orc2d_rtm.argtypes = [c_int, c_int, c_int, P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 5 + \
+            [c_float] * 5
         L.orc2d_forwardCD_f64.argtypes = [P] * 7 + [c_double] * 3 + [c_int, c_int]
         L.orc2d_backwardDC_f64.argtypes = [P] * 7 + [c_double] * 3 + [c_int, c_int]
         L._has2 = True
@@ -224,6 +226,26 @@ def fg2d(mode, born, data, c11, c13, c33, dc, wavelet, sloc, rloc, taper, nx, nz
                          _p(a[3]), _p(sloc), sloc.size // 4, _p(rloc), rloc.size // 2, _p(a[4]), nx, nz, nt, npad, dx,
                          dz, dt, rate, ot)
     return f, gc
+
+
+def rtm2d(mode, data, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0,
+          nhx=-1, ngpu=1):
+    """rtm_f (nhx < 0) or odcig_f (nhx >= 0) restated literally, as the
+    reference runs them on ngpu GPUs.  Returns the image (nnz, nnx) or the
+    offset gathers (2*nhx+1, nnz, nnx)."""
+    c11, c13, c33, wavelet, taper, data = (np.ascontiguousarray(a, np.float32).ravel()
+                                           for a in (c11, c13, c33, wavelet, taper, data))
+    sloc = np.ascontiguousarray(sloc, np.int32).ravel()
+    rloc = np.ascontiguousarray(rloc, np.int32).ravel()
+    nnx, nnz = nx + 2 * npad, nz + 2 * npad
+    shape = (nnz, nnx) if nhx < 0 else (2 * nhx + 1, nnz, nnx)
+    img = np.zeros(shape, np.float64 if mode == F64 else np.float32)
+    rc = _lib2().orc2d_rtm(mode, int(nhx >= 0), int(ngpu), _p(img), _p(data), _p(c11), _p(c13), _p(c33), _p(wavelet),
+                           _p(sloc), sloc.size // 4, _p(rloc), rloc.size // 2, _p(taper), nx, nz, max(nhx, 0), nt, npad,
+                           dx, dz, dt, rate, ot)
+    if rc != 0:
+        raise RuntimeError("orc2d_rtm failed")
+    return img
 
 
 def cpu3d_steps(mode, state, a, b, c, npad, dx, dy, dz, dt, nsteps, src_index=-1, wavelet=None):

@@ -7,6 +7,8 @@
  *   lib/vti-fwi-2d/synthetic_f.cu:13-152        synthetic_f
  *   lib/vti-fwi-2d/objFuncGradient_f.cu:15-371  objFuncGradient_f
  *   lib/vti-fwi-2d/hessian_f.cu:575-1010        GNhessian_f
+ *   lib/vti-fwi-2d/rtm_f.cu:13-269              rtm_f
+ *   lib/vti-fwi-2d/odcig_f.cu:12-247            odcig_f
  * mode 0: the reference's FP32 arithmetic; mode 2: FP64.
  */
 #include <math.h>
@@ -132,4 +134,27 @@ void orc2d_backwardDC_f64(double *p1, double *q1, const double *p0, const double
     backwardDC_f64(p1, q1, p0, q0, c11, c13, c33, dx2, dz2, dt2, nnx, nnz, sp, sq);
     free(sp);
     free(sq);
+}
+
+/* rtm_f (odcig = 0) / odcig_f (odcig = 1) as run on ngpu GPUs.  image:
+ * nnxz (rtm) or (2*nhx+1)*nnxz (odcig) of float (mode 0) or double (mode 2). */
+int orc2d_rtm(int mode, int odcig, int ngpu, void *image, const float *data, const float *c11, const float *c13,
+              const float *c33, const float *wavelet, const int *sloc, int ns, const int *rloc, int nr,
+              const float *taper, int nx, int nz, int nhx, int nt, int npad, float dx, float dz, float dt, float rate,
+              float ot)
+{
+    const long n = (long)(nx + 2 * npad) * (nz + 2 * npad);
+    if (mode == 0) {
+        rtm_ngpu_f32(odcig, ngpu, (float *)image, data, c11, c13, c33, wavelet, sloc, ns, rloc, nr, taper, nx, nz, nhx,
+                     nt, npad, dx, dz, dt, rate, ot);
+        return 0;
+    }
+    if (mode != 2) return -1;
+    double *a = widen(c11, n), *b = widen(c13, n), *c = widen(c33, n);
+    rtm_ngpu_f64(odcig, ngpu, (double *)image, data, a, b, c, wavelet, sloc, ns, rloc, nr, taper, nx, nz, nhx, nt, npad,
+                 dx, dz, dt, rate, ot);
+    free(a);
+    free(b);
+    free(c);
+    return 0;
 }

@@ -364,6 +364,171 @@ static double FN(fg_or_gn)(int born, int ngpu, T *gc11, T *gc13, T *gc33, const 
     return obj;
 }
 
+/* ---- rtm_f (lib/vti-fwi-2d/rtm_f.cu:13-269) and odcig_f (odcig_f.cu:12-247) */
+
+/* injectDipoleSource, kernels.cu:251-257 */
+static void FN(dipole_source)(T *p, T *q, float v, long s, int nnx)
+{
+    p[s] += (T)v;
+    q[s] += (T)v;
+    p[s + nnx] -= (T)v;
+    q[s + nnx] -= (T)v;
+}
+
+/* injectDipoleData, kernels.cu:232-243 */
+static void FN(dipole_data)(T *p, T *q, const T *d0, const T *d1, float f, const int *rloc, int nr, int nnx, T dt2)
+{
+    const T ff = (T)f;
+    for (int ir = 0; ir < nr; ++ir) {
+        const long i = rloc[2 * ir] + (long)rloc[2 * ir + 1] * nnx;
+        const T s = (T)0.5 * (((T)1 - ff) * d0[ir] + ff * d1[ir]);
+        p[i] += dt2 * s;
+        q[i] += dt2 * s;
+        p[i + nnx] -= dt2 * s;
+        q[i + nnx] -= dt2 * s;
+    }
+}
+
+/* recordWavefieldSlice, kernels.cu:335-343 */
+static void FN(slice)(T *d, const T *p, const T *q, int nnx, int nnz)
+{
+    INTERIOR(d[i] = (T)0.5 * (p[i] + q[i]);)
+}
+
+/* imagingCrossCor, kernels.cu:414-423 */
+static void FN(image)(T *image, const T *ap, const T *aq, const T *s0, const T *s1, float f, int nnx, int nnz)
+{
+    const T ff = (T)f;
+    INTERIOR(const T d = ((T)1 - ff) * s0[i] + ff * s1[i]; image[i] += (T)0.5 * (ap[i] + aq[i]) * d;)
+}
+
+/* extendedImagingCrossCor, kernels.cu:425-436, over the points whose
+ * contributions can be nonzero (both shifted points inside the interior) */
+static void FN(ximage)(T *image, const T *ap, const T *aq, const T *s0, const T *s1, float f, int nnx, int nnz,
+                       int nhx)
+{
+    const T ff = (T)f;
+    const long nnxz = (long)nnx * nnz;
+#pragma omp parallel for schedule(static)
+    for (int iz = 4; iz < nnz - 4; ++iz)
+        for (int ix = (nhx > 4 ? nhx : 4); ix < nnx - (nhx > 4 ? nhx : 4); ++ix) {
+            const long i = ix + (long)iz * nnx;
+            for (int h = -nhx; h <= nhx; ++h) {
+                const T d = ((T)1 - ff) * s0[i + h] + ff * s1[i + h];
+                image[i + (h + nhx) * nnxz] += (T)0.5 * (ap[i - h] + aq[i - h]) * d;
+            }
+        }
+}
+
+/* the shots is0, is0+stride, ... of rtm_f (odcig == 0) or odcig_f on one GPU,
+ * its image accumulated over them */
+static void FN(rtm_gpu)(int odcig, int is0, int stride, T *image, const float *data, const T *c11, const T *c13,
+                        const T *c33, const float *wavelet, const int *sloc, int ns, const int *rloc, int nr,
+                        const float *taper, int nx, int nz, int nhx, int nt, int npad, float dx, float dz, float dt,
+                        float rate, float ot)
+{
+    const int ratio = (int)(rate / dt + 0.5f);
+    const int ntNeg = (int)round(fabs(ot) / dt);
+    const int nnt = (nt - 1) / ratio + 1;
+    const int nnt_data = (nt - ntNeg - 1) / ratio + 1;
+    const int nnx = nx + 2 * npad, nnz = nz + 2 * npad;
+    const long nnxz = (long)nnx * nnz;
+    const float dx2f = dx * dx, dz2f = dz * dz, dt2f = dt * dt;
+    const T dx2 = (T)dx2f, dz2 = (T)dz2f, dt2 = (T)dt2f;
+    T *p0 = (T *)calloc(nnxz, sizeof(T)), *q0 = (T *)calloc(nnxz, sizeof(T));
+    T *p1 = (T *)calloc(nnxz, sizeof(T)), *q1 = (T *)calloc(nnxz, sizeof(T));
+    T *src = (T *)calloc(nnxz * (nnt + 1), sizeof(T)); /* one slot past the end, read when (nt-1)%ratio != 0 */
+    T *s0 = (T *)calloc(nnxz, sizeof(T)), *s1 = (T *)calloc(nnxz, sizeof(T));
+    for (int is = is0; is < ns; is += stride) {
+        const long slocxz = sloc[4 * is] + (long)sloc[4 * is + 1] * nnx;
+        const int nri = sloc[4 * is + 2], r0 = sloc[4 * is + 3];
+        const int *rl = rloc + 2 * r0;
+        T *d0 = (T *)calloc(nri > 0 ? nri : 1, sizeof(T)), *d1 = (T *)calloc(nri > 0 ? nri : 1, sizeof(T));
+        memset(p0, 0, sizeof(T) * nnxz);
+        memset(q0, 0, sizeof(T) * nnxz);
+        memset(p1, 0, sizeof(T) * nnxz);
+        memset(q1, 0, sizeof(T) * nnxz);
+        FN(dipole_source)(p1, q1, dt2f * wavelet[0], slocxz, nnx);
+        FN(abc2)(p1, q1, taper, nnx, nnz);
+        if (ratio == 1) FN(slice)(src + nnxz, p1, q1, nnx, nnz);
+        for (int it = 2; it < nt; ++it) {
+            FN(forwardCD)(p0, q0, p1, q1, c11, c13, c33, dx2, dz2, dt2, nnx, nnz);
+            /* odcig_f.cu:123 injects wavelet[it-1] without the dt2 of its first level */
+            FN(dipole_source)(p0, q0, odcig ? wavelet[it - 1] : dt2f * wavelet[it - 1], slocxz, nnx);
+            FN(abc4)(p1, q1, p0, q0, taper, nnx, nnz);
+            if (it % ratio == 0) FN(slice)(src + (long)(it / ratio) * nnxz, p0, q0, nnx, nnz);
+            SWAP(p0, p1);
+            SWAP(q0, q1);
+        }
+        memset(p0, 0, sizeof(T) * nnxz);
+        memset(q0, 0, sizeof(T) * nnxz);
+        memset(p1, 0, sizeof(T) * nnxz);
+        memset(q1, 0, sizeof(T) * nnxz);
+        for (int ir = 0; ir < nri; ++ir) d0[ir] = (T)data[(long)(nnt_data - 1) * nr + r0 + ir];
+        FN(dipole_data)(p0, q0, d0, d1, 0.f, rl, nri, nnx, dt2);
+        FN(abc2)(p0, q0, taper, nnx, nnz);
+        float f = (float)((nt - 2.) / ratio);
+        int i1 = (int)f, i2 = i1 + 1;
+        memcpy(s0, src + (long)i1 * nnxz, sizeof(T) * nnxz);
+        memcpy(s1, src + (long)i2 * nnxz, sizeof(T) * nnxz);
+        f = f - i1;
+        if (odcig) FN(ximage)(image, p0, q0, s0, s1, f, nnx, nnz, nhx);
+        else FN(image)(image, p0, q0, s0, s1, f, nnx, nnz);
+        for (int it = nt - 3; it >= 0; --it) {
+            FN(forwardCD)(p1, q1, p0, q0, c11, c13, c33, dx2, dz2, dt2, nnx, nnz);
+            if (it >= ntNeg) {
+                f = (float)((it - ntNeg + 1.) / ratio);
+                i1 = (int)f;
+                if ((it - ntNeg + 2) % ratio == 0) {
+                    for (int ir = 0; ir < nri; ++ir) d1[ir] = (T)data[(long)i1 * nr + r0 + ir];
+                    SWAP(d0, d1);
+                }
+                f = f - i1;
+                FN(dipole_data)(p1, q1, d0, d1, f, rl, nri, nnx, dt2);
+            }
+            FN(abc4)(p1, q1, p0, q0, taper, nnx, nnz);
+            f = (float)it / ratio;
+            i1 = (int)f;
+            if ((it + 1) % ratio == 0) {
+                memcpy(s1, src + (long)i1 * nnxz, sizeof(T) * nnxz);
+                SWAP(s0, s1);
+            }
+            f = f - i1;
+            if (odcig) FN(ximage)(image, p1, q1, s0, s1, f, nnx, nnz, nhx);
+            else FN(image)(image, p1, q1, s0, s1, f, nnx, nnz);
+            SWAP(p0, p1);
+            SWAP(q0, q1);
+        }
+        free(d0);
+        free(d1);
+    }
+    free(p0);
+    free(q0);
+    free(p1);
+    free(q1);
+    free(src);
+    free(s0);
+    free(s1);
+}
+
+/* rtm_f / odcig_f as run on ngpu GPUs: shots dealt round robin, each GPU's
+ * image accumulated over its shots, then the GPUs' images summed in order */
+static void FN(rtm_ngpu)(int odcig, int ngpu, T *image, const float *data, const T *c11, const T *c13, const T *c33,
+                         const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper,
+                         int nx, int nz, int nhx, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    const long n = (long)(nx + 2 * npad) * (nz + 2 * npad) * (odcig ? 2 * nhx + 1 : 1);
+    memset(image, 0, sizeof(T) * n);
+    T *one = (T *)calloc(n, sizeof(T));
+    for (int g = 0; g < ngpu && g < ns; ++g) {
+        memset(one, 0, sizeof(T) * n);
+        FN(rtm_gpu)(odcig, g, ngpu, one, data, c11, c13, c33, wavelet, sloc, ns, rloc, nr, taper, nx, nz, nhx, nt,
+                    npad, dx, dz, dt, rate, ot);
+        for (long i = 0; i < n; ++i) image[i] += one[i];
+    }
+    free(one);
+}
+
 #undef SWAP
 #undef INTERIOR
 #undef ST

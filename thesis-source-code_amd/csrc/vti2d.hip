@@ -84,6 +84,118 @@ __global__ void narrow2d_kernel(float *__restrict__ out, const double *__restric
         out[i] = (float)acc[i];
 }
 
+// ---- rtm_f / odcig_f (lib/vti-fwi-2d/rtm_f.cu, odcig_f.cu) -----------------
+
+// level 1 of a dipole source: injectDipoleSource into zero fields, then abc
+// (rtm_f.cu:121-123)
+__global__ void dipole_level1_kernel(float *p, float *q, const float *__restrict__ taper, long long s, int nnx, float v)
+{
+    p[s] = v * taper[s];
+    q[s] = v * taper[s];
+    p[s + nnx] = -v * taper[s + nnx];
+    q[s + nnx] = -v * taper[s + nnx];
+}
+
+// recordWavefieldSlice (kernels.cu:335-343)
+__global__ void slice2d_kernel(float *__restrict__ d, const float *__restrict__ p, const float *__restrict__ q, Geo2 g)
+{
+    const int ix = 4 + blockIdx.x * blockDim.x + threadIdx.x, iz = 4 + blockIdx.y;
+    if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
+    const long long i = ix + (long long)iz * g.nnx;
+    d[i] = 0.5f * (p[i] + q[i]);
+}
+
+// first adjoint level: injectDipoleData of the last data row (f = 0) into
+// zero fields, then abc (rtm_f.cu:160-164)
+__global__ void dipole_first_kernel(float *__restrict__ p, float *__restrict__ q, const float *__restrict__ taper,
+                                    const long long *__restrict__ idx, const float *__restrict__ row, int nr, int nnx,
+                                    float dt2)
+{
+    const int ir = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ir >= nr) return;
+    const long long i = idx[ir], j = i + nnx;
+    const float s = 0.5f * ((1.f - 0.f) * row[ir] + 0.f * 0.f);
+    const float v = dt2 * s;
+    p[i] = v * taper[i];
+    q[i] = v * taper[i];
+    p[j] = -v * taper[j];
+    q[j] = -v * taper[j];
+}
+
+// imagingCrossCor (kernels.cu:414-423) on its own, for the first adjoint level
+__global__ void image2d_kernel(float *__restrict__ image, const float *__restrict__ ap, const float *__restrict__ aq,
+                               const float *__restrict__ s0, const float *__restrict__ s1, float f, Geo2 g)
+{
+    const int ix = 4 + blockIdx.x * blockDim.x + threadIdx.x, iz = 4 + blockIdx.y;
+    if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
+    const long long i = ix + (long long)iz * g.nnx;
+    const float d = (1.f - f) * s0[i] + f * s1[i];
+    image[i] += 0.5f * (ap[i] + aq[i]) * d;
+}
+
+// extendedImagingCrossCor (kernels.cu:425-436) over the points whose
+// contributions can be nonzero: both shifted points inside the interior
+// (elsewhere the reference adds products with a zero factor)
+__global__ void ximage2d_kernel(float *__restrict__ image, const float *__restrict__ ap, const float *__restrict__ aq,
+                                const float *__restrict__ s0, const float *__restrict__ s1, float f, Geo2 g, int nhx,
+                                int xlo)
+{
+    const int ix = xlo + blockIdx.x * blockDim.x + threadIdx.x, iz = 4 + blockIdx.y;
+    if (ix >= g.nnx - xlo || iz >= g.nnz - 4) return;
+    const long long i = ix + (long long)iz * g.nnx, n = (long long)g.nnx * g.nnz;
+    for (int h = -nhx; h <= nhx; ++h) {
+        const float d = (1.f - f) * s0[i + h] + f * s1[i + h];
+        image[i + (h + nhx) * n] += 0.5f * (ap[i - h] + aq[i - h]) * d;
+    }
+}
+
+// One adjoint level of rtm_f/odcig_f: forwardCD (the reference propagates the
+// receiver side with the forward operator, rtm_f.cu:172-173), injectDipoleData
+// (recmap: +, dipmap: the point below a receiver, -), abc, and for RTM the
+// imaging condition fused (image += 0.5 (p+q) * interpolated slice).
+template <bool IMAGE>
+__global__ __launch_bounds__(64 * kBY2) void rtm_bwd2d_kernel(
+    float *__restrict__ p_n, float *__restrict__ q_n, const float *__restrict__ p_c, const float *__restrict__ q_c,
+    const float *__restrict__ c11, const float *__restrict__ c13, const float *__restrict__ c33,
+    const float *__restrict__ taper, const int *__restrict__ recmap, const int *__restrict__ dipmap, int inject,
+    const float *__restrict__ r0, const float *__restrict__ r1, float fr, const float *__restrict__ s0,
+    const float *__restrict__ s1, float f, float *__restrict__ image, Geo2 g, float dx2, float dz2, float dt2)
+{
+    __shared__ vti::Tiles2 t;
+    const int x0 = 4 + blockIdx.x * 64, z0 = 4 + blockIdx.y * kBY2;
+    const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
+    vti::load_tiles<0>(t, p_c, q_c, nullptr, nullptr, nullptr, g, x0, z0);
+    __syncthreads();
+    if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
+    const long long i = ix + (long long)iz * g.nnx;
+    const float tap = taper[i];
+    const float tp = vti::dxx(t, dx2), tq = vti::dzz(t, dz2);
+    float pn = 2.f * p_c[i] + dt2 * (c11[i] * tp + c13[i] * tq) - p_n[i] * tap;
+    float qn = 2.f * q_c[i] + dt2 * (c13[i] * tp + c33[i] * tq) - q_n[i] * tap;
+    if (inject) {
+        const int ir = recmap[i];
+        if (ir >= 0) {
+            const float s = 0.5f * ((1.f - fr) * r0[ir] + fr * r1[ir]);
+            pn += dt2 * s;
+            qn += dt2 * s;
+        }
+        const int jr = dipmap[i];
+        if (jr >= 0) {
+            const float s = 0.5f * ((1.f - fr) * r0[jr] + fr * r1[jr]);
+            pn -= dt2 * s;
+            qn -= dt2 * s;
+        }
+    }
+    pn = pn * tap;
+    qn = qn * tap;
+    p_n[i] = pn;
+    q_n[i] = qn;
+    if (IMAGE) {
+        const float d = (1.f - f) * s0[i] + f * s1[i];
+        image[i] += 0.5f * (pn + qn) * d;
+    }
+}
+
 struct Call2 {
     vti_ctx *ctx;
     Geo2 g;
@@ -239,7 +351,7 @@ int vti2d_synthetic(vti_ctx *ctx, float *data, const float *c11, const float *c1
             hipLaunchKernelGGL((vti::fwd2d_kernel<false>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
                                F.q[0].p, nullptr, nullptr, nullptr, nullptr, F.c11.p, F.c13.p, F.c33.p, nullptr,
                                nullptr, nullptr, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, c.dt2 * wavelet[it - 1],
-                               nullptr, nullptr);
+                               nullptr, nullptr, -1LL);
             std::swap(F.p[0].p, F.p[1].p);
             std::swap(F.q[0].p, F.q[1].p);
             if (t >= 0. && (it - c.ntNeg) % c.ratio == 0 && s.nri) {
@@ -263,6 +375,152 @@ int vti2d_synthetic(vti_ctx *ctx, float *data, const float *c11, const float *c1
 namespace {
 
 // objFuncGradient_f (born = false) and GNhessian_f (born = true).
+// rtm_f (nhx < 0) / odcig_f (nhx >= 0): see vti_amd.h
+int rtm2d(vti_ctx *ctx, int nhx, float *image, const float *data, const float *c11, const float *c13,
+          const float *c33, const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper,
+          int nx, int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    const bool odcig = nhx >= 0;
+    Call2 c;
+    int rc = setup(c, ctx, nx, nz, nt, npad, dx, dz, dt, rate, ot, true);
+    if (rc) return rc;
+    VTI_CHECK(image && data && c11 && c13 && c33 && wavelet && sloc && rloc && taper && ns >= 0, "null argument");
+    VTI_CHECK((nt - 1) % c.ratio == 0,
+              "(nt-1) %% (rate/dt) must be 0: otherwise the reference reads one source-wavefield slice past its "
+              "array (got nt %d, ratio %d)", nt, c.ratio);
+    VTI_CHECK(!odcig || 2 * nhx + 9 <= c.g.nnx, "offset half-width %d too large for %d columns", nhx, c.g.nnx);
+    hipStream_t st = ctx->stream;
+    const size_t n = (size_t)c.nnxz;
+    const int nh = odcig ? 2 * nhx + 1 : 1;
+    Fields2 F;
+    rc = load_model(c, F, c11, c13, c33, taper);
+    if (rc) return rc;
+    DBuf<float> slices, img;
+    DBuf<double> img64;
+    DBuf<int> recmap, dipmap;
+    rc = slices.alloc((size_t)c.nnt * n, st);  // slot 0 stays zero, as the reference's
+    if (!rc) rc = img.alloc(n * nh, st);
+    if (!rc) rc = img64.alloc(n * nh, st);
+    if (!rc) rc = recmap.alloc(n, st);
+    if (!rc) rc = dipmap.alloc(n, st);
+    if (rc) return rc;
+    const dim3 rows((c.g.nnx - 8 + 255) / 256, c.g.nnz - 8);
+    const int xlo = odcig ? std::max(4, nhx) : 4;
+    const dim3 xrows((std::max(1, c.g.nnx - 2 * xlo) + 255) / 256, c.g.nnz - 8);
+    for (int is = 0; is < ns; ++is) {
+        Shot2 s;
+        rc = shot_of(c, sloc, is, rloc, nr, s);
+        if (rc) return rc;
+        VTI_CHECK(interior(c, s.ix, s.iz + 1), "shot %d: the dipole's second pole (%d,%d) outside the interior", is,
+                  s.ix, s.iz + 1);
+        std::vector<int> hp(n, -1), hm(n, -1);
+        for (int ir = 0; ir < s.nri; ++ir) {
+            const int x = (int)(s.idx[ir] % c.g.nnx), z = (int)(s.idx[ir] / c.g.nnx);
+            VTI_CHECK(interior(c, x, z) && interior(c, x, z + 1),
+                      "receiver %d: (%d,%d) and the point below must lie inside the updated interior", s.r0 + ir, x,
+                      z);
+            VTI_CHECK(hp[s.idx[ir]] < 0, "receivers %d and %d share a grid point", hp[s.idx[ir]] + s.r0, s.r0 + ir);
+            hp[s.idx[ir]] = ir;
+            hm[s.idx[ir] + c.g.nnx] = ir;
+        }
+        for (size_t k = 0; k < n; ++k)
+            VTI_CHECK(hp[k] < 0 || hm[k] < 0,
+                      "receiver %d lies right below receiver %d: the reference's dipole injection races there",
+                      hp[k] + s.r0, hm[k] + s.r0);
+        DBuf<long long> idx;
+        DBuf<float> tr;
+        rc = idx.alloc(s.nri, st);
+        // the shot's data rows, time-major, and one zero row past the last
+        if (!rc) rc = tr.alloc((size_t)(c.nnt_data + 1) * std::max(1, s.nri), st);
+        if (!rc && s.nri) rc = upload(c, idx.p, s.idx.data(), sizeof(long long) * s.nri);
+        if (!rc) rc = upload(c, recmap.p, hp.data(), n * sizeof(int));
+        if (!rc) rc = upload(c, dipmap.p, hm.data(), n * sizeof(int));
+        std::vector<float> d((size_t)c.nnt_data * std::max(1, s.nri), 0.f);
+        for (int row = 0; row < c.nnt_data; ++row)
+            for (int ir = 0; ir < s.nri; ++ir) d[(size_t)row * s.nri + ir] = data[(size_t)row * nr + s.r0 + ir];
+        if (!rc && s.nri) rc = upload(c, tr.p, d.data(), d.size() * 4);
+        if (rc) return rc;
+        VTI_HIP(hipMemsetAsync(img.p, 0, n * nh * 4, st));
+
+        // ---- source side (rtm_f.cu:110-150)
+        rc = zero2(c, F.p, F.q);
+        if (rc) return rc;
+        hipLaunchKernelGGL(dipole_level1_kernel, dim3(1), dim3(1), 0, st, F.p[0].p, F.q[0].p, F.taper.p, s.src,
+                           c.g.nnx, c.dt2 * wavelet[0]);
+        if (c.ratio == 1)
+            hipLaunchKernelGGL(slice2d_kernel, rows, dim3(256), 0, st, slices.p + n, F.p[0].p, F.q[0].p, c.g);
+        for (int it = 2; it < nt; ++it) {
+            // odcig_f.cu:123 injects wavelet[it-1] without the dt2 of its first level
+            const float sv = odcig ? wavelet[it - 1] : c.dt2 * wavelet[it - 1];
+            hipLaunchKernelGGL((vti::fwd2d_kernel<false>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
+                               F.q[0].p, nullptr, nullptr, nullptr, nullptr, F.c11.p, F.c13.p, F.c33.p, nullptr,
+                               nullptr, nullptr, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv, nullptr, nullptr,
+                               s.src + c.g.nnx);
+            std::swap(F.p[0].p, F.p[1].p);
+            std::swap(F.q[0].p, F.q[1].p);
+            if (it % c.ratio == 0)
+                hipLaunchKernelGGL(slice2d_kernel, rows, dim3(256), 0, st, slices.p + (size_t)(it / c.ratio) * n,
+                                   F.p[0].p, F.q[0].p, c.g);
+        }
+        VTI_HIP(hipGetLastError());
+
+        // ---- receiver side and imaging (rtm_f.cu:152-216)
+        rc = zero2(c, F.p, F.q);
+        if (rc) return rc;
+        if (s.nri)
+            hipLaunchKernelGGL(dipole_first_kernel, dim3((s.nri + 255) / 256), dim3(256), 0, st, F.p[0].p, F.q[0].p,
+                               F.taper.p, idx.p, tr.p + (size_t)(c.nnt_data - 1) * s.nri, s.nri, c.g.nnx, c.dt2);
+        auto slot = [&](int k) { return slices.p + (size_t)k * n; };
+        {
+            float f = (float)((nt - 2.) / c.ratio);
+            const int i1 = (int)f;
+            f = f - i1;
+            if (odcig)
+                hipLaunchKernelGGL(ximage2d_kernel, xrows, dim3(256), 0, st, img.p, F.p[0].p, F.q[0].p, slot(i1),
+                                   slot(i1 + 1), f, c.g, nhx, xlo);
+            else
+                hipLaunchKernelGGL(image2d_kernel, rows, dim3(256), 0, st, img.p, F.p[0].p, F.q[0].p, slot(i1),
+                                   slot(i1 + 1), f, c.g);
+        }
+        for (int it = nt - 3; it >= 0; --it) {
+            // the data rows bracketing the level: row i1 and row i1+1 (zero past the last)
+            int inj = 0, ir1 = 0;
+            float fr = 0.f;
+            if (it >= c.ntNeg && s.nri) {
+                fr = (float)((it - c.ntNeg + 1.) / c.ratio);
+                ir1 = (int)fr;
+                fr = fr - ir1;
+                inj = 1;
+            }
+            float f = (float)it / c.ratio;
+            const int i1 = (int)f;
+            f = f - i1;
+            const float *r0 = tr.p + (size_t)ir1 * s.nri, *r1 = tr.p + (size_t)(ir1 + 1) * s.nri;
+            if (odcig)
+                hipLaunchKernelGGL((rtm_bwd2d_kernel<false>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
+                                   F.q[0].p, F.c11.p, F.c13.p, F.c33.p, F.taper.p, recmap.p, dipmap.p, inj, r0, r1, fr,
+                                   slot(i1), slot(i1 + 1), f, img.p, c.g, c.dx2, c.dz2, c.dt2);
+            else
+                hipLaunchKernelGGL((rtm_bwd2d_kernel<true>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
+                                   F.q[0].p, F.c11.p, F.c13.p, F.c33.p, F.taper.p, recmap.p, dipmap.p, inj, r0, r1, fr,
+                                   slot(i1), slot(i1 + 1), f, img.p, c.g, c.dx2, c.dz2, c.dt2);
+            std::swap(F.p[0].p, F.p[1].p);
+            std::swap(F.q[0].p, F.q[1].p);
+            if (odcig)
+                hipLaunchKernelGGL(ximage2d_kernel, xrows, dim3(256), 0, st, img.p, F.p[0].p, F.q[0].p, slot(i1),
+                                   slot(i1 + 1), f, c.g, nhx, xlo);
+        }
+        hipLaunchKernelGGL(acc64_2d_kernel, dim3(launch_blocks((long long)(n * nh))), dim3(256), 0, st, img64.p, img.p,
+                           (long long)(n * nh));
+        VTI_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(narrow2d_kernel, dim3(launch_blocks((long long)(n * nh))), dim3(256), 0, st, img.p, img64.p,
+                       (long long)(n * nh));
+    VTI_HIP(hipMemcpyAsync(image, img.p, n * nh * 4, hipMemcpyDeviceToHost, st));
+    VTI_HIP(hipStreamSynchronize(st));
+    return vti::kOk;
+}
+
 int fg2d(vti_ctx *ctx, bool born, float *objfunc, float *gc11, float *gc13, float *gc33, const float *data,
          const float *c11, const float *c13, const float *c33, const float *dc11, const float *dc13, const float *dc33,
          const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz,
@@ -348,11 +606,11 @@ int fg2d(vti_ctx *ctx, bool born, float *objfunc, float *gc11, float *gc13, floa
             if (born)
                 hipLaunchKernelGGL((vti::fwd2d_kernel<true>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
                                    F.q[0].p, B.p[1].p, B.q[1].p, B.p[0].p, B.q[0].p, F.c11.p, F.c13.p, F.c33.p,
-                                   dc[0].p, dc[1].p, dc[2].p, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv, sp, sq);
+                                   dc[0].p, dc[1].p, dc[2].p, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv, sp, sq, -1LL);
             else
                 hipLaunchKernelGGL((vti::fwd2d_kernel<false>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
                                    F.q[0].p, nullptr, nullptr, nullptr, nullptr, F.c11.p, F.c13.p, F.c33.p, nullptr,
-                                   nullptr, nullptr, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv, sp, sq);
+                                   nullptr, nullptr, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv, sp, sq, -1LL);
             std::swap(F.p[0].p, F.p[1].p);
             std::swap(F.q[0].p, F.q[1].p);
             if (born) {
@@ -468,6 +726,26 @@ int vti2d_gn_hessian(vti_ctx *ctx, float *gc11, float *gc13, float *gc33, const 
     (void)data;
     return fg2d(ctx, true, nullptr, gc11, gc13, gc33, nullptr, c11, c13, c33, dc11, dc13, dc33, wavelet, sloc, ns,
                 rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot);
+}
+
+// rtm_f, lib/vti-fwi-2d/rtm_f.cu:13-269 (wave.h:6).  image: nnx*nnz.
+int vti2d_rtm(vti_ctx *ctx, float *image, const float *data, const float *c11, const float *c13, const float *c33,
+              const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz,
+              int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    return rtm2d(ctx, -1, image, data, c11, c13, c33, wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt,
+                 rate, ot);
+}
+
+// odcig_f, lib/vti-fwi-2d/odcig_f.cu:12-247 (wave.h:16).  image: (2*nhx+1)
+// planes of nnx*nnz, offset -nhx first.
+int vti2d_odcig(vti_ctx *ctx, float *image, const float *data, const float *c11, const float *c13, const float *c33,
+                const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
+                int nz, int nhx, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    VTI_CHECK(nhx >= 0, "offset half-width must be >= 0, got %d", nhx);
+    return rtm2d(ctx, nhx, image, data, c11, c13, c33, wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz,
+                 dt, rate, ot);
 }
 
 }  // extern "C"

@@ -91,7 +91,8 @@ __device__ __forceinline__ float dzz(const Tiles2 &t, float dz2)
 }
 
 // Forward step: forwardCD / forwardC semantics (kernels.cu:452-498, 546-556)
-// + injectSource + abc; optionally writes D of the current level (the
+// + injectSource (or, with src_neg >= 0, injectDipoleSource) + abc;
+// optionally writes D of the current level (the
 // snapshot the reference's D kernel takes, kernels.cu:345-387).
 // BORN: also steps the Born field dp (GNhessian_f, hessian_f.cu:752-806) with
 // the scattering source dt2*(dC * D p) (kernels.cu:282-291).
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(64 * kBY2) void fwd2d_kernel(
     const float *__restrict__ dq_c, const float *__restrict__ c11, const float *__restrict__ c13,
     const float *__restrict__ c33, const float *__restrict__ dc11, const float *__restrict__ dc13,
     const float *__restrict__ dc33, const float *__restrict__ taper, Geo2 g, float dx2, float dz2, float dt2,
-    long long src, float src_val, float *__restrict__ snap_p, float *__restrict__ snap_q)
+    long long src, float src_val, float *__restrict__ snap_p, float *__restrict__ snap_q, long long src_neg)
 {
     __shared__ Tiles2 t;
     __shared__ Tiles2 tb;
@@ -125,6 +126,10 @@ __global__ __launch_bounds__(64 * kBY2) void fwd2d_kernel(
     if (i == src) {
         pn += src_val;
         qn += src_val;
+    }
+    if (i == src_neg) {  // the negative pole of injectDipoleSource (kernels.cu:251-257)
+        pn -= src_val;
+        qn -= src_val;
     }
     p_n[i] = pn * tap;
     q_n[i] = qn * tap;

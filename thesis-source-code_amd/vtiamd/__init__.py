@@ -86,6 +86,8 @@ _SIGS = {
         [P, P, P, c_int, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int, P],
     ),
     "vti2d_synthetic": (c_int, [P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
+    "vti2d_rtm": (c_int, [P] * 7 + [P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
+    "vti2d_odcig": (c_int, [P] * 7 + [P, c_int, P, c_int, P] + [c_int] * 5 + [c_float] * 5),
     "vti2d_objfunc_gradient": (
         c_int,
         [P, P, P, P, P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5,
@@ -397,6 +399,26 @@ def objfunc_gradient2d(ctx, data, c11, c13, c33, wavelet, sloc, rloc, taper, nx,
                                       rloc.size // 2, ptr(taper), nx, nz, nt, npad, dx, dz, dt, rate, ot),
           "vti2d_objfunc_gradient")
     return f.value, gc
+
+
+def rtm2d(ctx, data, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0,
+          nhx=-1):
+    """rtm_f (nhx < 0): the image (nnz, nnx); odcig_f (nhx >= 0): the offset
+    gathers (2*nhx+1, nnz, nnx), offset -nhx first."""
+    c11, c13, c33, wavelet, taper, data = (f32(a).ravel() for a in (c11, c13, c33, wavelet, taper, data))
+    sloc, rloc = _i32(sloc), _i32(rloc)
+    nnx, nnz = nx + 2 * npad, nz + 2 * npad
+    if nhx < 0:
+        img = np.zeros((nnz, nnx), np.float32)
+        check(_lib.vti2d_rtm(ctx.h, ptr(img), ptr(data), ptr(c11), ptr(c13), ptr(c33), ptr(wavelet), ptr(sloc),
+                             sloc.size // 4, ptr(rloc), rloc.size // 2, ptr(taper), nx, nz, nt, npad, dx, dz, dt, rate,
+                             ot), "vti2d_rtm")
+    else:
+        img = np.zeros((2 * nhx + 1, nnz, nnx), np.float32)
+        check(_lib.vti2d_odcig(ctx.h, ptr(img), ptr(data), ptr(c11), ptr(c13), ptr(c33), ptr(wavelet), ptr(sloc),
+                               sloc.size // 4, ptr(rloc), rloc.size // 2, ptr(taper), nx, nz, nhx, nt, npad, dx, dz, dt,
+                               rate, ot), "vti2d_odcig")
+    return img
 
 
 def gn_hessian2d(ctx, c11, c13, c33, dc11, dc13, dc33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate,
