@@ -171,6 +171,18 @@ int vti2d_objfunc_gradient(vti_ctx *ctx, float *objfunc, float *gc11, float *gc1
                            int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad,
                            float dx, float dz, float dt, float rate, float ot);
 
+/* hessian_f, lib/vti-fwi-2d/hessian_f.cu:15 (wave.h:32): the full
+ * Hessian-vector product H*dc -- the Gauss-Newton part plus the second-order
+ * terms of the residual's adjoint (scatteringa), with the exact adjoint
+ * backwardDC.  The reference fills its Born snapshot pipeline's first buffer
+ * from the background snapshots (hessian_f.cu:381-382); this keeps that, so
+ * the product is affine in dc.  Same schedule conditions as
+ * vti2d_objfunc_gradient, and (nt-1)/(rate/dt) >= 3. */
+int vti2d_hessian(vti_ctx *ctx, float *gc11, float *gc13, float *gc33, const float *data, const float *c11,
+                  const float *c13, const float *c33, const float *dc11, const float *dc13, const float *dc33,
+                  const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
+                  int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot);
+
 /* rtm_f, lib/vti-fwi-2d/rtm_f.cu:13 (wave.h:6): dipole source and receivers
  * (the pole one row below each point subtracted), the receiver side
  * propagated with the forward operator, image += 0.5 (p+q) x the source

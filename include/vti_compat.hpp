@@ -176,6 +176,10 @@ void GNhessian_f(float *gc11, float *gc13, float *gc33, const float *data, const
                  const float *c33, const float *dc11, const float *dc13, const float *dc33, const float *wavelet,
                  const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
                  int npad, float dx, float dz, float dt, float rate, float ot);
+void hessian_f(float *gc11, float *gc13, float *gc33, const float *data, const float *c11, const float *c13,
+               const float *c33, const float *dc11, const float *dc13, const float *dc33, const float *wavelet,
+               const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad,
+               float dx, float dz, float dt, float rate, float ot);
 void rtm_f(float *image, const float *data, const float *c11, const float *c13, const float *c33, const float *wavelet,
            const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad,
            float dx, float dz, float dt, float rate, float ot = 0.);
@@ -252,6 +256,17 @@ void GNhessian_f(float *gc11, float *gc13, float *gc33, const float *data, const
     vti_compat::report(vti2d_gn_hessian(vti_compat::context(), gc11, gc13, gc33, data, c11, c13, c33, dc11, dc13, dc33,
                                         wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot),
                        "GNhessian_f");
+}
+
+// hessian_f (wave.h:32, hessian_f.cu:15)
+void hessian_f(float *gc11, float *gc13, float *gc33, const float *data, const float *c11, const float *c13,
+               const float *c33, const float *dc11, const float *dc13, const float *dc33, const float *wavelet,
+               const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad,
+               float dx, float dz, float dt, float rate, float ot)
+{
+    vti_compat::report(vti2d_hessian(vti_compat::context(), gc11, gc13, gc33, data, c11, c13, c33, dc11, dc13, dc33,
+                                     wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot),
+                       "hessian_f");
 }
 
 // rtm_f (wave.h:6, rtm_f.cu:13)

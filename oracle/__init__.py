@@ -188,6 +188,8 @@ def _lib2():
             [c_float] * 5
         L.orc2d_rtm.argtypes = [c_int, c_int, c_int, P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 5 + \
             [c_float] * 5
+        L.orc2d_hessian.argtypes = [c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + \
+            [c_float] * 5
         L.orc2d_forwardCD_f64.argtypes = [P] * 7 + [c_double] * 3 + [c_int, c_int]
         L.orc2d_backwardDC_f64.argtypes = [P] * 7 + [c_double] * 3 + [c_int, c_int]
         L._has2 = True
@@ -226,6 +228,26 @@ def fg2d(mode, born, data, c11, c13, c33, dc, wavelet, sloc, rloc, taper, nx, nz
                          _p(a[3]), _p(sloc), sloc.size // 4, _p(rloc), rloc.size // 2, _p(a[4]), nx, nz, nt, npad, dx,
                          dz, dt, rate, ot)
     return f, gc
+
+
+def hessian2d(mode, data, c11, c13, c33, dc, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0,
+              ngpu=1, seed_bg=True):
+    """hessian_f restated literally (its two snapshot rotations included), as
+    the reference runs it on ngpu GPUs.  Returns H*dc as (3, nnz, nnx).
+    seed_bg=False fills the Born snapshot pipeline's first buffer from the
+    Born snapshots instead of the background ones (a check only)."""
+    a = [np.ascontiguousarray(x, np.float32).ravel() for x in (c11, c13, c33, wavelet, taper, data)]
+    dcs = [np.ascontiguousarray(x, np.float32).ravel() for x in dc]
+    sloc = np.ascontiguousarray(sloc, np.int32).ravel()
+    rloc = np.ascontiguousarray(rloc, np.int32).ravel()
+    nnx, nnz = nx + 2 * npad, nz + 2 * npad
+    gc = np.zeros((3, nnz, nnx), np.float64 if mode == F64 else np.float32)
+    rc = _lib2().orc2d_hessian(mode, int(seed_bg), int(ngpu), _p(gc), _p(a[5]), _p(a[0]), _p(a[1]), _p(a[2]), *[_p(x) for x in dcs],
+                               _p(a[3]), _p(sloc), sloc.size // 4, _p(rloc), rloc.size // 2, _p(a[4]), nx, nz, nt, npad,
+                               dx, dz, dt, rate, ot)
+    if rc != 0:
+        raise RuntimeError("orc2d_hessian failed")
+    return gc
 
 
 def rtm2d(mode, data, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0,

@@ -7,6 +7,7 @@
  *   lib/vti-fwi-2d/synthetic_f.cu:13-152        synthetic_f
  *   lib/vti-fwi-2d/objFuncGradient_f.cu:15-371  objFuncGradient_f
  *   lib/vti-fwi-2d/hessian_f.cu:575-1010        GNhessian_f
+ *   lib/vti-fwi-2d/hessian_f.cu:15-573          hessian_f
  *   lib/vti-fwi-2d/rtm_f.cu:13-269              rtm_f
  *   lib/vti-fwi-2d/odcig_f.cu:12-247            odcig_f
  * mode 0: the reference's FP32 arithmetic; mode 2: FP64.
@@ -156,5 +157,35 @@ int orc2d_rtm(int mode, int odcig, int ngpu, void *image, const float *data, con
     free(a);
     free(b);
     free(c);
+    return 0;
+}
+
+/* hessian_f as run on ngpu GPUs; gc: 3 * nnxz of float (mode 0) or double
+ * (mode 2); seed_bg = 1 as the reference, 0 with its Born snapshot buffer
+ * seeded from the Born snapshots (a check) */
+int orc2d_hessian(int mode, int seed_bg, int ngpu, void *gc, const float *data, const float *c11, const float *c13,
+                  const float *c33, const float *dc11, const float *dc13, const float *dc33, const float *wavelet,
+                  const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
+                  int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    const long n = (long)(nx + 2 * npad) * (nz + 2 * npad);
+    if (mode == 0) {
+        float *g = (float *)gc;
+        hess_ngpu_f32(seed_bg, ngpu, g, g + n, g + 2 * n, data, c11, c13, c33, dc11, dc13, dc33, wavelet, sloc, ns, rloc, nr,
+                      taper, nx, nz, nt, npad, dx, dz, dt, rate, ot);
+        return 0;
+    }
+    if (mode != 2) return -1;
+    double *a = widen(c11, n), *b = widen(c13, n), *c = widen(c33, n);
+    double *da = widen(dc11, n), *db = widen(dc13, n), *dc = widen(dc33, n);
+    double *g = (double *)gc;
+    hess_ngpu_f64(seed_bg, ngpu, g, g + n, g + 2 * n, data, a, b, c, da, db, dc, wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt,
+                  npad, dx, dz, dt, rate, ot);
+    free(a);
+    free(b);
+    free(c);
+    free(da);
+    free(db);
+    free(dc);
     return 0;
 }

@@ -529,6 +529,243 @@ static void FN(rtm_ngpu)(int odcig, int ngpu, T *image, const float *data, const
     free(one);
 }
 
+
+/* ---- hessian_f (lib/vti-fwi-2d/hessian_f.cu:15-573): the full Hessian-vector
+ * product, restated literally (the two snapshot rotations through host
+ * buffers included) */
+
+/* scatteringa, kernels.cu:150-195: dp += dt2 * D(dC lambda) */
+static void FN(scatteringa)(T *dp, T *dq, const T *p0, const T *q0, const T *dc11, const T *dc13, const T *dc33, T dx2,
+                            T dz2, T dt2, int nnx, int nnz, T *sp, T *sq)
+{
+    const long n = (long)nnx * nnz;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        sp[i] = dc11[i] * p0[i] + dc13[i] * q0[i];
+        sq[i] = dc13[i] * p0[i] + dc33[i] * q0[i];
+    }
+    INTERIOR(const T tp = ST(sp, i, 1) / dx2; const T tq = ST(sq, i, nnx) / dz2; dp[i] += dt2 * tp;
+             dq[i] += dt2 * tq;)
+}
+
+static void FN(hess_gpu)(int seed_bg, int is0, int stride, T *gc11, T *gc13, T *gc33, const float *data, const T *c11,
+                         const T *c13, const T *c33, const T *dc11, const T *dc13, const T *dc33, const float *wavelet,
+                         const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
+                         int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    const int ratio = (int)(rate / dt + 0.5f);
+    const int ntNeg = (int)round(fabs(ot) / dt);
+    const int nnt = (nt - 1) / ratio + 1;
+    const int nnt_data = (nt - ntNeg - 1) / ratio + 1;
+    const int nnx = nx + 2 * npad, nnz = nz + 2 * npad;
+    const long nnxz = (long)nnx * nnz;
+    const float dx2f = dx * dx, dz2f = dz * dz, dt2f = dt * dt;
+    const T dx2 = (T)dx2f, dz2 = (T)dz2f, dt2 = (T)dt2f;
+
+    T *p0 = (T *)calloc(nnxz, sizeof(T)), *q0 = (T *)calloc(nnxz, sizeof(T));
+    T *p1 = (T *)calloc(nnxz, sizeof(T)), *q1 = (T *)calloc(nnxz, sizeof(T));
+    T *dp0 = (T *)calloc(nnxz, sizeof(T)), *dq0 = (T *)calloc(nnxz, sizeof(T));
+    T *dp1 = (T *)calloc(nnxz, sizeof(T)), *dq1 = (T *)calloc(nnxz, sizeof(T));
+    T *sp = (T *)calloc(nnxz, sizeof(T)), *sq = (T *)calloc(nnxz, sizeof(T));
+    /* device snapshot buffers of the background (D) and Born (dD) fields */
+    T *d_Dpq = (T *)calloc(2 * nnxz, sizeof(T)), *d_Dpqa = (T *)calloc(2 * nnxz, sizeof(T));
+    T *d_Dpqb = (T *)calloc(2 * nnxz, sizeof(T));
+    T *d_Ddpq = (T *)calloc(2 * nnxz, sizeof(T)), *d_Ddpqa = (T *)calloc(2 * nnxz, sizeof(T));
+    T *d_Ddpqb = (T *)calloc(2 * nnxz, sizeof(T));
+    /* host copies of every snapshot and the two pinned staging buffers */
+    T *Dp = (T *)calloc(nnxz * nnt, sizeof(T)), *Dq = (T *)calloc(nnxz * nnt, sizeof(T));
+    T *Ddp = (T *)calloc(nnxz * nnt, sizeof(T)), *Ddq = (T *)calloc(nnxz * nnt, sizeof(T));
+    T *Dpqa = (T *)calloc(2 * nnxz, sizeof(T)), *Dpqb = (T *)calloc(2 * nnxz, sizeof(T));
+    T *Ddpqa = (T *)calloc(2 * nnxz, sizeof(T)), *Ddpqb = (T *)calloc(2 * nnxz, sizeof(T));
+    T *gd11 = (T *)calloc(nnxz, sizeof(T)), *gd13 = (T *)calloc(nnxz, sizeof(T)), *gd33 = (T *)calloc(nnxz, sizeof(T));
+    T *d_Dp1 = d_Dpq, *d_Dq1 = d_Dpq + nnxz, *d_Dp0 = d_Dpqa, *d_Dq0 = d_Dpqa + nnxz;
+    T *d_Ddp1 = d_Ddpq, *d_Ddq1 = d_Ddpq + nnxz, *d_Ddp0 = d_Ddpqa, *d_Ddq0 = d_Ddpqa + nnxz;
+
+    for (int is = is0; is < ns; is += stride) {
+        const long slocxz = sloc[4 * is] + (long)sloc[4 * is + 1] * nnx;
+        const int nri = sloc[4 * is + 2];
+        const int *rl = rloc + 2 * sloc[4 * is + 3];
+        const long nrd = (long)(nri > 0 ? nri : 1);
+        T *bg = (T *)calloc(nrd * nnt_data, sizeof(T)), *bd = (T *)calloc(nrd * nnt_data, sizeof(T));
+        T *drec = (T *)calloc(nrd, sizeof(T));
+        memset(p0, 0, sizeof(T) * nnxz);
+        memset(q0, 0, sizeof(T) * nnxz);
+        memset(p1, 0, sizeof(T) * nnxz);
+        memset(q1, 0, sizeof(T) * nnxz);
+        memset(dp0, 0, sizeof(T) * nnxz);
+        memset(dq0, 0, sizeof(T) * nnxz);
+        memset(dp1, 0, sizeof(T) * nnxz);
+        memset(dq1, 0, sizeof(T) * nnxz);
+        {
+            const float v0 = dt2f * wavelet[0];
+            p1[slocxz] += (T)v0;
+            q1[slocxz] += (T)v0;
+        }
+        FN(D)(d_Dpq, d_Dpq + nnxz, p1, q1, dx2, dz2, nnx, nnz);
+        for (int it = 2; it < nt; ++it) {
+            const float t = it * dt + ot;
+            FN(forwardC)(p0, q0, p1, q1, d_Dpq, d_Dpq + nnxz, c11, c13, c33, dt2, nnx, nnz);
+            FN(forwardC)(dp0, dq0, dp1, dq1, d_Ddpq, d_Ddpq + nnxz, c11, c13, c33, dt2, nnx, nnz);
+            const float v = dt2f * wavelet[it - 1];
+            p0[slocxz] += (T)v;
+            q0[slocxz] += (T)v;
+            FN(scattering)(dp0, dq0, dc11, dc13, dc33, d_Dpq, d_Dpq + nnxz, dt2, nnx, nnz);
+            FN(abc4)(p1, q1, p0, q0, taper, nnx, nnz);
+            FN(abc4)(dp1, dq1, dp0, dq0, taper, nnx, nnz);
+            FN(D)(d_Dpq, d_Dpq + nnxz, p0, q0, dx2, dz2, nnx, nnz);
+            FN(D)(d_Ddpq, d_Ddpq + nnxz, dp0, dq0, dx2, dz2, nnx, nnz);
+            if (t >= 0.f && (it - ntNeg) % ratio == 0) {
+                FN(record)(drec, p0, q0, rl, nri, nnx);
+                memcpy(bg + (long)((it - ntNeg) / ratio) * nri, drec, sizeof(T) * nri);
+                FN(record)(drec, dp0, dq0, rl, nri, nnx);
+                memcpy(bd + (long)((it - ntNeg) / ratio) * nri, drec, sizeof(T) * nri);
+            }
+            if (it % ratio == 0) {
+                if (it != nt - 1) {
+                    memcpy(d_Dpqa, d_Dpq, sizeof(T) * 2 * nnxz);
+                    memcpy(d_Ddpqa, d_Ddpq, sizeof(T) * 2 * nnxz);
+                }
+                if (it > ratio) {
+                    memcpy(Dp + (long)(it / ratio - 1) * nnxz, Dpqa, sizeof(T) * nnxz);
+                    memcpy(Dq + (long)(it / ratio - 1) * nnxz, Dpqa + nnxz, sizeof(T) * nnxz);
+                    memcpy(Ddp + (long)(it / ratio - 1) * nnxz, Ddpqa, sizeof(T) * nnxz);
+                    memcpy(Ddq + (long)(it / ratio - 1) * nnxz, Ddpqa + nnxz, sizeof(T) * nnxz);
+                }
+            }
+            if (it % ratio == 1 && it != nt - ratio) {
+                memcpy(Dpqa, d_Dpqa, sizeof(T) * 2 * nnxz);
+                memcpy(Ddpqa, d_Ddpqa, sizeof(T) * 2 * nnxz);
+            }
+            SWAP(p0, p1);
+            SWAP(q0, q1);
+            SWAP(dp0, dp1);
+            SWAP(dq0, dq1);
+        }
+        for (int it = 0; it < nnt_data; ++it)
+            for (int ir = 0; ir < nri; ++ir)
+                bg[(long)it * nri + ir] -= (T)data[(long)it * nr + sloc[4 * is + 3] + ir];
+        memset(p0, 0, sizeof(T) * nnxz);
+        memset(q0, 0, sizeof(T) * nnxz);
+        memset(p1, 0, sizeof(T) * nnxz);
+        memset(q1, 0, sizeof(T) * nnxz);
+        memset(dp0, 0, sizeof(T) * nnxz);
+        memset(dq0, 0, sizeof(T) * nnxz);
+        memset(dp1, 0, sizeof(T) * nnxz);
+        memset(dq1, 0, sizeof(T) * nnxz);
+        T *b0 = (T *)calloc(nrd, sizeof(T)), *b1 = (T *)calloc(nrd, sizeof(T));
+        T *n0 = (T *)calloc(nrd, sizeof(T)), *n1 = (T *)calloc(nrd, sizeof(T));
+        memcpy(b0, bg + (long)(nnt_data - 1) * nri, sizeof(T) * nri);
+        FN(inject_data)(p0, q0, b0, b1, 0.f, rl, nri, nnx, dt2);
+        memcpy(n0, bd + (long)(nnt_data - 1) * nri, sizeof(T) * nri);
+        FN(inject_data)(dp0, dq0, n0, n1, 0.f, rl, nri, nnx, dt2);
+        float f = (nt - 2.) / ratio;
+        int i1 = (int)f;
+        memcpy(Dpqb, Dp + (long)(nnt - 3) * nnxz, sizeof(T) * nnxz);
+        memcpy(Dpqb + nnxz, Dq + (long)(nnt - 3) * nnxz, sizeof(T) * nnxz);
+        /* hessian_f.cu:381-382 fills the Born buffer from the background
+         * snapshots (seed_bg = 1, the reference); seed_bg = 0 takes the Born
+         * snapshots, the exact derivative the tests check by finite differences */
+        memcpy(Ddpqb, (seed_bg ? Dp : Ddp) + (long)(nnt - 3) * nnxz, sizeof(T) * nnxz);
+        memcpy(Ddpqb + nnxz, (seed_bg ? Dq : Ddq) + (long)(nnt - 3) * nnxz, sizeof(T) * nnxz);
+        f = f - i1;
+        FN(xcor)(gd11, gd13, gd33, p0, q0, d_Ddp0, d_Ddq0, d_Ddp1, d_Ddq1, f, nnx, nnz);
+        FN(xcor)(gd11, gd13, gd33, dp0, dq0, d_Dp0, d_Dq0, d_Dp1, d_Dq1, f, nnx, nnz);
+        for (int it = nt - 3; it >= 0; --it) {
+            FN(backwardDC)(p1, q1, p0, q0, c11, c13, c33, dx2, dz2, dt2, nnx, nnz, sp, sq);
+            FN(backwardDC)(dp1, dq1, dp0, dq0, c11, c13, c33, dx2, dz2, dt2, nnx, nnz, sp, sq);
+            if (it % ratio == 0) {
+                SWAP(Dpqa, Dpqb);
+                SWAP(Ddpqa, Ddpqb);
+                memcpy(d_Dpqb, Dpqa, sizeof(T) * 2 * nnxz);
+                memcpy(d_Ddpqb, Ddpqa, sizeof(T) * 2 * nnxz);
+                if (it >= 2 * ratio) {
+                    memcpy(Dpqb, Dp + (long)(it / ratio - 2) * nnxz, sizeof(T) * nnxz);
+                    memcpy(Dpqb + nnxz, Dq + (long)(it / ratio - 2) * nnxz, sizeof(T) * nnxz);
+                    memcpy(Ddpqb, Ddp + (long)(it / ratio - 2) * nnxz, sizeof(T) * nnxz);
+                    memcpy(Ddpqb + nnxz, Ddq + (long)(it / ratio - 2) * nnxz, sizeof(T) * nnxz);
+                }
+            }
+            if (it >= ntNeg) {
+                f = (it - ntNeg + 1.) / ratio;
+                i1 = (int)f;
+                if ((it - ntNeg + 2) % ratio == 0) {
+                    memcpy(b1, bg + (long)i1 * nri, sizeof(T) * nri);
+                    SWAP(b0, b1);
+                    memcpy(n1, bd + (long)i1 * nri, sizeof(T) * nri);
+                    SWAP(n0, n1);
+                }
+                f = f - i1;
+                FN(inject_data)(p1, q1, b0, b1, f, rl, nri, nnx, dt2);
+                FN(inject_data)(dp1, dq1, n0, n1, f, rl, nri, nnx, dt2);
+                FN(scatteringa)(dp1, dq1, p0, q0, dc11, dc13, dc33, dx2, dz2, dt2, nnx, nnz, sp, sq);
+            }
+            FN(abc4)(p1, q1, p0, q0, taper, nnx, nnz);
+            FN(abc4)(dp1, dq1, dp0, dq0, taper, nnx, nnz);
+            f = (float)it / ratio;
+            i1 = (int)f;
+            if ((it + 1) % ratio == 0) {
+                T *tt = d_Dpqb;
+                d_Dpqb = d_Dpq;
+                d_Dpq = d_Dpqa;
+                d_Dpqa = tt;
+                d_Dp1 = d_Dpq;
+                d_Dq1 = d_Dpq + nnxz;
+                d_Dp0 = d_Dpqa;
+                d_Dq0 = d_Dpqa + nnxz;
+                tt = d_Ddpqb;
+                d_Ddpqb = d_Ddpq;
+                d_Ddpq = d_Ddpqa;
+                d_Ddpqa = tt;
+                d_Ddp1 = d_Ddpq;
+                d_Ddq1 = d_Ddpq + nnxz;
+                d_Ddp0 = d_Ddpqa;
+                d_Ddq0 = d_Ddpqa + nnxz;
+            }
+            f = f - i1;
+            FN(xcor)(gd11, gd13, gd33, p1, q1, d_Ddp0, d_Ddq0, d_Ddp1, d_Ddq1, f, nnx, nnz);
+            FN(xcor)(gd11, gd13, gd33, dp1, dq1, d_Dp0, d_Dq0, d_Dp1, d_Dq1, f, nnx, nnz);
+            SWAP(p0, p1);
+            SWAP(q0, q1);
+            SWAP(dp0, dp1);
+            SWAP(dq0, dq1);
+        }
+        free(b0);
+        free(b1);
+        free(n0);
+        free(n1);
+        free(bg);
+        free(bd);
+        free(drec);
+    }
+    for (long i = 0; i < nnxz; ++i) {
+        gc11[i] += gd11[i];
+        gc13[i] += gd13[i];
+        gc33[i] += gd33[i];
+    }
+    free(p0); free(q0); free(p1); free(q1);
+    free(dp0); free(dq0); free(dp1); free(dq1);
+    free(sp); free(sq);
+    free(d_Dpq); free(d_Dpqa); free(d_Dpqb);
+    free(d_Ddpq); free(d_Ddpqa); free(d_Ddpqb);
+    free(Dp); free(Dq); free(Ddp); free(Ddq);
+    free(Dpqa); free(Dpqb); free(Ddpqa); free(Ddpqb);
+    free(gd11); free(gd13); free(gd33);
+}
+
+static void FN(hess_ngpu)(int seed_bg, int ngpu, T *gc11, T *gc13, T *gc33, const float *data, const T *c11, const T *c13,
+                          const T *c33, const T *dc11, const T *dc13, const T *dc33, const float *wavelet,
+                          const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
+                          int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    const long nnxz = (long)(nx + 2 * npad) * (nz + 2 * npad);
+    memset(gc11, 0, sizeof(T) * nnxz);
+    memset(gc13, 0, sizeof(T) * nnxz);
+    memset(gc33, 0, sizeof(T) * nnxz);
+    for (int g = 0; g < ngpu && g < ns; ++g)
+        FN(hess_gpu)(seed_bg, g, ngpu, gc11, gc13, gc33, data, c11, c13, c33, dc11, dc13, dc33, wavelet, sloc, ns, rloc, nr,
+                     taper, nx, nz, nt, npad, dx, dz, dt, rate, ot);
+}
+
 #undef SWAP
 #undef INTERIOR
 #undef ST

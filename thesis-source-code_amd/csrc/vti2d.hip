@@ -351,7 +351,7 @@ int vti2d_synthetic(vti_ctx *ctx, float *data, const float *c11, const float *c1
             hipLaunchKernelGGL((vti::fwd2d_kernel<false>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
                                F.q[0].p, nullptr, nullptr, nullptr, nullptr, F.c11.p, F.c13.p, F.c33.p, nullptr,
                                nullptr, nullptr, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, c.dt2 * wavelet[it - 1],
-                               nullptr, nullptr, -1LL);
+                               nullptr, nullptr, -1LL, nullptr, nullptr);
             std::swap(F.p[0].p, F.p[1].p);
             std::swap(F.q[0].p, F.q[1].p);
             if (t >= 0. && (it - c.ntNeg) % c.ratio == 0 && s.nri) {
@@ -455,7 +455,7 @@ int rtm2d(vti_ctx *ctx, int nhx, float *image, const float *data, const float *c
             hipLaunchKernelGGL((vti::fwd2d_kernel<false>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
                                F.q[0].p, nullptr, nullptr, nullptr, nullptr, F.c11.p, F.c13.p, F.c33.p, nullptr,
                                nullptr, nullptr, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv, nullptr, nullptr,
-                               s.src + c.g.nnx);
+                               s.src + c.g.nnx, nullptr, nullptr);
             std::swap(F.p[0].p, F.p[1].p);
             std::swap(F.q[0].p, F.q[1].p);
             if (it % c.ratio == 0)
@@ -606,11 +606,11 @@ int fg2d(vti_ctx *ctx, bool born, float *objfunc, float *gc11, float *gc13, floa
             if (born)
                 hipLaunchKernelGGL((vti::fwd2d_kernel<true>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
                                    F.q[0].p, B.p[1].p, B.q[1].p, B.p[0].p, B.q[0].p, F.c11.p, F.c13.p, F.c33.p,
-                                   dc[0].p, dc[1].p, dc[2].p, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv, sp, sq, -1LL);
+                                   dc[0].p, dc[1].p, dc[2].p, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv, sp, sq, -1LL, nullptr, nullptr);
             else
                 hipLaunchKernelGGL((vti::fwd2d_kernel<false>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
                                    F.q[0].p, nullptr, nullptr, nullptr, nullptr, F.c11.p, F.c13.p, F.c33.p, nullptr,
-                                   nullptr, nullptr, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv, sp, sq, -1LL);
+                                   nullptr, nullptr, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv, sp, sq, -1LL, nullptr, nullptr);
             std::swap(F.p[0].p, F.p[1].p);
             std::swap(F.q[0].p, F.q[1].p);
             if (born) {
@@ -701,6 +701,190 @@ int fg2d(vti_ctx *ctx, bool born, float *objfunc, float *gc11, float *gc13, floa
     return vti::kOk;
 }
 
+// hessian_f (lib/vti-fwi-2d/hessian_f.cu:15-573): the full Hessian-vector
+// product, per shot:
+//  - forward: background and Born fields as GNhessian_f, recording both traces
+//    and the D snapshots of both at every rate/dt levels;
+//  - the residual r = background traces - data;
+//  - backward: the residual's adjoint lambda and its perturbation dlambda
+//    (driven by the Born traces and by dt2 * D(dC lambda)), both with the
+//    exact adjoint backwardDC, imaged against the Born and the background
+//    snapshots respectively.
+// The reference routes the Born snapshots through its pipeline with the first
+// host buffer filled from the background ones (hessian_f.cu:381-382); every use
+// of that buffer is an image of Born slot nnt-3, so the background slot
+// nnt-3 is copied over the Born one before the backward pass.
+int hess2d_full(vti_ctx *ctx, float *gc11, float *gc13, float *gc33, const float *data, const float *c11,
+                const float *c13, const float *c33, const float *dc11, const float *dc13, const float *dc33,
+                const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
+                int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    Call2 c;
+    int rc = setup(c, ctx, nx, nz, nt, npad, dx, dz, dt, rate, ot, true);
+    if (rc) return rc;
+    VTI_CHECK(gc11 && gc13 && gc33 && data && c11 && c13 && c33 && dc11 && dc13 && dc33 && wavelet && sloc && rloc &&
+                  taper && ns >= 0,
+              "null argument");
+    VTI_CHECK(c.ratio >= 2 && (nt - 1) % c.ratio == 0 && c.ntNeg == 0,
+              "the reference's snapshot rotation lines up only for rate/dt >= 2, (nt-1) %% (rate/dt) == 0 and ot == 0 "
+              "(got ratio %d, nt %d, ot %g)",
+              c.ratio, nt, (double)ot);
+    VTI_CHECK(c.nnt >= 4, "hessian_f reads snapshot slot nnt-3 of at least 4 (got nnt %d)", c.nnt);
+    hipStream_t st = ctx->stream;
+    const size_t n = (size_t)c.nnxz;
+    Fields2 F, B;
+    rc = load_model(c, F, c11, c13, c33, taper);
+    if (rc) return rc;
+    DBuf<float> dc[3], snap, bsnap, acc[3];
+    DBuf<double> acc64[3];
+    DBuf<int> recmap;
+    for (int k = 0; k < 2 && !rc; ++k) {
+        rc = B.p[k].alloc(n, st);
+        if (!rc) rc = B.q[k].alloc(n, st);
+    }
+    const float *src[3] = {dc11, dc13, dc33};
+    for (int k = 0; k < 3 && !rc; ++k) {
+        rc = dc[k].alloc(n, st);
+        if (!rc) rc = upload(c, dc[k].p, src[k], n * 4);
+    }
+    if (!rc) rc = snap.alloc((size_t)c.nnt * 2 * n, st);
+    if (!rc) rc = bsnap.alloc((size_t)c.nnt * 2 * n, st);
+    for (int k = 0; k < 3 && !rc; ++k) {
+        rc = acc[k].alloc(n, st);
+        if (!rc) rc = acc64[k].alloc(n, st);
+    }
+    if (!rc) rc = recmap.alloc(n, st);
+    if (rc) return rc;
+    auto sp = [&](DBuf<float> &b, int k) { return b.p + (size_t)k * 2 * n; };
+    auto sq = [&](DBuf<float> &b, int k) { return b.p + (size_t)k * 2 * n + n; };
+    std::vector<int> hmap(n, -1);
+    for (int is = 0; is < ns; ++is) {
+        Shot2 s;
+        rc = shot_of(c, sloc, is, rloc, nr, s);
+        if (rc) return rc;
+        for (int ir = 0; ir < s.nri; ++ir) {
+            const int x = (int)(s.idx[ir] % c.g.nnx), z = (int)(s.idx[ir] / c.g.nnx);
+            VTI_CHECK(interior(c, x, z), "receiver %d outside the updated interior", s.r0 + ir);
+            VTI_CHECK(hmap[s.idx[ir]] < 0, "receivers %d and %d share a grid point", hmap[s.idx[ir]] + s.r0, s.r0 + ir);
+            hmap[s.idx[ir]] = ir;
+        }
+        DBuf<long long> idx;
+        DBuf<float> bg, bd, obs, zrow;
+        DBuf<double> part;
+        const int nblk = launch_blocks((long long)s.nri * c.nnt_data);
+        rc = idx.alloc(s.nri, st);
+        if (!rc) rc = bg.alloc((size_t)s.nri * c.nnt_data, st);
+        if (!rc) rc = bd.alloc((size_t)s.nri * c.nnt_data, st);
+        if (!rc) rc = zrow.alloc(s.nri, st);
+        if (!rc) rc = part.alloc(nblk, st);
+        if (!rc && s.nri) rc = upload(c, idx.p, s.idx.data(), sizeof(long long) * s.nri);
+        if (!rc) rc = upload(c, recmap.p, hmap.data(), n * sizeof(int));
+        if (rc) return rc;
+
+        // ---- forward (hessian_f.cu:204-280)
+        rc = zero2(c, F.p, F.q);
+        if (!rc) rc = zero2(c, B.p, B.q);
+        if (rc) return rc;
+        hipLaunchKernelGGL(set2_kernel, dim3(1), dim3(1), 0, st, F.p[0].p, F.q[0].p, s.src, c.dt2 * wavelet[0]);
+        for (int it = 2; it < nt; ++it) {
+            const int lc = it - 1;
+            const bool due = lc % c.ratio == 0;
+            const float sv = c.dt2 * wavelet[it - 1];
+            hipLaunchKernelGGL((vti::fwd2d_kernel<true>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
+                               F.q[0].p, B.p[1].p, B.q[1].p, B.p[0].p, B.q[0].p, F.c11.p, F.c13.p, F.c33.p, dc[0].p,
+                               dc[1].p, dc[2].p, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, s.src, sv,
+                               due ? sp(snap, lc / c.ratio) : nullptr, due ? sq(snap, lc / c.ratio) : nullptr, -1LL,
+                               due ? sp(bsnap, lc / c.ratio) : nullptr, due ? sq(bsnap, lc / c.ratio) : nullptr);
+            std::swap(F.p[0].p, F.p[1].p);
+            std::swap(F.q[0].p, F.q[1].p);
+            std::swap(B.p[0].p, B.p[1].p);
+            std::swap(B.q[0].p, B.q[1].p);
+            if (it % c.ratio == 0 && s.nri) {
+                const size_t row = (size_t)(it / c.ratio) * s.nri;
+                hipLaunchKernelGGL(record2d_kernel, dim3((s.nri + 255) / 256), dim3(256), 0, st, bg.p + row,
+                                   F.p[0].p, F.q[0].p, idx.p, s.nri);
+                hipLaunchKernelGGL(record2d_kernel, dim3((s.nri + 255) / 256), dim3(256), 0, st, bd.p + row,
+                                   B.p[0].p, B.q[0].p, idx.p, s.nri);
+            }
+        }
+        hipLaunchKernelGGL(vti::dsnap2d_kernel, c.grid, c.block, 0, st, F.p[0].p, F.q[0].p, c.g, c.dx2, c.dz2,
+                           sp(snap, c.nnt - 1), sq(snap, c.nnt - 1));
+        hipLaunchKernelGGL(vti::dsnap2d_kernel, c.grid, c.block, 0, st, B.p[0].p, B.q[0].p, c.g, c.dx2, c.dz2,
+                           sp(bsnap, c.nnt - 1), sq(bsnap, c.nnt - 1));
+        VTI_HIP(hipMemcpyAsync(sp(bsnap, c.nnt - 3), sp(snap, c.nnt - 3), 2 * n * 4, hipMemcpyDeviceToDevice, st));
+        VTI_HIP(hipGetLastError());
+
+        // ---- residual (hessian_f.cu:300-304)
+        {
+            std::vector<float> o((size_t)s.nri * c.nnt_data);
+            for (int row = 0; row < c.nnt_data; ++row)
+                for (int ir = 0; ir < s.nri; ++ir) o[(size_t)row * s.nri + ir] = data[(size_t)row * nr + s.r0 + ir];
+            rc = obs.alloc(o.size(), st);
+            if (!rc && !o.empty()) rc = upload(c, obs.p, o.data(), o.size() * 4);
+            if (rc) return rc;
+            hipLaunchKernelGGL(residual2d_kernel, dim3(nblk), dim3(256), 0, st, bg.p, obs.p,
+                               (long long)s.nri * c.nnt_data, part.p);
+        }
+
+        // ---- backward (hessian_f.cu:306-443)
+        rc = zero2(c, F.p, F.q);
+        if (!rc) rc = zero2(c, B.p, B.q);
+        if (rc) return rc;
+        if (s.nri) {
+            hipLaunchKernelGGL(inject2d_kernel, dim3((s.nri + 255) / 256), dim3(256), 0, st, F.p[0].p, F.q[0].p,
+                               idx.p, bg.p + (size_t)(c.nnt_data - 1) * s.nri, zrow.p, 0.f, s.nri, c.dt2);
+            hipLaunchKernelGGL(inject2d_kernel, dim3((s.nri + 255) / 256), dim3(256), 0, st, B.p[0].p, B.q[0].p,
+                               idx.p, bd.p + (size_t)(c.nnt_data - 1) * s.nri, zrow.p, 0.f, s.nri, c.dt2);
+        }
+        {
+            float f = (float)((nt - 2.) / c.ratio);
+            const int i1 = (int)f;
+            f = f - i1;
+            const dim3 xg((c.g.nnx - 8 + 255) / 256, c.g.nnz - 8);
+            hipLaunchKernelGGL(vti::xcor2d_kernel, xg, dim3(256), 0, st, F.p[0].p, F.q[0].p, sp(bsnap, i1),
+                               sq(bsnap, i1), sp(bsnap, i1 + 1), sq(bsnap, i1 + 1), f, acc[0].p, acc[1].p, acc[2].p,
+                               c.g);
+            hipLaunchKernelGGL(vti::xcor2d_kernel, xg, dim3(256), 0, st, B.p[0].p, B.q[0].p, sp(snap, i1),
+                               sq(snap, i1), sp(snap, i1 + 1), sq(snap, i1 + 1), f, acc[0].p, acc[1].p, acc[2].p, c.g);
+        }
+        for (int it = nt - 3; it >= 0; --it) {
+            float fr = (float)((it + 1.) / c.ratio);
+            const int ir1 = (int)fr;
+            fr = fr - ir1;
+            float f = (float)it / c.ratio;
+            const int i1 = (int)f;
+            f = f - i1;
+            const size_t o0 = (size_t)ir1 * s.nri, o1 = (size_t)(ir1 + 1) * s.nri;
+            hipLaunchKernelGGL(vti::hess_bwd2d_kernel, c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
+                               F.q[0].p, B.p[1].p, B.q[1].p, B.p[0].p, B.q[0].p, F.c11.p, F.c13.p, F.c33.p, dc[0].p,
+                               dc[1].p, dc[2].p, F.taper.p, recmap.p, bg.p + o0, bg.p + o1, bd.p + o0, bd.p + o1, fr,
+                               sp(bsnap, i1), sq(bsnap, i1), sp(bsnap, i1 + 1), sq(bsnap, i1 + 1), sp(snap, i1),
+                               sq(snap, i1), sp(snap, i1 + 1), sq(snap, i1 + 1), f, acc[0].p, acc[1].p, acc[2].p, c.g,
+                               c.dx2, c.dz2, c.dt2);
+            std::swap(F.p[0].p, F.p[1].p);
+            std::swap(F.q[0].p, F.q[1].p);
+            std::swap(B.p[0].p, B.p[1].p);
+            std::swap(B.q[0].p, B.q[1].p);
+        }
+        VTI_HIP(hipGetLastError());
+        for (int k = 0; k < 3; ++k) {
+            hipLaunchKernelGGL(acc64_2d_kernel, dim3(launch_blocks((long long)n)), dim3(256), 0, st, acc64[k].p,
+                               acc[k].p, (long long)n);
+            VTI_HIP(hipMemsetAsync(acc[k].p, 0, n * 4, st));
+        }
+        for (int ir = 0; ir < s.nri; ++ir) hmap[s.idx[ir]] = -1;
+        VTI_HIP(hipStreamSynchronize(st));
+    }
+    float *out[3] = {gc11, gc13, gc33};
+    for (int k = 0; k < 3; ++k) {
+        hipLaunchKernelGGL(narrow2d_kernel, dim3(launch_blocks((long long)n)), dim3(256), 0, st, acc[k].p, acc64[k].p,
+                           (long long)n);
+        VTI_HIP(hipMemcpyAsync(out[k], acc[k].p, n * 4, hipMemcpyDeviceToHost, st));
+    }
+    VTI_HIP(hipStreamSynchronize(st));
+    return vti::kOk;
+}
+
 }  // namespace
 
 extern "C" {
@@ -726,6 +910,16 @@ int vti2d_gn_hessian(vti_ctx *ctx, float *gc11, float *gc13, float *gc33, const 
     (void)data;
     return fg2d(ctx, true, nullptr, gc11, gc13, gc33, nullptr, c11, c13, c33, dc11, dc13, dc33, wavelet, sloc, ns,
                 rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot);
+}
+
+// hessian_f, lib/vti-fwi-2d/hessian_f.cu:15-573 (wave.h:32).
+int vti2d_hessian(vti_ctx *ctx, float *gc11, float *gc13, float *gc33, const float *data, const float *c11,
+                  const float *c13, const float *c33, const float *dc11, const float *dc13, const float *dc33,
+                  const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
+                  int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    return hess2d_full(ctx, gc11, gc13, gc33, data, c11, c13, c33, dc11, dc13, dc33, wavelet, sloc, ns, rloc, nr,
+                       taper, nx, nz, nt, npad, dx, dz, dt, rate, ot);
 }
 
 // rtm_f, lib/vti-fwi-2d/rtm_f.cu:13-269 (wave.h:6).  image: nnx*nnz.

@@ -95,7 +95,8 @@ __device__ __forceinline__ float dzz(const Tiles2 &t, float dz2)
 // optionally writes D of the current level (the
 // snapshot the reference's D kernel takes, kernels.cu:345-387).
 // BORN: also steps the Born field dp (GNhessian_f, hessian_f.cu:752-806) with
-// the scattering source dt2*(dC * D p) (kernels.cu:282-291).
+// the scattering source dt2*(dC * D p) (kernels.cu:282-291), and optionally
+// writes D of the Born level read (bsnap).
 template <bool BORN>
 __global__ __launch_bounds__(64 * kBY2) void fwd2d_kernel(
     float *__restrict__ p_n, float *__restrict__ q_n, const float *__restrict__ p_c, const float *__restrict__ q_c,
@@ -103,7 +104,8 @@ __global__ __launch_bounds__(64 * kBY2) void fwd2d_kernel(
     const float *__restrict__ dq_c, const float *__restrict__ c11, const float *__restrict__ c13,
     const float *__restrict__ c33, const float *__restrict__ dc11, const float *__restrict__ dc13,
     const float *__restrict__ dc33, const float *__restrict__ taper, Geo2 g, float dx2, float dz2, float dt2,
-    long long src, float src_val, float *__restrict__ snap_p, float *__restrict__ snap_q, long long src_neg)
+    long long src, float src_val, float *__restrict__ snap_p, float *__restrict__ snap_q, long long src_neg,
+    float *__restrict__ bsnap_p, float *__restrict__ bsnap_q)
 {
     __shared__ Tiles2 t;
     __shared__ Tiles2 tb;
@@ -135,6 +137,10 @@ __global__ __launch_bounds__(64 * kBY2) void fwd2d_kernel(
     q_n[i] = qn * tap;
     if (BORN) {
         const float btp = dxx(tb, dx2), btq = dzz(tb, dz2);
+        if (bsnap_p) {  // D of the Born level read (hessian_f.cu:236)
+            bsnap_p[i] = btp;
+            bsnap_q[i] = btq;
+        }
         float dpn = 2.f * dp_c[i] + dt2 * (a11 * btp + a13 * btq) - dp_n[i] * tap;
         float dqn = 2.f * dq_c[i] + dt2 * (a13 * btp + a33 * btq) - dq_n[i] * tap;
         dpn += dt2 * (dc11[i] * tp + dc13[i] * tq);
@@ -209,6 +215,74 @@ __global__ __launch_bounds__(64 * kBY2) void bwd2d_kernel(
     g11[i] += pn * Dp;
     g13[i] += pn * Dq + qn * Dp;
     g33[i] += qn * Dq;
+}
+
+// hessian_f's adjoint level (hessian_f.cu:387-443): the residual's adjoint
+// lambda and its perturbation dlambda each take a backwardDC step
+// (kernels.cu:104-148) with their injected traces (the residual, the Born
+// data), dlambda the second-order source dt2 * D(dC lambda) (scatteringa,
+// kernels.cu:150-195), abc, and the image of both: lambda against the Born
+// snapshots, then dlambda against the background ones.
+static __global__ __attribute__((unused)) __launch_bounds__(64 * kBY2) void hess_bwd2d_kernel(
+    float *__restrict__ p_n, float *__restrict__ q_n, const float *__restrict__ p_c, const float *__restrict__ q_c,
+    float *__restrict__ dp_n, float *__restrict__ dq_n, const float *__restrict__ dp_c,
+    const float *__restrict__ dq_c, const float *__restrict__ c11, const float *__restrict__ c13,
+    const float *__restrict__ c33, const float *__restrict__ dc11, const float *__restrict__ dc13,
+    const float *__restrict__ dc33, const float *__restrict__ taper, const int *__restrict__ recmap,
+    const float *__restrict__ r0, const float *__restrict__ r1, const float *__restrict__ b0,
+    const float *__restrict__ b1, float fr, const float *__restrict__ BD0p, const float *__restrict__ BD0q,
+    const float *__restrict__ BD1p, const float *__restrict__ BD1q, const float *__restrict__ D0p,
+    const float *__restrict__ D0q, const float *__restrict__ D1p, const float *__restrict__ D1q, float f,
+    float *__restrict__ g11, float *__restrict__ g13, float *__restrict__ g33, Geo2 g, float dx2, float dz2,
+    float dt2)
+{
+    __shared__ Tiles2 ta, tb, ts;
+    const int x0 = 4 + blockIdx.x * 64, z0 = 4 + blockIdx.y * kBY2;
+    const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
+    load_tiles<1>(ta, p_c, q_c, c11, c13, c33, g, x0, z0);
+    load_tiles<1>(tb, dp_c, dq_c, c11, c13, c33, g, x0, z0);
+    load_tiles<1>(ts, p_c, q_c, dc11, dc13, dc33, g, x0, z0);
+    __syncthreads();
+    if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
+    const long long i = ix + (long long)iz * g.nnx;
+    const float tap = taper[i];
+    float pn = 2.f * p_c[i] + dt2 * dxx(ta, dx2) - p_n[i] * tap;
+    float qn = 2.f * q_c[i] + dt2 * dzz(ta, dz2) - q_n[i] * tap;
+    float dpn = 2.f * dp_c[i] + dt2 * dxx(tb, dx2) - dp_n[i] * tap;
+    float dqn = 2.f * dq_c[i] + dt2 * dzz(tb, dz2) - dq_n[i] * tap;
+    const int ir = recmap[i];
+    if (ir >= 0) {
+        const float s = 0.5f * ((1.f - fr) * r0[ir] + fr * r1[ir]);
+        pn += dt2 * s;
+        qn += dt2 * s;
+        const float sb = 0.5f * ((1.f - fr) * b0[ir] + fr * b1[ir]);
+        dpn += dt2 * sb;
+        dqn += dt2 * sb;
+    }
+    dpn += dt2 * dxx(ts, dx2);
+    dqn += dt2 * dzz(ts, dz2);
+    pn = pn * tap;
+    qn = qn * tap;
+    dpn = dpn * tap;
+    dqn = dqn * tap;
+    p_n[i] = pn;
+    q_n[i] = qn;
+    dp_n[i] = dpn;
+    dq_n[i] = dqn;
+    const float BDp = (1.f - f) * BD0p[i] + f * BD1p[i];
+    const float BDq = (1.f - f) * BD0q[i] + f * BD1q[i];
+    const float Dp = (1.f - f) * D0p[i] + f * D1p[i];
+    const float Dq = (1.f - f) * D0q[i] + f * D1q[i];
+    float a11 = g11[i], a13 = g13[i], a33 = g33[i];
+    a11 += pn * BDp;
+    a13 += pn * BDq + qn * BDp;
+    a33 += qn * BDq;
+    a11 += dpn * Dp;
+    a13 += dpn * Dq + dqn * Dp;
+    a33 += dqn * Dq;
+    g11[i] = a11;
+    g13[i] = a13;
+    g33[i] = a33;
 }
 
 // gradientCrossCor on a given level (the first adjoint level, which the

@@ -86,6 +86,7 @@ _SIGS = {
         [P, P, P, c_int, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int, P],
     ),
     "vti2d_synthetic": (c_int, [P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
+    "vti2d_hessian": (c_int, [P] * 13 + [c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
     "vti2d_rtm": (c_int, [P] * 7 + [P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
     "vti2d_odcig": (c_int, [P] * 7 + [P, c_int, P, c_int, P] + [c_int] * 5 + [c_float] * 5),
     "vti2d_objfunc_gradient": (
@@ -399,6 +400,19 @@ def objfunc_gradient2d(ctx, data, c11, c13, c33, wavelet, sloc, rloc, taper, nx,
                                       rloc.size // 2, ptr(taper), nx, nz, nt, npad, dx, dz, dt, rate, ot),
           "vti2d_objfunc_gradient")
     return f.value, gc
+
+
+def hessian2d(ctx, data, c11, c13, c33, dc11, dc13, dc33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt,
+              rate, ot=0.0):
+    """hessian_f: the full Hessian-vector product H*dc as (3, nnz, nnx)."""
+    arrs = [f32(a).ravel() for a in (data, c11, c13, c33, dc11, dc13, dc33, wavelet, taper)]
+    sloc, rloc = _i32(sloc), _i32(rloc)
+    nnx, nnz = nx + 2 * npad, nz + 2 * npad
+    gc = np.zeros((3, nnz, nnx), np.float32)
+    check(_lib.vti2d_hessian(ctx.h, ptr(gc[0]), ptr(gc[1]), ptr(gc[2]), *[ptr(a) for a in arrs[:8]], ptr(sloc),
+                             sloc.size // 4, ptr(rloc), rloc.size // 2, ptr(arrs[8]), nx, nz, nt, npad, dx, dz, dt,
+                             rate, ot), "vti2d_hessian")
+    return gc
 
 
 def rtm2d(ctx, data, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0,
