@@ -292,6 +292,8 @@ def main():
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--small", action="store_true", help="small grid (profiling runs)")
     ap.add_argument("--no-extra", action="store_true", help="only the headline kernel (profiling runs)")
+    ap.add_argument("--only", choices=["shot2d", "batch2d", "serve"],
+                    help="run one secondary workload alone and print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -302,6 +304,17 @@ def main():
         sys.exit(2)
 
     import vtiamd
+
+    if args.only:
+        with vtiamd.Context(0) as ctx:
+            if args.only == "shot2d":
+                r = shot_gradient_2d(vtiamd, ctx)
+            elif args.only == "batch2d":
+                r = batch_gradient_2d(vtiamd, ctx)
+            else:
+                r = serve_stream(vtiamd)
+        print(json.dumps({"workload": args.only, **r}), flush=True)
+        return
 
     nz, ny, nx, npad = (460, 572, 572, 30) if not args.small else (132, 140, 140, 10)
     dx, dt = 12.5, 1.0e-3
