@@ -125,8 +125,11 @@ def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7, variants=None):
     rloc = np.array(rloc, np.int32)
     nnt = (nt - 1) // int(rate / dt + 0.5) + 1
     data = np.zeros(int(sum(nnt * s[4] for s in shots)), np.float32)
+    tuned = vtiamd.tune_batch2d(ctx, shots, rloc, NX, nz, nt, npad, dx, dx, dt, rate)
+    if variants is None:
+        variants = {"autotuned": {k: tuned[k] for k in ("levels", "tile_rows", "max_wave")}}
     timed = {}
-    for name, opts in (variants or {"default": {}}).items():
+    for name, opts in variants.items():
         t0 = time.perf_counter()
         vtiamd.objfunc_gradient2d_batch(ctx, data, c11, c13, c33, w, shots, rloc, NX, nz, nt, npad, dx, dx, dt, rate,
                                         **opts)
@@ -150,7 +153,7 @@ def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7, variants=None):
     return {"shots": ns, "model_padded": [nnz, NNX], "aperture_cols": [300, 900], "steps": nt - 1,
             "batched_shot_gradients_per_s": round(ns / tb, 3), "batched_s": round(tb, 3),
             "plain_shot_gradients_per_s": round(nplain / tp, 3), "plain_shots_timed": nplain,
-            "speedup": round((ns / tb) / (nplain / tp), 2),
+            "speedup": round((ns / tb) / (nplain / tp), 2), "launch_shape": tuned,
             **({"variants_shot_gradients_per_s": {k: round(ns / v, 3) for k, v in timed.items()}}
                if variants else {})}
 

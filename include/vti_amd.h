@@ -262,6 +262,16 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
                                  int nt, int npad, float dx, float dz, float dt, float rate, int max_wave, int levels,
                                  int tile_rows, int snap_fmt);
 
+/* Launch-shape autotuner for vti2d_objfunc_gradient_batch: times short runs
+ * (24 snapshot intervals) of the same shots on a synthetic model over levels
+ * per launch (1, 2), tile rows (8, 16) and wave size (all shots that fit, at
+ * most 8) and returns the fastest in *levels, *tile_rows, *max_wave (and its
+ * trial time in *best_ms when not NULL).  Cached per geometry for the life of
+ * the process.  No counterpart in the reference. */
+int vti2d_batch_tune(vti_ctx *ctx, const int *shots, int ns, const int *rloc, int nr, int NX, int nz, int nt,
+                     int npad, float dx, float dz, float dt, float rate, int *levels, int *tile_rows, int *max_wave,
+                     double *best_ms);
+
 /* ---- L-BFGS (FP64) -----------------------------------------------------------
  * lbfgs, lib/lbfgs/lbfgs_c.cpp:11 (lbfgs.h:32) with mcsrch/mcstep,
  * lib/lbfgs/lsearch.cpp:9-606: reverse communication.  Start with *iflag = 0

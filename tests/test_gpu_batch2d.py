@@ -152,3 +152,22 @@ def test_16bit_snapshots_vs_fp64_oracle(vti, orc, ctx, snap, tol):
                                            B["nt"], B["npad"], B["dx"], B["dz"], B["dt"], B["rate"], snap=snap,
                                            levels=1)
     assert np.array_equal(g16, g16b)
+
+
+def test_autotuned_launch_shape(vti, ctx):
+    """The tuner returns one of its candidates, caches it, and the batch run
+    with it gives the same bits as the default launch shape."""
+    B = batch_problem(vti, seed=13, ns=5)
+    a = (B["shots"], B["rloc"], B["NX"], B["nz"], B["nt"], B["npad"], B["dx"], B["dz"], B["dt"], B["rate"])
+    t = vti.tune_batch2d(ctx, *a)
+    _say("batch2d tuned", t)
+    assert t["levels"] in (1, 2) and t["tile_rows"] in (8, 16) and t["max_wave"] in (0, 8)
+    assert t["trial_ms"] > 0
+    assert vti.tune_batch2d(ctx, *a) == t  # cached
+    data = observed(vti, ctx, B)
+    run = lambda **kw: vti.objfunc_gradient2d_batch(ctx, data, *B["c"], B["w"], B["shots"], B["rloc"], B["NX"],
+                                                    B["nz"], B["nt"], B["npad"], B["dx"], B["dz"], B["dt"], B["rate"],
+                                                    **kw)
+    f0, g0 = run()
+    f1, g1 = run(levels=t["levels"], tile_rows=t["tile_rows"], max_wave=t["max_wave"])
+    assert np.array_equal(g0, g1) and f0 == f1

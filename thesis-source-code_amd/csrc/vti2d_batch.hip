@@ -11,8 +11,12 @@
 // columns, shot by shot in input order.  The batch is split into waves that
 // fit the device memory budget; within a wave all shots advance together.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "vti2d_batch_kernels.h"
@@ -464,6 +468,86 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
     for (int k = 0; k < 3; ++k) VTI_HIP(hipMemcpyAsync(out[k], A[k].p, gsize * 8, hipMemcpyDeviceToHost, st));
     VTI_HIP(hipStreamSynchronize(st));
     *objfunc = 0.5 * obj;
+    return vti::kOk;
+}
+
+// Launch-shape autotuner for vti2d_objfunc_gradient_batch (see vti_amd.h).
+// Times short runs of the same shots (nt cut to 24 snapshot intervals) with a
+// synthetic model and zero data over: levels per launch 1/2, tile rows 8/16,
+// waves of all shots that fit / at most 8.  The pick is cached per geometry
+// for the life of the process.
+int vti2d_batch_tune(vti_ctx *ctx, const int *shots, int ns, const int *rloc, int nr, int NX, int nz, int nt,
+                     int npad, float dx, float dz, float dt, float rate, int *levels, int *tile_rows, int *max_wave,
+                     double *best_ms)
+{
+    VTI_CHECK(ctx && shots && rloc && levels && tile_rows && max_wave, "vti2d_batch_tune: null argument");
+    VTI_CHECK(ns >= 1 && NX > 0 && nz > 0 && npad >= 0 && nt >= 3 && dt > 0 && rate > 0,
+              "vti2d_batch_tune: bad sizes");
+    const int ratio = (int)(rate / dt + 0.5f);
+    VTI_CHECK(ratio >= 2 && (nt - 1) % ratio == 0, "vti2d_batch_tune: schedule needs rate/dt >= 2 and "
+                                                   "(nt-1) %% (rate/dt) == 0");
+    const int nt_t = std::min(nt, 24 * ratio + 1);
+    using Key = std::tuple<int, int, int, int, int, int, long long>;
+    static std::mutex mu;
+    static std::map<Key, std::tuple<int, int, int, double>> cache;
+    long long h = 1469598103934665603LL;
+    for (int k = 0; k < 6 * ns; ++k) h = (h ^ shots[k]) * 1099511628211LL;
+    for (int k = 0; k < 2 * nr; ++k) h = (h ^ rloc[k]) * 1099511628211LL;
+    const Key key{ctx->device, ns, NX, nz, npad, ratio, h};
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            std::tie(*levels, *tile_rows, *max_wave, std::ignore) = it->second;
+            if (best_ms) *best_ms = std::get<3>(it->second);
+            return vti::kOk;
+        }
+    }
+    const int NNX = NX + 2 * npad, nnz = nz + 2 * npad;
+    const size_t g = (size_t)NNX * nnz;
+    std::vector<float> c33(g, 2000.f * 2000.f), c11(g), c13(g);
+    for (size_t i = 0; i < g; ++i) {
+        c11[i] = 1.2f * c33[i];
+        c13[i] = 1.05f * c33[i];
+    }
+    std::vector<float> w(nt_t, 0.f);
+    if (nt_t > 1) w[1] = 1.f;
+    const int nnt = (nt_t - 1) / ratio + 1;
+    long long ndata = 0;
+    for (int s = 0; s < ns; ++s) ndata += (long long)nnt * shots[6 * s + 4];
+    std::vector<float> data((size_t)std::max<long long>(ndata, 1), 0.f);
+    std::vector<double> gc(3 * g);
+    double obj = 0;
+    int bl = 2, br = 16, bw = 0;
+    double best = 1e300;
+    const int waves[2] = {0, 8};
+    for (int lv = 1; lv <= 2; ++lv)
+        for (int rows : {8, 16})
+            for (int wv : waves) {
+                if (lv == 1 && rows != 8) continue;  // the tile height only shapes two-level launches
+                if (wv && wv >= ns) continue;
+                VTI_HIP(hipStreamSynchronize(ctx->stream));
+                const auto t0 = std::chrono::steady_clock::now();
+                const int rc = vti2d_objfunc_gradient_batch(ctx, &obj, gc.data(), gc.data() + g, gc.data() + 2 * g,
+                                                            data.data(), c11.data(), c13.data(), c33.data(), w.data(),
+                                                            shots, ns, rloc, nr, NX, nz, nt_t, npad, dx, dz, dt, rate,
+                                                            wv, lv, rows, 0);
+                if (rc) return rc;
+                const double ms =
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (ms < best) {
+                    best = ms;
+                    bl = lv;
+                    br = rows;
+                    bw = wv;
+                }
+            }
+    *levels = bl;
+    *tile_rows = br;
+    *max_wave = bw;
+    if (best_ms) *best_ms = best;
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = std::make_tuple(bl, br, bw, best);
     return vti::kOk;
 }
 

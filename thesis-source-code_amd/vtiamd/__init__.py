@@ -98,6 +98,8 @@ _SIGS = {
         [P] * 13 + [c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5,
     ),
     "vti_check_eps_del": (c_int, [P, P, c_float, c_float, c_longlong, P]),
+    "vti2d_batch_tune": (c_int, [P, P, c_int, P, c_int] + [c_int] * 4 + [c_float] * 4
+                         + [POINTER(c_int)] * 3 + [POINTER(ctypes.c_double)]),
     "vti2d_objfunc_gradient_batch": (
         c_int,
         [P] * 11 + [c_int, P] + [c_int] * 5 + [c_float] * 4 + [c_int] * 4,
@@ -549,6 +551,19 @@ class LBFGS:
 
 
 SNAP_FORMATS = {"fp32": 0, "fp16": 1, "bf16": 2}
+
+
+def tune_batch2d(ctx, shots, rloc, NX, nz, nt, npad, dx, dz, dt, rate):
+    """The launch shape vti2d_batch_tune picks for these shots: a dict of the
+    objfunc_gradient2d_batch keyword arguments (levels, tile_rows, max_wave)
+    and the winning trial's time."""
+    shots, rloc = _i32(shots), _i32(rloc)
+    lv, rows, wv = c_int(), c_int(), c_int()
+    ms = ctypes.c_double()
+    check(_lib.vti2d_batch_tune(ctx.h, ptr(shots), shots.size // 6, ptr(rloc), rloc.size // 2, NX, nz, nt, npad, dx,
+                                dz, dt, rate, ctypes.byref(lv), ctypes.byref(rows), ctypes.byref(wv),
+                                ctypes.byref(ms)), "vti2d_batch_tune")
+    return {"levels": lv.value, "tile_rows": rows.value, "max_wave": wv.value, "trial_ms": ms.value}
 
 
 def objfunc_gradient2d_batch(ctx, data, c11, c13, c33, wavelet, shots, rloc, NX, nz, nt, npad, dx, dz, dt, rate,
