@@ -295,6 +295,7 @@ def main():
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--small", action="store_true", help="small grid (profiling runs)")
     ap.add_argument("--no-extra", action="store_true", help="only the headline kernel (profiling runs)")
+    ap.add_argument("--no-2d", action="store_true", help="skip the 2D and serving workloads (profiling runs)")
     ap.add_argument("--only", choices=["shot2d", "batch2d", "serve"],
                     help="run one secondary workload alone and print its JSON (profiling runs)")
     args = ap.parse_args()
@@ -396,7 +397,7 @@ def main():
     f32ms = prop.forward_steps(max(1, args.steps // 2), absorbing=True, timed=True)
     extra["fwd_step_fp32_arith"] = {"ms": round(f32ms, 4), "gpts_s": round(npts / (f32ms * 1e-3) / 1e9, 3)}
     prop.close()
-    if not args.small:
+    if not args.small and not args.no_2d:
         extra["2d_shot_gradient_512x1024_3000steps"] = shot_gradient_2d(vtiamd, ctx)
         extra["2d_batched_vs_plain_gradient"] = batch_gradient_2d(vtiamd, ctx)
         extra["served_stream"] = serve_stream(vtiamd)
