@@ -127,6 +127,9 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
     (void)hipGetLastError();
     hipStream_t st = ctx->stream;
     const int NNX = NX + 2 * npad, nnz = nz + 2 * npad;
+    VTI_CHECK((long long)NNX * nnz < (1LL << 30),
+              "vti2d_objfunc_gradient_batch: padded model of %d x %d points exceeds the 2^30 the kernels index", nnz,
+              NNX);
     const int ratio = (int)(rate / dt + 0.5f);  // objFuncGradient_f.cu:30 rounds in float
     VTI_CHECK(ratio >= 2 && (nt - 1) % ratio == 0,
               "the reference's snapshot rotation lines up only for rate/dt >= 2 and (nt-1) %% (rate/dt) == 0 "
@@ -177,8 +180,9 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
         if (!rc) VTI_HIP(hipMemcpyAsync(C[k].p, csrc[k], gsize * 4, hipMemcpyHostToDevice, st));
         if (!rc) rc = A[k].alloc(gsize, st);
     }
-    std::vector<float> t0(std::max(npad, 1));
+    std::vector<float> t0(npad + 1);  // t0[npad] = 1: the factor away from the edges
     vti::taper_profile2d(t0.data(), npad);
+    t0[npad] = 1.f;
     if (!rc) rc = t0d.alloc(t0.size(), st);
     if (!rc) VTI_HIP(hipMemcpyAsync(t0d.p, t0.data(), t0.size() * 4, hipMemcpyHostToDevice, st));
     const int nblk_r = 1024;

@@ -68,14 +68,61 @@ struct SnapR {
 };
 
 // taper of the point (ix, iz) of a sub-grid nnx wide: t0 along x, then along z
-__device__ __forceinline__ float taper_at(const float *__restrict__ t0, int npad, int nnx, int nnz, int ix, int iz)
+// The taper of point (ix, iz) as two loaded factors whose product it is:
+// t0[d] for a distance d < npad from an edge, else t0[npad], which holds 1.
+// The loads carry no branch and their product is formed where the taper is
+// used, so that no load is waited for while the others are in flight.
+struct TapF {
+    float x, z;
+};
+__device__ __forceinline__ TapF taper_ld(const float *__restrict__ t0, int npad, int nnx, int nnz, int ix, int iz)
 {
-    float t = 1.f;
-    if (ix < npad) t = t * t0[ix];
-    else if (ix >= nnx - npad) t = t * t0[nnx - 1 - ix];
-    if (iz < npad) t = t * t0[iz];
-    else if (iz >= nnz - npad) t = t * t0[nnz - 1 - iz];
-    return t;
+    const int dx = ix < npad ? ix : (ix >= nnx - npad ? nnx - 1 - ix : npad);
+    const int dz = iz < npad ? iz : (iz >= nnz - npad ? nnz - 1 - iz : npad);
+    return {t0[dx], t0[dz]};
+}
+__device__ __forceinline__ float taper_of(TapF f)
+{
+    return f.x * f.z;
+}
+
+// base[idx] through a 32-bit byte offset from a workgroup-uniform base, the
+// form gfx950's global loads and stores take as SGPR base + VGPR offset with
+// no 64-bit address arithmetic per lane (the host keeps the model, and so
+// every shot plane, under 2^30 points)
+template <class T>
+__device__ __forceinline__ T ld32(const T *__restrict__ base, unsigned idx)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(T *__restrict__ base, unsigned idx, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (unsigned)sizeof(T)) = v;
+}
+__device__ __forceinline__ TapF taper_ld32(const float *__restrict__ t0, int npad, int nnx, int nnz, int ix, int iz)
+{
+    const int dx = ix < npad ? ix : (ix >= nnx - npad ? nnx - 1 - ix : npad);
+    const int dz = iz < npad ? iz : (iz >= nnz - npad ? nnz - 1 - iz : npad);
+    return {ld32(t0, (unsigned)dx), ld32(t0, (unsigned)dz)};
+}
+// the snapshot plane of component c at a slot, and its element i widened
+template <int SF>
+__device__ __forceinline__ float snap_raw32(const SnapR &r, const ShotB &sh, long long plane, int slot, int c,
+                                            unsigned i)
+{
+    const long long o = sh.snap_off + (long long)slot * 2 * plane + c * plane;
+    if (SF == 0) return ld32(r.f + o, i);
+    const unsigned short h = ld32(r.h + o, i);
+    if (SF == 1) return __half2float(__ushort_as_half(h));
+    return __uint_as_float((unsigned)h << 16);
+}
+__device__ __forceinline__ void snap_put32(const SnapW &w, const ShotB &sh, long long plane, int slot, unsigned i,
+                                           float tp, float tq)
+{
+    float *sp = w.amax ? w.d + sh.stage_off : w.d + sh.snap_off + (long long)slot * 2 * plane;
+    st32(sp, i, tp);
+    st32(sp + plane, i, tq);
 }
 
 // write D of point i (shot-relative) into its slot or the staging planes
@@ -110,6 +157,25 @@ __device__ __forceinline__ void snap_amax(const SnapW &w, const ShotB &sh, int s
         if (mp > 0.f) atomicMax(a, __float_as_uint(mp));
         if (mq > 0.f) atomicMax(a + 1, __float_as_uint(mq));
     }
+}
+
+// D component c of point i at a slot, in two parts: the stored value
+// (widened to float) and the slot's scale, multiplied by snap_mul where the
+// value is used, so the load is not waited for where it is issued
+template <int SF>
+__device__ __forceinline__ float snap_raw(const SnapR &r, const ShotB &sh, long long plane, int slot, int c,
+                                          long long i)
+{
+    const long long k = sh.snap_off + (long long)slot * 2 * plane + c * plane + i;
+    if (SF == 0) return r.f[k];
+    if (SF == 1) return __half2float(__ushort_as_half(r.h[k]));
+    return __uint_as_float((unsigned)r.h[k] << 16);
+}
+template <int SF>
+__device__ __forceinline__ float snap_mul(const SnapR &r, const ShotB &sh, int slot, int c, float raw)
+{
+    if (SF == 0) return raw;
+    return raw * r.scale[sh.scale_off + 2 * slot + c];
 }
 
 // D component c of point i at a slot
@@ -234,7 +300,8 @@ static __global__ __launch_bounds__(64 * kBY2) void fwd2db_kernel(
     const bool act = ix < nnx - 4 && iz < nnz - 4;
     const long long i = ix + (long long)iz * nnx;
     float *p_n = P_n + sh.f_off, *q_n = Q_n + sh.f_off;
-    float pp = 0.f, qp = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f, tap = 0.f, pc, qc;
+    float pp = 0.f, qp = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f, pc, qc;
+    TapF tw{0.f, 0.f};
     if (act) {
         const long long c = sh.c_off + ix + (long long)iz * bg.cpitch;
         pp = p_n[i];
@@ -242,12 +309,13 @@ static __global__ __launch_bounds__(64 * kBY2) void fwd2db_kernel(
         a11 = c11[c];
         a13 = c13[c];
         a33 = c33[c];
-        tap = taper_at(t0, bg.npad, nnx, nnz, ix, iz);
+        tw = taper_ld(t0, bg.npad, nnx, nnz, ix, iz);
     }
     load_tiles_b<0>(t, P_c + sh.f_off, Q_c + sh.f_off, nullptr, nullptr, nullptr, nnx, nnz, 0, x0, z0, pc, qc);
     __syncthreads();
     float mp = 0.f, mq = 0.f;
     if (act) {
+        const float tap = taper_of(tw);
         const float tp = dxx(t, dx2), tq = dzz(t, dz2);
         if (slot >= 0) {
             snap_put(sw, sh, (long long)nnx * nnz, slot, i, tp, tq);
@@ -318,30 +386,36 @@ static __global__ __launch_bounds__(64 * kBY2) void bwd2db_kernel(
     const long long i = ix + (long long)iz * nnx, k = sh.f_off + i;
     const long long plane = (long long)nnx * nnz;
     float *p_n = P_n + sh.f_off, *q_n = Q_n + sh.f_off;
-    float pp = 0.f, qp = 0.f, tap = 0.f, inj = 0.f, g11 = 0.f, g13 = 0.f, g33 = 0.f;
+    float pp = 0.f, qp = 0.f, inj = 0.f, g11 = 0.f, g13 = 0.f, g33 = 0.f;
     float d0p = 0.f, d1p = 0.f, d0q = 0.f, d1q = 0.f, pc, qc;
+    TapF tw{0.f, 0.f};
     int ir = -1;
     if (act) {
         pp = p_n[i];
         qp = q_n[i];
-        tap = taper_at(t0, bg.npad, nnx, nnz, ix, iz);
-        ir = (iz >= sh.rz0 && iz <= sh.rz1) ? recmap[k] : -1;
-        if (ir >= 0) {
-            const float *r0 = traces + sh.r_off + (long long)row0 * sh.nr;
-            inj = 0.5f * ((1.f - fr) * r0[ir] + fr * r0[ir + sh.nr]);
-        }
-        d0p = snap_get<SF>(sr, sh, plane, slot0, 0, i);
-        d1p = snap_get<SF>(sr, sh, plane, slot0 + 1, 0, i);
-        d0q = snap_get<SF>(sr, sh, plane, slot0, 1, i);
-        d1q = snap_get<SF>(sr, sh, plane, slot0 + 1, 1, i);
+        tw = taper_ld(t0, bg.npad, nnx, nnz, ix, iz);
+        if (iz >= sh.rz0 && iz <= sh.rz1) ir = recmap[k];
+        d0p = snap_raw<SF>(sr, sh, plane, slot0, 0, i);
+        d1p = snap_raw<SF>(sr, sh, plane, slot0 + 1, 0, i);
+        d0q = snap_raw<SF>(sr, sh, plane, slot0, 1, i);
+        d1q = snap_raw<SF>(sr, sh, plane, slot0 + 1, 1, i);
         g11 = G11[k];
         g13 = G13[k];
         g33 = G33[k];
     }
     load_tiles_b<1>(t, P_c + sh.f_off, Q_c + sh.f_off, c11 + sh.c_off, c13 + sh.c_off, c33 + sh.c_off, nnx, nnz,
                     bg.cpitch, x0, z0, pc, qc);
+    if (ir >= 0) {  // depends on recmap: issued after the other loads
+        const float *r0 = traces + sh.r_off + (long long)row0 * sh.nr;
+        inj = 0.5f * ((1.f - fr) * r0[ir] + fr * r0[ir + sh.nr]);
+    }
     __syncthreads();
     if (!act) return;
+    const float tap = taper_of(tw);
+    d0p = snap_mul<SF>(sr, sh, slot0, 0, d0p);
+    d1p = snap_mul<SF>(sr, sh, slot0 + 1, 0, d1p);
+    d0q = snap_mul<SF>(sr, sh, slot0, 1, d0q);
+    d1q = snap_mul<SF>(sr, sh, slot0 + 1, 1, d1q);
     const float tp0 = dxx(t, dx2), tq0 = dzz(t, dz2);
     float pn = 2.f * pc + dt2 * tp0 - pp * tap;
     float qn = 2.f * qc + dt2 * tq0 - qp * tap;
@@ -420,42 +494,98 @@ struct LevelB {
     float f;   // weight of the second slot
 };
 
-template <int BY, bool ADJ>
-__device__ __forceinline__ void load_tiles_t2(TB2<BY, ADJ> &t, const float *__restrict__ P,
-                                              const float *__restrict__ Q, const float *__restrict__ c11,
-                                              const float *__restrict__ c13, const float *__restrict__ c33, int nnx,
-                                              int nnz, int cpitch, int x0, int z0, int tid)
-{
-    constexpr int NT = 64 * BY;
-    for (int k = tid; k < (BY + 8) * 80; k += NT) {
-        const int c = k % 80, r = k / 80;
-        const int x = x0 - 8 + c, z = z0 - 4 + r;
-        float v = 0.f;
-        if (x >= 0 && x < nnx && z >= 0 && z < nnz) {
-            const long long i = x + (long long)z * nnx;
-            if (ADJ) {
-                const long long cc = x + (long long)z * cpitch;
-                v = c11[cc] * P[i] + c13[cc] * Q[i];
-            } else {
-                v = P[i];
-            }
+// The tiles' points outside R1: columns x0-8..x0-5 and x0+68..x0+71 of the
+// x tile (NA points), rows z0-8..z0-5 and z0+BY+4..z0+BY+7 of the z tile
+// (NB points).  Each thread loads at most NH of them into registers with the
+// rest of its operands, before any LDS write, so that the workgroup waits for
+// one memory latency rather than one per load loop.  The R1 part of the tiles
+// is written from the operands the threads already hold.
+template <int BY>
+struct HaloT2 {
+    // tile a's points take the first NAW slots (whole wavefronts), tile b's
+    // the next NB, so that which tile a wavefront serves is uniform across it
+    static constexpr int NT = 64 * BY, NA = 8 * (BY + 8), NAW = (NA + 63) / 64 * 64, NB = 8 * 72,
+                         NH = (NAW + NB + NT - 1) / NT;
+    // LDS position (tile a or b, row, column) and grid point of halo slot h;
+    // false for a slot that holds no point (the outputs then name point 0)
+    __device__ static __forceinline__ bool at(int h, int x0, int z0, bool &in_a, int &r, int &c, int &x, int &z)
+    {
+        const bool ok = h < NA || (h >= NAW && h < NAW + NB);
+        h = ok ? (h < NA ? h : h - NAW + NA) : 0;
+        if (h < NA) {
+            in_a = true;
+            r = h >> 3;
+            c = h & 7;
+            c = c < 4 ? c : c + 72;
+            x = x0 - 8 + c;
+            z = z0 - 4 + r;
+        } else {
+            h -= NA;
+            in_a = false;
+            r = h / 72;
+            c = h - r * 72;
+            r = r < 4 ? r : r + BY + 8;
+            x = x0 - 4 + c;
+            z = z0 - 8 + r;
         }
-        t.a[r][c] = v;
+        return ok;
     }
-    for (int k = tid; k < (BY + 16) * 72; k += NT) {
-        const int c = k % 72, r = k / 72;
-        const int x = x0 - 4 + c, z = z0 - 8 + r;
-        float v = 0.f;
-        if (x >= 0 && x < nnx && z >= 0 && z < nnz) {
-            const long long i = x + (long long)z * nnx;
-            if (ADJ) {
-                const long long cc = x + (long long)z * cpitch;
-                v = c13[cc] * P[i] + c33[cc] * Q[i];
-            } else {
-                v = Q[i];
-            }
+};
+
+// The loads behind one halo point, made at the point clamped into the grid:
+// p (tile a) or q (tile b) in p; for the adjoint p, q and the two
+// coefficients that weight them (c11, c13 in tile a; c13, c33 in tile b).
+// The tile value is formed at the LDS write, 0 outside the grid.
+template <bool ADJ>
+struct HaloV {
+    float p, q, u, w;
+};
+
+template <int BY, bool ADJ>
+__device__ __forceinline__ void halo_load_t2(HaloV<ADJ> (&hv)[HaloT2<BY>::NH], const float *__restrict__ P,
+                                             const float *__restrict__ Q, const float *__restrict__ c11,
+                                             const float *__restrict__ c13, const float *__restrict__ c33, int nnx,
+                                             int nnz, int cpitch, int x0, int z0, int tid)
+{
+#pragma unroll
+    for (int j = 0; j < HaloT2<BY>::NH; ++j) {
+        bool in_a;
+        int r, c, x, z;
+        (void)HaloT2<BY>::at(tid + j * HaloT2<BY>::NT, x0, z0, in_a, r, c, x, z);
+        x = min(max(x, 0), nnx - 1);
+        z = min(max(z, 0), nnz - 1);
+        const unsigned i = x + z * nnx;
+        // the wavefront's tile, uniform across it by the slot layout
+        const bool wa = __builtin_amdgcn_readfirstlane((tid + j * HaloT2<BY>::NT) < HaloT2<BY>::NAW);
+        if (!ADJ) {
+            hv[j].p = ld32(wa ? P : Q, i);
+        } else {
+            const unsigned cc = x + z * cpitch;
+            hv[j].p = ld32(P, i);
+            hv[j].q = ld32(Q, i);
+            hv[j].u = ld32(wa ? c11 : c13, cc);
+            hv[j].w = ld32(wa ? c13 : c33, cc);
         }
-        t.b[r][c] = v;
+    }
+}
+
+template <int BY, bool ADJ>
+__device__ __forceinline__ void halo_store_t2(TB2<BY, ADJ> &t, const HaloV<ADJ> (&hv)[HaloT2<BY>::NH], int nnx,
+                                              int nnz, int x0, int z0, int tid)
+{
+#pragma unroll
+    for (int j = 0; j < HaloT2<BY>::NH; ++j) {
+        bool in_a;
+        int r, c, x, z;
+        if (HaloT2<BY>::at(tid + j * HaloT2<BY>::NT, x0, z0, in_a, r, c, x, z)) {
+            // x tile: p (adjoint c11 p + c13 q); z tile: q (adjoint c13 p + c33 q)
+            float v = ADJ ? hv[j].u * hv[j].p + hv[j].w * hv[j].q : hv[j].p;
+            v = (x >= 0 && x < nnx && z >= 0 && z < nnz) ? v : 0.f;
+            if (in_a)
+                t.a[r][c] = v;
+            else
+                t.b[r][c] = v;
+        }
     }
 }
 
@@ -496,35 +626,51 @@ static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
     Qa += sh.f_off;
     Pb += sh.f_off;
     Qb += sh.f_off;
-    // operands of this thread's points of R1 (level L-1, coefficients, taper)
-    float pa[NK], qa[NK], k11[NK], k13[NK], k33[NK], tk[NK];
+    // operands of this thread's points of R1 (levels L and L-1, coefficients,
+    // taper) and of its own point of the tile.  Every load is made, at the
+    // point clamped into the grid, without a branch: values of points outside
+    // the grid (or the interior) are replaced or left unused afterwards.
+    float pb[NK], qb[NK], pa[NK], qa[NK], k11[NK], k13[NK], k33[NK];
+    TapF tk[NK];
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+        const int k = min(tid + j * NT, N1 - 1);
+        const int x = min(x0 - 4 + k % 72, nnx - 1), z = min(z0 - 4 + k / 72, nnz - 1);
+        const unsigned i = x + z * nnx, c = x + z * bg.cpitch;
+        pb[j] = ld32(Pb, i);
+        qb[j] = ld32(Qb, i);
+        pa[j] = ld32(Pa, i);
+        qa[j] = ld32(Qa, i);
+        k11[j] = ld32(cc11, c);
+        k13[j] = ld32(cc13, c);
+        k33[j] = ld32(cc33, c);
+        tk[j] = taper_ld32(t0, bg.npad, nnx, nnz, x, z);
+    }
+    HaloV<false> hv[HaloT2<BY>::NH];
+    halo_load_t2<BY, false>(hv, Pb, Qb, nullptr, nullptr, nullptr, nnx, nnz, 0, x0, z0, tid);
+    const int x = x0 + threadIdx.x, z = z0 + threadIdx.y;
+    const bool act = x < nnx - 4 && z < nnz - 4;
+    float a11, a13, a33;
+    TapF tw;
+    {
+        const int xo = min(x, nnx - 1), zo = min(z, nnz - 1);
+        const unsigned c = xo + zo * bg.cpitch;
+        a11 = ld32(cc11, c);
+        a13 = ld32(cc13, c);
+        a33 = ld32(cc33, c);
+        tw = taper_ld32(t0, bg.npad, nnx, nnz, xo, zo);
+    }
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
         const int k = tid + j * NT;
-        pa[j] = qa[j] = k11[j] = k13[j] = k33[j] = tk[j] = 0.f;
-        const int x = x0 - 4 + k % 72, z = z0 - 4 + k / 72;
-        if (k < N1 && x >= 4 && x < nnx - 4 && z >= 4 && z < nnz - 4) {
-            const long long i = x + (long long)z * nnx, c = x + (long long)z * bg.cpitch;
-            pa[j] = Pa[i];
-            qa[j] = Qa[i];
-            k11[j] = cc11[c];
-            k13[j] = cc13[c];
-            k33[j] = cc33[c];
-            tk[j] = taper_at(t0, bg.npad, nnx, nnz, x, z);
+        if (k < N1) {
+            const int rx = k % 72, rz = k / 72;
+            const bool in = x0 - 4 + rx < nnx && z0 - 4 + rz < nnz;
+            t.a[rz][rx + 4] = in ? pb[j] : 0.f;
+            t.b[rz + 4][rx] = in ? qb[j] : 0.f;
         }
     }
-    // and of its own point of the tile
-    const int x = x0 + threadIdx.x, z = z0 + threadIdx.y;
-    const bool act = x < nnx - 4 && z < nnz - 4;
-    float a11 = 0.f, a13 = 0.f, a33 = 0.f, tap = 0.f;
-    if (act) {
-        const long long c = x + (long long)z * bg.cpitch;
-        a11 = cc11[c];
-        a13 = cc13[c];
-        a33 = cc33[c];
-        tap = taper_at(t0, bg.npad, nnx, nnz, x, z);
-    }
-    load_tiles_t2<BY, false>(t, Pb, Qb, nullptr, nullptr, nullptr, nnx, nnz, 0, x0, z0, tid);
+    halo_store_t2<BY, false>(t, hv, nnx, nnz, x0, z0, tid);
     __syncthreads();
     float mp = 0.f, mq = 0.f;
     // level L+1 on R1
@@ -536,22 +682,23 @@ static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
             const int xx = x0 - 4 + rx, zz = z0 - 4 + rz;
             float p1 = 0.f, q1 = 0.f;
             if (xx >= 4 && xx < nnx - 4 && zz >= 4 && zz < nnz - 4) {
-                const long long i = xx + (long long)zz * nnx;
+                const int i = xx + zz * nnx;
                 const float tp = st9_row(t.a[rz], rx + 4) / dx2;
                 const float tq = st9_col<72>(t.b, rz + 4, rx) / dz2;
                 if (slot1 >= 0 && rx >= 4 && rx < 68 && rz >= 4 && rz < BY + 4) {
-                    snap_put(sw, sh, plane, slot1, i, tp, tq);
+                    snap_put32(sw, sh, plane, slot1, i, tp, tq);
                     mp = fmaxf(mp, fabsf(tp));
                     mq = fmaxf(mq, fabsf(tq));
                 }
-                float pn = 2.f * t.a[rz][rx + 4] + dt2 * (k11[j] * tp + k13[j] * tq) - pa[j] * tk[j];
-                float qn = 2.f * t.b[rz + 4][rx] + dt2 * (k13[j] * tp + k33[j] * tq) - qa[j] * tk[j];
+                const float tkj = taper_of(tk[j]);
+                float pn = 2.f * pb[j] + dt2 * (k11[j] * tp + k13[j] * tq) - pa[j] * tkj;
+                float qn = 2.f * qb[j] + dt2 * (k13[j] * tp + k33[j] * tq) - qa[j] * tkj;
                 if (i == sh.src) {
                     pn += src1;
                     qn += src1;
                 }
-                p1 = pn * tk[j];
-                q1 = qn * tk[j];
+                p1 = pn * tkj;
+                q1 = qn * tkj;
             }
             t.p1[rz][rx] = p1;
             t.q1[rz][rx] = q1;
@@ -563,12 +710,13 @@ static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
     mp = 0.f;
     mq = 0.f;
     if (act) {
+        const float tap = taper_of(tw);
         const int rx = threadIdx.x + 4, rz = threadIdx.y + 4;
-        const long long i = x + (long long)z * nnx;
+        const int i = x + z * nnx;
         const float tp = st9_row(t.p1[rz], rx) / dx2;
         const float tq = st9_col<72>(t.q1, rz, rx) / dz2;
         if (slot2 >= 0) {
-            snap_put(sw, sh, plane, slot2, i, tp, tq);
+            snap_put32(sw, sh, plane, slot2, i, tp, tq);
             mp = fabsf(tp);
             mq = fabsf(tq);
         }
@@ -579,10 +727,10 @@ static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
             pn += src2;
             qn += src2;
         }
-        Pc[sh.f_off + i] = p1;
-        Qc[sh.f_off + i] = q1;
-        Pd[sh.f_off + i] = pn * tap;
-        Qd[sh.f_off + i] = qn * tap;
+        st32(Pc + sh.f_off, i, p1);
+        st32(Qc + sh.f_off, i, q1);
+        st32(Pd + sh.f_off, i, pn * tap);
+        st32(Qd + sh.f_off, i, qn * tap);
     }
     if (sw.amax && slot2 >= 0) snap_amax(sw, sh, slot2, mp, mq, red);
 }
@@ -611,64 +759,81 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
     Qa += sh.f_off;
     Pb += sh.f_off;
     Qb += sh.f_off;
-    // operands of this thread's points of R1: levels L and L-1, coefficients,
-    // taper, injected residual
-    float pb[NK], qb[NK], pa[NK], qa[NK], k11[NK], k13[NK], k33[NK], tk[NK], inj[NK];
-    bool rec[NK];
-    const float *tr1 = traces + sh.r_off + (long long)l1.row * sh.nr;
+    // operands of this thread's points of R1 (levels L and L-1, coefficients,
+    // taper, receiver index) and of its own point of the tile (level L, taper,
+    // receiver index, the snapshots of both levels, the gradient).  Every load
+    // is made, at the point clamped into the grid, without a branch: values of
+    // points outside the grid (or the interior) are replaced or left unused
+    // afterwards.  Receiver indices are read only by workgroups whose rows
+    // meet the receiver rows.
+    const bool rrows = z0 - 4 <= sh.rz1 && z0 + BY + 4 > sh.rz0;
+    float pb[NK], qb[NK], pa[NK], qa[NK], k11[NK], k13[NK], k33[NK], inj[NK];
+    TapF tk[NK];
+    int irj[NK];
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+        const int k = min(tid + j * NT, N1 - 1);
+        const int x = min(x0 - 4 + k % 72, nnx - 1), z = min(z0 - 4 + k / 72, nnz - 1);
+        const unsigned i = x + z * nnx, c = x + z * bg.cpitch;
+        pb[j] = ld32(Pb, i);
+        qb[j] = ld32(Qb, i);
+        pa[j] = ld32(Pa, i);
+        qa[j] = ld32(Qa, i);
+        k11[j] = ld32(cc11, c);
+        k13[j] = ld32(cc13, c);
+        k33[j] = ld32(cc33, c);
+        tk[j] = taper_ld32(t0, bg.npad, nnx, nnz, x, z);
+        irj[j] = -1;
+        inj[j] = 0.f;
+    }
+    const int x = x0 + threadIdx.x, z = z0 + threadIdx.y;
+    const bool act = x < nnx - 4 && z < nnz - 4;
+    const unsigned i = min(x, nnx - 1) + min(z, nnz - 1) * nnx;
+    const float pbc = ld32(Pb, i), qbc = ld32(Qb, i);
+    const TapF tw = taper_ld32(t0, bg.npad, nnx, nnz, min(x, nnx - 1), min(z, nnz - 1));
+    const int *rmap = recmap + sh.f_off;
+    float *g11p = G11 + sh.f_off, *g13p = G13 + sh.f_off, *g33p = G33 + sh.f_off;
+    HaloV<true> hv[HaloT2<BY>::NH];
+    halo_load_t2<BY, true>(hv, Pb, Qb, cc11, cc13, cc33, nnx, nnz, bg.cpitch, x0, z0, tid);
+    // the injected residual: recmap, then the trace reads that depend on it
+    float inj2 = 0.f;
+    int ir2 = -1;
+    if (rrows) {
+        int rraw[NK];
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+            const int k = min(tid + j * NT, N1 - 1);
+            const int xr = min(x0 - 4 + k % 72, nnx - 1), zr = min(z0 - 4 + k / 72, nnz - 1);
+            rraw[j] = ld32(rmap, xr + zr * nnx);
+        }
+        const int r2raw = ld32(rmap, i);
+        const float *tr1 = traces + sh.r_off + (long long)l1.row * sh.nr;
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+            const int k = tid + j * NT;
+            const int xr = x0 - 4 + k % 72, zr = z0 - 4 + k / 72;
+            const bool in = k < N1 && xr >= 4 && xr < nnx - 4 && zr >= 4 && zr < nnz - 4 && zr >= sh.rz0 && zr <= sh.rz1;
+            irj[j] = in ? rraw[j] : -1;
+            if (irj[j] >= 0)
+                inj[j] = 0.5f * ((1.f - l1.fr) * ld32(tr1, irj[j]) + l1.fr * ld32(tr1, irj[j] + sh.nr));
+        }
+        ir2 = (act && z >= sh.rz0 && z <= sh.rz1) ? r2raw : -1;
+        if (ir2 >= 0) {
+            const float *tr2 = traces + sh.r_off + (long long)l2.row * sh.nr;
+            inj2 = 0.5f * ((1.f - l2.fr) * ld32(tr2, ir2) + l2.fr * ld32(tr2, ir2 + sh.nr));
+        }
+    }
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
         const int k = tid + j * NT;
-        pb[j] = qb[j] = pa[j] = qa[j] = k11[j] = k13[j] = k33[j] = tk[j] = inj[j] = 0.f;
-        rec[j] = false;
-        const int x = x0 - 4 + k % 72, z = z0 - 4 + k / 72;
-        if (k < N1 && x >= 4 && x < nnx - 4 && z >= 4 && z < nnz - 4) {
-            const long long i = x + (long long)z * nnx, c = x + (long long)z * bg.cpitch;
-            pb[j] = Pb[i];
-            qb[j] = Qb[i];
-            pa[j] = Pa[i];
-            qa[j] = Qa[i];
-            k11[j] = cc11[c];
-            k13[j] = cc13[c];
-            k33[j] = cc33[c];
-            tk[j] = taper_at(t0, bg.npad, nnx, nnz, x, z);
-            const int ir = (z >= sh.rz0 && z <= sh.rz1) ? recmap[sh.f_off + i] : -1;
-            if (ir >= 0) {
-                rec[j] = true;
-                inj[j] = 0.5f * ((1.f - l1.fr) * tr1[ir] + l1.fr * tr1[ir + sh.nr]);
-            }
+        if (k < N1) {
+            const int rx = k % 72, rz = k / 72;
+            const bool in = x0 - 4 + rx < nnx && z0 - 4 + rz < nnz;
+            t.a[rz][rx + 4] = in ? k11[j] * pb[j] + k13[j] * qb[j] : 0.f;
+            t.b[rz + 4][rx] = in ? k13[j] * pb[j] + k33[j] * qb[j] : 0.f;
         }
     }
-    // and of its own point of the tile: level L, taper, residual, the
-    // snapshots of both levels, the gradient
-    const int x = x0 + threadIdx.x, z = z0 + threadIdx.y;
-    const bool act = x < nnx - 4 && z < nnz - 4;
-    const long long i = x + (long long)z * nnx, kk = sh.f_off + i;
-    float pbc = 0.f, qbc = 0.f, tap = 0.f, inj2 = 0.f, g11 = 0.f, g13 = 0.f, g33 = 0.f;
-    float a0p = 0.f, a1p = 0.f, a0q = 0.f, a1q = 0.f, b0p = 0.f, b1p = 0.f, b0q = 0.f, b1q = 0.f;
-    int ir2 = -1;
-    if (act) {
-        pbc = Pb[i];
-        qbc = Qb[i];
-        tap = taper_at(t0, bg.npad, nnx, nnz, x, z);
-        ir2 = (z >= sh.rz0 && z <= sh.rz1) ? recmap[kk] : -1;
-        if (ir2 >= 0) {
-            const float *tr2 = traces + sh.r_off + (long long)l2.row * sh.nr;
-            inj2 = 0.5f * ((1.f - l2.fr) * tr2[ir2] + l2.fr * tr2[ir2 + sh.nr]);
-        }
-        a0p = snap_get<SF>(sr, sh, plane, l1.slot, 0, i);
-        a1p = snap_get<SF>(sr, sh, plane, l1.slot + 1, 0, i);
-        a0q = snap_get<SF>(sr, sh, plane, l1.slot, 1, i);
-        a1q = snap_get<SF>(sr, sh, plane, l1.slot + 1, 1, i);
-        b0p = snap_get<SF>(sr, sh, plane, l2.slot, 0, i);
-        b1p = snap_get<SF>(sr, sh, plane, l2.slot + 1, 0, i);
-        b0q = snap_get<SF>(sr, sh, plane, l2.slot, 1, i);
-        b1q = snap_get<SF>(sr, sh, plane, l2.slot + 1, 1, i);
-        g11 = G11[kk];
-        g13 = G13[kk];
-        g33 = G33[kk];
-    }
-    load_tiles_t2<BY, true>(t, Pb, Qb, cc11, cc13, cc33, nnx, nnz, bg.cpitch, x0, z0, tid);
+    halo_store_t2<BY, true>(t, hv, nnx, nnz, x0, z0, tid);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
@@ -680,14 +845,15 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
             if (xx >= 4 && xx < nnx - 4 && zz >= 4 && zz < nnz - 4) {
                 const float tp0 = st9_row(t.a[rz], rx + 4) / dx2;
                 const float tq0 = st9_col<72>(t.b, rz + 4, rx) / dz2;
-                float pn = 2.f * pb[j] + dt2 * tp0 - pa[j] * tk[j];
-                float qn = 2.f * qb[j] + dt2 * tq0 - qa[j] * tk[j];
-                if (rec[j]) {
+                const float tkj = taper_of(tk[j]);
+                float pn = 2.f * pb[j] + dt2 * tp0 - pa[j] * tkj;
+                float qn = 2.f * qb[j] + dt2 * tq0 - qa[j] * tkj;
+                if (irj[j] >= 0) {
                     pn += dt2 * inj[j];
                     qn += dt2 * inj[j];
                 }
-                p1 = pn * tk[j];
-                q1 = qn * tk[j];
+                p1 = pn * tkj;
+                q1 = qn * tkj;
                 pp = k11[j] * p1 + k13[j] * q1;
                 qq = k13[j] * p1 + k33[j] * q1;
             }
@@ -699,6 +865,18 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
     }
     __syncthreads();
     if (!act) return;
+    // the snapshots of both levels and the gradient, read while the step to
+    // L+2 is computed (kept out of the first phase to save registers there)
+    float a0p = snap_raw32<SF>(sr, sh, plane, l1.slot, 0, i);
+    float a1p = snap_raw32<SF>(sr, sh, plane, l1.slot + 1, 0, i);
+    float a0q = snap_raw32<SF>(sr, sh, plane, l1.slot, 1, i);
+    float a1q = snap_raw32<SF>(sr, sh, plane, l1.slot + 1, 1, i);
+    float b0p = snap_raw32<SF>(sr, sh, plane, l2.slot, 0, i);
+    float b1p = snap_raw32<SF>(sr, sh, plane, l2.slot + 1, 0, i);
+    float b0q = snap_raw32<SF>(sr, sh, plane, l2.slot, 1, i);
+    float b1q = snap_raw32<SF>(sr, sh, plane, l2.slot + 1, 1, i);
+    float g11 = ld32(g11p, i), g13 = ld32(g13p, i), g33 = ld32(g33p, i);
+    const float tap = taper_of(tw);
     const int rx = threadIdx.x + 4, rz = threadIdx.y + 4;
     const float tp0 = st9_row(t.pp1[rz], rx) / dx2;
     const float tq0 = st9_col<72>(t.qq1, rz, rx) / dz2;
@@ -711,10 +889,18 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
     }
     pn = pn * tap;
     qn = qn * tap;
-    Pc[kk] = p1;
-    Qc[kk] = q1;
-    Pd[kk] = pn;
-    Qd[kk] = qn;
+    st32(Pc + sh.f_off, i, p1);
+    st32(Qc + sh.f_off, i, q1);
+    st32(Pd + sh.f_off, i, pn);
+    st32(Qd + sh.f_off, i, qn);
+    a0p = snap_mul<SF>(sr, sh, l1.slot, 0, a0p);
+    a1p = snap_mul<SF>(sr, sh, l1.slot + 1, 0, a1p);
+    a0q = snap_mul<SF>(sr, sh, l1.slot, 1, a0q);
+    a1q = snap_mul<SF>(sr, sh, l1.slot + 1, 1, a1q);
+    b0p = snap_mul<SF>(sr, sh, l2.slot, 0, b0p);
+    b1p = snap_mul<SF>(sr, sh, l2.slot + 1, 0, b1p);
+    b0q = snap_mul<SF>(sr, sh, l2.slot, 1, b0q);
+    b1q = snap_mul<SF>(sr, sh, l2.slot + 1, 1, b1q);
     const float Dp1 = (1.f - l1.f) * a0p + l1.f * a1p;
     const float Dq1 = (1.f - l1.f) * a0q + l1.f * a1q;
     const float Dp2 = (1.f - l2.f) * b0p + l2.f * b1p;
@@ -725,9 +911,9 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
     g11 += pn * Dp2;
     g13 += pn * Dq2 + qn * Dp2;
     g33 += qn * Dq2;
-    G11[kk] = g11;
-    G13[kk] = g13;
-    G33[kk] = g33;
+    st32(g11p, i, g11);
+    st32(g13p, i, g13);
+    st32(g33p, i, g33);
 }
 
 }  // namespace vti

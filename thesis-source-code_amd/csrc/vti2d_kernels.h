@@ -75,6 +75,35 @@ __device__ __forceinline__ void load_tiles(Tiles2 &t, const float *__restrict__ 
     }
 }
 
+// load_tiles<0> that also returns the raw p and q of the thread's own point
+// (0 outside the grid), so the step need not read them again
+__device__ __forceinline__ void load_tiles_c(Tiles2 &t, const float *__restrict__ P, const float *__restrict__ Q,
+                                             const Geo2 &g, int x0, int z0, float &pc, float &qc)
+{
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    auto pv = [&](int x, int z) -> float {
+        if (x < 0 || x >= g.nnx || z < 0 || z >= g.nnz) return 0.f;
+        return P[x + (long long)z * g.nnx];
+    };
+    auto qv = [&](int x, int z) -> float {
+        if (x < 0 || x >= g.nnx || z < 0 || z >= g.nnz) return 0.f;
+        return Q[x + (long long)z * g.nnx];
+    };
+    const int ix = x0 + tx, iz = z0 + ty;
+    pc = pv(ix, iz);
+    qc = qv(ix, iz);
+    t.p[ty][tx + 4] = pc;
+    if (tx < 4) {
+        t.p[ty][tx] = pv(x0 - 4 + tx, iz);
+        t.p[ty][tx + 68] = pv(x0 + 64 + tx, iz);
+    }
+    t.q[ty + 4][tx] = qc;
+    if (ty < 4) {
+        t.q[ty][tx] = qv(ix, z0 - 4 + ty);
+        t.q[ty + kBY2 + 4][tx] = qv(ix, z0 + kBY2 + ty);
+    }
+}
+
 __device__ __forceinline__ float dxx(const Tiles2 &t, float dx2)
 {
     const int c = threadIdx.x + 4, r = threadIdx.y;
@@ -111,20 +140,29 @@ __global__ __launch_bounds__(64 * kBY2) void fwd2d_kernel(
     __shared__ Tiles2 tb;
     const int x0 = 4 + blockIdx.x * 64, z0 = 4 + blockIdx.y * kBY2;
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
-    load_tiles<0>(t, p_c, q_c, nullptr, nullptr, nullptr, g, x0, z0);
+    const bool act = ix < g.nnx - 4 && iz < g.nnz - 4;
+    const long long i = ix + (long long)iz * g.nnx;
+    // the point's own operands, loaded with the tiles before the barrier
+    float tap = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f, pp = 0.f, qp = 0.f, pcur, qcur;
+    if (act) {
+        tap = taper[i];
+        a11 = c11[i];
+        a13 = c13[i];
+        a33 = c33[i];
+        pp = p_n[i];
+        qp = q_n[i];
+    }
+    load_tiles_c(t, p_c, q_c, g, x0, z0, pcur, qcur);
     if (BORN) load_tiles<0>(tb, dp_c, dq_c, nullptr, nullptr, nullptr, g, x0, z0);
     __syncthreads();
-    if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
-    const long long i = ix + (long long)iz * g.nnx;
+    if (!act) return;
     const float tp = dxx(t, dx2), tq = dzz(t, dz2);
     if (snap_p) {
         snap_p[i] = tp;
         snap_q[i] = tq;
     }
-    const float tap = taper[i];
-    const float a11 = c11[i], a13 = c13[i], a33 = c33[i];
-    float pn = 2.f * p_c[i] + dt2 * (a11 * tp + a13 * tq) - p_n[i] * tap;
-    float qn = 2.f * q_c[i] + dt2 * (a13 * tp + a33 * tq) - q_n[i] * tap;
+    float pn = 2.f * pcur + dt2 * (a11 * tp + a13 * tq) - pp * tap;
+    float qn = 2.f * qcur + dt2 * (a13 * tp + a33 * tq) - qp * tap;
     if (i == src) {
         pn += src_val;
         qn += src_val;
@@ -185,24 +223,47 @@ __global__ __launch_bounds__(64 * kBY2) void bwd2d_kernel(
     __shared__ Tiles2 t;
     const int x0 = 4 + blockIdx.x * 64, z0 = 4 + blockIdx.y * kBY2;
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
+    const bool act = ix < g.nnx - 4 && iz < g.nnz - 4;
+    const long long i = ix + (long long)iz * g.nnx;
+    // the point's own operands, loaded with the tiles before the barrier
+    float tap = 0.f, pc = 0.f, qc = 0.f, pp = 0.f, qp = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f, s = 0.f;
+    float d0p = 0.f, d0q = 0.f, d1p = 0.f, d1q = 0.f, o11 = 0.f, o13 = 0.f, o33 = 0.f;
+    int ir = -1;
+    if (act) {
+        tap = taper[i];
+        pc = p_c[i];
+        qc = q_c[i];
+        pp = p_n[i];
+        qp = q_n[i];
+        if (ADJ == 1) {
+            a11 = c11[i];
+            a13 = c13[i];
+            a33 = c33[i];
+        }
+        ir = recmap[i];
+        if (ir >= 0) s = 0.5f * ((1.f - fr) * r0[ir] + fr * r1[ir]);
+        d0p = D0p[i];
+        d0q = D0q[i];
+        d1p = D1p[i];
+        d1q = D1q[i];
+        o11 = g11[i];
+        o13 = g13[i];
+        o33 = g33[i];
+    }
     load_tiles<ADJ == 0 ? 1 : 0>(t, p_c, q_c, c11, c13, c33, g, x0, z0);
     __syncthreads();
-    if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
-    const long long i = ix + (long long)iz * g.nnx;
-    const float tap = taper[i];
+    if (!act) return;
     float pn, qn;
     if (ADJ == 0) {
         const float tp0 = dxx(t, dx2), tq0 = dzz(t, dz2);
-        pn = 2.f * p_c[i] + dt2 * tp0 - p_n[i] * tap;
-        qn = 2.f * q_c[i] + dt2 * tq0 - q_n[i] * tap;
+        pn = 2.f * pc + dt2 * tp0 - pp * tap;
+        qn = 2.f * qc + dt2 * tq0 - qp * tap;
     } else {
         const float tp = dxx(t, dx2), tq = dzz(t, dz2);
-        pn = 2.f * p_c[i] + dt2 * (c11[i] * tp + c13[i] * tq) - p_n[i] * tap;
-        qn = 2.f * q_c[i] + dt2 * (c13[i] * tp + c33[i] * tq) - q_n[i] * tap;
+        pn = 2.f * pc + dt2 * (a11 * tp + a13 * tq) - pp * tap;
+        qn = 2.f * qc + dt2 * (a13 * tp + a33 * tq) - qp * tap;
     }
-    const int ir = recmap[i];
     if (ir >= 0) {
-        const float s = 0.5f * ((1.f - fr) * r0[ir] + fr * r1[ir]);
         pn += dt2 * s;
         qn += dt2 * s;
     }
@@ -210,11 +271,11 @@ __global__ __launch_bounds__(64 * kBY2) void bwd2d_kernel(
     qn = qn * tap;
     p_n[i] = pn;
     q_n[i] = qn;
-    const float Dp = (1.f - f) * D0p[i] + f * D1p[i];
-    const float Dq = (1.f - f) * D0q[i] + f * D1q[i];
-    g11[i] += pn * Dp;
-    g13[i] += pn * Dq + qn * Dp;
-    g33[i] += qn * Dq;
+    const float Dp = (1.f - f) * d0p + f * d1p;
+    const float Dq = (1.f - f) * d0q + f * d1q;
+    g11[i] = o11 + pn * Dp;
+    g13[i] = o13 + (pn * Dq + qn * Dp);
+    g33[i] = o33 + qn * Dq;
 }
 
 // hessian_f's adjoint level (hessian_f.cu:387-443): the residual's adjoint
