@@ -49,6 +49,18 @@ struct ShotB {
     int scale_off;        // first of the shot's 2*nnt slice scales / maxima
 };
 
+// The tile a workgroup works on.  Workgroups are dealt to the 8 XCDs in
+// turn (workgroup b to XCD b % 8) and each XCD has its own L2, so this gives
+// XCD x the contiguous run of tiles [x q + min(x, r), ...) of the nb = 8 q + r
+// in the launch, in order: neighbouring tiles, which read each other's
+// halos, then run on one XCD close together in time and share its L2.  A
+// bijection of [0, nb) for any nb.
+__device__ __forceinline__ int xcd_tile()
+{
+    const int nb = gridDim.x, b = blockIdx.x, x = b & 7, q = nb >> 3, r = nb & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 struct BatchGeo {
     int nnz, npad, cpitch;
     long long nnxz_total;
@@ -292,8 +304,9 @@ static __global__ __launch_bounds__(64 * kBY2) void fwd2db_kernel(
 {
     __shared__ Tiles2 t;
     __shared__ float2 red[kBY2];
-    const ShotB sh = shots[tile_shot[blockIdx.x]];
-    const int tl = blockIdx.x - sh.tile0;
+    const int tile = xcd_tile();
+    const ShotB sh = shots[tile_shot[tile]];
+    const int tl = tile - sh.tile0;
     const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * kBY2;
     const int nnx = sh.nnx, nnz = bg.nnz;
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
@@ -343,8 +356,9 @@ static __global__ __launch_bounds__(64 * kBY2) void dsnap2db_kernel(const float 
 {
     __shared__ Tiles2 t;
     __shared__ float2 red[kBY2];
-    const ShotB sh = shots[tile_shot[blockIdx.x]];
-    const int tl = blockIdx.x - sh.tile0;
+    const int tile = xcd_tile();
+    const ShotB sh = shots[tile_shot[tile]];
+    const int tl = tile - sh.tile0;
     const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * kBY2;
     const int nnx = sh.nnx, nnz = bg.nnz;
     float pc, qc;
@@ -377,8 +391,9 @@ static __global__ __launch_bounds__(64 * kBY2) void bwd2db_kernel(
     float dt2)
 {
     __shared__ Tiles2 t;
-    const ShotB sh = shots[tile_shot[blockIdx.x]];
-    const int tl = blockIdx.x - sh.tile0;
+    const int tile = xcd_tile();
+    const ShotB sh = shots[tile_shot[tile]];
+    const int tl = tile - sh.tile0;
     const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * kBY2;
     const int nnx = sh.nnx, nnz = bg.nnz;
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
@@ -441,8 +456,9 @@ static __global__ void xcor2db_kernel(const float *__restrict__ P, const float *
                                       SnapR sr, int slot0, float f, float *__restrict__ G11, float *__restrict__ G13,
                                       float *__restrict__ G33)
 {
-    const ShotB sh = shots[tile_shot[blockIdx.x]];
-    const int tl = blockIdx.x - sh.tile0;
+    const int tile = xcd_tile();
+    const ShotB sh = shots[tile_shot[tile]];
+    const int tl = tile - sh.tile0;
     const int ix = 4 + (tl % sh.ntx) * 64 + threadIdx.x, iz = 4 + (tl / sh.ntx) * kBY2 + threadIdx.y;
     const int nnx = sh.nnx, nnz = bg.nnz;
     if (ix >= nnx - 4 || iz >= nnz - 4) return;
@@ -616,8 +632,9 @@ static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
     constexpr int NT = 64 * BY, N1 = (BY + 8) * 72, NK = (N1 + NT - 1) / NT;
     __shared__ TB2<BY, false> t;
     __shared__ float2 red[BY];
-    const ShotB sh = shots[tile_shot[blockIdx.x]];
-    const int tl = blockIdx.x - sh.tile0;
+    const int tile = xcd_tile();
+    const ShotB sh = shots[tile_shot[tile]];
+    const int tl = tile - sh.tile0;
     const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * BY;
     const int nnx = sh.nnx, nnz = bg.nnz, tid = threadIdx.x + 64 * threadIdx.y;
     const long long plane = (long long)nnx * nnz;
@@ -749,8 +766,9 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
 {
     constexpr int NT = 64 * BY, N1 = (BY + 8) * 72, NK = (N1 + NT - 1) / NT;
     __shared__ TB2<BY, true> t;
-    const ShotB sh = shots[tile_shot[blockIdx.x]];
-    const int tl = blockIdx.x - sh.tile0;
+    const int tile = xcd_tile();
+    const ShotB sh = shots[tile_shot[tile]];
+    const int tl = tile - sh.tile0;
     const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * BY;
     const int nnx = sh.nnx, nnz = bg.nnz, tid = threadIdx.x + 64 * threadIdx.y;
     const long long plane = (long long)nnx * nnz;
