@@ -135,6 +135,11 @@ def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7, variants=None):
                                         **opts)
         timed[name] = time.perf_counter() - t0
     tb = timed[next(iter(timed))]
+    # the same shots captured once (hipGraph of the whole launch sequence) and
+    # replayed on new models, as the iterations of an inversion do
+    shape = {k: tuned[k] for k in ("levels", "tile_rows", "max_wave")}
+    plan = graph_plan_rate(vtiamd, ctx, data, w, shots, rloc, (NX, nz, nt, npad, dx, dx, dt, rate), (c11, c13, c33),
+                           shape, direct_s=tb)
     # the plain driver, one call per shot on the cropped model
     tp = 0.0
     for k in range(nplain):
@@ -154,8 +159,77 @@ def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7, variants=None):
             "batched_shot_gradients_per_s": round(ns / tb, 3), "batched_s": round(tb, 3),
             "plain_shot_gradients_per_s": round(nplain / tp, 3), "plain_shots_timed": nplain,
             "speedup": round((ns / tb) / (nplain / tp), 2), "launch_shape": tuned,
+            "graph_plan": plan, "graph_plan_speedup_over_plain": round(plan["shot_gradients_per_s"] / (nplain / tp), 2),
+            "small_grid": batch_small_2d(vtiamd, ctx),
             **({"variants_shot_gradients_per_s": {k: round(ns / v, 3) for k, v in timed.items()}}
                if variants else {})}
+
+
+def graph_plan_rate(vtiamd, ctx, data, w, shots, rloc, geo, model, shape, direct_s, replays=3):
+    """Shot-gradients/s of a vtiamd.BatchPlan replayed `replays` times on
+    models scaled by 1 + 1e-3 k (capture time reported apart)."""
+    ns = len(shots)
+    t0 = time.perf_counter()
+    with vtiamd.BatchPlan(ctx, data, w, shots, rloc, *geo, **shape) as plan:
+        t_cap = time.perf_counter() - t0
+        times = []
+        for k in range(replays):
+            m = [(c * np.float32(1 + 1e-3 * k)).astype(np.float32) for c in model]
+            t1 = time.perf_counter()
+            plan.run(*m)
+            times.append(time.perf_counter() - t1)
+    tr = min(times)
+    return {"shot_gradients_per_s": round(ns / tr, 3), "replay_s": round(tr, 4), "capture_s": round(t_cap, 3),
+            "direct_call_s": round(direct_s, 4), "replays": replays}
+
+
+def batch_small_2d(vtiamd, ctx, ns=32, seed=5):
+    """Where launches are short: 32 shots of 64-160 columns on a 1024 x 96
+    model (+20 boundary), 1000 steps -- direct batched calls against graph
+    replays of the same shots, and the plain one-shot driver."""
+    rng = np.random.default_rng(seed)
+    NX, nz, npad, nt, dx, dt, rate = 1024, 96, 20, 1001, 10.0, 1e-3, 4e-3
+    NNX, nnz = NX + 2 * npad, nz + 2 * npad
+    v = np.broadcast_to(1500.0 + 2000.0 * np.arange(nnz, dtype=np.float32)[:, None] / nnz, (nnz, NNX))
+    c33 = np.ascontiguousarray(v * v, dtype=np.float32)
+    c11, c13 = (c33 * 1.2).astype(np.float32), (c33 * 1.05).astype(np.float32)
+    t = np.arange(nt) * dt - 0.1
+    a = (np.pi * 15.0 * t) ** 2
+    w = ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
+    shots, rloc = [], []
+    for _ in range(ns):
+        nx = int(rng.integers(64, 161))
+        c0 = int(rng.integers(0, NX - nx + 1))
+        shots.append((c0, nx, npad + nx // 2, npad + 2, nx, len(rloc)))
+        rloc += [(npad + i, npad + 1) for i in range(nx)]
+    shots, rloc = np.array(shots, np.int32), np.array(rloc, np.int32)
+    nnt = (nt - 1) // int(rate / dt + 0.5) + 1
+    data = np.zeros(int(sum(nnt * s[4] for s in shots)), np.float32)
+    geo = (NX, nz, nt, npad, dx, dx, dt, rate)
+    vtiamd.objfunc_gradient2d_batch(ctx, data, c11, c13, c33, w, shots, rloc, *geo)
+    td = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        vtiamd.objfunc_gradient2d_batch(ctx, data, c11, c13, c33, w, shots, rloc, *geo)
+        td.append(time.perf_counter() - t0)
+    plan = graph_plan_rate(vtiamd, ctx, data, w, shots, rloc, geo, (c11, c13, c33), {}, direct_s=min(td))
+    tp = 0.0
+    for k in range(4):
+        c0, nx, ix, iz, nr, r0 = (int(q) for q in shots[k])
+        wd = nx + 2 * npad
+        taper = vtiamd.init_taper2d(nx, nz, npad)
+        crop = [np.ascontiguousarray(c[:, c0:c0 + wd]) for c in (c11, c13, c33)]
+        d = np.zeros((nnt, nr), np.float32)
+        sl = np.array([[ix, iz, nr, 0]], np.int32)
+        t0 = time.perf_counter()
+        vtiamd.objfunc_gradient2d(ctx, d, *crop, w, sl, rloc[r0:r0 + nr], taper, nx, nz, nt, npad, dx, dx, dt, rate)
+        if k:
+            tp += time.perf_counter() - t0
+    plain = 3 / tp
+    return {"shots": ns, "model_padded": [nnz, NNX], "aperture_cols": [64, 160], "steps": nt - 1,
+            "direct_shot_gradients_per_s": round(ns / min(td), 3), "graph_plan": plan,
+            "plain_shot_gradients_per_s": round(plain, 3),
+            "graph_plan_speedup_over_plain": round(plan["shot_gradients_per_s"] / plain, 2)}
 
 
 def shot_gradient_2d(vtiamd, ctx, nshots=2):

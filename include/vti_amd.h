@@ -28,6 +28,7 @@ extern "C" {
 
 typedef struct vti_ctx vti_ctx;
 typedef struct vti3d_prop vti3d_prop;
+typedef struct vti2d_batch_plan vti2d_batch_plan;
 
 /* ---- status and context ------------------------------------------------ */
 const char *vti_last_error(void);
@@ -261,6 +262,29 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
                                  const float *wavelet, const int *shots, int ns, const int *rloc, int nr, int NX, int nz,
                                  int nt, int npad, float dx, float dz, float dt, float rate, int max_wave, int levels,
                                  int tile_rows, int snap_fmt);
+
+/* The batched gradient captured once and replayed: for the iterations of an
+ * inversion, which evaluate the same shots (geometry, observed data, wavelet)
+ * on a new model each time.  vti2d_batch_plan_create checks the arguments as
+ * vti2d_objfunc_gradient_batch does, allocates the device state of every
+ * wave (waves of at most max_wave shots when > 0, else one; all held at once,
+ * an error when they do not fit the free device memory), uploads the shot
+ * layout, receiver maps and observed data, and records the call's whole
+ * launch sequence -- forward pass, residual, adjoint pass, FP64 accumulation,
+ * every buffer clearing done by a kernel -- into a hipGraph on the context's
+ * stream.  vti2d_batch_plan_run uploads c11/c13/c33, replays the graph and
+ * returns what vti2d_objfunc_gradient_batch returns for the same arguments,
+ * bit for bit.  The plan holds its device memory until
+ * vti2d_batch_plan_destroy, which must come before vti_ctx_destroy.  Runs of
+ * one plan, or of plans on one context, are serialised by the caller.
+ * No counterpart in the reference, which sets every call up anew. */
+int vti2d_batch_plan_create(vti_ctx *ctx, vti2d_batch_plan **plan, const float *data, const float *wavelet,
+                            const int *shots, int ns, const int *rloc, int nr, int NX, int nz, int nt, int npad,
+                            float dx, float dz, float dt, float rate, int max_wave, int levels, int tile_rows,
+                            int snap_fmt);
+int vti2d_batch_plan_run(vti2d_batch_plan *plan, double *objfunc, double *gc11, double *gc13, double *gc33,
+                         const float *c11, const float *c13, const float *c33);
+int vti2d_batch_plan_destroy(vti2d_batch_plan *plan);
 
 /* Launch-shape autotuner for vti2d_objfunc_gradient_batch: times short runs
  * (24 snapshot intervals) of the same shots on a synthetic model over levels

@@ -171,3 +171,28 @@ def test_autotuned_launch_shape(vti, ctx):
     f0, g0 = run()
     f1, g1 = run(levels=t["levels"], tile_rows=t["tile_rows"], max_wave=t["max_wave"])
     assert np.array_equal(g0, g1) and f0 == f1
+
+
+@pytest.mark.parametrize("snap,levels,max_wave", [("fp32", 2, 0), ("fp32", 1, 3), ("bf16", 2, 2)])
+def test_graph_plan_replays_equal_direct_calls(vti, ctx, snap, levels, max_wave):
+    """A captured plan (hipGraph of the whole launch sequence) replayed on
+    three models, with other drivers' work and synchronisations between the
+    replays, gives each time the direct batched call's bits on that model --
+    also the third replay, back on the first model."""
+    B = batch_problem(vti, seed=14, ns=4)
+    _say("batch2d plan", snap, "levels", levels, "max_wave", max_wave)
+    data = observed(vti, ctx, B)
+    a = (B["NX"], B["nz"], B["nt"], B["npad"], B["dx"], B["dz"], B["dt"], B["rate"])
+    models = [B["c"], tuple((c * np.float32(1.01)).astype(np.float32) for c in B["c"]), B["c"]]
+    with vti.BatchPlan(ctx, data, B["w"], B["shots"], B["rloc"], *a, max_wave=max_wave, levels=levels,
+                       snap=snap) as plan:
+        for k, m in enumerate(models):
+            fg, gg = plan.run(*m)
+            fd, gd = vti.objfunc_gradient2d_batch(ctx, data, *m, B["w"], B["shots"], B["rloc"], *a,
+                                                  max_wave=max_wave, levels=levels, snap=snap)
+            _say("  replay", k, fg, fd)
+            assert np.abs(gd).max() > 0
+            assert np.array_equal(gg, gd) and fg == fd
+            # eager work of another driver between replays
+            vti.objfunc_gradient2d_batch(ctx, data[::-1].copy(), *B["ct"], B["w"], B["shots"], B["rloc"], *a,
+                                         max_wave=2)

@@ -15,11 +15,13 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <tuple>
 #include <vector>
 
 #include "vti2d_batch_kernels.h"
+#include "vti_amd.h"
 #include "vti_ctx.h"
 #include "vti_dbuf.h"
 #include "vti_host.h"
@@ -98,11 +100,488 @@ __global__ void accum_crop_kernel(double *__restrict__ A11, double *__restrict__
     A33[c] += (double)G33[i];
 }
 
+// zero n 4-byte words with a kernel (what a captured graph may record,
+// unlike hipMemsetAsync)
+__global__ void fill0_kernel(unsigned *__restrict__ p, long long n)
+{
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+        p[k] = 0u;
+}
+void zero_words(void *p, size_t nwords, hipStream_t st)
+{
+    const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>((nwords + 255) / 256, 4096));
+    hipLaunchKernelGGL(fill0_kernel, dim3(nb), dim3(256), 0, st, (unsigned *)p, (long long)nwords);
+}
+
 struct ShotIn {
     int c0, nx, ix, iz, nr, r0;
 };
 
+// The call's checked arguments and derived sizes.
+struct BatchArgs {
+    int ns, nr, NX, nz, nt, npad, NNX, nnz, ratio, nnt, nnt_data, levels, BYT, snap_fmt;
+    bool half;
+    float dx2, dz2, dt2;
+    std::vector<ShotIn> in;
+    std::vector<long long> data_off;
+    long long shot_bytes(int s) const
+    {
+        // 8 wavefield planes, 3 gradient planes, the receiver map, the
+        // snapshots (and for 16-bit ones two staging planes and the slice
+        // scales/maxima) and two trace copies
+        const long long nxz = (long long)(in[s].nx + 2 * npad) * nnz;
+        const long long snapb = half ? 2LL * nxz * 2 * nnt + 4LL * nxz * 2 + 16LL * nnt : 4LL * nxz * 2 * nnt;
+        return 4LL * nxz * (8 + 3 + 1) + snapb + 8LL * nnt_data * in[s].nr + 64;
+    }
+};
+
+int parse_args(BatchArgs &a, const int *shots, int ns, const int *rloc, int nr, int NX, int nz, int nt, int npad,
+               float dx, float dz, float dt, float rate, int levels, int tile_rows, int snap_fmt)
+{
+    VTI_CHECK(ns >= 1 && nr >= 0 && NX > 0 && nz > 0 && npad >= 0 && nt >= 3 && dx > 0 && dz > 0 && dt > 0,
+              "vti2d_objfunc_gradient_batch: bad sizes");
+    if (levels == 0) levels = 2;
+    if (tile_rows == 0) tile_rows = 16;
+    VTI_CHECK(levels == 1 || levels == 2, "levels per launch must be 1 or 2, got %d", levels);
+    VTI_CHECK(tile_rows == 4 || tile_rows == 8 || tile_rows == 16, "tile rows must be 4, 8 or 16, got %d", tile_rows);
+    VTI_CHECK(snap_fmt >= 0 && snap_fmt <= 2, "snapshot format must be 0 (FP32), 1 (FP16) or 2 (BF16), got %d",
+              snap_fmt);
+    a.ns = ns;
+    a.nr = nr;
+    a.NX = NX;
+    a.nz = nz;
+    a.nt = nt;
+    a.npad = npad;
+    a.NNX = NX + 2 * npad;
+    a.nnz = nz + 2 * npad;
+    a.levels = levels;
+    a.BYT = tile_rows;
+    a.snap_fmt = snap_fmt;
+    a.half = snap_fmt != 0;
+    VTI_CHECK((long long)a.NNX * a.nnz < (1LL << 30),
+              "vti2d_objfunc_gradient_batch: padded model of %d x %d points exceeds the 2^30 the kernels index", a.nnz,
+              a.NNX);
+    a.ratio = (int)(rate / dt + 0.5f);  // objFuncGradient_f.cu:30 rounds in float
+    VTI_CHECK(a.ratio >= 2 && (nt - 1) % a.ratio == 0,
+              "the reference's snapshot rotation lines up only for rate/dt >= 2 and (nt-1) %% (rate/dt) == 0 "
+              "(got ratio %d, nt %d)", a.ratio, nt);
+    a.nnt = (nt - 1) / a.ratio + 1;
+    a.nnt_data = a.nnt;
+    a.dx2 = dx * dx;
+    a.dz2 = dz * dz;
+    a.dt2 = dt * dt;
+    a.in.resize(ns);
+    a.data_off.resize(ns);
+    long long data_total = 0;
+    const int nnz = a.nnz;
+    for (int s = 0; s < ns; ++s) {
+        ShotIn &q = a.in[s];
+        q.c0 = shots[6 * s];
+        q.nx = shots[6 * s + 1];
+        q.ix = shots[6 * s + 2];
+        q.iz = shots[6 * s + 3];
+        q.nr = shots[6 * s + 4];
+        q.r0 = shots[6 * s + 5];
+        const int w = q.nx + 2 * npad;
+        VTI_CHECK(q.nx > 0 && q.c0 >= 0 && q.c0 + w <= a.NNX,
+                  "shot %d: sub-grid [%d, %d) outside the model's %d columns", s, q.c0, q.c0 + w, a.NNX);
+        VTI_CHECK(w >= 9 && nnz >= 9, "shot %d: sub-grid too small", s);
+        VTI_CHECK(q.ix >= 4 && q.ix < w - 4 && q.iz >= 4 && q.iz < nnz - 4, "shot %d: source outside the interior",
+                  s);
+        VTI_CHECK(q.nr >= 0 && q.r0 >= 0 && q.r0 + q.nr <= nr, "shot %d: receiver range outside rloc", s);
+        std::vector<char> seen((size_t)w * nnz, 0);
+        for (int k = 0; k < q.nr; ++k) {
+            const int x = rloc[2 * (q.r0 + k)], z = rloc[2 * (q.r0 + k) + 1];
+            VTI_CHECK(x >= 4 && x < w - 4 && z >= 4 && z < nnz - 4, "shot %d receiver %d outside the interior", s, k);
+            VTI_CHECK(!seen[x + (size_t)z * w], "shot %d: two receivers share a grid point", s);
+            seen[x + (size_t)z * w] = 1;
+        }
+        a.data_off[s] = data_total;
+        data_total += (long long)a.nnt_data * q.nr;
+    }
+    return vti::kOk;
+}
+
+// Device state of a call shared by its waves: the model, the taper profile
+// (t0[npad] = 1, the factor away from the edges) and the FP64 gradient.
+constexpr int kNblkR = 1024;
+struct BatchShared {
+    DBuf<float> C[3], t0d;
+    DBuf<double> A[3];
+};
+
+int setup_shared(BatchShared &S, const BatchArgs &a, hipStream_t st)
+{
+    const size_t gsize = (size_t)a.NNX * a.nnz;
+    int rc = vti::kOk;
+    for (int k = 0; k < 3 && !rc; ++k) {
+        rc = S.C[k].alloc(gsize, st);
+        if (!rc) rc = S.A[k].alloc(gsize, st);
+    }
+    std::vector<float> t0(a.npad + 1);
+    vti::taper_profile2d(t0.data(), a.npad);
+    t0[a.npad] = 1.f;
+    if (!rc) rc = S.t0d.alloc(t0.size(), st);
+    if (!rc) VTI_HIP(hipMemcpyAsync(S.t0d.p, t0.data(), t0.size() * 4, hipMemcpyHostToDevice, st));
+    if (!rc) VTI_HIP(hipStreamSynchronize(st));  // t0 is a host temporary
+    return rc;
+}
+
+int upload_model(BatchShared &S, const BatchArgs &a, const float *c11, const float *c13, const float *c33,
+                 hipStream_t st)
+{
+    const size_t gsize = (size_t)a.NNX * a.nnz;
+    const float *c[3] = {c11, c13, c33};
+    for (int k = 0; k < 3; ++k) VTI_HIP(hipMemcpyAsync(S.C[k].p, c[k], gsize * 4, hipMemcpyHostToDevice, st));
+    return vti::kOk;
+}
+
+// One wave of shots on the device: descriptors and tile maps (sb/tile_shot
+// for the one-level kernels' 64 x 8 tiles, sbt/tile_shot_t for the
+// two-level ones' 64 x tile_rows), receiver maps, wavefields, snapshots,
+// traces, the observed data and the residual's partial sums.
+struct BatchWave {
+    int nw = 0, nrec = 0;
+    long long f_total = 0, snap_total = 0, tr_total = 0, max_plane = 0;
+    std::vector<ShotB> sb, sbt;
+    std::vector<int> tile_shot, tile_shot_t;
+    DBuf<ShotB> sbd, sbtd;
+    DBuf<int> tsh, tsht, recmap, tsdd;
+    DBuf<long long> fidxd, tbd;
+    DBuf<float> P[4], Q[4], G[3], snapf, stage, scale, tr, obs;
+    DBuf<unsigned short> snaph;
+    DBuf<unsigned> amax;
+    DBuf<double> part;
+};
+
+int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *data, const int *rloc, hipStream_t st)
+{
+    const int nw = s1 - s0, nnz = a.nnz, npad = a.npad, nnt = a.nnt;
+    const int ntz = (nnz - 8 + kBY2 - 1) / kBY2, ntz_t = (nnz - 8 + a.BYT - 1) / a.BYT;
+    W.nw = nw;
+    W.sb.assign(nw, ShotB{});
+    W.sbt.assign(nw, ShotB{});
+    W.tile_shot.clear();
+    W.tile_shot_t.clear();
+    W.f_total = W.snap_total = W.tr_total = W.max_plane = 0;
+    W.nrec = 0;
+    for (int j = 0; j < nw; ++j) {
+        const ShotIn &q = a.in[s0 + j];
+        ShotB &b = W.sb[j];
+        b.nnx = q.nx + 2 * npad;
+        b.ntx = (b.nnx - 8 + 63) / 64;
+        b.tile0 = (int)W.tile_shot.size();
+        for (int t = 0; t < b.ntx * ntz; ++t) W.tile_shot.push_back(j);
+        b.nr = q.nr;
+        b.f_off = W.f_total;
+        b.c_off = q.c0;
+        b.snap_off = W.snap_total;
+        b.r_off = W.tr_total;
+        b.stage_off = 2 * W.f_total;
+        b.src = q.ix + q.iz * b.nnx;
+        b.rz0 = nnz;
+        b.rz1 = -1;
+        for (int r = 0; r < q.nr; ++r) {
+            const int z = rloc[2 * (q.r0 + r) + 1];
+            b.rz0 = std::min(b.rz0, z);
+            b.rz1 = std::max(b.rz1, z);
+        }
+        b.scale_off = 2 * nnt * j;
+        W.sbt[j] = b;
+        W.sbt[j].tile0 = (int)W.tile_shot_t.size();
+        for (int t = 0; t < b.ntx * ntz_t; ++t) W.tile_shot_t.push_back(j);
+        const long long plane = (long long)b.nnx * nnz;
+        W.max_plane = std::max(W.max_plane, plane);
+        W.f_total += plane;
+        W.snap_total += (long long)nnt * 2 * plane;
+        W.tr_total += (long long)a.nnt_data * q.nr;
+        W.nrec += q.nr;
+    }
+    const int nrec = W.nrec;
+    std::vector<long long> fidx(nrec), tb(nrec);
+    std::vector<int> tsd(nrec);
+    std::vector<int> hmap((size_t)W.f_total, -1);
+    for (int j = 0, k = 0; j < nw; ++j) {
+        const ShotIn &q = a.in[s0 + j];
+        for (int r = 0; r < q.nr; ++r, ++k) {
+            const int x = rloc[2 * (q.r0 + r)], z = rloc[2 * (q.r0 + r) + 1];
+            fidx[k] = W.sb[j].f_off + x + (long long)z * W.sb[j].nnx;
+            tb[k] = W.sb[j].r_off + r;
+            tsd[k] = q.nr;
+            hmap[(size_t)fidx[k]] = r;
+        }
+    }
+    std::vector<float> obs_h((size_t)std::max<long long>(W.tr_total, 1));
+    for (int j = 0; j < nw; ++j)
+        if (a.in[s0 + j].nr)
+            std::memcpy(obs_h.data() + W.sb[j].r_off, data + a.data_off[s0 + j],
+                        sizeof(float) * (size_t)a.nnt_data * a.in[s0 + j].nr);
+    const long long f_total = W.f_total, tr_total = W.tr_total;
+    int rc = W.sbd.alloc(nw, st);
+    if (!rc) rc = W.sbtd.alloc(nw, st);
+    if (!rc) rc = W.tsh.alloc(W.tile_shot.size(), st);
+    if (!rc) rc = W.tsht.alloc(W.tile_shot_t.size(), st);
+    if (!rc) rc = W.recmap.alloc((size_t)f_total, st);
+    if (!rc) rc = W.tsdd.alloc(nrec, st);
+    if (!rc) rc = W.fidxd.alloc(nrec, st);
+    if (!rc) rc = W.tbd.alloc(nrec, st);
+    for (int k = 0; k < 4 && !rc; ++k) {
+        rc = W.P[k].alloc((size_t)f_total, st);
+        if (!rc) rc = W.Q[k].alloc((size_t)f_total, st);
+    }
+    for (int k = 0; k < 3 && !rc; ++k) rc = W.G[k].alloc((size_t)f_total, st);
+    if (a.half) {
+        if (!rc) rc = W.snaph.alloc((size_t)W.snap_total, st);
+        if (!rc) rc = W.stage.alloc((size_t)(2 * f_total), st);
+        if (!rc) rc = W.scale.alloc((size_t)(2 * nnt * nw), st);
+        if (!rc) rc = W.amax.alloc((size_t)(2 * nnt * nw), st);
+    } else {
+        if (!rc) rc = W.snapf.alloc((size_t)W.snap_total, st);
+    }
+    if (!rc) rc = W.tr.alloc((size_t)std::max<long long>(tr_total, 1), st);
+    if (!rc) rc = W.obs.alloc((size_t)std::max<long long>(tr_total, 1), st);
+    if (!rc) rc = W.part.alloc(kNblkR, st);
+    if (rc) return rc;
+    VTI_HIP(hipMemcpyAsync(W.sbd.p, W.sb.data(), sizeof(ShotB) * nw, hipMemcpyHostToDevice, st));
+    VTI_HIP(hipMemcpyAsync(W.sbtd.p, W.sbt.data(), sizeof(ShotB) * nw, hipMemcpyHostToDevice, st));
+    VTI_HIP(hipMemcpyAsync(W.tsh.p, W.tile_shot.data(), sizeof(int) * W.tile_shot.size(), hipMemcpyHostToDevice, st));
+    VTI_HIP(hipMemcpyAsync(W.tsht.p, W.tile_shot_t.data(), sizeof(int) * W.tile_shot_t.size(),
+                           hipMemcpyHostToDevice, st));
+    VTI_HIP(hipMemcpyAsync(W.recmap.p, hmap.data(), sizeof(int) * hmap.size(), hipMemcpyHostToDevice, st));
+    if (nrec) {
+        VTI_HIP(hipMemcpyAsync(W.tsdd.p, tsd.data(), sizeof(int) * nrec, hipMemcpyHostToDevice, st));
+        VTI_HIP(hipMemcpyAsync(W.fidxd.p, fidx.data(), sizeof(long long) * nrec, hipMemcpyHostToDevice, st));
+        VTI_HIP(hipMemcpyAsync(W.tbd.p, tb.data(), sizeof(long long) * nrec, hipMemcpyHostToDevice, st));
+    }
+    if (tr_total)
+        VTI_HIP(hipMemcpyAsync(W.obs.p, obs_h.data(), sizeof(float) * tr_total, hipMemcpyHostToDevice, st));
+    VTI_HIP(hipStreamSynchronize(st));  // the host staging vectors end here
+    return vti::kOk;
+}
+
+// Enqueue one wave's whole objective + gradient on `st`: the forward pass
+// with snapshots and traces, the residual (its partial sums into W.part),
+// the adjoint pass with the imaging condition and the shots' gradients
+// added in FP64 into S.A, shot by shot.  Every buffer a pass reads before
+// writing is cleared here by a kernel, so the sequence can be captured into a
+// graph and replayed.  Host work is limited to computing launch arguments.
+int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *wavelet, hipStream_t st)
+{
+    const int nw = W.nw, nrec = W.nrec, nt = a.nt, ratio = a.ratio, nnt = a.nnt, nnz = a.nnz, BYT = a.BYT;
+    const long long f_total = W.f_total, tr_total = W.tr_total;
+    const float dx2 = a.dx2, dz2 = a.dz2, dt2 = a.dt2;
+    const int levels = a.levels, snap_fmt = a.snap_fmt;
+    const bool half = a.half;
+    DBuf<float> *P = W.P, *Q = W.Q, *G = W.G, *C = S.C;
+    for (int k = 0; k < 4; ++k) {
+        zero_words(P[k].p, (size_t)f_total, st);
+        zero_words(Q[k].p, (size_t)f_total, st);
+    }
+    for (int k = 0; k < 3; ++k) zero_words(G[k].p, (size_t)f_total, st);
+    zero_words(W.tr.p, W.tr.n, st);  // row 0 is never recorded
+    if (half) zero_words(W.amax.p, W.amax.n, st);
+
+    BatchGeo bg;
+    bg.nnz = nnz;
+    bg.npad = a.npad;
+    bg.cpitch = a.NNX;
+    bg.nnxz_total = f_total;
+    const dim3 grid((unsigned)W.tile_shot.size()), block(64, kBY2);
+    const dim3 gridt((unsigned)W.tile_shot_t.size()), blockt(64, BYT);
+    const int rb = (nrec + 255) / 256;
+    const vti::SnapW sw{half ? W.stage.p : W.snapf.p, half ? W.amax.p : nullptr};
+    const vti::SnapR sr{W.snapf.p, W.snaph.p, W.scale.p};
+    const dim3 gpack((unsigned)std::min<long long>((2 * W.max_plane + 255) / 256, 1024), (unsigned)nw);
+    auto pack = [&](int slot) {
+        if (half && slot >= 0)
+            hipLaunchKernelGGL(vti::pack_snap_kernel, gpack, dim3(256), 0, st, W.snaph.p, W.scale.p, W.stage.p,
+                               W.amax.p, W.sbd.p, nnz, slot, snap_fmt == 2 ? 1 : 0);
+    };
+    auto record = [&](int it, int buf) {
+        if (it % ratio == 0 && nrec)
+            hipLaunchKernelGGL(record_b_kernel, dim3(rb), dim3(256), 0, st, W.tr.p, P[buf].p, Q[buf].p, W.fidxd.p,
+                               W.tbd.p, W.tsdd.p, nrec, it / ratio);
+    };
+    auto snap_slot = [&](int lc) { return lc % ratio == 0 ? lc / ratio : -1; };
+    ShotB *sbd = W.sbd.p, *sbtd = W.sbtd.p;
+    int *tsh = W.tsh.p, *tsht = W.tsht.p;
+    float *t0d = S.t0d.p;
+
+    // ---- forward (objFuncGradient_f.cu:160-210 for every shot).  Buffers
+    // ip/ic hold the previous and current levels; s0b/s1b are free.
+    int ip = 1, ic = 0, s0b = 2, s1b = 3;
+    hipLaunchKernelGGL(set_src_b_kernel, dim3((nw + 63) / 64), dim3(64), 0, st, P[ic].p, Q[ic].p, sbd, nw,
+                       dt2 * wavelet[0]);
+    for (int it = 2; it < nt;) {
+        if (levels == 2 && it + 1 < nt) {
+            // levels it and it+1, reading levels it-1 and it
+            const int sl1 = snap_slot(it - 1), sl2 = snap_slot(it);
+#define FWD_T2(BYV)                                                                                                 \
+    hipLaunchKernelGGL(vti::fwd2db_t2_kernel<BYV>, gridt, blockt, 0, st, P[s0b].p, Q[s0b].p, P[s1b].p, Q[s1b].p,     \
+                       P[ip].p, Q[ip].p, P[ic].p, Q[ic].p, C[0].p, C[1].p, C[2].p, t0d, sbtd, tsht, bg, dx2, dz2, \
+                       dt2, dt2 * wavelet[it - 1], dt2 * wavelet[it], sw, sl1, sl2)
+            if (BYT == 4) FWD_T2(4);
+            else if (BYT == 8) FWD_T2(8);
+            else FWD_T2(16);
+#undef FWD_T2
+            pack(sl1 >= 0 ? sl1 : sl2);
+            record(it, s0b);
+            record(it + 1, s1b);
+            std::swap(ip, s0b);
+            std::swap(ic, s1b);
+            it += 2;
+        } else {
+            const int sl = snap_slot(it - 1);
+            hipLaunchKernelGGL(vti::fwd2db_kernel, grid, block, 0, st, P[ip].p, Q[ip].p, P[ic].p, Q[ic].p, C[0].p,
+                               C[1].p, C[2].p, t0d, sbd, tsh, bg, dx2, dz2, dt2, dt2 * wavelet[it - 1], sw, sl);
+            pack(sl);
+            std::swap(ip, ic);
+            record(it, ic);
+            it += 1;
+        }
+    }
+    hipLaunchKernelGGL(vti::dsnap2db_kernel, grid, block, 0, st, P[ic].p, Q[ic].p, sbd, tsh, bg, dx2, dz2, sw,
+                       nnt - 1);
+    pack(nnt - 1);
+
+    // ---- residual and its partial sums (objFuncGradient_f.cu:214-219)
+    if (tr_total)
+        hipLaunchKernelGGL(residual_b_kernel, dim3(kNblkR), dim3(256), 0, st, W.tr.p, W.obs.p, tr_total, W.part.p);
+
+    // ---- backward (objFuncGradient_f.cu:225-300 for every shot)
+    for (int k = 0; k < 4; ++k) {
+        zero_words(P[k].p, (size_t)f_total, st);
+        zero_words(Q[k].p, (size_t)f_total, st);
+    }
+    ic = 0;
+    ip = 1;
+    s0b = 2;
+    s1b = 3;
+    if (nrec)
+        hipLaunchKernelGGL(inject_first_b_kernel, dim3(rb), dim3(256), 0, st, P[ic].p, Q[ic].p, W.tr.p, W.fidxd.p,
+                           W.tbd.p, W.tsdd.p, nrec, a.nnt_data - 1, dt2);
+    {
+        float f = (float)((nt - 2.) / ratio);
+        const int i1 = (int)f;
+        f = f - i1;
+#define XCOR(SFV)                                                                                                  \
+    hipLaunchKernelGGL(vti::xcor2db_kernel<SFV>, grid, block, 0, st, P[ic].p, Q[ic].p, sbd, tsh, bg, sr, i1, f, \
+                       G[0].p, G[1].p, G[2].p)
+        if (snap_fmt == 0) XCOR(0);
+        else if (snap_fmt == 1) XCOR(1);
+        else XCOR(2);
+#undef XCOR
+    }
+    // trace rows and snapshot slots bracketing adjoint level it
+    auto level = [&](int it) {
+        vti::LevelB l;
+        float fr = (float)((it + 1.) / ratio);
+        l.row = (int)fr;
+        l.fr = fr - l.row;
+        float f = (float)it / ratio;
+        l.slot = (int)f;
+        l.f = f - l.slot;
+        return l;
+    };
+    int *recmap = W.recmap.p;
+    float *tr = W.tr.p;
+    for (int it = nt - 3; it >= 0;) {
+        if (levels == 2 && it >= 1) {
+            const vti::LevelB l1 = level(it), l2 = level(it - 1);
+#define BWD_T2(BYV, SFV)                                                                                         \
+    vti::bwd2db_t2_kernel<BYV, SFV><<<gridt, blockt, 0, st>>>(P[s0b].p, Q[s0b].p, P[s1b].p, Q[s1b].p, P[ip].p,  \
+                                                              Q[ip].p, P[ic].p, Q[ic].p, C[0].p, C[1].p, C[2].p, \
+                                                              t0d, sbtd, tsht, bg, recmap, tr, l1, l2, sr, G[0].p, \
+                                                              G[1].p, G[2].p, dx2, dz2, dt2)
+#define BWD_T2_SF(BYV)                      \
+    if (snap_fmt == 0) BWD_T2(BYV, 0);      \
+    else if (snap_fmt == 1) BWD_T2(BYV, 1); \
+    else BWD_T2(BYV, 2)
+            if (BYT == 4) {
+                BWD_T2_SF(4);
+            } else if (BYT == 8) {
+                BWD_T2_SF(8);
+            } else {
+                BWD_T2_SF(16);
+            }
+#undef BWD_T2_SF
+#undef BWD_T2
+            std::swap(ip, s0b);
+            std::swap(ic, s1b);
+            it -= 2;
+        } else {
+            const vti::LevelB l = level(it);
+#define BWD1(SFV)                                                                                                  \
+    hipLaunchKernelGGL(vti::bwd2db_kernel<SFV>, grid, block, 0, st, P[ip].p, Q[ip].p, P[ic].p, Q[ic].p, C[0].p,    \
+                       C[1].p, C[2].p, t0d, sbd, tsh, bg, recmap, tr, l.row, l.fr, sr, l.slot, l.f, G[0].p, G[1].p, \
+                       G[2].p, dx2, dz2, dt2)
+            if (snap_fmt == 0) BWD1(0);
+            else if (snap_fmt == 1) BWD1(1);
+            else BWD1(2);
+#undef BWD1
+            std::swap(ip, ic);
+            it -= 1;
+        }
+    }
+    // ---- per-shot FP32 gradients into the FP64 global sum, in shot order
+    for (int j = 0; j < nw; ++j)
+        hipLaunchKernelGGL(accum_crop_kernel, dim3((W.sb[j].nnx + 255) / 256, nnz), dim3(256), 0, st, S.A[0].p,
+                           S.A[1].p, S.A[2].p, G[0].p, G[1].p, G[2].p, W.sb[j].f_off, W.sb[j].c_off, W.sb[j].nnx, nnz,
+                           a.NNX);
+    VTI_HIP(hipGetLastError());
+    return vti::kOk;
+}
+
+// the objective's share of a wave: the residual's partial sums, in order
+int wave_objective(const BatchWave &W, hipStream_t st, double *obj)
+{
+    if (!W.tr_total) return vti::kOk;
+    std::vector<double> hp(kNblkR);
+    VTI_HIP(hipMemcpyAsync(hp.data(), W.part.p, sizeof(double) * kNblkR, hipMemcpyDeviceToHost, st));
+    VTI_HIP(hipStreamSynchronize(st));
+    for (double v : hp) *obj += v;
+    return vti::kOk;
+}
+
+int download_gradient(const BatchShared &S, const BatchArgs &a, double *gc11, double *gc13, double *gc33,
+                      hipStream_t st)
+{
+    const size_t gsize = (size_t)a.NNX * a.nnz;
+    double *out[3] = {gc11, gc13, gc33};
+    for (int k = 0; k < 3; ++k) VTI_HIP(hipMemcpyAsync(out[k], S.A[k].p, gsize * 8, hipMemcpyDeviceToHost, st));
+    VTI_HIP(hipStreamSynchronize(st));
+    return vti::kOk;
+}
+
+// The shots [s0, s1) of each wave: consecutive shots, at most max_wave of
+// them (when > 0) and within `budget` bytes (at least one shot per wave).
+std::vector<std::pair<int, int>> split_waves(const BatchArgs &a, int max_wave, long long budget)
+{
+    std::vector<std::pair<int, int>> w;
+    for (int s0 = 0; s0 < a.ns;) {
+        long long used = 0;
+        int s1 = s0;
+        while (s1 < a.ns && (max_wave <= 0 || s1 - s0 < max_wave) && used + a.shot_bytes(s1) <= budget)
+            used += a.shot_bytes(s1++);
+        if (s1 == s0) s1 = s0 + 1;
+        w.emplace_back(s0, s1);
+        s0 = s1;
+    }
+    return w;
+}
+
 }  // namespace
+
+// A captured batch: the device state of every wave (all held at once) and
+// the graph of the whole launch sequence, wave after wave.
+struct vti2d_batch_plan {
+    vti_ctx *ctx = nullptr;
+    BatchArgs a;
+    BatchShared S;
+    std::vector<std::unique_ptr<BatchWave>> W;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
 
 extern "C" {
 
@@ -115,363 +594,121 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
 {
     VTI_CHECK(ctx && objfunc && gc11 && gc13 && gc33 && data && c11 && c13 && c33 && wavelet && shots && rloc,
               "vti2d_objfunc_gradient_batch: null argument");
-    VTI_CHECK(ns >= 1 && nr >= 0 && NX > 0 && nz > 0 && npad >= 0 && nt >= 3 && dx > 0 && dz > 0 && dt > 0,
-              "vti2d_objfunc_gradient_batch: bad sizes");
-    if (levels == 0) levels = 2;
-    if (tile_rows == 0) tile_rows = 16;
-    VTI_CHECK(levels == 1 || levels == 2, "levels per launch must be 1 or 2, got %d", levels);
-    VTI_CHECK(tile_rows == 4 || tile_rows == 8 || tile_rows == 16, "tile rows must be 4, 8 or 16, got %d", tile_rows);
-    VTI_CHECK(snap_fmt >= 0 && snap_fmt <= 2, "snapshot format must be 0 (FP32), 1 (FP16) or 2 (BF16), got %d",
-              snap_fmt);
+    BatchArgs a;
+    int rc = parse_args(a, shots, ns, rloc, nr, NX, nz, nt, npad, dx, dz, dt, rate, levels, tile_rows, snap_fmt);
+    if (rc) return rc;
     VTI_HIP(hipSetDevice(ctx->device));
     (void)hipGetLastError();
     hipStream_t st = ctx->stream;
-    const int NNX = NX + 2 * npad, nnz = nz + 2 * npad;
-    VTI_CHECK((long long)NNX * nnz < (1LL << 30),
-              "vti2d_objfunc_gradient_batch: padded model of %d x %d points exceeds the 2^30 the kernels index", nnz,
-              NNX);
-    const int ratio = (int)(rate / dt + 0.5f);  // objFuncGradient_f.cu:30 rounds in float
-    VTI_CHECK(ratio >= 2 && (nt - 1) % ratio == 0,
-              "the reference's snapshot rotation lines up only for rate/dt >= 2 and (nt-1) %% (rate/dt) == 0 "
-              "(got ratio %d, nt %d)", ratio, nt);
-    const int nnt = (nt - 1) / ratio + 1, nnt_data = nnt;
-    const float dx2 = dx * dx, dz2 = dz * dz, dt2 = dt * dt;
-    const bool half = snap_fmt != 0;
-
-    // ---- shots: check and lay out
-    std::vector<ShotIn> in(ns);
-    long long data_total = 0;
-    std::vector<long long> data_off(ns);
-    for (int s = 0; s < ns; ++s) {
-        ShotIn &a = in[s];
-        a.c0 = shots[6 * s];
-        a.nx = shots[6 * s + 1];
-        a.ix = shots[6 * s + 2];
-        a.iz = shots[6 * s + 3];
-        a.nr = shots[6 * s + 4];
-        a.r0 = shots[6 * s + 5];
-        const int w = a.nx + 2 * npad;
-        VTI_CHECK(a.nx > 0 && a.c0 >= 0 && a.c0 + w <= NNX, "shot %d: sub-grid [%d, %d) outside the model's %d columns",
-                  s, a.c0, a.c0 + w, NNX);
-        VTI_CHECK(w >= 9 && nnz >= 9, "shot %d: sub-grid too small", s);
-        VTI_CHECK(a.ix >= 4 && a.ix < w - 4 && a.iz >= 4 && a.iz < nnz - 4, "shot %d: source outside the interior",
-                  s);
-        VTI_CHECK(a.nr >= 0 && a.r0 >= 0 && a.r0 + a.nr <= nr, "shot %d: receiver range outside rloc", s);
-        std::vector<char> seen((size_t)w * nnz, 0);
-        for (int k = 0; k < a.nr; ++k) {
-            const int x = rloc[2 * (a.r0 + k)], z = rloc[2 * (a.r0 + k) + 1];
-            VTI_CHECK(x >= 4 && x < w - 4 && z >= 4 && z < nnz - 4, "shot %d receiver %d outside the interior", s, k);
-            VTI_CHECK(!seen[x + (size_t)z * w], "shot %d: two receivers share a grid point", s);
-            seen[x + (size_t)z * w] = 1;
-        }
-        data_off[s] = data_total;
-        data_total += (long long)nnt_data * a.nr;
-    }
-
-    // ---- global model and FP64 gradient
-    const size_t gsize = (size_t)NNX * nnz;
-    DBuf<float> C[3], t0d;
-    DBuf<double> A[3];
-    DBuf<double> part;
-    int rc = vti::kOk;
-    const float *csrc[3] = {c11, c13, c33};
-    for (int k = 0; k < 3 && !rc; ++k) {
-        rc = C[k].alloc(gsize, st);
-        if (!rc) VTI_HIP(hipMemcpyAsync(C[k].p, csrc[k], gsize * 4, hipMemcpyHostToDevice, st));
-        if (!rc) rc = A[k].alloc(gsize, st);
-    }
-    std::vector<float> t0(npad + 1);  // t0[npad] = 1: the factor away from the edges
-    vti::taper_profile2d(t0.data(), npad);
-    t0[npad] = 1.f;
-    if (!rc) rc = t0d.alloc(t0.size(), st);
-    if (!rc) VTI_HIP(hipMemcpyAsync(t0d.p, t0.data(), t0.size() * 4, hipMemcpyHostToDevice, st));
-    const int nblk_r = 1024;
-    if (!rc) rc = part.alloc(nblk_r, st);
+    BatchShared S;
+    rc = setup_shared(S, a, st);
+    if (!rc) rc = upload_model(S, a, c11, c13, c33, st);
     if (rc) return rc;
 
-    // ---- waves within the memory budget: per shot 8 wavefield planes, 3
-    // gradient planes, the receiver map, the snapshots (and for 16-bit ones
-    // two staging planes and the slice scales/maxima) and two trace copies
-    auto shot_bytes = [&](int s) {
-        const long long nxz = (long long)(in[s].nx + 2 * npad) * nnz;
-        const long long snapb = half ? 2LL * nxz * 2 * nnt + 4LL * nxz * 2 + 16LL * nnt : 4LL * nxz * 2 * nnt;
-        return 4LL * nxz * (8 + 3 + 1) + snapb + 8LL * nnt_data * in[s].nr + 64;
-    };
+    // waves within the memory budget
     size_t mfree = 0, mtotal = 0;
     VTI_HIP(hipMemGetInfo(&mfree, &mtotal));
     const long long budget = (long long)mfree - (long long)(2ULL << 30);
-    VTI_CHECK(budget > shot_bytes(0), "not enough device memory for one shot");
+    VTI_CHECK(budget > a.shot_bytes(0), "not enough device memory for one shot");
     double obj = 0.0;
-    int s0 = 0;
-    while (s0 < ns) {
-        long long used = 0;
-        int s1 = s0;
-        while (s1 < ns && (max_wave <= 0 || s1 - s0 < max_wave) && used + shot_bytes(s1) <= budget)
-            used += shot_bytes(s1++);
-        if (s1 == s0) s1 = s0 + 1;
-        const int nw = s1 - s0;
-
-        // layout of the wave: sb/tile_shot for the one-level kernels (64 x 8
-        // tiles), sbt/tile_shot_t for the two-level ones (64 x tile_rows)
-        std::vector<ShotB> sb(nw), sbt(nw);
-        std::vector<int> tile_shot, tile_shot_t;
-        const int BYT = tile_rows, ntz = (nnz - 8 + kBY2 - 1) / kBY2, ntz_t = (nnz - 8 + BYT - 1) / BYT;
-        long long f_total = 0, snap_total = 0, tr_total = 0, max_plane = 0;
-        int nrec = 0;
-        for (int j = 0; j < nw; ++j) {
-            const ShotIn &a = in[s0 + j];
-            ShotB &b = sb[j];
-            b.nnx = a.nx + 2 * npad;
-            b.ntx = (b.nnx - 8 + 63) / 64;
-            b.tile0 = (int)tile_shot.size();
-            for (int t = 0; t < b.ntx * ntz; ++t) tile_shot.push_back(j);
-            b.nr = a.nr;
-            b.f_off = f_total;
-            b.c_off = a.c0;
-            b.snap_off = snap_total;
-            b.r_off = tr_total;
-            b.stage_off = 2 * f_total;
-            b.src = a.ix + a.iz * b.nnx;
-            b.rz0 = nnz;
-            b.rz1 = -1;
-            for (int r = 0; r < a.nr; ++r) {
-                const int z = rloc[2 * (a.r0 + r) + 1];
-                b.rz0 = std::min(b.rz0, z);
-                b.rz1 = std::max(b.rz1, z);
-            }
-            b.scale_off = 2 * nnt * j;
-            sbt[j] = b;
-            sbt[j].tile0 = (int)tile_shot_t.size();
-            for (int t = 0; t < b.ntx * ntz_t; ++t) tile_shot_t.push_back(j);
-            const long long plane = (long long)b.nnx * nnz;
-            max_plane = std::max(max_plane, plane);
-            f_total += plane;
-            snap_total += (long long)nnt * 2 * plane;
-            tr_total += (long long)nnt_data * a.nr;
-            nrec += a.nr;
-        }
-        std::vector<long long> fidx(nrec), tb(nrec);
-        std::vector<int> tsd(nrec);
-        std::vector<int> hmap((size_t)f_total, -1);
-        {
-            int k = 0;
-            for (int j = 0; j < nw; ++j) {
-                const ShotIn &a = in[s0 + j];
-                for (int r = 0; r < a.nr; ++r, ++k) {
-                    const int x = rloc[2 * (a.r0 + r)], z = rloc[2 * (a.r0 + r) + 1];
-                    fidx[k] = sb[j].f_off + x + (long long)z * sb[j].nnx;
-                    tb[k] = sb[j].r_off + r;
-                    tsd[k] = a.nr;
-                    hmap[(size_t)fidx[k]] = r;
-                }
-            }
-        }
-        std::vector<float> obs_h((size_t)std::max<long long>(tr_total, 1));
-        for (int j = 0; j < nw; ++j)
-            if (in[s0 + j].nr)
-                std::memcpy(obs_h.data() + sb[j].r_off, data + data_off[s0 + j],
-                            sizeof(float) * (size_t)nnt_data * in[s0 + j].nr);
-
-        DBuf<ShotB> sbd, sbtd;
-        DBuf<int> tsh, tsht, recmap, tsdd;
-        DBuf<long long> fidxd, tbd;
-        DBuf<float> P[4], Q[4], G[3], snapf, stage, scale, tr, obs;
-        DBuf<unsigned short> snaph;
-        DBuf<unsigned> amax;
-        rc = sbd.alloc(nw, st);
-        if (!rc) rc = sbtd.alloc(nw, st);
-        if (!rc) rc = tsh.alloc(tile_shot.size(), st);
-        if (!rc) rc = tsht.alloc(tile_shot_t.size(), st);
-        if (!rc) rc = recmap.alloc((size_t)f_total, st);
-        if (!rc) rc = tsdd.alloc(nrec, st);
-        if (!rc) rc = fidxd.alloc(nrec, st);
-        if (!rc) rc = tbd.alloc(nrec, st);
-        for (int k = 0; k < 4 && !rc; ++k) {
-            rc = P[k].alloc((size_t)f_total, st);
-            if (!rc) rc = Q[k].alloc((size_t)f_total, st);
-        }
-        for (int k = 0; k < 3 && !rc; ++k) rc = G[k].alloc((size_t)f_total, st);
-        if (half) {
-            if (!rc) rc = snaph.alloc((size_t)snap_total, st);
-            if (!rc) rc = stage.alloc((size_t)(2 * f_total), st);
-            if (!rc) rc = scale.alloc((size_t)(2 * nnt * nw), st);
-            if (!rc) rc = amax.alloc((size_t)(2 * nnt * nw), st);
-        } else {
-            if (!rc) rc = snapf.alloc((size_t)snap_total, st);
-        }
-        if (!rc) rc = tr.alloc((size_t)std::max<long long>(tr_total, 1), st);
-        if (!rc) rc = obs.alloc((size_t)std::max<long long>(tr_total, 1), st);
+    for (const auto &w : split_waves(a, max_wave, budget)) {
+        BatchWave W;
+        rc = setup_wave(W, a, w.first, w.second, data, rloc, st);
+        if (!rc) rc = enqueue_wave(W, S, a, wavelet, st);
+        if (!rc) rc = wave_objective(W, st, &obj);
         if (rc) return rc;
-        VTI_HIP(hipMemcpyAsync(sbd.p, sb.data(), sizeof(ShotB) * nw, hipMemcpyHostToDevice, st));
-        VTI_HIP(hipMemcpyAsync(sbtd.p, sbt.data(), sizeof(ShotB) * nw, hipMemcpyHostToDevice, st));
-        VTI_HIP(hipMemcpyAsync(tsh.p, tile_shot.data(), sizeof(int) * tile_shot.size(), hipMemcpyHostToDevice, st));
-        VTI_HIP(hipMemcpyAsync(tsht.p, tile_shot_t.data(), sizeof(int) * tile_shot_t.size(), hipMemcpyHostToDevice,
-                               st));
-        VTI_HIP(hipMemcpyAsync(recmap.p, hmap.data(), sizeof(int) * hmap.size(), hipMemcpyHostToDevice, st));
-        if (nrec) {
-            VTI_HIP(hipMemcpyAsync(tsdd.p, tsd.data(), sizeof(int) * nrec, hipMemcpyHostToDevice, st));
-            VTI_HIP(hipMemcpyAsync(fidxd.p, fidx.data(), sizeof(long long) * nrec, hipMemcpyHostToDevice, st));
-            VTI_HIP(hipMemcpyAsync(tbd.p, tb.data(), sizeof(long long) * nrec, hipMemcpyHostToDevice, st));
-        }
-        if (tr_total)
-            VTI_HIP(hipMemcpyAsync(obs.p, obs_h.data(), sizeof(float) * tr_total, hipMemcpyHostToDevice, st));
-
-        BatchGeo bg;
-        bg.nnz = nnz;
-        bg.npad = npad;
-        bg.cpitch = NNX;
-        bg.nnxz_total = f_total;
-        const dim3 grid((unsigned)tile_shot.size()), block(64, kBY2);
-        const dim3 gridt((unsigned)tile_shot_t.size()), blockt(64, BYT);
-        const int rb = (nrec + 255) / 256;
-        const vti::SnapW sw{half ? stage.p : snapf.p, half ? amax.p : nullptr};
-        const vti::SnapR sr{snapf.p, snaph.p, scale.p};
-        const dim3 gpack((unsigned)std::min<long long>((2 * max_plane + 255) / 256, 1024), (unsigned)nw);
-        auto pack = [&](int slot) {
-            if (half && slot >= 0)
-                hipLaunchKernelGGL(vti::pack_snap_kernel, gpack, dim3(256), 0, st, snaph.p, scale.p, stage.p, amax.p,
-                                   sbd.p, nnz, slot, snap_fmt == 2 ? 1 : 0);
-        };
-        auto record = [&](int it, int buf) {
-            if (it % ratio == 0 && nrec)
-                hipLaunchKernelGGL(record_b_kernel, dim3(rb), dim3(256), 0, st, tr.p, P[buf].p, Q[buf].p, fidxd.p,
-                                   tbd.p, tsdd.p, nrec, it / ratio);
-        };
-        auto snap_slot = [&](int lc) { return lc % ratio == 0 ? lc / ratio : -1; };
-
-        // ---- forward (objFuncGradient_f.cu:160-210 for every shot).  Buffers
-        // ip/ic hold the previous and current levels; s0b/s1b are free.
-        int ip = 1, ic = 0, s0b = 2, s1b = 3;
-        hipLaunchKernelGGL(set_src_b_kernel, dim3((nw + 63) / 64), dim3(64), 0, st, P[ic].p, Q[ic].p, sbd.p, nw,
-                           dt2 * wavelet[0]);
-        for (int it = 2; it < nt;) {
-            if (levels == 2 && it + 1 < nt) {
-                // levels it and it+1, reading levels it-1 and it
-                const int sl1 = snap_slot(it - 1), sl2 = snap_slot(it);
-#define FWD_T2(BYV)                                                                                                 \
-    hipLaunchKernelGGL(vti::fwd2db_t2_kernel<BYV>, gridt, blockt, 0, st, P[s0b].p, Q[s0b].p, P[s1b].p, Q[s1b].p,     \
-                       P[ip].p, Q[ip].p, P[ic].p, Q[ic].p, C[0].p, C[1].p, C[2].p, t0d.p, sbtd.p, tsht.p, bg, dx2, \
-                       dz2, dt2, dt2 * wavelet[it - 1], dt2 * wavelet[it], sw, sl1, sl2)
-                if (BYT == 4) FWD_T2(4);
-                else if (BYT == 8) FWD_T2(8);
-                else FWD_T2(16);
-#undef FWD_T2
-                pack(sl1 >= 0 ? sl1 : sl2);
-                record(it, s0b);
-                record(it + 1, s1b);
-                std::swap(ip, s0b);
-                std::swap(ic, s1b);
-                it += 2;
-            } else {
-                const int sl = snap_slot(it - 1);
-                hipLaunchKernelGGL(vti::fwd2db_kernel, grid, block, 0, st, P[ip].p, Q[ip].p, P[ic].p, Q[ic].p,
-                                   C[0].p, C[1].p, C[2].p, t0d.p, sbd.p, tsh.p, bg, dx2, dz2, dt2,
-                                   dt2 * wavelet[it - 1], sw, sl);
-                pack(sl);
-                std::swap(ip, ic);
-                record(it, ic);
-                it += 1;
-            }
-        }
-        hipLaunchKernelGGL(vti::dsnap2db_kernel, grid, block, 0, st, P[ic].p, Q[ic].p, sbd.p, tsh.p, bg, dx2, dz2, sw,
-                           nnt - 1);
-        pack(nnt - 1);
-        VTI_HIP(hipGetLastError());
-
-        // ---- residual and objective (objFuncGradient_f.cu:214-219)
-        if (tr_total) {
-            hipLaunchKernelGGL(residual_b_kernel, dim3(nblk_r), dim3(256), 0, st, tr.p, obs.p, tr_total, part.p);
-            std::vector<double> hp(nblk_r);
-            VTI_HIP(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * nblk_r, hipMemcpyDeviceToHost, st));
-            VTI_HIP(hipStreamSynchronize(st));
-            for (double v : hp) obj += v;
-        }
-
-        // ---- backward (objFuncGradient_f.cu:225-300 for every shot)
-        for (int k = 0; k < 4; ++k) {
-            VTI_HIP(hipMemsetAsync(P[k].p, 0, (size_t)f_total * 4, st));
-            VTI_HIP(hipMemsetAsync(Q[k].p, 0, (size_t)f_total * 4, st));
-        }
-        ic = 0;
-        ip = 1;
-        s0b = 2;
-        s1b = 3;
-        if (nrec)
-            hipLaunchKernelGGL(inject_first_b_kernel, dim3(rb), dim3(256), 0, st, P[ic].p, Q[ic].p, tr.p, fidxd.p,
-                               tbd.p, tsdd.p, nrec, nnt_data - 1, dt2);
-        {
-            float f = (float)((nt - 2.) / ratio);
-            const int i1 = (int)f;
-            f = f - i1;
-#define XCOR(SFV)                                                                                                  \
-    hipLaunchKernelGGL(vti::xcor2db_kernel<SFV>, grid, block, 0, st, P[ic].p, Q[ic].p, sbd.p, tsh.p, bg, sr, i1, f, \
-                       G[0].p, G[1].p, G[2].p)
-            if (snap_fmt == 0) XCOR(0);
-            else if (snap_fmt == 1) XCOR(1);
-            else XCOR(2);
-#undef XCOR
-        }
-        // trace rows and snapshot slots bracketing adjoint level it
-        auto level = [&](int it) {
-            vti::LevelB l;
-            float fr = (float)((it + 1.) / ratio);
-            l.row = (int)fr;
-            l.fr = fr - l.row;
-            float f = (float)it / ratio;
-            l.slot = (int)f;
-            l.f = f - l.slot;
-            return l;
-        };
-        for (int it = nt - 3; it >= 0;) {
-            if (levels == 2 && it >= 1) {
-                const vti::LevelB l1 = level(it), l2 = level(it - 1);
-#define BWD_T2(BYV, SFV)                                                                                             \
-    vti::bwd2db_t2_kernel<BYV, SFV><<<gridt, blockt, 0, st>>>(P[s0b].p, Q[s0b].p, P[s1b].p,                         \
-                       Q[s1b].p, P[ip].p, Q[ip].p, P[ic].p, Q[ic].p, C[0].p, C[1].p, C[2].p, t0d.p, sbtd.p, tsht.p, \
-                       bg, recmap.p, tr.p, l1, l2, sr, G[0].p, G[1].p, G[2].p, dx2, dz2, dt2)
-#define BWD_T2_SF(BYV)                  \
-    if (snap_fmt == 0) BWD_T2(BYV, 0);  \
-    else if (snap_fmt == 1) BWD_T2(BYV, 1); \
-    else BWD_T2(BYV, 2)
-                if (BYT == 4) {
-                    BWD_T2_SF(4);
-                } else if (BYT == 8) {
-                    BWD_T2_SF(8);
-                } else {
-                    BWD_T2_SF(16);
-                }
-#undef BWD_T2_SF
-#undef BWD_T2
-                std::swap(ip, s0b);
-                std::swap(ic, s1b);
-                it -= 2;
-            } else {
-                const vti::LevelB l = level(it);
-#define BWD1(SFV)                                                                                                    \
-    hipLaunchKernelGGL(vti::bwd2db_kernel<SFV>, grid, block, 0, st, P[ip].p, Q[ip].p, P[ic].p, Q[ic].p, C[0].p,      \
-                       C[1].p, C[2].p, t0d.p, sbd.p, tsh.p, bg, recmap.p, tr.p, l.row, l.fr, sr, l.slot, l.f, G[0].p, \
-                       G[1].p, G[2].p, dx2, dz2, dt2)
-                if (snap_fmt == 0) BWD1(0);
-                else if (snap_fmt == 1) BWD1(1);
-                else BWD1(2);
-#undef BWD1
-                std::swap(ip, ic);
-                it -= 1;
-            }
-        }
-        VTI_HIP(hipGetLastError());
-        // ---- per-shot FP32 gradients into the FP64 global sum, in shot order
-        for (int j = 0; j < nw; ++j)
-            hipLaunchKernelGGL(accum_crop_kernel, dim3((sb[j].nnx + 255) / 256, nnz), dim3(256), 0, st, A[0].p,
-                               A[1].p, A[2].p, G[0].p, G[1].p, G[2].p, sb[j].f_off, sb[j].c_off, sb[j].nnx, nnz, NNX);
-        VTI_HIP(hipGetLastError());
-        VTI_HIP(hipStreamSynchronize(st));
-        s0 = s1;
     }
-    double *out[3] = {gc11, gc13, gc33};
-    for (int k = 0; k < 3; ++k) VTI_HIP(hipMemcpyAsync(out[k], A[k].p, gsize * 8, hipMemcpyDeviceToHost, st));
-    VTI_HIP(hipStreamSynchronize(st));
+    rc = download_gradient(S, a, gc11, gc13, gc33, st);
+    if (rc) return rc;
     *objfunc = 0.5 * obj;
+    return vti::kOk;
+}
+
+// Captured batch (see vti_amd.h).
+int vti2d_batch_plan_create(vti_ctx *ctx, vti2d_batch_plan **plan, const float *data, const float *wavelet,
+                            const int *shots, int ns, const int *rloc, int nr, int NX, int nz, int nt, int npad,
+                            float dx, float dz, float dt, float rate, int max_wave, int levels, int tile_rows,
+                            int snap_fmt)
+{
+    VTI_CHECK(ctx && plan && data && wavelet && shots && rloc, "vti2d_batch_plan_create: null argument");
+    *plan = nullptr;
+    auto *pl = new vti2d_batch_plan;
+    pl->ctx = ctx;
+    int rc = parse_args(pl->a, shots, ns, rloc, nr, NX, nz, nt, npad, dx, dz, dt, rate, levels, tile_rows, snap_fmt);
+    hipStream_t st = ctx->stream;
+    if (!rc) {
+        const hipError_t e = hipSetDevice(ctx->device);
+        if (e != hipSuccess) {
+            vti::set_error("hipSetDevice: %s", hipGetErrorString(e));
+            rc = vti::kErrHip;
+        }
+    }
+    if (!rc) {
+        size_t mfree = 0, mtotal = 0;
+        (void)hipMemGetInfo(&mfree, &mtotal);
+        long long need = 0;
+        for (int s = 0; s < ns; ++s) need += pl->a.shot_bytes(s);
+        if (need > (long long)mfree - (long long)(2ULL << 30)) {
+            vti::set_error("vti2d_batch_plan_create: the %d shots need %.1f GB held at once, %.1f GB free", ns,
+                           need / 1e9, mfree / 1e9);
+            rc = vti::kErrArg;
+        }
+    }
+    if (!rc) rc = setup_shared(pl->S, pl->a, st);
+    if (!rc)
+        for (const auto &w : split_waves(pl->a, max_wave, 1LL << 62)) {
+            pl->W.emplace_back(new BatchWave);
+            rc = setup_wave(*pl->W.back(), pl->a, w.first, w.second, data, rloc, st);
+            if (rc) break;
+        }
+    if (!rc) {
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            for (int k = 0; k < 3; ++k) zero_words(pl->S.A[k].p, 2 * pl->S.A[k].n, st);
+            for (auto &W : pl->W)
+                if (!rc) rc = enqueue_wave(*W, pl->S, pl->a, wavelet, st);
+            const hipError_t e2 = hipStreamEndCapture(st, &pl->graph);
+            if (!rc && e2 != hipSuccess) e = e2;
+        }
+        if (!rc && e == hipSuccess) e = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0);
+        if (!rc && e != hipSuccess) {
+            vti::set_error("vti2d_batch_plan_create: graph capture: %s", hipGetErrorString(e));
+            rc = vti::kErrHip;
+        }
+    }
+    if (rc) {
+        vti2d_batch_plan_destroy(pl);
+        return rc;
+    }
+    *plan = pl;
+    return vti::kOk;
+}
+
+int vti2d_batch_plan_run(vti2d_batch_plan *plan, double *objfunc, double *gc11, double *gc13, double *gc33,
+                         const float *c11, const float *c13, const float *c33)
+{
+    VTI_CHECK(plan && objfunc && gc11 && gc13 && gc33 && c11 && c13 && c33, "vti2d_batch_plan_run: null argument");
+    VTI_HIP(hipSetDevice(plan->ctx->device));
+    hipStream_t st = plan->ctx->stream;
+    int rc = upload_model(plan->S, plan->a, c11, c13, c33, st);
+    if (rc) return rc;
+    VTI_HIP(hipGraphLaunch(plan->exec, st));
+    double obj = 0.0;
+    for (const auto &W : plan->W)
+        if (!rc) rc = wave_objective(*W, st, &obj);
+    if (!rc) rc = download_gradient(plan->S, plan->a, gc11, gc13, gc33, st);
+    if (rc) return rc;
+    *objfunc = 0.5 * obj;
+    return vti::kOk;
+}
+
+int vti2d_batch_plan_destroy(vti2d_batch_plan *plan)
+{
+    if (!plan) return vti::kOk;
+    if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    (void)hipSetDevice(plan->ctx->device);
+    delete plan;  // the buffers go back to the stream's pool
     return vti::kOk;
 }
 

@@ -104,6 +104,10 @@ _SIGS = {
         c_int,
         [P] * 11 + [c_int, P] + [c_int] * 5 + [c_float] * 4 + [c_int] * 4,
     ),
+    "vti2d_batch_plan_create": (c_int, [P, POINTER(c_void_p), P, P, P, c_int, P, c_int] + [c_int] * 4
+                                + [c_float] * 4 + [c_int] * 4),
+    "vti2d_batch_plan_run": (c_int, [P] * 8),
+    "vti2d_batch_plan_destroy": (c_int, [P]),
     "vti_lbfgs_create": (c_int, [POINTER(c_void_p), c_longlong, c_int, c_int]),
     "vti_lbfgs_destroy": (c_int, [P]),
     "vti_lbfgs_step": (c_int, [P, P, ctypes.c_double, P, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
@@ -584,6 +588,44 @@ def objfunc_gradient2d_batch(ctx, data, c11, c13, c33, wavelet, shots, rloc, NX,
                                             levels, tile_rows, SNAP_FORMATS[snap]),
           "vti2d_objfunc_gradient_batch")
     return f.value, gc
+
+
+class BatchPlan:
+    """vti2d_batch_plan_*: the batched gradient of fixed shots (geometry,
+    observed data, wavelet) captured once into a hipGraph; run(c11, c13, c33)
+    replays it on a new model and returns what objfunc_gradient2d_batch
+    returns for the same arguments."""
+
+    def __init__(self, ctx, data, wavelet, shots, rloc, NX, nz, nt, npad, dx, dz, dt, rate, max_wave=0, levels=0,
+                 tile_rows=0, snap="fp32"):
+        wavelet, data = f32(wavelet).ravel(), f32(data).ravel()
+        shots, rloc = _i32(shots), _i32(rloc)
+        self.shape = (3, nz + 2 * npad, NX + 2 * npad)
+        h = ctypes.c_void_p()
+        check(_lib.vti2d_batch_plan_create(ctx.h, ctypes.byref(h), ptr(data), ptr(wavelet), ptr(shots),
+                                           shots.size // 6, ptr(rloc), rloc.size // 2, NX, nz, nt, npad, dx, dz, dt,
+                                           rate, max_wave, levels, tile_rows, SNAP_FORMATS[snap]),
+              "vti2d_batch_plan_create")
+        self.h, self._ctx = h, ctx
+
+    def run(self, c11, c13, c33):
+        c11, c13, c33 = (f32(a).ravel() for a in (c11, c13, c33))
+        gc = np.zeros(self.shape, np.float64)
+        f = ctypes.c_double(0.0)
+        check(_lib.vti2d_batch_plan_run(self.h, ctypes.byref(f), ptr(gc[0]), ptr(gc[1]), ptr(gc[2]), ptr(c11),
+                                        ptr(c13), ptr(c33)), "vti2d_batch_plan_run")
+        return f.value, gc
+
+    def close(self):
+        if self.h:
+            check(_lib.vti2d_batch_plan_destroy(self.h), "vti2d_batch_plan_destroy")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class Ticket:
