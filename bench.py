@@ -98,7 +98,7 @@ def cpu_baseline(ved, npad, dx, dt, budget_s=15.0):
     }
 
 
-def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7, variants=None):
+def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7, variants=None, graph=True):
     """Shot-gradients/s of the batched 2D gradient against the plain
     one-shot-per-call driver on the same shots: a 4096-wide seeded model
     (nz 512, 40-point boundary), each shot on its own aperture of 300-900
@@ -139,7 +139,7 @@ def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7, variants=None):
     # replayed on new models, as the iterations of an inversion do
     shape = {k: tuned[k] for k in ("levels", "tile_rows", "max_wave")}
     plan = graph_plan_rate(vtiamd, ctx, data, w, shots, rloc, (NX, nz, nt, npad, dx, dx, dt, rate), (c11, c13, c33),
-                           shape, direct_s=tb)
+                           shape, direct_s=tb) if graph else None
     # the plain driver, one call per shot on the cropped model
     tp = 0.0
     for k in range(nplain):
@@ -159,8 +159,9 @@ def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7, variants=None):
             "batched_shot_gradients_per_s": round(ns / tb, 3), "batched_s": round(tb, 3),
             "plain_shot_gradients_per_s": round(nplain / tp, 3), "plain_shots_timed": nplain,
             "speedup": round((ns / tb) / (nplain / tp), 2), "launch_shape": tuned,
-            "graph_plan": plan, "graph_plan_speedup_over_plain": round(plan["shot_gradients_per_s"] / (nplain / tp), 2),
-            "small_grid": batch_small_2d(vtiamd, ctx),
+            **({"graph_plan": plan,
+                "graph_plan_speedup_over_plain": round(plan["shot_gradients_per_s"] / (nplain / tp), 2),
+                "small_grid": batch_small_2d(vtiamd, ctx)} if graph else {}),
             **({"variants_shot_gradients_per_s": {k: round(ns / v, 3) for k, v in timed.items()}}
                if variants else {})}
 
@@ -370,6 +371,8 @@ def main():
     ap.add_argument("--small", action="store_true", help="small grid (profiling runs)")
     ap.add_argument("--no-extra", action="store_true", help="only the headline kernel (profiling runs)")
     ap.add_argument("--no-2d", action="store_true", help="skip the 2D and serving workloads (profiling runs)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the graph replays of the batched workload (kernel-trace profiling runs)")
     ap.add_argument("--only", choices=["shot2d", "batch2d", "serve"],
                     help="run one secondary workload alone and print its JSON (profiling runs)")
     args = ap.parse_args()
@@ -388,7 +391,7 @@ def main():
             if args.only == "shot2d":
                 r = shot_gradient_2d(vtiamd, ctx)
             elif args.only == "batch2d":
-                r = batch_gradient_2d(vtiamd, ctx)
+                r = batch_gradient_2d(vtiamd, ctx, graph=not args.no_graph)
             else:
                 r = serve_stream(vtiamd)
         print(json.dumps({"workload": args.only, **r}), flush=True)
