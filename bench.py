@@ -259,12 +259,15 @@ def shot_gradient_2d(vtiamd, ctx, nshots=2):
             "ms_per_shot": round(el * 1e3 / nshots, 2), "shot_gradients_per_s": round(nshots / el, 3)}
 
 
-def serve_trace(vtiamd, nreq=40, rate=5.0, frac3d=0.3, seed=5):
+def serve_trace(vtiamd, nreq=40, rate=5.0, frac3d=0.3, seed=5, surveys=False):
     """A seeded request trace: Poisson arrivals at `rate` req/s, a fraction
     frac3d of 3D modeling requests (each dimension 128-400, 1000-3000
     steps), the rest 2D gradient requests (nz 256-512, nx 256-1024 + a
     40-point boundary, 1000-3000 steps), on seeded layered models (four 2D
-    and three 3D grids) that the server keeps resident."""
+    and three 3D grids) that the server keeps resident.  surveys: the 2D
+    requests are single shots of two surveys instead -- the first two 2D
+    grids, each with its own fixed step count (2001, 1501), a source at a
+    random column -- as an inversion service sees them."""
     rng = np.random.default_rng(seed)
     npad2 = 40
     grids2 = sorted({(int(rng.integers(256, 513)), int(rng.integers(256, 1025))) for _ in range(4)})
@@ -281,6 +284,12 @@ def serve_trace(vtiamd, nreq=40, rate=5.0, frac3d=0.3, seed=5):
     for _ in range(nreq):
         t += float(rng.exponential(1.0 / rate))
         if rng.random() >= frac3d:
+            if surveys:
+                k = int(rng.integers(2))
+                g = grids2[k]
+                reqs.append({"t": t, "kind": "2d", "grid": g, "nt": (2001, 1501)[k],
+                             "ix": npad2 + int(rng.integers(8, g[1] - 8))})
+                continue
             g = grids2[int(rng.integers(len(grids2)))]
             reqs.append({"t": t, "kind": "2d", "grid": g, "nt": 4 * int(rng.integers(250, 751)) + 1})
         else:
@@ -290,9 +299,10 @@ def serve_trace(vtiamd, nreq=40, rate=5.0, frac3d=0.3, seed=5):
             "npad2": npad2, "rate": rate, "frac3d": frac3d}
 
 
-def serve_run(vtiamd, tr, slots, realtime):
-    """Run a trace through a server with `slots` workers; realtime: submit at
-    the arrival times, else all at once.  Returns latency/throughput stats."""
+def serve_run(vtiamd, tr, slots, realtime, batch=8):
+    """Run a trace through a server with `slots` workers merging up to
+    `batch` 2D requests per run (1: none); realtime: submit at the arrival
+    times, else all at once.  Returns latency/throughput stats."""
     npad2, dx2, dt, rate_s = tr["npad2"], 10.0, 1e-3, 4e-3
 
     def wavelet(nt):
@@ -308,7 +318,7 @@ def serve_run(vtiamd, tr, slots, realtime):
         sou = np.array([[nx // 2 * d + 0.1, ny // 2 * d + 0.1, 34 * d + 0.1, len(rec), 0]], np.float32)
         return sou, np.array(rec, np.float32)
 
-    with vtiamd.Server(0, slots=slots) as srv:
+    with vtiamd.Server(0, slots=slots, batch=batch) as srv:
         h2 = {g: srv.put_model(*tr["models2"][g][:3]) for g in tr["grids2"]}
         h3 = {g: srv.put_model(*tr["models3"][g]) for g in tr["models3"]}
         tickets = []
@@ -319,7 +329,7 @@ def serve_run(vtiamd, tr, slots, realtime):
             if r["kind"] == "2d":
                 nz, nx = r["grid"]
                 nnt = (r["nt"] - 1) // 4 + 1
-                sl = np.array([[npad2 + nx // 2, npad2 + 2, nx, 0]], np.int32)
+                sl = np.array([[r.get("ix", npad2 + nx // 2), npad2 + 2, nx, 0]], np.int32)
                 rl = np.array([[npad2 + i, npad2 + 1] for i in range(nx)], np.int32)
                 tickets.append(srv.grad2d(h2[r["grid"]], np.zeros((nnt, nx), np.float32), wavelet(r["nt"]), sl, rl,
                                           tr["models2"][r["grid"]][3], nx, nz, r["nt"], npad2, dx2, dx2, dt, rate_s))
@@ -331,33 +341,44 @@ def serve_run(vtiamd, tr, slots, realtime):
             srv.wait(tk)
         st = srv.stats()
     return {"p50_s": round(st["p50_s"], 4), "p99_s": round(st["p99_s"], 4), "req_per_s": round(st["req_per_s"], 3),
-            "peak_gb": round(st["peak_bytes"] / 2**30, 2)}
+            "peak_gb": round(st["peak_bytes"] / 2**30, 2), "merged_requests": st["merged_requests"]}
 
 
 def serve_stream(vtiamd, nreq=40):
     """The served stream (70% 2D gradients, 30% 3D modeling) at 5 req/s and
-    at 12 req/s through a 4-slot server and through one slot (each request
-    alone, in arrival order: the plain per-request launch), and the
-    saturation throughput (all requests at once) of both, also for a stream
-    of 2D gradient requests only."""
+    at 12 req/s through a 4-slot server and through one slot without merging
+    (each request alone, in arrival order: the plain per-request launch), and
+    the saturation throughput (all requests at once) of both, also for a
+    stream of 2D gradient requests only; then the same for a stream whose 2D
+    requests are single shots of two surveys, which the server merges into
+    batched runs."""
     out = {"requests": nreq, "mix": "70% 2D gradient, 30% 3D modeling"}
     for rate in (5.0, 12.0):
         tr = serve_trace(vtiamd, nreq, rate)
         out.setdefault("grids_2d", [list(g) for g in tr["grids2"]])
         out.setdefault("grids_3d", [list(g) for g in tr["grids3"]])
-        served, plain = serve_run(vtiamd, tr, 4, True), serve_run(vtiamd, tr, 1, True)
+        served, plain = serve_run(vtiamd, tr, 4, True), serve_run(vtiamd, tr, 1, True, batch=1)
         out[f"poisson_{rate:g}rps"] = {"served_4slots": served, "plain_1slot": plain,
                                        "p99_ratio_plain_over_served": round(plain["p99_s"] / max(served["p99_s"],
                                                                                                  1e-9), 2)}
-    sat = {"served_4slots": serve_run(vtiamd, tr, 4, False)["req_per_s"],
-           "plain_1slot": serve_run(vtiamd, tr, 1, False)["req_per_s"]}
-    sat["ratio"] = round(sat["served_4slots"] / max(sat["plain_1slot"], 1e-9), 2)
-    out["saturation_req_s"] = sat
-    tr2 = serve_trace(vtiamd, nreq, 5.0, frac3d=0.0, seed=6)
-    sat2 = {"served_4slots": serve_run(vtiamd, tr2, 4, False)["req_per_s"],
-            "plain_1slot": serve_run(vtiamd, tr2, 1, False)["req_per_s"]}
-    sat2["ratio"] = round(sat2["served_4slots"] / max(sat2["plain_1slot"], 1e-9), 2)
-    out["saturation_2d_only_shot_gradients_per_s"] = sat2
+    def saturation(tr):
+        sat = {"served_4slots": serve_run(vtiamd, tr, 4, False)["req_per_s"],
+               "plain_1slot": serve_run(vtiamd, tr, 1, False, batch=1)["req_per_s"]}
+        sat["ratio"] = round(sat["served_4slots"] / max(sat["plain_1slot"], 1e-9), 2)
+        return sat
+
+    out["saturation_req_s"] = saturation(tr)
+    out["saturation_2d_only_shot_gradients_per_s"] = saturation(serve_trace(vtiamd, nreq, 5.0, frac3d=0.0, seed=6))
+    sv = {"mix": "70% single-shot 2D gradients of two surveys (grids_2d[0] at 2000 steps, grids_2d[1] at 1500), "
+                 "30% 3D modeling"}
+    tr = serve_trace(vtiamd, nreq, 12.0, surveys=True, seed=7)
+    served, plain = serve_run(vtiamd, tr, 4, True), serve_run(vtiamd, tr, 1, True, batch=1)
+    sv["poisson_12rps"] = {"served_4slots": served, "plain_1slot": plain,
+                           "p99_ratio_plain_over_served": round(plain["p99_s"] / max(served["p99_s"], 1e-9), 2)}
+    sv["saturation_req_s"] = saturation(tr)
+    sv["saturation_2d_only_shot_gradients_per_s"] = saturation(
+        serve_trace(vtiamd, nreq, 5.0, frac3d=0.0, seed=8, surveys=True))
+    out["surveys"] = sv
     return out
 
 

@@ -343,6 +343,16 @@ int vti_serve_model3d(vti_server *s, long long *id, int model, float *ddata, con
                       const int *shotid, int nshot, const float *recloc, int nr, const float *wavelet, int nx, int ny,
                       int nz, int nt, int npad, float ox, float oy, float oz, float ot, float dx, float dy, float dz,
                       float dt, float samplingRate);
+/* Kernel-level batching of 2D gradient requests: a worker merges up to
+ * max_requests queued requests that share a schedule (nz, npad, nt, dx, dz,
+ * dt, rate, wavelet) and use the standard taper into one batched run over
+ * their models placed side by side; each still gets the bits
+ * vti2d_objfunc_gradient returns for it alone.  1 turns merging off; the
+ * default is 8.  vti_serve_batch_stats returns how many merged runs there
+ * were and how many requests they held.  No counterpart in the reference. */
+int vti_serve_set_batching(vti_server *s, int max_requests);
+int vti_serve_batch_stats(vti_server *s, long long *merged_runs, long long *merged_requests);
+
 /* blocks until request `id` is done; returns its status (vti_last_error()
  * then holds its message) and its latency and service time in seconds */
 int vti_serve_wait(vti_server *s, long long id, double *latency_s, double *service_s);

@@ -129,3 +129,50 @@ def test_failed_request_leaves_others_intact(vti, ctx):
             assert np.array_equal(g, alone[1])
         with pytest.raises(vti.VtiError, match="no request"):
             srv.wait(tb)
+
+
+def test_merged_requests_equal_alone(vti, ctx):
+    """2D gradient requests of one schedule on models of different widths and
+    shot counts, queued behind a 3D request on a single slot, run as merged
+    batched runs; each result equals the same call made alone, bit for bit."""
+    Ps = [problem2d(vti, nx=nx, nz=48, npad=10, nt=161, ns=ns, seed=sd)
+          for nx, ns, sd in ((64, 2, 21), (96, 1, 22), (80, 3, 23))]
+    obs, alone = [], []
+    for P in Ps:
+        o = (_obs2d(vti, ctx, P) * np.float32(0.9)).astype(np.float32)
+        obs.append(o)
+        alone.append(vti.objfunc_gradient2d(ctx, o, *P["c"], P["w"], P["sloc"], P["rloc"], P["taper"], *args2d(P)))
+    P3 = problem(nz=36, ny=40, nx=48, npad=8, nt=110, ns=1, seed=12)
+    _say("serve merged: 3 models x 2 rounds behind one 3D request")
+    with vti.Server(0, slots=1, batch=8) as srv:
+        h2 = [srv.put_model(*P["c"]) for P in Ps]
+        h3 = srv.put_model(*P3["cij"])
+        shot = np.arange(len(P3["souloc"]))
+        t3 = srv.model3d(h3, P3["souloc"], shot, P3["recloc"], P3["w"], P3["cij"].shape[1:], P3["nt"], P3["npad"],
+                         P3["o"], P3["d"], P3["dt"], P3["rate"])
+        tickets = [srv.grad2d(h, o, P["w"], P["sloc"], P["rloc"], P["taper"], *args2d(P))
+                   for _ in range(2) for P, h, o in zip(Ps, h2, obs)]
+        srv.wait(t3)
+        for k, t in enumerate(tickets):
+            (f, g), lat, svc = srv.wait(t)
+            fa, ga = alone[k % 3]
+            assert lat >= svc >= 0.0
+            assert np.abs(ga).max() > 0
+            assert np.float32(f) == np.float32(fa)
+            assert np.array_equal(g, ga)
+        st = srv.stats()
+        _say("  stats", st)
+        assert st["merged_runs"] >= 1 and st["merged_requests"] >= 2
+
+
+def test_batching_off_runs_requests_alone(vti, ctx):
+    P = problem2d(vti)
+    obs = _obs2d(vti, ctx, P)
+    alone = vti.objfunc_gradient2d(ctx, obs, *P["c"], P["w"], P["sloc"], P["rloc"], P["taper"], *args2d(P))
+    with vti.Server(0, slots=1, batch=1) as srv:
+        h = srv.put_model(*P["c"])
+        ts = [srv.grad2d(h, obs, P["w"], P["sloc"], P["rloc"], P["taper"], *args2d(P)) for _ in range(3)]
+        for t in ts:
+            (f, g), _, _ = srv.wait(t)
+            assert np.array_equal(g, alone[1])
+        assert srv.stats()["merged_runs"] == 0

@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -252,7 +253,16 @@ struct BatchWave {
     DBuf<unsigned short> snaph;
     DBuf<unsigned> amax;
     DBuf<double> part;
+    // residual partial sums per shot (merged requests): the one-shot
+    // driver's block count for each shot's traces, and their offsets in part
+    std::vector<int> pblk, poff;
 };
+
+// the one-shot driver's residual grid for n trace samples
+int shot_residual_blocks(long long n)
+{
+    return (int)std::min<long long>(2048, std::max<long long>(1, (n + 255) / 256));
+}
 
 int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *data, const int *rloc, hipStream_t st)
 {
@@ -340,7 +350,15 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
     }
     if (!rc) rc = W.tr.alloc((size_t)std::max<long long>(tr_total, 1), st);
     if (!rc) rc = W.obs.alloc((size_t)std::max<long long>(tr_total, 1), st);
-    if (!rc) rc = W.part.alloc(kNblkR, st);
+    W.pblk.assign(nw, 0);
+    W.poff.assign(nw, 0);
+    int ptot = 0;
+    for (int j = 0; j < nw; ++j) {
+        W.pblk[j] = shot_residual_blocks((long long)a.nnt_data * a.in[s0 + j].nr);
+        W.poff[j] = ptot;
+        ptot += W.pblk[j];
+    }
+    if (!rc) rc = W.part.alloc(std::max(kNblkR, ptot), st);
     if (rc) return rc;
     VTI_HIP(hipMemcpyAsync(W.sbd.p, W.sb.data(), sizeof(ShotB) * nw, hipMemcpyHostToDevice, st));
     VTI_HIP(hipMemcpyAsync(W.sbtd.p, W.sbt.data(), sizeof(ShotB) * nw, hipMemcpyHostToDevice, st));
@@ -365,7 +383,8 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
 // added in FP64 into S.A, shot by shot.  Every buffer a pass reads before
 // writing is cleared here by a kernel, so the sequence can be captured into a
 // graph and replayed.  Host work is limited to computing launch arguments.
-int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *wavelet, hipStream_t st)
+int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *wavelet, hipStream_t st,
+                 bool per_shot_obj = false)
 {
     const int nw = W.nw, nrec = W.nrec, nt = a.nt, ratio = a.ratio, nnt = a.nnt, nnz = a.nnz, BYT = a.BYT;
     const long long f_total = W.f_total, tr_total = W.tr_total;
@@ -445,8 +464,16 @@ int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *
     pack(nnt - 1);
 
     // ---- residual and its partial sums (objFuncGradient_f.cu:214-219)
-    if (tr_total)
+    if (per_shot_obj) {
+        // each shot's residual on its own, with the one-shot driver's grid, so
+        // that its partial sums are that driver's
+        for (int j = 0; j < nw; ++j)
+            if (W.sb[j].nr)
+                hipLaunchKernelGGL(residual_b_kernel, dim3(W.pblk[j]), dim3(256), 0, st, W.tr.p + W.sb[j].r_off,
+                                   W.obs.p + W.sb[j].r_off, (long long)a.nnt_data * W.sb[j].nr, W.part.p + W.poff[j]);
+    } else if (tr_total) {
         hipLaunchKernelGGL(residual_b_kernel, dim3(kNblkR), dim3(256), 0, st, W.tr.p, W.obs.p, tr_total, W.part.p);
+    }
 
     // ---- backward (objFuncGradient_f.cu:225-300 for every shot)
     for (int k = 0; k < 4; ++k) {
@@ -711,6 +738,127 @@ int vti2d_batch_plan_destroy(vti2d_batch_plan *plan)
     delete plan;  // the buffers go back to the stream's pool
     return vti::kOk;
 }
+
+}  // extern "C"
+
+namespace vti {
+
+// Device bytes of grad2d_merged: the side-by-side model and its FP64
+// gradient, and the largest wave (the max_wave largest shots).
+long long grad2d_merged_bytes(const Grad2dPart *p, int n, int nz, int nt, int npad, float dt, float rate,
+                              int max_wave)
+{
+    const int ratio = std::max(1, (int)(rate / dt + 0.5f));
+    const long long nnt = (nt - 1) / ratio + 1, nnz = nz + 2 * npad;
+    long long total = 0;
+    std::vector<long long> shot;
+    for (int r = 0; r < n; ++r) {
+        const long long nxz = (long long)(p[r].nx + 2 * npad) * nnz;
+        total += 4LL * nxz * 3 + 8LL * nxz * 3;
+        for (int k = 0; k < p[r].ns; ++k)
+            shot.push_back(4LL * nxz * 12 + 8LL * nxz * nnt + 8LL * nnt * p[r].sloc[4 * k + 2] + 64);
+    }
+    std::sort(shot.begin(), shot.end(), std::greater<long long>());
+    for (size_t k = 0; k < shot.size() && (max_wave <= 0 || (int)k < max_wave); ++k) total += shot[k];
+    return total + (1LL << 20);
+}
+
+// Several 2D objective + gradient requests of one schedule (nz, npad, nt,
+// dx, dz, dt, rate, wavelet; the standard taper) in one batched run.  Their
+// models sit side by side in one model of their total width; each request's
+// shots become batch shots whose sub-grid is the request's whole model, so
+// every shot's FP32 gradient is the one-shot driver's.  The shots' FP64 sums
+// stay within their request's columns, in shot order, and each shot's
+// residual runs with the one-shot driver's grid, so every request gets the
+// objective and gradient vti2d_objfunc_gradient returns for it alone, bit for
+// bit (FP32 outputs, as that driver's).
+int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet, int nz, int nt, int npad,
+                  float dx, float dz, float dt, float rate, int max_wave)
+{
+    VTI_CHECK(ctx && p && n >= 1 && wavelet, "grad2d_merged: bad arguments");
+    const int nnz = nz + 2 * npad;
+    std::vector<int> off(n);
+    int NNX = 0;
+    for (int r = 0; r < n; ++r) {
+        off[r] = NNX;
+        NNX += p[r].nx + 2 * npad;
+    }
+    const int ratio = (int)(rate / dt + 0.5f);
+    VTI_CHECK(ratio >= 1, "grad2d_merged: rate/dt below 1");
+    const int nnt = (nt - 1) / ratio + 1;
+    std::vector<int> shots, rl, owner;
+    std::vector<float> data;
+    for (int r = 0; r < n; ++r)
+        for (int k = 0; k < p[r].ns; ++k) {
+            const int ix = p[r].sloc[4 * k], iz = p[r].sloc[4 * k + 1], nri = p[r].sloc[4 * k + 2],
+                      r0 = p[r].sloc[4 * k + 3];
+            VTI_CHECK(nri >= 0 && r0 >= 0 && r0 + nri <= p[r].nr, "request %d shot %d: receiver range outside rloc",
+                      r, k);
+            shots.insert(shots.end(), {off[r], p[r].nx, ix, iz, nri, (int)(rl.size() / 2)});
+            for (int ir = 0; ir < nri; ++ir) {
+                rl.push_back(p[r].rloc[2 * (r0 + ir)]);
+                rl.push_back(p[r].rloc[2 * (r0 + ir) + 1]);
+            }
+            for (int row = 0; row < nnt; ++row)
+                for (int ir = 0; ir < nri; ++ir) data.push_back(p[r].data[(size_t)row * p[r].nr + r0 + ir]);
+            owner.push_back(r);
+        }
+    const int ns = (int)owner.size();
+    std::vector<double> obj(n, 0.0);
+    std::vector<double> A;
+    if (ns) {
+        BatchArgs a;
+        int rc = parse_args(a, shots.data(), ns, rl.data(), (int)(rl.size() / 2), NNX - 2 * npad, nz, nt, npad, dx,
+                            dz, dt, rate, 2, 16, 0);
+        if (rc) return rc;
+        VTI_HIP(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        BatchShared S;
+        rc = setup_shared(S, a, st);
+        if (rc) return rc;
+        for (int r = 0; r < n; ++r) {
+            const int w = p[r].nx + 2 * npad;
+            for (int k = 0; k < 3; ++k)
+                VTI_HIP(hipMemcpy2DAsync(S.C[k].p + off[r], (size_t)NNX * 4, p[r].c[k], (size_t)w * 4, (size_t)w * 4,
+                                         nnz, hipMemcpyDefault, st));
+        }
+        size_t mfree = 0, mtotal = 0;
+        VTI_HIP(hipMemGetInfo(&mfree, &mtotal));
+        const long long budget = (long long)mfree - (long long)(2ULL << 30);
+        for (const auto &wv : split_waves(a, max_wave, budget)) {
+            BatchWave W;
+            rc = setup_wave(W, a, wv.first, wv.second, data.data(), rl.data(), st);
+            if (!rc) rc = enqueue_wave(W, S, a, wavelet, st, true);
+            if (rc) return rc;
+            std::vector<double> hp(W.part.n);
+            VTI_HIP(hipMemcpyAsync(hp.data(), W.part.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st));
+            VTI_HIP(hipStreamSynchronize(st));
+            for (int j = 0; j < W.nw; ++j)
+                if (W.sb[j].nr)
+                    for (int b = 0; b < W.pblk[j]; ++b) obj[owner[wv.first + j]] += hp[W.poff[j] + b];
+        }
+        A.resize((size_t)3 * NNX * nnz);
+        const size_t g = (size_t)NNX * nnz;
+        for (int k = 0; k < 3; ++k)
+            VTI_HIP(hipMemcpyAsync(A.data() + k * g, S.A[k].p, g * 8, hipMemcpyDeviceToHost, st));
+        VTI_HIP(hipStreamSynchronize(st));
+    }
+    for (int r = 0; r < n; ++r) {
+        const int w = p[r].nx + 2 * npad;
+        for (int k = 0; k < 3; ++k)
+            for (int z = 0; z < nnz; ++z)
+                for (int x = 0; x < w; ++x)
+                    p[r].g[k][(size_t)z * w + x] =
+                        ns ? (float)A[(size_t)k * NNX * nnz + (size_t)z * NNX + off[r] + x] : 0.f;
+        // objFuncGradient_f returns 0.5f*objfunc as float (objFuncGradient_f.cu:370)
+        if (p[r].objfunc) *p[r].objfunc = (float)(0.5f * obj[r]);
+    }
+    return vti::kOk;
+}
+
+}  // namespace vti
+
+extern "C" {
 
 // Launch-shape autotuner for vti2d_objfunc_gradient_batch (see vti_amd.h).
 // Times short runs of the same shots (nt cut to 24 snapshot intervals) with a

@@ -22,6 +22,8 @@
 #define VTI_ONETHIRD_F ((float)VTI_ONETHIRD_D)
 #define VTI_TWOTHIRD_F ((float)VTI_TWOTHIRD_D)
 
+struct vti_ctx;
+
 namespace vti {
 
 // Status codes returned by every extern "C" entry point (0 = success).
@@ -42,6 +44,25 @@ const char *get_error();
 // cached for the next call.  Unlike hipMalloc / hipFree neither waits for the
 // work of other streams, so calls on different streams overlap.  dmalloc
 // returns kOk, or kErrNoMem with the error message set.
+// One request of a merged 2D objective + gradient (vti2d_batch.hip): the
+// arguments vti2d_objfunc_gradient takes for it (sloc {ix, iz, nr, r0} per
+// shot; data nnt rows of nr), its model in any memory, its outputs on the host.
+struct Grad2dPart {
+    const float *c[3];
+    const float *data;
+    const int *sloc;
+    int ns;
+    const int *rloc;
+    int nr;
+    int nx;
+    float *objfunc;
+    float *g[3];
+};
+long long grad2d_merged_bytes(const Grad2dPart *p, int n, int nz, int nt, int npad, float dt, float rate,
+                              int max_wave);
+int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet, int nz, int nt, int npad,
+                  float dx, float dz, float dt, float rate, int max_wave);
+
 int dmalloc(void **p, size_t bytes, hipStream_t s);
 void dfree(void *p, hipStream_t s);
 

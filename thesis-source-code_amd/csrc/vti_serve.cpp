@@ -12,6 +12,15 @@
 // a request names its model by handle and the drivers copy it device to
 // device.
 //
+// 2D gradient requests that share a schedule (grid height, padding, steps,
+// spacings, time step, snapshot rate, wavelet) and use the standard taper are
+// batched at kernel level: a worker that takes such a request also takes the
+// queued requests it can merge with (up to `batch` of them, within the memory
+// budget) and runs them as one batched launch sequence over their models
+// placed side by side (vti::grad2d_merged).  Each request still gets, bit for
+// bit, what vti2d_objfunc_gradient returns for it alone.  If the merged run
+// fails, its requests are run one by one.
+//
 // Latency of a request = completion - submission (steady clock, seconds).
 #include <algorithm>
 #include <chrono>
@@ -28,6 +37,7 @@
 
 #include "vti_amd.h"
 #include "vti_ctx.h"
+#include "vti_host.h"
 #include "vti_internal.h"
 
 namespace {
@@ -59,6 +69,7 @@ struct Request {
     const int *shotid = nullptr;
     int nsou = 0, nshot = 0;
     float ox = 0, oy = 0, oz = 0, ot = 0;
+    bool mergeable = false;  // 2D: may join a merged batch
     // outcome
     Clock::time_point t_submit, t_start, t_done;
     bool done = false;
@@ -85,6 +96,8 @@ struct vti_server {
     Clock::time_point first_submit, last_done;
     bool any = false;
     vti_ctx *upload_ctx = nullptr;
+    int batch = 8;                        // most requests merged into one batched run
+    long long groups = 0, grouped = 0;    // merged runs and the requests in them
 };
 
 namespace {
@@ -106,6 +119,74 @@ long long est_model3d(const Request &r)
     const long long nnt = (long long)((r.nt - 1) / std::max(1, (int)(r.rate / r.dt + 0.5f))) + 1;
     // model 3, two levels of two fields, damping; traces
     return 4LL * vox * 7 + 16LL * nnt * std::max(r.nr, 1) + (1LL << 20);
+}
+
+constexpr int kMergeWave = 8;  // shots per wave of a merged run
+
+// Whether a 2D request can join a merged batch: the batched driver's schedule
+// conditions, the standard init_abc taper, source and receivers in the
+// interior, no two receivers of a shot on one point.  A request that fails
+// here runs alone through vti2d_objfunc_gradient, which reports its error.
+bool mergeable_grad2d(const Request &r)
+{
+    const int ratio = (int)(r.rate / r.dt + 0.5f);
+    if (ratio < 2 || (r.nt - 1) % ratio != 0) return false;
+    const int nnx = r.nx + 2 * r.npad, nnz = r.nz + 2 * r.npad;
+    if (nnx < 9 || nnz < 9 || (long long)nnx * nnz >= (1LL << 26)) return false;
+    std::vector<float> t((size_t)nnx * nnz);
+    vti::init_taper2d(t.data(), r.nx, r.nz, r.npad);
+    if (std::memcmp(t.data(), r.taper, t.size() * sizeof(float)) != 0) return false;
+    std::vector<char> seen;
+    for (int k = 0; k < r.ns; ++k) {
+        const int ix = r.sloc[4 * k], iz = r.sloc[4 * k + 1], nri = r.sloc[4 * k + 2], r0 = r.sloc[4 * k + 3];
+        if (nri < 0 || r0 < 0 || r0 + nri > r.nr) return false;
+        if (ix < 4 || ix >= nnx - 4 || iz < 4 || iz >= nnz - 4) return false;
+        seen.assign((size_t)nnx * nnz, 0);
+        for (int ir = 0; ir < nri; ++ir) {
+            const int x = r.rloc[2 * (r0 + ir)], z = r.rloc[2 * (r0 + ir) + 1];
+            if (x < 4 || x >= nnx - 4 || z < 4 || z >= nnz - 4 || seen[x + (size_t)z * nnx]) return false;
+            seen[x + (size_t)z * nnx] = 1;
+        }
+    }
+    return true;
+}
+
+bool same_schedule(const Request &a, const Request &b)
+{
+    auto bits = [](float v) {
+        unsigned u;
+        std::memcpy(&u, &v, 4);
+        return u;
+    };
+    return a.nz == b.nz && a.npad == b.npad && a.nt == b.nt && bits(a.dx) == bits(b.dx) &&
+           bits(a.dz) == bits(b.dz) && bits(a.dt) == bits(b.dt) && bits(a.rate) == bits(b.rate) &&
+           (a.wavelet == b.wavelet || std::memcmp(a.wavelet, b.wavelet, sizeof(float) * a.nt) == 0);
+}
+
+vti::Grad2dPart part_of(const Request &r, const Model &m)
+{
+    vti::Grad2dPart p;
+    for (int k = 0; k < 3; ++k) {
+        p.c[k] = m.c[k];
+        p.g[k] = r.g[k];
+    }
+    p.data = r.data;
+    p.sloc = r.sloc;
+    p.ns = r.ns;
+    p.rloc = r.rloc;
+    p.nr = r.nr;
+    p.nx = r.nx;
+    p.objfunc = r.objfunc;
+    return p;
+}
+
+long long merged_bytes(const std::vector<Request *> &g)
+{
+    std::vector<vti::Grad2dPart> parts;
+    const Model none;
+    for (const Request *q : g) parts.push_back(part_of(*q, none));
+    const Request &r = *g[0];
+    return vti::grad2d_merged_bytes(parts.data(), (int)parts.size(), r.nz, r.nt, r.npad, r.dt, r.rate, kMergeWave);
 }
 
 int run(vti_ctx *ctx, const Model &m, Request &r)
@@ -135,28 +216,75 @@ void worker(vti_server *s)
         if (s->queue.empty()) break;
         Request *r = s->queue.front();
         s->queue.pop_front();
-        s->inflight += r->bytes;
-        s->peak = std::max(s->peak, s->inflight + s->resident);
-        r->t_start = Clock::now();
-        const Model m = s->models.at(r->model);
-        lk.unlock();
-        int rc = crc;
-        std::string err;
-        if (crc == 0) {
-            rc = run(ctx, m, *r);
-        } else {
-            err = vti_last_error();
+        // queued requests that merge with it, oldest first, while the merged
+        // run fits beside those in flight
+        std::vector<Request *> grp{r};
+        long long gbytes = r->bytes;
+        if (s->batch > 1 && r->kind == kGrad2d && r->mergeable) {
+            const long long room = s->budget - s->resident;
+            long long width = r->nx + 2 * r->npad;
+            for (auto it = s->queue.begin(); it != s->queue.end() && (int)grp.size() < s->batch;) {
+                Request *q = *it;
+                const long long w2 = width + q->nx + 2 * q->npad;
+                if (q->kind == kGrad2d && q->mergeable && same_schedule(*r, *q) &&
+                    w2 * (r->nz + 2 * r->npad) < (1LL << 30)) {
+                    grp.push_back(q);
+                    const long long b = merged_bytes(grp);
+                    if (b <= room && (s->inflight == 0 || s->inflight + b <= room)) {
+                        gbytes = b;
+                        width = w2;
+                        it = s->queue.erase(it);
+                        continue;
+                    }
+                    grp.pop_back();
+                }
+                ++it;
+            }
         }
-        if (rc && err.empty()) err = vti_last_error();
+        s->inflight += gbytes;
+        s->peak = std::max(s->peak, s->inflight + s->resident);
+        std::vector<Model> ms;
+        for (Request *q : grp) {
+            q->t_start = Clock::now();
+            ms.push_back(s->models.at(q->model));
+        }
+        lk.unlock();
+        std::vector<int> rcs(grp.size(), crc);
+        std::vector<std::string> errs(grp.size());
+        if (crc == 0) {
+            bool alone = grp.size() == 1;
+            if (!alone) {
+                std::vector<vti::Grad2dPart> parts;
+                for (size_t i = 0; i < grp.size(); ++i) parts.push_back(part_of(*grp[i], ms[i]));
+                alone = vti::grad2d_merged(ctx, parts.data(), (int)parts.size(), r->wavelet, r->nz, r->nt, r->npad,
+                                           r->dx, r->dz, r->dt, r->rate, kMergeWave) != 0;
+                if (!alone) std::fill(rcs.begin(), rcs.end(), 0);
+            }
+            if (alone)
+                for (size_t i = 0; i < grp.size(); ++i) {
+                    rcs[i] = run(ctx, ms[i], *grp[i]);
+                    if (rcs[i]) errs[i] = vti_last_error();
+                }
+        } else {
+            std::fill(errs.begin(), errs.end(), std::string(vti_last_error()));
+        }
         lk.lock();
-        r->status = rc;
-        r->error = err;
-        r->t_done = Clock::now();
-        r->done = true;
-        s->inflight -= r->bytes;
-        s->models[r->model].users -= 1;
-        s->latencies.push_back(std::chrono::duration<double>(r->t_done - r->t_submit).count());
-        s->last_done = std::max(s->last_done, r->t_done);
+        const Clock::time_point t_done = Clock::now();
+        for (size_t i = 0; i < grp.size(); ++i) {
+            Request *q = grp[i];
+            q->status = rcs[i];
+            q->error = errs[i];
+            q->t_done = t_done;
+            q->done = true;
+            s->models[q->model].users -= 1;
+            s->latencies.push_back(std::chrono::duration<double>(q->t_done - q->t_submit).count());
+        }
+        if (grp.size() > 1) {
+            s->groups += 1;
+            s->grouped += (long long)grp.size();
+        }
+        s->inflight -= gbytes;
+        s->last_done = std::max(s->last_done, t_done);
         s->done_cv.notify_all();
         s->work_cv.notify_all();
     }
@@ -303,6 +431,7 @@ int vti_serve_grad2d(vti_server *s, long long *id, int model, float *objfunc, fl
     r->dt = dt;
     r->rate = rate;
     r->bytes = est_grad2d(*r);
+    r->mergeable = mergeable_grad2d(*r);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->models.find(model);
@@ -374,6 +503,23 @@ int vti_serve_wait(vti_server *s, long long id, double *latency_s, double *servi
     if (service_s) *service_s = std::chrono::duration<double>(r->t_done - r->t_start).count();
     if (r->status) vti::set_error("request %lld: %s", id, r->error.c_str());
     return r->status;
+}
+
+int vti_serve_set_batching(vti_server *s, int max_requests)
+{
+    VTI_CHECK(s && max_requests >= 1 && max_requests <= 64, "vti_serve_set_batching: max_requests in [1, 64]");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->batch = max_requests;
+    return vti::kOk;
+}
+
+int vti_serve_batch_stats(vti_server *s, long long *merged_runs, long long *merged_requests)
+{
+    VTI_CHECK(s, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (merged_runs) *merged_runs = s->groups;
+    if (merged_requests) *merged_requests = s->grouped;
+    return vti::kOk;
 }
 
 int vti_serve_stats(vti_server *s, double *p50, double *p99, double *req_per_s, long long *completed,

@@ -121,6 +121,8 @@ _SIGS = {
                           + [c_float] * 9),
     "vti_serve_wait": (c_int, [P, c_longlong, POINTER(ctypes.c_double), POINTER(ctypes.c_double)]),
     "vti_serve_stats": (c_int, [P] + [POINTER(ctypes.c_double)] * 3 + [POINTER(c_longlong), POINTER(ctypes.c_double)]),
+    "vti_serve_set_batching": (c_int, [P, c_int]),
+    "vti_serve_batch_stats": (c_int, [P, POINTER(c_longlong), POINTER(c_longlong)]),
     "vti2d_objfunc_gradient_vepsdel": (
         c_int,
         [P] * 9 + [c_float] * 3 + [P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 6 + [P],
@@ -638,14 +640,17 @@ class Ticket:
 
 class Server:
     """Request server on one device (vti_serve_*): `slots` concurrent
-    workers, admission by device-memory budget, resident models.  Submit
-    with grad2d / model3d, collect with wait."""
+    workers, admission by device-memory budget, resident models, 2D
+    gradient requests of one schedule merged into batched runs (at most
+    `batch` per run; 1 = off).  Submit with grad2d / model3d, collect with
+    wait."""
 
-    def __init__(self, device=0, slots=4, budget_bytes=0.0):
+    def __init__(self, device=0, slots=4, budget_bytes=0.0, batch=8):
         h = ctypes.c_void_p()
         check(_lib.vti_serve_create(ctypes.byref(h), device, slots, float(budget_bytes)), "vti_serve_create")
         self.h = h
         self._models = {}
+        check(_lib.vti_serve_set_batching(self.h, batch), "vti_serve_set_batching")
 
     def close(self):
         if self.h:
@@ -716,5 +721,7 @@ class Server:
         n = c_longlong()
         check(_lib.vti_serve_stats(self.h, ctypes.byref(p50), ctypes.byref(p99), ctypes.byref(rate), ctypes.byref(n),
                                    ctypes.byref(peak)), "vti_serve_stats")
+        runs, merged = c_longlong(), c_longlong()
+        check(_lib.vti_serve_batch_stats(self.h, ctypes.byref(runs), ctypes.byref(merged)), "vti_serve_batch_stats")
         return {"p50_s": p50.value, "p99_s": p99.value, "req_per_s": rate.value, "completed": n.value,
-                "peak_bytes": peak.value}
+                "peak_bytes": peak.value, "merged_runs": runs.value, "merged_requests": merged.value}
