@@ -376,8 +376,12 @@ def serve_stream(vtiamd, nreq=40):
     sv["poisson_12rps"] = {"served_4slots": served, "plain_1slot": plain,
                            "p99_ratio_plain_over_served": round(plain["p99_s"] / max(served["p99_s"], 1e-9), 2)}
     sv["saturation_req_s"] = saturation(tr)
-    sv["saturation_2d_only_shot_gradients_per_s"] = saturation(
-        serve_trace(vtiamd, nreq, 5.0, frac3d=0.0, seed=8, surveys=True))
+    tr = serve_trace(vtiamd, 2 * nreq, 5.0, frac3d=0.0, seed=8, surveys=True)
+    sat = saturation(tr)
+    sat["served_2slots"] = serve_run(vtiamd, tr, 2, False)["req_per_s"]
+    sat["ratio_2slots"] = round(sat["served_2slots"] / max(sat["plain_1slot"], 1e-9), 2)
+    sv["saturation_2d_only_shot_gradients_per_s"] = sat
+    sv["saturation_2d_only_requests"] = 2 * nreq
     out["surveys"] = sv
     return out
 

@@ -87,6 +87,13 @@ __global__ void residual_b_kernel(float *__restrict__ tr, const float *__restric
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// the FP64 gradient rounded to FP32 (what the one-shot driver returns)
+__global__ void narrow_b_kernel(float *__restrict__ out, const double *__restrict__ a, long long n)
+{
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+        out[k] = (float)a[k];
+}
+
 // one shot's FP32 sub-grid gradient into the FP64 global gradient
 __global__ void accum_crop_kernel(double *__restrict__ A11, double *__restrict__ A13, double *__restrict__ A33,
                                   const float *__restrict__ G11, const float *__restrict__ G13,
@@ -805,7 +812,6 @@ int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet
         }
     const int ns = (int)owner.size();
     std::vector<double> obj(n, 0.0);
-    std::vector<double> A;
     if (ns) {
         BatchArgs a;
         int rc = parse_args(a, shots.data(), ns, rl.data(), (int)(rl.size() / 2), NNX - 2 * npad, nz, nt, npad, dx,
@@ -837,19 +843,30 @@ int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet
                 if (W.sb[j].nr)
                     for (int b = 0; b < W.pblk[j]; ++b) obj[owner[wv.first + j]] += hp[W.poff[j] + b];
         }
-        A.resize((size_t)3 * NNX * nnz);
+        // rounded to FP32 on the device
         const size_t g = (size_t)NNX * nnz;
+        DBuf<float> G32;
+        rc = G32.alloc(3 * g, st);
+        if (rc) return rc;
         for (int k = 0; k < 3; ++k)
-            VTI_HIP(hipMemcpyAsync(A.data() + k * g, S.A[k].p, g * 8, hipMemcpyDeviceToHost, st));
+            hipLaunchKernelGGL(narrow_b_kernel, dim3((unsigned)std::min<size_t>((g + 255) / 256, 4096)), dim3(256), 0,
+                               st, G32.p + k * g, S.A[k].p, (long long)g);
+        // one contiguous transfer (a strided copy into pageable memory goes
+        // row by row), then the columns split on the host
+        std::vector<float> h(3 * g);
+        VTI_HIP(hipMemcpyAsync(h.data(), G32.p, 3 * g * 4, hipMemcpyDeviceToHost, st));
         VTI_HIP(hipStreamSynchronize(st));
+        for (int r = 0; r < n; ++r) {
+            const size_t w = (size_t)(p[r].nx + 2 * npad);
+            for (int k = 0; k < 3; ++k)
+                for (int z = 0; z < nnz; ++z)
+                    std::memcpy(p[r].g[k] + (size_t)z * w, h.data() + k * g + (size_t)z * NNX + off[r], w * 4);
+        }
     }
     for (int r = 0; r < n; ++r) {
-        const int w = p[r].nx + 2 * npad;
-        for (int k = 0; k < 3; ++k)
-            for (int z = 0; z < nnz; ++z)
-                for (int x = 0; x < w; ++x)
-                    p[r].g[k][(size_t)z * w + x] =
-                        ns ? (float)A[(size_t)k * NNX * nnz + (size_t)z * NNX + off[r] + x] : 0.f;
+        if (!ns)
+            for (int k = 0; k < 3; ++k)
+                std::fill(p[r].g[k], p[r].g[k] + (size_t)(p[r].nx + 2 * npad) * nnz, 0.f);
         // objFuncGradient_f returns 0.5f*objfunc as float (objFuncGradient_f.cu:370)
         if (p[r].objfunc) *p[r].objfunc = (float)(0.5f * obj[r]);
     }

@@ -121,8 +121,6 @@ long long est_model3d(const Request &r)
     return 4LL * vox * 7 + 16LL * nnt * std::max(r.nr, 1) + (1LL << 20);
 }
 
-constexpr int kMergeWave = 8;  // shots per wave of a merged run
-
 // Whether a 2D request can join a merged batch: the batched driver's schedule
 // conditions, the standard init_abc taper, source and receivers in the
 // interior, no two receivers of a shot on one point.  A request that fails
@@ -180,13 +178,14 @@ vti::Grad2dPart part_of(const Request &r, const Model &m)
     return p;
 }
 
-long long merged_bytes(const std::vector<Request *> &g)
+// shots per wave of a merged run: as many as requests it may merge
+long long merged_bytes(const std::vector<Request *> &g, int wave)
 {
     std::vector<vti::Grad2dPart> parts;
     const Model none;
     for (const Request *q : g) parts.push_back(part_of(*q, none));
     const Request &r = *g[0];
-    return vti::grad2d_merged_bytes(parts.data(), (int)parts.size(), r.nz, r.nt, r.npad, r.dt, r.rate, kMergeWave);
+    return vti::grad2d_merged_bytes(parts.data(), (int)parts.size(), r.nz, r.nt, r.npad, r.dt, r.rate, wave);
 }
 
 int run(vti_ctx *ctx, const Model &m, Request &r)
@@ -229,7 +228,7 @@ void worker(vti_server *s)
                 if (q->kind == kGrad2d && q->mergeable && same_schedule(*r, *q) &&
                     w2 * (r->nz + 2 * r->npad) < (1LL << 30)) {
                     grp.push_back(q);
-                    const long long b = merged_bytes(grp);
+                    const long long b = merged_bytes(grp, s->batch);
                     if (b <= room && (s->inflight == 0 || s->inflight + b <= room)) {
                         gbytes = b;
                         width = w2;
@@ -248,6 +247,7 @@ void worker(vti_server *s)
             q->t_start = Clock::now();
             ms.push_back(s->models.at(q->model));
         }
+        const int wave = s->batch;
         lk.unlock();
         std::vector<int> rcs(grp.size(), crc);
         std::vector<std::string> errs(grp.size());
@@ -257,7 +257,7 @@ void worker(vti_server *s)
                 std::vector<vti::Grad2dPart> parts;
                 for (size_t i = 0; i < grp.size(); ++i) parts.push_back(part_of(*grp[i], ms[i]));
                 alone = vti::grad2d_merged(ctx, parts.data(), (int)parts.size(), r->wavelet, r->nz, r->nt, r->npad,
-                                           r->dx, r->dz, r->dt, r->rate, kMergeWave) != 0;
+                                           r->dx, r->dz, r->dt, r->rate, wave) != 0;
                 if (!alone) std::fill(rcs.begin(), rcs.end(), 0);
             }
             if (alone)
