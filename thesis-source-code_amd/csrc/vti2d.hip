@@ -162,7 +162,8 @@ __global__ __launch_bounds__(64 * kBY2) void rtm_bwd2d_kernel(
     const float *__restrict__ s1, float f, float *__restrict__ image, Geo2 g, float dx2, float dz2, float dt2)
 {
     __shared__ vti::Tiles2 t;
-    const int x0 = 4 + blockIdx.x * 64, z0 = 4 + blockIdx.y * kBY2;
+    int x0, z0;
+    vti::tile_origin(x0, z0);
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
     vti::load_tiles<0>(t, p_c, q_c, nullptr, nullptr, nullptr, g, x0, z0);
     __syncthreads();

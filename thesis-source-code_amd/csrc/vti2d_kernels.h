@@ -37,6 +37,20 @@ __device__ __forceinline__ float st9(float c, float a1, float b1, float a2, floa
     return VTI_C0 * c + VTI_C1 * (a1 + b1) + VTI_C2 * (a2 + b2) + VTI_C3 * (a3 + b3) + VTI_C4 * (a4 + b4);
 }
 
+// Origin of the workgroup's 64 x kBY2 tile in a (tiles across) x (tiles
+// down) grid, in XCD-aware order.  Workgroups are dealt to the 8 XCDs in
+// turn (linear id b to XCD b % 8) and each XCD has its own L2, so XCD x
+// takes the contiguous run [x q + min(x, r), ...) of the nb = 8 q + r tiles
+// in row-major order: tiles that read each other's halos share one L2.
+__device__ __forceinline__ void tile_origin(int &x0, int &z0)
+{
+    const int nb = gridDim.x * gridDim.y, b = blockIdx.x + blockIdx.y * gridDim.x;
+    const int x = b & 7, q = nb >> 3, r = nb & 7;
+    const int t = x * q + min(x, r) + (b >> 3);
+    x0 = 4 + (t % (int)gridDim.x) * 64;
+    z0 = 4 + (t / (int)gridDim.x) * kBY2;
+}
+
 struct Tiles2 {
     float p[kBY2][72];      // x-halo tile
     float q[kBY2 + 8][64];  // z-halo tile
@@ -138,7 +152,8 @@ __global__ __launch_bounds__(64 * kBY2) void fwd2d_kernel(
 {
     __shared__ Tiles2 t;
     __shared__ Tiles2 tb;
-    const int x0 = 4 + blockIdx.x * 64, z0 = 4 + blockIdx.y * kBY2;
+    int x0, z0;
+    tile_origin(x0, z0);
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
     const bool act = ix < g.nnx - 4 && iz < g.nnz - 4;
     const long long i = ix + (long long)iz * g.nnx;
@@ -194,7 +209,8 @@ static __global__ __attribute__((unused)) __launch_bounds__(64 * kBY2) void dsna
                                                            float *__restrict__ sq)
 {
     __shared__ Tiles2 t;
-    const int x0 = 4 + blockIdx.x * 64, z0 = 4 + blockIdx.y * kBY2;
+    int x0, z0;
+    tile_origin(x0, z0);
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
     load_tiles<0>(t, p, q, nullptr, nullptr, nullptr, g, x0, z0);
     __syncthreads();
@@ -221,7 +237,8 @@ __global__ __launch_bounds__(64 * kBY2) void bwd2d_kernel(
     float *__restrict__ g13, float *__restrict__ g33, Geo2 g, float dx2, float dz2, float dt2)
 {
     __shared__ Tiles2 t;
-    const int x0 = 4 + blockIdx.x * 64, z0 = 4 + blockIdx.y * kBY2;
+    int x0, z0;
+    tile_origin(x0, z0);
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
     const bool act = ix < g.nnx - 4 && iz < g.nnz - 4;
     const long long i = ix + (long long)iz * g.nnx;
@@ -298,7 +315,8 @@ static __global__ __attribute__((unused)) __launch_bounds__(64 * kBY2) void hess
     float dt2)
 {
     __shared__ Tiles2 ta, tb, ts;
-    const int x0 = 4 + blockIdx.x * 64, z0 = 4 + blockIdx.y * kBY2;
+    int x0, z0;
+    tile_origin(x0, z0);
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
     load_tiles<1>(ta, p_c, q_c, c11, c13, c33, g, x0, z0);
     load_tiles<1>(tb, dp_c, dq_c, c11, c13, c33, g, x0, z0);
