@@ -557,7 +557,10 @@ int fg2d(vti_ctx *ctx, bool born, float *objfunc, float *gc11, float *gc13, floa
             if (!rc) rc = upload(c, dc[k].p, src[k], n * 4);
         }
     }
-    if (!rc) rc = snap.alloc((size_t)c.nnt * 2 * n, st);  // [k][p | q], slot 0 stays zero
+    // [k][p | q]: slots 1.. are written before they are read; slot 0 (D of
+    // the zero level 0) is never written and is the only one cleared
+    if (!rc) rc = snap.alloc((size_t)c.nnt * 2 * n, st, false);
+    if (!rc) VTI_HIP(hipMemsetAsync(snap.p, 0, 2 * n * sizeof(float), st));
     for (int k = 0; k < 3 && !rc; ++k) {
         rc = acc[k].alloc(n, st);
         if (!rc) rc = acc64[k].alloc(n, st);

@@ -347,13 +347,23 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
         if (!rc) rc = W.Q[k].alloc((size_t)f_total, st);
     }
     for (int k = 0; k < 3 && !rc; ++k) rc = W.G[k].alloc((size_t)f_total, st);
+    // snapshot slots 1.. are written by the forward pass before the adjoint
+    // reads them; slot 0 (D of the zero level 0) is never written and is
+    // cleared below, so the slots themselves need no clearing
     if (a.half) {
-        if (!rc) rc = W.snaph.alloc((size_t)W.snap_total, st);
+        if (!rc) rc = W.snaph.alloc((size_t)W.snap_total, st, false);
         if (!rc) rc = W.stage.alloc((size_t)(2 * f_total), st);
         if (!rc) rc = W.scale.alloc((size_t)(2 * nnt * nw), st);
         if (!rc) rc = W.amax.alloc((size_t)(2 * nnt * nw), st);
     } else {
-        if (!rc) rc = W.snapf.alloc((size_t)W.snap_total, st);
+        if (!rc) rc = W.snapf.alloc((size_t)W.snap_total, st, false);
+    }
+    for (int j = 0; j < nw && !rc; ++j) {
+        const size_t plane2 = (size_t)2 * W.sb[j].nnx * nnz;
+        if (a.half)
+            VTI_HIP(hipMemsetAsync(W.snaph.p + W.sb[j].snap_off, 0, plane2 * sizeof(unsigned short), st));
+        else
+            VTI_HIP(hipMemsetAsync(W.snapf.p + W.sb[j].snap_off, 0, plane2 * sizeof(float), st));
     }
     if (!rc) rc = W.tr.alloc((size_t)std::max<long long>(tr_total, 1), st);
     if (!rc) rc = W.obs.alloc((size_t)std::max<long long>(tr_total, 1), st);

@@ -20,12 +20,14 @@ struct DBuf {
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { dfree(p, s); }
-    int alloc(size_t count, hipStream_t st)
+    // zero: clear the buffer (the default); a caller that writes every
+    // element before reading it, or clears what it reads, passes false
+    int alloc(size_t count, hipStream_t st, bool zero = true)
     {
         n = std::max<size_t>(count, 1);
         s = st;
         int rc = dmalloc((void **)&p, n * sizeof(T), st);
-        if (rc) return rc;
+        if (rc || !zero) return rc;
         const hipError_t e = hipMemsetAsync(p, 0, n * sizeof(T), st);
         if (e != hipSuccess) {
             vti::set_error("hipMemsetAsync: %s", hipGetErrorString(e));
