@@ -233,9 +233,11 @@ def batch_small_2d(vtiamd, ctx, ns=32, seed=5):
             "graph_plan_speedup_over_plain": round(plan["shot_gradients_per_s"] / plain, 2)}
 
 
-def shot_gradient_2d(vtiamd, ctx, nshots=2):
-    """Plain one-shot-per-call 2D gradients on the 512x1024 (+40 boundary)
-    grid, 3000 steps, 1024 receivers, rate 4 ms: shot-gradients/s."""
+def shot_gradient_2d(vtiamd, ctx, nshots=2, ns_call=32):
+    """2D gradients on the 512x1024 (+40 boundary) grid, 3000 steps, 1024
+    receivers, rate 4 ms: one shot per call (the one-shot loop), and one call
+    with 32 shots as the reference's configuration makes it (run as one batch
+    inside objfunc_gradient2d): shot-gradients/s of both."""
     nx, nz, npad, nt, dx, dt, rate = 1024, 512, 40, 3001, 10.0, 1e-3, 4e-3
     nnx, nnz = nx + 2 * npad, nz + 2 * npad
     z = np.arange(nnz, dtype=np.float32)[:, None] * np.ones((1, nnx), np.float32)
@@ -247,16 +249,27 @@ def shot_gradient_2d(vtiamd, ctx, nshots=2):
     t = np.arange(nt) * dt - 0.12
     a = (np.pi * 10.0 * t) ** 2
     w = ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
-    sloc = np.array([[npad + (k + 1) * nx // (nshots + 1), npad + 2, nx, 0] for k in range(nshots)], np.int32)
     rloc = np.array([[npad + i, npad + 1] for i in range(nx)], np.int32)
     nnt = (nt - 1) // int(rate / dt + 0.5) + 1
     data = np.zeros((nnt, nx), np.float32)
-    vtiamd.objfunc_gradient2d(ctx, data, c11, c13, c33, w, sloc[:1], rloc, taper, nx, nz, nt, npad, dx, dx, dt, rate)
+    args = (taper, nx, nz, nt, npad, dx, dx, dt, rate)
+
+    def sources(n):
+        return np.array([[npad + 8 + (k * (nx - 16)) // max(n - 1, 1), npad + 2, nx, 0] for k in range(n)], np.int32)
+
+    sl = sources(nshots)
+    vtiamd.objfunc_gradient2d(ctx, data, c11, c13, c33, w, sl[:1], rloc, *args)
     t0 = time.perf_counter()
-    vtiamd.objfunc_gradient2d(ctx, data, c11, c13, c33, w, sloc, rloc, taper, nx, nz, nt, npad, dx, dx, dt, rate)
+    for k in range(nshots):
+        vtiamd.objfunc_gradient2d(ctx, data, c11, c13, c33, w, sl[k:k + 1], rloc, *args)
     el = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    vtiamd.objfunc_gradient2d(ctx, data, c11, c13, c33, w, sources(ns_call), rloc, *args)
+    el32 = time.perf_counter() - t0
     return {"shots": nshots, "grid_padded": [nnz, nnx], "steps": nt - 1, "receivers": nx,
-            "ms_per_shot": round(el * 1e3 / nshots, 2), "shot_gradients_per_s": round(nshots / el, 3)}
+            "ms_per_shot": round(el * 1e3 / nshots, 2), "shot_gradients_per_s": round(nshots / el, 3),
+            f"call_of_{ns_call}_shots": {"s": round(el32, 3), "shot_gradients_per_s": round(ns_call / el32, 3),
+                                          "speedup_over_one_shot_calls": round((ns_call / el32) / (nshots / el), 2)}}
 
 
 def serve_trace(vtiamd, nreq=40, rate=5.0, frac3d=0.3, seed=5, surveys=False):

@@ -162,3 +162,30 @@ def test_vepsdel_gn_wrapper(vti, ctx):
         gk[:rows] = 0
         assert np.array_equal(h[k], gk)
     assert np.abs(h[0]).max() > 0
+
+
+def test_multishot_call_runs_as_batch_bit_exact(vti, ctx):
+    """A call with several shots and the standard taper runs them as one
+    batch: its gradient equals the FP64 sum of one-shot calls rounded once,
+    and with a taper that differs from the standard one only at a corner
+    point no step reads (so the call takes the one-shot loop) the result is
+    the same bit for bit."""
+    P3 = problem2d(vti, ns=3, seed=4)
+    a = args2d(P3)
+    data = vti.synthetic2d(ctx, *P3["ctrue"], P3["w"], P3["sloc"], P3["rloc"], P3["taper"], *a)
+    data = (data * np.float32(0.9)).astype(np.float32)
+    f, g = vti.objfunc_gradient2d(ctx, data, *P3["c"], P3["w"], P3["sloc"], P3["rloc"], P3["taper"], *a)
+    acc = np.zeros(g.shape, np.float64)
+    fs = 0.0
+    for k in range(3):
+        fk, gk = vti.objfunc_gradient2d(ctx, data, *P3["c"], P3["w"], P3["sloc"][k:k + 1], P3["rloc"], P3["taper"],
+                                        *a)
+        acc += gk.astype(np.float64)
+        fs += fk
+    assert np.abs(g).max() > 0
+    assert np.array_equal(g, acc.astype(np.float32))
+    assert abs(f - fs) <= 1e-6 * abs(fs)
+    t2 = P3["taper"].copy()
+    t2.flat[0] = np.float32(0.5)
+    f2, g2 = vti.objfunc_gradient2d(ctx, data, *P3["c"], P3["w"], P3["sloc"], P3["rloc"], t2, *a)
+    assert np.array_equal(g2, g) and np.float32(f2) == np.float32(f)

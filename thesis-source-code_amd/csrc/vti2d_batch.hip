@@ -760,6 +760,37 @@ int vti2d_batch_plan_destroy(vti2d_batch_plan *plan)
 
 namespace vti {
 
+// Whether vti2d_objfunc_gradient's arguments can run through grad2d_merged:
+// the batched driver's schedule conditions (rate/dt >= 2, (nt-1) a multiple
+// of it, ot = 0), the standard init_abc taper (compared bit for bit),
+// sources and receivers in the interior, one receiver per point per shot.
+bool grad2d_batchable(const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
+                      int npad, float dt, float rate, float ot)
+{
+    if (!sloc || !rloc || !taper || ns < 1 || nx <= 0 || nz <= 0 || npad < 0 || nt < 3 || !(dt > 0) || ot != 0.f)
+        return false;
+    const int ratio = (int)(rate / dt + 0.5f);
+    if (ratio < 2 || (nt - 1) % ratio != 0) return false;
+    const int nnx = nx + 2 * npad, nnz = nz + 2 * npad;
+    if (nnx < 9 || nnz < 9 || (long long)nnx * nnz >= (1LL << 26)) return false;
+    std::vector<float> t((size_t)nnx * nnz);
+    vti::init_taper2d(t.data(), nx, nz, npad);
+    if (std::memcmp(t.data(), taper, t.size() * sizeof(float)) != 0) return false;
+    std::vector<char> seen;
+    for (int k = 0; k < ns; ++k) {
+        const int ix = sloc[4 * k], iz = sloc[4 * k + 1], nri = sloc[4 * k + 2], r0 = sloc[4 * k + 3];
+        if (nri < 0 || r0 < 0 || r0 + nri > nr) return false;
+        if (ix < 4 || ix >= nnx - 4 || iz < 4 || iz >= nnz - 4) return false;
+        seen.assign((size_t)nnx * nnz, 0);
+        for (int ir = 0; ir < nri; ++ir) {
+            const int x = rloc[2 * (r0 + ir)], z = rloc[2 * (r0 + ir) + 1];
+            if (x < 4 || x >= nnx - 4 || z < 4 || z >= nnz - 4 || seen[x + (size_t)z * nnx]) return false;
+            seen[x + (size_t)z * nnx] = 1;
+        }
+    }
+    return true;
+}
+
 // Device bytes of grad2d_merged: the side-by-side model and its FP64
 // gradient, and the largest wave (the max_wave largest shots).
 long long grad2d_merged_bytes(const Grad2dPart *p, int n, int nz, int nt, int npad, float dt, float rate,
@@ -835,8 +866,11 @@ int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet
         for (int r = 0; r < n; ++r) {
             const int w = p[r].nx + 2 * npad;
             for (int k = 0; k < 3; ++k)
-                VTI_HIP(hipMemcpy2DAsync(S.C[k].p + off[r], (size_t)NNX * 4, p[r].c[k], (size_t)w * 4, (size_t)w * 4,
-                                         nnz, hipMemcpyDefault, st));
+                if (n == 1)  // one model: a plain copy (a strided one from pageable memory goes row by row)
+                    VTI_HIP(hipMemcpyAsync(S.C[k].p, p[r].c[k], (size_t)w * nnz * 4, hipMemcpyDefault, st));
+                else
+                    VTI_HIP(hipMemcpy2DAsync(S.C[k].p + off[r], (size_t)NNX * 4, p[r].c[k], (size_t)w * 4,
+                                             (size_t)w * 4, nnz, hipMemcpyDefault, st));
         }
         size_t mfree = 0, mtotal = 0;
         VTI_HIP(hipMemGetInfo(&mfree, &mtotal));

@@ -58,6 +58,8 @@ struct Grad2dPart {
     float *objfunc;
     float *g[3];
 };
+bool grad2d_batchable(const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
+                      int npad, float dt, float rate, float ot);
 long long grad2d_merged_bytes(const Grad2dPart *p, int n, int nz, int nt, int npad, float dt, float rate,
                               int max_wave);
 int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet, int nz, int nt, int npad,

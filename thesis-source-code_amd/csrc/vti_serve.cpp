@@ -121,32 +121,12 @@ long long est_model3d(const Request &r)
     return 4LL * vox * 7 + 16LL * nnt * std::max(r.nr, 1) + (1LL << 20);
 }
 
-// Whether a 2D request can join a merged batch: the batched driver's schedule
-// conditions, the standard init_abc taper, source and receivers in the
-// interior, no two receivers of a shot on one point.  A request that fails
-// here runs alone through vti2d_objfunc_gradient, which reports its error.
+// Whether a 2D request can join a merged batch (vti::grad2d_batchable); a
+// request that fails here runs alone through vti2d_objfunc_gradient, which
+// reports its error.
 bool mergeable_grad2d(const Request &r)
 {
-    const int ratio = (int)(r.rate / r.dt + 0.5f);
-    if (ratio < 2 || (r.nt - 1) % ratio != 0) return false;
-    const int nnx = r.nx + 2 * r.npad, nnz = r.nz + 2 * r.npad;
-    if (nnx < 9 || nnz < 9 || (long long)nnx * nnz >= (1LL << 26)) return false;
-    std::vector<float> t((size_t)nnx * nnz);
-    vti::init_taper2d(t.data(), r.nx, r.nz, r.npad);
-    if (std::memcmp(t.data(), r.taper, t.size() * sizeof(float)) != 0) return false;
-    std::vector<char> seen;
-    for (int k = 0; k < r.ns; ++k) {
-        const int ix = r.sloc[4 * k], iz = r.sloc[4 * k + 1], nri = r.sloc[4 * k + 2], r0 = r.sloc[4 * k + 3];
-        if (nri < 0 || r0 < 0 || r0 + nri > r.nr) return false;
-        if (ix < 4 || ix >= nnx - 4 || iz < 4 || iz >= nnz - 4) return false;
-        seen.assign((size_t)nnx * nnz, 0);
-        for (int ir = 0; ir < nri; ++ir) {
-            const int x = r.rloc[2 * (r0 + ir)], z = r.rloc[2 * (r0 + ir) + 1];
-            if (x < 4 || x >= nnx - 4 || z < 4 || z >= nnz - 4 || seen[x + (size_t)z * nnx]) return false;
-            seen[x + (size_t)z * nnx] = 1;
-        }
-    }
-    return true;
+    return vti::grad2d_batchable(r.sloc, r.ns, r.rloc, r.nr, r.taper, r.nx, r.nz, r.nt, r.npad, r.dt, r.rate, 0.f);
 }
 
 bool same_schedule(const Request &a, const Request &b)
