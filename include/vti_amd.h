@@ -288,10 +288,12 @@ int vti2d_batch_plan_destroy(vti2d_batch_plan *plan);
 
 /* Launch-shape autotuner for vti2d_objfunc_gradient_batch: times short runs
  * (24 snapshot intervals) of the same shots on a synthetic model over levels
- * per launch (1, 2), tile rows (8, 16) and wave size (all shots that fit, at
- * most 8) and returns the fastest in *levels, *tile_rows, *max_wave (and its
- * trial time in *best_ms when not NULL).  Cached per geometry for the life of
- * the process.  No counterpart in the reference. */
+ * per launch (1, 2) and tile rows (8, 16), and returns the fastest in
+ * *levels, *tile_rows (and its trial time in *best_ms when not NULL).
+ * *max_wave is modelled, not timed: the most shots whose adjoint working set
+ * (16 FP32 planes per shot) fits 192 MB of the Infinity Cache, or 0 when all
+ * shots do.  Cached per geometry for the life of the process.  No
+ * counterpart in the reference. */
 int vti2d_batch_tune(vti_ctx *ctx, const int *shots, int ns, const int *rloc, int nr, int NX, int nz, int nt,
                      int npad, float dx, float dz, float dt, float rate, int *levels, int *tile_rows, int *max_wave,
                      double *best_ms);

@@ -161,7 +161,8 @@ def test_autotuned_launch_shape(vti, ctx):
     a = (B["shots"], B["rloc"], B["NX"], B["nz"], B["nt"], B["npad"], B["dx"], B["dz"], B["dt"], B["rate"])
     t = vti.tune_batch2d(ctx, *a)
     _say("batch2d tuned", t)
-    assert t["levels"] in (1, 2) and t["tile_rows"] in (8, 16) and t["max_wave"] in (0, 8)
+    # the wave size is modelled from the Infinity Cache: these small shots all fit
+    assert t["levels"] in (1, 2) and t["tile_rows"] in (8, 16) and t["max_wave"] == 0
     assert t["trial_ms"] > 0
     assert vti.tune_batch2d(ctx, *a) == t  # cached
     data = observed(vti, ctx, B)

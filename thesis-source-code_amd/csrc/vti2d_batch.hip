@@ -614,6 +614,21 @@ std::vector<std::pair<int, int>> split_waves(const BatchArgs &a, int max_wave, l
     return w;
 }
 
+// Shots per wave whose adjoint working set stays in the 256 MB Infinity
+// Cache: per shot and point 16 FP32 planes (four buffers of p and q, the
+// three gradients, the two snapshot slots a level reads, the receiver map),
+// 192 MB in all, at the shots' mean sub-grid size.  At full length a wave
+// that fits runs about 12% faster than one that does not; the short timed
+// trials of the tuner cannot see this, so the wave size is modelled.
+int cache_wave(const int *shots, int ns, int npad, int nz)
+{
+    double mean = 0.0;
+    for (int s = 0; s < ns; ++s) mean += (double)(shots[6 * s + 1] + 2 * npad) * (nz + 2 * npad);
+    mean /= std::max(ns, 1);
+    const int w = (int)(192e6 / (64.0 * std::max(mean, 1.0)));
+    return std::max(1, std::min(w, ns));
+}
+
 }  // namespace
 
 // A captured batch: the device state of every wave (all held at once) and
@@ -875,7 +890,10 @@ int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet
         size_t mfree = 0, mtotal = 0;
         VTI_HIP(hipMemGetInfo(&mfree, &mtotal));
         const long long budget = (long long)mfree - (long long)(2ULL << 30);
-        for (const auto &wv : split_waves(a, max_wave, budget)) {
+        // waves of at most max_wave shots, fewer when the cache model asks
+        const int cw = cache_wave(shots.data(), ns, npad, nz);
+        const int wave = max_wave > 0 ? std::min(max_wave, cw) : cw;
+        for (const auto &wv : split_waves(a, wave, budget)) {
             BatchWave W;
             rc = setup_wave(W, a, wv.first, wv.second, data.data(), rl.data(), st);
             if (!rc) rc = enqueue_wave(W, S, a, wavelet, st, true);
@@ -924,8 +942,8 @@ extern "C" {
 // Launch-shape autotuner for vti2d_objfunc_gradient_batch (see vti_amd.h).
 // Times short runs of the same shots (nt cut to 24 snapshot intervals) with a
 // synthetic model and zero data over: levels per launch 1/2, tile rows 8/16,
-// waves of all shots that fit / at most 8.  The pick is cached per geometry
-// for the life of the process.
+// with waves sized by cache_wave.  The pick is cached per geometry for the
+// life of the process.
 int vti2d_batch_tune(vti_ctx *ctx, const int *shots, int ns, const int *rloc, int nr, int NX, int nz, int nt,
                      int npad, float dx, float dz, float dt, float rate, int *levels, int *tile_rows, int *max_wave,
                      double *best_ms)
@@ -968,14 +986,13 @@ int vti2d_batch_tune(vti_ctx *ctx, const int *shots, int ns, const int *rloc, in
     std::vector<float> data((size_t)std::max<long long>(ndata, 1), 0.f);
     std::vector<double> gc(3 * g);
     double obj = 0;
-    int bl = 2, br = 16, bw = 0;
+    const int wave = cache_wave(shots, ns, npad, nz);
+    int bl = 2, br = 16, bw = wave >= ns ? 0 : wave;
     double best = 1e300;
-    const int waves[2] = {0, 8};
     for (int lv = 1; lv <= 2; ++lv)
         for (int rows : {8, 16})
-            for (int wv : waves) {
+            for (int wv : {bw}) {
                 if (lv == 1 && rows != 8) continue;  // the tile height only shapes two-level launches
-                if (wv && wv >= ns) continue;
                 VTI_HIP(hipStreamSynchronize(ctx->stream));
                 const auto t0 = std::chrono::steady_clock::now();
                 const int rc = vti2d_objfunc_gradient_batch(ctx, &obj, gc.data(), gc.data() + g, gc.data() + 2 * g,
