@@ -665,8 +665,7 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
     if (rc) return rc;
 
     // waves within the memory budget
-    size_t mfree = 0, mtotal = 0;
-    VTI_HIP(hipMemGetInfo(&mfree, &mtotal));
+    const size_t mfree = vti::device_free_bytes();
     const long long budget = (long long)mfree - (long long)(2ULL << 30);
     VTI_CHECK(budget > a.shot_bytes(0), "not enough device memory for one shot");
     double obj = 0.0;
@@ -703,8 +702,7 @@ int vti2d_batch_plan_create(vti_ctx *ctx, vti2d_batch_plan **plan, const float *
         }
     }
     if (!rc) {
-        size_t mfree = 0, mtotal = 0;
-        (void)hipMemGetInfo(&mfree, &mtotal);
+        const size_t mfree = vti::device_free_bytes();
         long long need = 0;
         for (int s = 0; s < ns; ++s) need += pl->a.shot_bytes(s);
         if (need > (long long)mfree - (long long)(2ULL << 30)) {
@@ -887,8 +885,7 @@ int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet
                     VTI_HIP(hipMemcpy2DAsync(S.C[k].p + off[r], (size_t)NNX * 4, p[r].c[k], (size_t)w * 4,
                                              (size_t)w * 4, nnz, hipMemcpyDefault, st));
         }
-        size_t mfree = 0, mtotal = 0;
-        VTI_HIP(hipMemGetInfo(&mfree, &mtotal));
+        const size_t mfree = vti::device_free_bytes();
         const long long budget = (long long)mfree - (long long)(2ULL << 30);
         // waves of at most max_wave shots, fewer when the cache model asks
         const int cw = cache_wave(shots.data(), ns, npad, nz);

@@ -66,6 +66,9 @@ int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet
                   float dx, float dz, float dt, float rate, int max_wave);
 
 int dmalloc(void **p, size_t bytes, hipStream_t s);
+// free device memory of the current device, counting the large blocks
+// dmalloc keeps for reuse
+size_t device_free_bytes();
 void dfree(void *p, hipStream_t s);
 
 }  // namespace vti
