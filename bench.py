@@ -272,6 +272,34 @@ def shot_gradient_2d(vtiamd, ctx, nshots=2, ns_call=32):
                                           "speedup_over_one_shot_calls": round((ns_call / el32) / (nshots / el), 2)}}
 
 
+def synthetic_cfg1(vtiamd, ctx, ns=16):
+    """synthetic_f on the 256x256 (+40 boundary) grid, 2000 steps, 256
+    receivers, rate 4 ms: 16 shots in one call (run as one batch) against
+    one shot per call: shots/s of both."""
+    nx, nz, npad, nt, dx, dt, rate = 256, 256, 40, 2001, 10.0, 1e-3, 4e-3
+    nnx, nnz = nx + 2 * npad, nz + 2 * npad
+    v = 1500.0 + 2500.0 * np.arange(nnz, dtype=np.float32)[:, None] / nnz * np.ones((1, nnx), np.float32)
+    c33 = (v * v).astype(np.float32)
+    c11, c13 = (c33 * 1.2).astype(np.float32), (c33 * 1.05).astype(np.float32)
+    taper = vtiamd.init_taper2d(nx, nz, npad)
+    t = np.arange(nt) * dt - 0.1
+    a = (np.pi * 15.0 * t) ** 2
+    w = ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
+    rloc = np.array([[npad + i, npad + 1] for i in range(nx)], np.int32)
+    sl = np.array([[npad + 8 + (k * (nx - 16)) // (ns - 1), npad + 2, nx, 0] for k in range(ns)], np.int32)
+    args = (taper, nx, nz, nt, npad, dx, dx, dt, rate)
+    vtiamd.synthetic2d(ctx, c11, c13, c33, w, sl[:2], rloc, *args)
+    t0 = time.perf_counter()
+    for k in range(4):
+        vtiamd.synthetic2d(ctx, c11, c13, c33, w, sl[k:k + 1], rloc, *args)
+    one = 4 / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    vtiamd.synthetic2d(ctx, c11, c13, c33, w, sl, rloc, *args)
+    many = ns / (time.perf_counter() - t0)
+    return {"grid_padded": [nnz, nnx], "steps": nt - 1, "receivers": nx, "one_shot_per_call_shots_per_s": round(one, 2),
+            f"call_of_{ns}_shots_shots_per_s": round(many, 2), "speedup": round(many / one, 2)}
+
+
 def serve_trace(vtiamd, nreq=40, rate=5.0, frac3d=0.3, seed=5, surveys=False):
     """A seeded request trace: Poisson arrivals at `rate` req/s, a fraction
     frac3d of 3D modeling requests (each dimension 128-400, 1000-3000
@@ -428,6 +456,7 @@ def main():
         with vtiamd.Context(0) as ctx:
             if args.only == "shot2d":
                 r = shot_gradient_2d(vtiamd, ctx)
+                r["synthetic_256x256_2000steps"] = synthetic_cfg1(vtiamd, ctx)
             elif args.only == "batch2d":
                 r = batch_gradient_2d(vtiamd, ctx, graph=not args.no_graph)
             else:
@@ -514,6 +543,7 @@ def main():
     prop.close()
     if not args.small and not args.no_2d:
         extra["2d_shot_gradient_512x1024_3000steps"] = shot_gradient_2d(vtiamd, ctx)
+        extra["2d_synthetic_256x256_2000steps"] = synthetic_cfg1(vtiamd, ctx)
         extra["2d_batched_vs_plain_gradient"] = batch_gradient_2d(vtiamd, ctx)
         extra["served_stream"] = serve_stream(vtiamd)
     out["extra"] = extra

@@ -189,3 +189,29 @@ def test_multishot_call_runs_as_batch_bit_exact(vti, ctx):
     t2.flat[0] = np.float32(0.5)
     f2, g2 = vti.objfunc_gradient2d(ctx, data, *P3["c"], P3["w"], P3["sloc"], P3["rloc"], t2, *a)
     assert np.array_equal(g2, g) and np.float32(f2) == np.float32(f)
+
+
+def test_multishot_synthetic_runs_as_batch_bit_exact(vti, ctx):
+    """synthetic_f with several shots and the standard taper runs them as one
+    batch: the traces equal one-shot calls' bit for bit, the row the
+    reference never records keeps the caller's values, and a taper that
+    differs only at a corner point nothing reads (the one-shot loop) gives
+    the same bits."""
+    P3 = problem2d(vti, ns=3, seed=6)
+    a = args2d(P3)
+    nnt = (P3["nt"] - 1) // 4 + 1
+    fill = np.full((nnt, P3["nr"]), -7.0, np.float32)
+    d = vti.synthetic2d(ctx, *P3["ctrue"], P3["w"], P3["sloc"], P3["rloc"], P3["taper"], *a, data=fill.copy())
+    ref = fill.copy()
+    for k in range(3):
+        dk = vti.synthetic2d(ctx, *P3["ctrue"], P3["w"], P3["sloc"][k:k + 1], P3["rloc"], P3["taper"], *a,
+                             data=fill.copy())
+        c0, n = P3["sloc"][k, 3], P3["sloc"][k, 2]
+        ref[:, c0:c0 + n] = dk[:, c0:c0 + n]
+    assert np.abs(d[1:]).max() > 0
+    assert np.array_equal(d, ref)
+    assert np.all(d[0] == -7.0)
+    t2 = P3["taper"].copy()
+    t2.flat[0] = np.float32(0.5)
+    d2 = vti.synthetic2d(ctx, *P3["ctrue"], P3["w"], P3["sloc"], P3["rloc"], t2, *a, data=fill.copy())
+    assert np.array_equal(d2, d)

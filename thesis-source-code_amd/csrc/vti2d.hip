@@ -317,6 +317,13 @@ int vti2d_synthetic(vti_ctx *ctx, float *data, const float *c11, const float *c1
                     const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
                     int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
 {
+    // Several shots with the standard taper (and rate/dt rounding the same in
+    // float as in synthetic_f's double) run as one batch, bit for bit.
+    if (ctx && ns >= 2 && data && c11 && c13 && c33 && wavelet && dx > 0 && dz > 0 && dt > 0 &&
+        (int)(rate / dt + 0.5f) == (int)(rate / dt + 0.5) &&
+        vti::grad2d_batchable(sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dt, rate, ot))
+        return vti::synth2d_batched(ctx, data, c11, c13, c33, wavelet, sloc, ns, rloc, nr, nx, nz, nt, npad, dx, dz,
+                                    dt, rate);
     Call2 c;
     int rc = setup(c, ctx, nx, nz, nt, npad, dx, dz, dt, rate, ot, false);
     if (rc) return rc;

@@ -46,6 +46,21 @@ __global__ void set_src_b_kernel(float *__restrict__ P, float *__restrict__ Q, c
     }
 }
 
+// synthetic_f's level 1: dt2*w[0] at each shot's source times the taper
+// there (synthetic_f.cu:86-89), the taper formed as the step kernels form it
+__global__ void set_src_taper_b_kernel(float *__restrict__ P, float *__restrict__ Q, const ShotB *__restrict__ shots,
+                                       int ns, float v, const float *__restrict__ t0, int npad, int nnz)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < ns) {
+        const ShotB sh = shots[s];
+        const float tap = vti::taper_of(vti::taper_ld(t0, npad, sh.nnx, nnz, sh.src % sh.nnx, sh.src / sh.nnx));
+        const long long i = sh.f_off + sh.src;
+        P[i] = v * tap;
+        Q[i] = v * tap;
+    }
+}
+
 // recordData of every receiver of the batch into row `row` of its shot's traces
 __global__ void record_b_kernel(float *__restrict__ tr, const float *__restrict__ P, const float *__restrict__ Q,
                                 const long long *__restrict__ fidx, const long long *__restrict__ tbase,
@@ -576,6 +591,66 @@ int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *
     return vti::kOk;
 }
 
+// Enqueue one wave of synthetic_f: the forward pass of enqueue_wave with
+// synthetic_f's level 1 and no snapshots, recording every rate/dt levels.
+int enqueue_model_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *wavelet, hipStream_t st)
+{
+    const int nw = W.nw, nrec = W.nrec, nt = a.nt, ratio = a.ratio, BYT = a.BYT;
+    const long long f_total = W.f_total;
+    const float dx2 = a.dx2, dz2 = a.dz2, dt2 = a.dt2;
+    DBuf<float> *P = W.P, *Q = W.Q, *C = S.C;
+    for (int k = 0; k < 4; ++k) {
+        zero_words(P[k].p, (size_t)f_total, st);
+        zero_words(Q[k].p, (size_t)f_total, st);
+    }
+    zero_words(W.tr.p, W.tr.n, st);
+    BatchGeo bg;
+    bg.nnz = a.nnz;
+    bg.npad = a.npad;
+    bg.cpitch = a.NNX;
+    bg.nnxz_total = f_total;
+    const dim3 grid((unsigned)W.tile_shot.size()), block(64, kBY2);
+    const dim3 gridt((unsigned)W.tile_shot_t.size()), blockt(64, BYT);
+    const int rb = (nrec + 255) / 256;
+    const vti::SnapW sw{nullptr, nullptr};  // no snapshots (every slot argument is -1)
+    auto record = [&](int it, int buf) {
+        if (it % ratio == 0 && nrec)
+            hipLaunchKernelGGL(record_b_kernel, dim3(rb), dim3(256), 0, st, W.tr.p, P[buf].p, Q[buf].p, W.fidxd.p,
+                               W.tbd.p, W.tsdd.p, nrec, it / ratio);
+    };
+    ShotB *sbd = W.sbd.p, *sbtd = W.sbtd.p;
+    int *tsh = W.tsh.p, *tsht = W.tsht.p;
+    float *t0d = S.t0d.p;
+    int ip = 1, ic = 0, s0b = 2, s1b = 3;
+    hipLaunchKernelGGL(set_src_taper_b_kernel, dim3((nw + 63) / 64), dim3(64), 0, st, P[ic].p, Q[ic].p, sbd, nw,
+                       dt2 * wavelet[0], t0d, a.npad, a.nnz);
+    for (int it = 2; it < nt;) {
+        if (a.levels == 2 && it + 1 < nt) {
+#define FWD_T2(BYV)                                                                                                 \
+    hipLaunchKernelGGL(vti::fwd2db_t2_kernel<BYV>, gridt, blockt, 0, st, P[s0b].p, Q[s0b].p, P[s1b].p, Q[s1b].p,     \
+                       P[ip].p, Q[ip].p, P[ic].p, Q[ic].p, C[0].p, C[1].p, C[2].p, t0d, sbtd, tsht, bg, dx2, dz2, \
+                       dt2, dt2 * wavelet[it - 1], dt2 * wavelet[it], sw, -1, -1)
+            if (BYT == 4) FWD_T2(4);
+            else if (BYT == 8) FWD_T2(8);
+            else FWD_T2(16);
+#undef FWD_T2
+            record(it, s0b);
+            record(it + 1, s1b);
+            std::swap(ip, s0b);
+            std::swap(ic, s1b);
+            it += 2;
+        } else {
+            hipLaunchKernelGGL(vti::fwd2db_kernel, grid, block, 0, st, P[ip].p, Q[ip].p, P[ic].p, Q[ic].p, C[0].p,
+                               C[1].p, C[2].p, t0d, sbd, tsh, bg, dx2, dz2, dt2, dt2 * wavelet[it - 1], sw, -1);
+            std::swap(ip, ic);
+            record(it, ic);
+            it += 1;
+        }
+    }
+    VTI_HIP(hipGetLastError());
+    return vti::kOk;
+}
+
 // the objective's share of a wave: the residual's partial sums, in order
 int wave_objective(const BatchWave &W, hipStream_t st, double *obj)
 {
@@ -775,12 +850,13 @@ namespace vti {
 
 // Whether vti2d_objfunc_gradient's arguments can run through grad2d_merged:
 // the batched driver's schedule conditions (rate/dt >= 2, (nt-1) a multiple
-// of it, ot = 0), the standard init_abc taper (compared bit for bit),
+// of it, ot = 0), npad >= 4 as the one-shot drivers require, the standard
+// init_abc taper (compared bit for bit),
 // sources and receivers in the interior, one receiver per point per shot.
 bool grad2d_batchable(const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
                       int npad, float dt, float rate, float ot)
 {
-    if (!sloc || !rloc || !taper || ns < 1 || nx <= 0 || nz <= 0 || npad < 0 || nt < 3 || !(dt > 0) || ot != 0.f)
+    if (!sloc || !rloc || !taper || ns < 1 || nx <= 0 || nz <= 0 || npad < 4 || nt < 3 || !(dt > 0) || ot != 0.f)
         return false;
     const int ratio = (int)(rate / dt + 0.5f);
     if (ratio < 2 || (nt - 1) % ratio != 0) return false;
@@ -802,6 +878,63 @@ bool grad2d_batchable(const int *sloc, int ns, const int *rloc, int nr, const fl
         }
     }
     return true;
+}
+
+// synthetic_f (vti2d_synthetic) for several shots as one batch: the forward
+// pass of the batched driver with synthetic_f's level 1 (the source times
+// the taper) and no snapshots.  Each shot's traces are the one-shot loop's,
+// bit for bit; rows the loop never records (row 0) keep their input values.
+// The caller has checked grad2d_batchable and synthetic_f's own rounding of
+// rate/dt.
+int synth2d_batched(vti_ctx *ctx, float *out, const float *c11, const float *c13, const float *c33,
+                    const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, int nx, int nz, int nt,
+                    int npad, float dx, float dz, float dt, float rate)
+{
+    VTI_CHECK(ctx && out && c11 && c13 && c33 && wavelet && sloc && rloc && ns >= 1, "synth2d_batched: bad arguments");
+    const int nnx = nx + 2 * npad, nnz = nz + 2 * npad;
+    const int ratio = (int)(rate / dt + 0.5f);
+    const int nnt = (nt - 1) / ratio + 1;
+    std::vector<int> shots, rl;
+    for (int k = 0; k < ns; ++k) {
+        const int ix = sloc[4 * k], iz = sloc[4 * k + 1], nri = sloc[4 * k + 2], r0 = sloc[4 * k + 3];
+        VTI_CHECK(nri >= 0 && r0 >= 0 && r0 + nri <= nr, "shot %d: receiver range outside rloc", k);
+        shots.insert(shots.end(), {0, nx, ix, iz, nri, (int)(rl.size() / 2)});
+        for (int ir = 0; ir < nri; ++ir) {
+            rl.push_back(rloc[2 * (r0 + ir)]);
+            rl.push_back(rloc[2 * (r0 + ir) + 1]);
+        }
+    }
+    BatchArgs a;
+    int rc = parse_args(a, shots.data(), ns, rl.data(), (int)(rl.size() / 2), nx, nz, nt, npad, dx, dz, dt, rate, 2,
+                        16, 0);
+    if (rc) return rc;
+    VTI_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    BatchShared S;
+    rc = setup_shared(S, a, st);
+    if (rc) return rc;
+    const float *cs[3] = {c11, c13, c33};
+    for (int k = 0; k < 3; ++k)
+        VTI_HIP(hipMemcpyAsync(S.C[k].p, cs[k], (size_t)nnx * nnz * 4, hipMemcpyDefault, st));
+    const std::vector<float> nodata((size_t)std::max<long long>((long long)nnt * (rl.size() / 2), 1), 0.f);
+    const int cw = cache_wave(shots.data(), ns, npad, nz);
+    const long long budget = (long long)vti::device_free_bytes() - (long long)(2ULL << 30);
+    for (const auto &wv : split_waves(a, cw, budget)) {
+        BatchWave W;
+        rc = setup_wave(W, a, wv.first, wv.second, nodata.data(), rl.data(), st);
+        if (!rc) rc = enqueue_model_wave(W, S, a, wavelet, st);
+        if (rc) return rc;
+        std::vector<float> h((size_t)std::max<long long>(W.tr_total, 1));
+        if (W.tr_total) VTI_HIP(hipMemcpyAsync(h.data(), W.tr.p, W.tr_total * 4, hipMemcpyDeviceToHost, st));
+        VTI_HIP(hipStreamSynchronize(st));
+        for (int j = 0; j < W.nw; ++j) {
+            const int k = wv.first + j, nri = sloc[4 * k + 2], r0 = sloc[4 * k + 3];
+            for (int row = 1; row < nnt; ++row)  // row 0 is never recorded
+                for (int ir = 0; ir < nri; ++ir)
+                    out[(size_t)row * nr + r0 + ir] = h[W.sb[j].r_off + (size_t)row * nri + ir];
+        }
+    }
+    return vti::kOk;
 }
 
 // Device bytes of grad2d_merged: the side-by-side model and its FP64

@@ -60,6 +60,9 @@ struct Grad2dPart {
 };
 bool grad2d_batchable(const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz, int nt,
                       int npad, float dt, float rate, float ot);
+int synth2d_batched(vti_ctx *ctx, float *out, const float *c11, const float *c13, const float *c33,
+                    const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, int nx, int nz, int nt,
+                    int npad, float dx, float dz, float dt, float rate);
 long long grad2d_merged_bytes(const Grad2dPart *p, int n, int nz, int nt, int npad, float dt, float rate,
                               int max_wave);
 int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet, int nz, int nt, int npad,

@@ -381,15 +381,21 @@ def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32).ravel()
 
 
-def synthetic2d(ctx, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0):
-    """synthetic_f: returns data (nnt_data, nr) time-major, zeros where never recorded."""
+def synthetic2d(ctx, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0,
+                data=None):
+    """synthetic_f: returns data (nnt_data, nr) time-major.  Rows the
+    reference never records keep the values of `data` (written in place when
+    given as a C-contiguous float32 array of that shape), else zeros."""
     c11, c13, c33, wavelet, taper = (f32(a).ravel() for a in (c11, c13, c33, wavelet, taper))
     sloc, rloc = _i32(sloc), _i32(rloc)
     ns, nr = sloc.size // 4, rloc.size // 2
     ratio = int(rate / dt + 0.5)
     ntneg = int(round(abs(ot) / dt))
     nnt_data = (nt - ntneg - 1) // ratio + 1
-    data = np.zeros((nnt_data, nr), np.float32)
+    if data is None:
+        data = np.zeros((nnt_data, nr), np.float32)
+    elif data.dtype != np.float32 or data.shape != (nnt_data, nr) or not data.flags["C_CONTIGUOUS"]:
+        raise ValueError(f"data must be a C-contiguous float32 array of shape {(nnt_data, nr)}")
     check(_lib.vti2d_synthetic(ctx.h, ptr(data), ptr(c11), ptr(c13), ptr(c33), ptr(wavelet), ptr(sloc), ns, ptr(rloc),
                                nr, ptr(taper), nx, nz, nt, npad, dx, dz, dt, rate, ot), "vti2d_synthetic")
     return data
