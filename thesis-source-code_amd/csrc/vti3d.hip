@@ -610,6 +610,15 @@ int vti3d_prop_create(vti_ctx *ctx, vti3d_prop **out, int nx, int ny, int nz, in
         VTI_CHECK(false, "vti3d_prop_create: x-y plane %dx%d too large for 32-bit chunk offsets", nx, ny);
     }
     p->zc = std::min(p->zc, max_zchunk(g));
+    if (!getenv("VTI_ZCHUNK")) {
+        // a small grid gets a shorter z-chunk, so that a launch still has
+        // about 2048 workgroups (8 per CU) to spread over the 256 CUs; each
+        // chunk re-reads a 4-plane halo of sigma_z on either side, which the
+        // parallelism pays for below 64 planes
+        const long long cols = (long long)((g.nx - 4 + 63) / 64) * ((g.ny - 8 + kBY - 1) / kBY);
+        const long long zc = g.zmax * cols / 2048 / 4 * 4;
+        p->zc = (int)std::max<long long>(8, std::min<long long>(p->zc, zc));
+    }
     const size_t nv = (size_t)vox(g);
     int rc = vti::kOk;
     for (int k = 0; k < 3 && rc == vti::kOk; ++k) rc = dalloc(p, &p->cm[0][k], nv);

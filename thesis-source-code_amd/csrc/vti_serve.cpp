@@ -97,6 +97,7 @@ struct vti_server {
     bool any = false;
     vti_ctx *upload_ctx = nullptr;
     int batch = 8;                        // most requests merged into one batched run
+    int idle = 0;                         // workers waiting for work
     long long groups = 0, grouped = 0;    // merged runs and the requests in them
 };
 
@@ -187,11 +188,13 @@ void worker(vti_server *s)
         std::unique_lock<std::mutex> lk(s->mu);
         // the oldest request, once it fits beside those in flight (or runs
         // alone); on shutdown the queue is drained first
+        s->idle += 1;
         s->work_cv.wait(lk, [&] {
             if (s->queue.empty()) return s->stop;
             const long long b = s->queue.front()->bytes;
             return s->inflight == 0 || s->inflight + b <= s->budget - s->resident;
         });
+        s->idle -= 1;
         if (s->queue.empty()) break;
         Request *r = s->queue.front();
         s->queue.pop_front();
@@ -202,7 +205,13 @@ void worker(vti_server *s)
         if (s->batch > 1 && r->kind == kGrad2d && r->mergeable) {
             const long long room = s->budget - s->resident;
             long long width = r->nx + 2 * r->npad;
-            for (auto it = s->queue.begin(); it != s->queue.end() && (int)grp.size() < s->batch;) {
+            // this worker's share of the mergeable requests, so that idle
+            // workers take the rest as runs of their own
+            int compatible = 1;
+            for (const Request *q : s->queue)
+                if (q->kind == kGrad2d && q->mergeable && same_schedule(*r, *q)) ++compatible;
+            const int share = std::min(s->batch, (compatible + s->idle) / (s->idle + 1));
+            for (auto it = s->queue.begin(); it != s->queue.end() && (int)grp.size() < share;) {
                 Request *q = *it;
                 const long long w2 = width + q->nx + 2 * q->npad;
                 if (q->kind == kGrad2d && q->mergeable && same_schedule(*r, *q) &&
