@@ -315,8 +315,7 @@ int vti_serve_create(vti_server **out, int device, int slots, double budget_byte
     vti_ctx *up = nullptr;
     int rc = vti_ctx_create(&up, device);
     if (rc) return rc;
-    size_t mfree = 0, mtotal = 0;
-    VTI_HIP(hipMemGetInfo(&mfree, &mtotal));
+    const size_t mfree = vti::device_free_bytes();  // vti_ctx_create made `device` current
     auto *s = new vti_server();
     s->device = device;
     s->budget = budget_bytes > 0 ? (long long)budget_bytes : (long long)mfree - (2LL << 30);
