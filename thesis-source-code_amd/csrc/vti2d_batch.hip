@@ -837,10 +837,11 @@ int vti2d_batch_plan_run(vti2d_batch_plan *plan, double *objfunc, double *gc11, 
 int vti2d_batch_plan_destroy(vti2d_batch_plan *plan)
 {
     if (!plan) return vti::kOk;
+    (void)hipSetDevice(plan->ctx->device);
+    (void)hipStreamSynchronize(plan->ctx->stream);  // no replay still running
     if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
     if (plan->graph) (void)hipGraphDestroy(plan->graph);
-    (void)hipSetDevice(plan->ctx->device);
-    delete plan;  // the buffers go back to the stream's pool
+    delete plan;  // the buffers go back to the allocator on the context's stream
     return vti::kOk;
 }
 
