@@ -170,7 +170,7 @@ __global__ __launch_bounds__(64 * kBY2) void rtm_bwd2d_kernel(
     if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
     const long long i = ix + (long long)iz * g.nnx;
     const float tap = taper[i];
-    const float tp = vti::dxx(t, dx2), tq = vti::dzz(t, dz2);
+    const float tp = vti::dxx(t, dx2, g.rx2), tq = vti::dzz(t, dz2, g.rz2);
     float pn = 2.f * p_c[i] + dt2 * (c11[i] * tp + c13[i] * tq) - p_n[i] * tap;
     float qn = 2.f * q_c[i] + dt2 * (c13[i] * tp + c33[i] * tq) - q_n[i] * tap;
     if (inject) {
@@ -231,6 +231,8 @@ int setup(Call2 &c, vti_ctx *ctx, int nx, int nz, int nt, int npad, float dx, fl
     c.dx2 = dx * dx;
     c.dz2 = dz * dz;
     c.dt2 = dt * dt;
+    c.g.rx2 = vti::div_reciprocal(c.dx2);
+    c.g.rz2 = vti::div_reciprocal(c.dz2);
     c.block = dim3(64, kBY2);
     c.grid = dim3((c.g.nnx - 8 + 63) / 64, (c.g.nnz - 8 + kBY2 - 1) / kBY2);
     VTI_HIP(hipSetDevice(ctx->device));

@@ -436,6 +436,8 @@ int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *
     bg.nnz = nnz;
     bg.npad = a.npad;
     bg.cpitch = a.NNX;
+    bg.rx2 = vti::div_reciprocal(dx2);
+    bg.rz2 = vti::div_reciprocal(dz2);
     bg.nnxz_total = f_total;
     const dim3 grid((unsigned)W.tile_shot.size()), block(64, kBY2);
     const dim3 gridt((unsigned)W.tile_shot_t.size()), blockt(64, BYT);
@@ -608,6 +610,8 @@ int enqueue_model_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const f
     bg.nnz = a.nnz;
     bg.npad = a.npad;
     bg.cpitch = a.NNX;
+    bg.rx2 = vti::div_reciprocal(dx2);
+    bg.rz2 = vti::div_reciprocal(dz2);
     bg.nnxz_total = f_total;
     const dim3 grid((unsigned)W.tile_shot.size()), block(64, kBY2);
     const dim3 gridt((unsigned)W.tile_shot_t.size()), blockt(64, BYT);

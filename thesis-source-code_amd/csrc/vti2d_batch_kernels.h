@@ -64,6 +64,7 @@ __device__ __forceinline__ int xcd_tile()
 struct BatchGeo {
     int nnz, npad, cpitch;
     long long nnxz_total;
+    float rx2, rz2;  // RN(1/dx2), RN(1/dz2) when qdiv may use them, else 0
 };
 
 // Where a forward kernel puts the D snapshot of the level it reads.
@@ -329,7 +330,7 @@ static __global__ __launch_bounds__(64 * kBY2) void fwd2db_kernel(
     float mp = 0.f, mq = 0.f;
     if (act) {
         const float tap = taper_of(tw);
-        const float tp = dxx(t, dx2), tq = dzz(t, dz2);
+        const float tp = dxx(t, dx2, bg.rx2), tq = dzz(t, dz2, bg.rz2);
         if (slot >= 0) {
             snap_put(sw, sh, (long long)nnx * nnz, slot, i, tp, tq);
             mp = fabsf(tp);
@@ -368,7 +369,7 @@ static __global__ __launch_bounds__(64 * kBY2) void dsnap2db_kernel(const float 
     float mp = 0.f, mq = 0.f;
     if (ix < nnx - 4 && iz < nnz - 4) {
         const long long i = ix + (long long)iz * nnx;
-        const float tp = dxx(t, dx2), tq = dzz(t, dz2);
+        const float tp = dxx(t, dx2, bg.rx2), tq = dzz(t, dz2, bg.rz2);
         snap_put(sw, sh, (long long)nnx * nnz, slot, i, tp, tq);
         mp = fabsf(tp);
         mq = fabsf(tq);
@@ -431,7 +432,7 @@ static __global__ __launch_bounds__(64 * kBY2) void bwd2db_kernel(
     d1p = snap_mul<SF>(sr, sh, slot0 + 1, 0, d1p);
     d0q = snap_mul<SF>(sr, sh, slot0, 1, d0q);
     d1q = snap_mul<SF>(sr, sh, slot0 + 1, 1, d1q);
-    const float tp0 = dxx(t, dx2), tq0 = dzz(t, dz2);
+    const float tp0 = dxx(t, dx2, bg.rx2), tq0 = dzz(t, dz2, bg.rz2);
     float pn = 2.f * pc + dt2 * tp0 - pp * tap;
     float qn = 2.f * qc + dt2 * tq0 - qp * tap;
     if (ir >= 0) {
@@ -700,8 +701,8 @@ static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
             float p1 = 0.f, q1 = 0.f;
             if (xx >= 4 && xx < nnx - 4 && zz >= 4 && zz < nnz - 4) {
                 const int i = xx + zz * nnx;
-                const float tp = st9_row(t.a[rz], rx + 4) / dx2;
-                const float tq = st9_col<72>(t.b, rz + 4, rx) / dz2;
+                const float tp = qdiv(st9_row(t.a[rz], rx + 4), dx2, bg.rx2);
+                const float tq = qdiv(st9_col<72>(t.b, rz + 4, rx), dz2, bg.rz2);
                 if (slot1 >= 0 && rx >= 4 && rx < 68 && rz >= 4 && rz < BY + 4) {
                     snap_put32(sw, sh, plane, slot1, i, tp, tq);
                     mp = fmaxf(mp, fabsf(tp));
@@ -730,8 +731,8 @@ static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
         const float tap = taper_of(tw);
         const int rx = threadIdx.x + 4, rz = threadIdx.y + 4;
         const int i = x + z * nnx;
-        const float tp = st9_row(t.p1[rz], rx) / dx2;
-        const float tq = st9_col<72>(t.q1, rz, rx) / dz2;
+        const float tp = qdiv(st9_row(t.p1[rz], rx), dx2, bg.rx2);
+        const float tq = qdiv(st9_col<72>(t.q1, rz, rx), dz2, bg.rz2);
         if (slot2 >= 0) {
             snap_put32(sw, sh, plane, slot2, i, tp, tq);
             mp = fabsf(tp);
@@ -861,8 +862,8 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
             const int xx = x0 - 4 + rx, zz = z0 - 4 + rz;
             float p1 = 0.f, q1 = 0.f, pp = 0.f, qq = 0.f;
             if (xx >= 4 && xx < nnx - 4 && zz >= 4 && zz < nnz - 4) {
-                const float tp0 = st9_row(t.a[rz], rx + 4) / dx2;
-                const float tq0 = st9_col<72>(t.b, rz + 4, rx) / dz2;
+                const float tp0 = qdiv(st9_row(t.a[rz], rx + 4), dx2, bg.rx2);
+                const float tq0 = qdiv(st9_col<72>(t.b, rz + 4, rx), dz2, bg.rz2);
                 const float tkj = taper_of(tk[j]);
                 float pn = 2.f * pb[j] + dt2 * tp0 - pa[j] * tkj;
                 float qn = 2.f * qb[j] + dt2 * tq0 - qa[j] * tkj;
@@ -896,8 +897,8 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
     float g11 = ld32(g11p, i), g13 = ld32(g13p, i), g33 = ld32(g33p, i);
     const float tap = taper_of(tw);
     const int rx = threadIdx.x + 4, rz = threadIdx.y + 4;
-    const float tp0 = st9_row(t.pp1[rz], rx) / dx2;
-    const float tq0 = st9_col<72>(t.qq1, rz, rx) / dz2;
+    const float tp0 = qdiv(st9_row(t.pp1[rz], rx), dx2, bg.rx2);
+    const float tq0 = qdiv(st9_col<72>(t.qq1, rz, rx), dz2, bg.rz2);
     const float p1 = t.p1[rz][rx], q1 = t.q1[rz][rx];
     float pn = 2.f * p1 + dt2 * tp0 - pbc * tap;
     float qn = 2.f * q1 + dt2 * tq0 - qbc * tap;

@@ -26,6 +26,7 @@ namespace vti {
 
 struct Geo2 {
     int nnx, nnz;
+    float rx2, rz2;  // RN(1/dx2), RN(1/dz2) when qdiv may use them, else 0
 };
 
 constexpr int kBY2 = 8;
@@ -118,19 +119,44 @@ __device__ __forceinline__ void load_tiles_c(Tiles2 &t, const float *__restrict_
     }
 }
 
-__device__ __forceinline__ float dxx(const Tiles2 &t, float dx2)
+// x / d correctly rounded, for a divisor d fixed per launch.  y is RN(1/d)
+// when the host has shown (vti::div_reciprocal) that Markstein's correction
+// step -- q0 = RN(x y), r = x - q0 d (exact, one fma), q = RN(q0 + r y) --
+// gives RN(x/d) for every x in [1,2).  For d in [2^-20, 2^20] every
+// intermediate then scales exactly with x's exponent, which carries the
+// result to all |x| in [2^-100, 2^100).  A zero x comes out as +0 and
+// takes x's sign bit (a nonzero q has x's sign already).  Other magnitudes,
+// and y == 0 (no proof for this d), take the division.
+__device__ __forceinline__ float qdiv(float x, float d, float y)
+{
+    if (y == 0.f) return x / d;
+    const float q0 = x * y;
+    const float r = __builtin_fmaf(-q0, d, x);
+    const unsigned xb = __float_as_uint(x), a = xb & 0x7fffffffu;
+    float q = __uint_as_float((__float_as_uint(__builtin_fmaf(r, y, q0)) & 0x7fffffffu) | (xb & 0x80000000u));
+    if (a - 0x0d800000u >= 0x64000000u && a != 0u) {
+        // the empty volatile statement keeps the compiler from computing the
+        // division for every lane and selecting it afterwards
+        float xs = x;
+        asm volatile("" : "+v"(xs));
+        q = xs / d;
+    }
+    return q;
+}
+
+__device__ __forceinline__ float dxx(const Tiles2 &t, float dx2, float rx2)
 {
     const int c = threadIdx.x + 4, r = threadIdx.y;
     const float *a = t.p[r];
-    return st9(a[c], a[c - 1], a[c + 1], a[c - 2], a[c + 2], a[c - 3], a[c + 3], a[c - 4], a[c + 4]) / dx2;
+    return qdiv(st9(a[c], a[c - 1], a[c + 1], a[c - 2], a[c + 2], a[c - 3], a[c + 3], a[c - 4], a[c + 4]), dx2, rx2);
 }
 
-__device__ __forceinline__ float dzz(const Tiles2 &t, float dz2)
+__device__ __forceinline__ float dzz(const Tiles2 &t, float dz2, float rz2)
 {
     const int c = threadIdx.x, r = threadIdx.y + 4;
-    return st9(t.q[r][c], t.q[r - 1][c], t.q[r + 1][c], t.q[r - 2][c], t.q[r + 2][c], t.q[r - 3][c], t.q[r + 3][c],
-               t.q[r - 4][c], t.q[r + 4][c]) /
-           dz2;
+    return qdiv(st9(t.q[r][c], t.q[r - 1][c], t.q[r + 1][c], t.q[r - 2][c], t.q[r + 2][c], t.q[r - 3][c],
+                    t.q[r + 3][c], t.q[r - 4][c], t.q[r + 4][c]),
+                dz2, rz2);
 }
 
 // Forward step: forwardCD / forwardC semantics (kernels.cu:452-498, 546-556)
@@ -171,7 +197,7 @@ __global__ __launch_bounds__(64 * kBY2) void fwd2d_kernel(
     if (BORN) load_tiles<0>(tb, dp_c, dq_c, nullptr, nullptr, nullptr, g, x0, z0);
     __syncthreads();
     if (!act) return;
-    const float tp = dxx(t, dx2), tq = dzz(t, dz2);
+    const float tp = dxx(t, dx2, g.rx2), tq = dzz(t, dz2, g.rz2);
     if (snap_p) {
         snap_p[i] = tp;
         snap_q[i] = tq;
@@ -189,7 +215,7 @@ __global__ __launch_bounds__(64 * kBY2) void fwd2d_kernel(
     p_n[i] = pn * tap;
     q_n[i] = qn * tap;
     if (BORN) {
-        const float btp = dxx(tb, dx2), btq = dzz(tb, dz2);
+        const float btp = dxx(tb, dx2, g.rx2), btq = dzz(tb, dz2, g.rz2);
         if (bsnap_p) {  // D of the Born level read (hessian_f.cu:236)
             bsnap_p[i] = btp;
             bsnap_q[i] = btq;
@@ -216,8 +242,8 @@ static __global__ __attribute__((unused)) __launch_bounds__(64 * kBY2) void dsna
     __syncthreads();
     if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
     const long long i = ix + (long long)iz * g.nnx;
-    sp[i] = dxx(t, dx2);
-    sq[i] = dzz(t, dz2);
+    sp[i] = dxx(t, dx2, g.rx2);
+    sq[i] = dzz(t, dz2, g.rz2);
 }
 
 // Adjoint step fused with injectData, abc and gradientCrossCor
@@ -272,11 +298,11 @@ __global__ __launch_bounds__(64 * kBY2) void bwd2d_kernel(
     if (!act) return;
     float pn, qn;
     if (ADJ == 0) {
-        const float tp0 = dxx(t, dx2), tq0 = dzz(t, dz2);
+        const float tp0 = dxx(t, dx2, g.rx2), tq0 = dzz(t, dz2, g.rz2);
         pn = 2.f * pc + dt2 * tp0 - pp * tap;
         qn = 2.f * qc + dt2 * tq0 - qp * tap;
     } else {
-        const float tp = dxx(t, dx2), tq = dzz(t, dz2);
+        const float tp = dxx(t, dx2, g.rx2), tq = dzz(t, dz2, g.rz2);
         pn = 2.f * pc + dt2 * (a11 * tp + a13 * tq) - pp * tap;
         qn = 2.f * qc + dt2 * (a13 * tp + a33 * tq) - qp * tap;
     }
@@ -325,10 +351,10 @@ static __global__ __attribute__((unused)) __launch_bounds__(64 * kBY2) void hess
     if (ix >= g.nnx - 4 || iz >= g.nnz - 4) return;
     const long long i = ix + (long long)iz * g.nnx;
     const float tap = taper[i];
-    float pn = 2.f * p_c[i] + dt2 * dxx(ta, dx2) - p_n[i] * tap;
-    float qn = 2.f * q_c[i] + dt2 * dzz(ta, dz2) - q_n[i] * tap;
-    float dpn = 2.f * dp_c[i] + dt2 * dxx(tb, dx2) - dp_n[i] * tap;
-    float dqn = 2.f * dq_c[i] + dt2 * dzz(tb, dz2) - dq_n[i] * tap;
+    float pn = 2.f * p_c[i] + dt2 * dxx(ta, dx2, g.rx2) - p_n[i] * tap;
+    float qn = 2.f * q_c[i] + dt2 * dzz(ta, dz2, g.rz2) - q_n[i] * tap;
+    float dpn = 2.f * dp_c[i] + dt2 * dxx(tb, dx2, g.rx2) - dp_n[i] * tap;
+    float dqn = 2.f * dq_c[i] + dt2 * dzz(tb, dz2, g.rz2) - dq_n[i] * tap;
     const int ir = recmap[i];
     if (ir >= 0) {
         const float s = 0.5f * ((1.f - fr) * r0[ir] + fr * r1[ir]);
@@ -338,8 +364,8 @@ static __global__ __attribute__((unused)) __launch_bounds__(64 * kBY2) void hess
         dpn += dt2 * sb;
         dqn += dt2 * sb;
     }
-    dpn += dt2 * dxx(ts, dx2);
-    dqn += dt2 * dzz(ts, dz2);
+    dpn += dt2 * dxx(ts, dx2, g.rx2);
+    dqn += dt2 * dzz(ts, dz2, g.rz2);
     pn = pn * tap;
     qn = qn * tap;
     dpn = dpn * tap;

@@ -1,5 +1,7 @@
 // Error reporting and the execution context shared by all entry points.
+#include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -169,9 +171,54 @@ void dfree(void *p, hipStream_t s)
     (void)hipFreeAsync(p, s);
 }
 
+// Markstein's correction step for a quotient with a known reciprocal
+// (P. Markstein, "Computation of elementary functions on the IBM RISC
+// System/6000 processor", IBM J. Res. Dev. 34(1), 1990), run in FP32 with
+// the operations the device uses: q0 = RN(x y), r = fma(-q0, d, x),
+// q = fma(r, y, q0).  A case fails when q differs from RN(x/d) or when r
+// is not the exact remainder x - q0 d (exactness is what lets the result
+// scale with x's exponent).
+long long div_mismatches(float d, float y)
+{
+    long long bad = 0;
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+        const uint32_t b = 0x3f800000u | m;
+        float x;
+        std::memcpy(&x, &b, 4);
+        const float q0 = x * y;
+        const float r = std::fma(-q0, d, x);
+        const float q = std::fma(r, y, q0);
+        const float want = x / d;
+        const double rex = (double)x - (double)q0 * (double)d;
+        if ((double)r != rex || std::memcmp(&q, &want, 4) != 0) ++bad;
+    }
+    return bad;
+}
+
+float div_reciprocal(float d)
+{
+    static std::mutex mu;
+    static std::unordered_map<uint32_t, float> known;
+    if (!(d >= 0x1p-20f && d <= 0x1p20f)) return 0.f;
+    uint32_t key;
+    std::memcpy(&key, &d, 4);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(key);
+    if (it != known.end()) return it->second;
+    const float y = 1.f / d;
+    const float out = div_mismatches(d, y) == 0 ? y : 0.f;
+    known.emplace(key, out);
+    return out;
+}
+
 }  // namespace vti
 
 extern "C" {
+
+// The divisor proof, for tests: RN(1/d) or 0 (see vti::div_reciprocal), and
+// the number of failing cases of a given reciprocal.
+float vti_div_reciprocal(float d) { return vti::div_reciprocal(d); }
+long long vti_div_mismatches(float d, float y) { return vti::div_mismatches(d, y); }
 
 const char *vti_last_error(void) { return vti::get_error(); }
 

@@ -74,6 +74,14 @@ int dmalloc(void **p, size_t bytes, hipStream_t s);
 size_t device_free_bytes();
 void dfree(void *p, hipStream_t s);
 
+// RN(1/d) when the kernels' qdiv (x / d by one multiply and two fmas) is
+// shown, over every mantissa, to give the correctly rounded quotient; 0 when
+// it is not, or d lies outside [2^-20, 2^20] (qdiv then divides).  The proof
+// runs once per distinct d in the process (about 8M cases, some 50 ms).
+float div_reciprocal(float d);
+// mismatches of that sequence, with reciprocal y, over x in [1,2)
+long long div_mismatches(float d, float y);
+
 }  // namespace vti
 
 #define VTI_HIP(call)                                                                       \
