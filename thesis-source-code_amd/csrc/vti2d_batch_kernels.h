@@ -854,6 +854,24 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
     }
     halo_store_t2<BY, true>(t, hv, nnx, nnz, x0, z0, tid);
     __syncthreads();
+    // the snapshots of both levels and the gradient at this thread's point
+    // (clamped into the grid), issued here so that they arrive while level
+    // L+1 is computed on R1.  Both levels usually interpolate between the
+    // same two slots; the second level's are then not read again.
+    float a0p = snap_raw32<SF>(sr, sh, plane, l1.slot, 0, i);
+    float a1p = snap_raw32<SF>(sr, sh, plane, l1.slot + 1, 0, i);
+    float a0q = snap_raw32<SF>(sr, sh, plane, l1.slot, 1, i);
+    float a1q = snap_raw32<SF>(sr, sh, plane, l1.slot + 1, 1, i);
+    // (no copies here: a copy would wait for the loads)
+    const bool same = l2.slot == l1.slot;
+    float b0p = 0.f, b1p = 0.f, b0q = 0.f, b1q = 0.f;
+    if (!same) {
+        b0p = snap_raw32<SF>(sr, sh, plane, l2.slot, 0, i);
+        b1p = snap_raw32<SF>(sr, sh, plane, l2.slot + 1, 0, i);
+        b0q = snap_raw32<SF>(sr, sh, plane, l2.slot, 1, i);
+        b1q = snap_raw32<SF>(sr, sh, plane, l2.slot + 1, 1, i);
+    }
+    float g11 = ld32(g11p, i), g13 = ld32(g13p, i), g33 = ld32(g33p, i);
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
         const int k = tid + j * NT;
@@ -884,17 +902,6 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
     }
     __syncthreads();
     if (!act) return;
-    // the snapshots of both levels and the gradient, read while the step to
-    // L+2 is computed (kept out of the first phase to save registers there)
-    float a0p = snap_raw32<SF>(sr, sh, plane, l1.slot, 0, i);
-    float a1p = snap_raw32<SF>(sr, sh, plane, l1.slot + 1, 0, i);
-    float a0q = snap_raw32<SF>(sr, sh, plane, l1.slot, 1, i);
-    float a1q = snap_raw32<SF>(sr, sh, plane, l1.slot + 1, 1, i);
-    float b0p = snap_raw32<SF>(sr, sh, plane, l2.slot, 0, i);
-    float b1p = snap_raw32<SF>(sr, sh, plane, l2.slot + 1, 0, i);
-    float b0q = snap_raw32<SF>(sr, sh, plane, l2.slot, 1, i);
-    float b1q = snap_raw32<SF>(sr, sh, plane, l2.slot + 1, 1, i);
-    float g11 = ld32(g11p, i), g13 = ld32(g13p, i), g33 = ld32(g33p, i);
     const float tap = taper_of(tw);
     const int rx = threadIdx.x + 4, rz = threadIdx.y + 4;
     const float tp0 = qdiv(st9_row(t.pp1[rz], rx), dx2, bg.rx2);
@@ -912,6 +919,12 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
     st32(Qc + sh.f_off, i, q1);
     st32(Pd + sh.f_off, i, pn);
     st32(Qd + sh.f_off, i, qn);
+    if (same) {  // the raw values, scaled below like the others
+        b0p = a0p;
+        b1p = a1p;
+        b0q = a0q;
+        b1q = a1q;
+    }
     a0p = snap_mul<SF>(sr, sh, l1.slot, 0, a0p);
     a1p = snap_mul<SF>(sr, sh, l1.slot + 1, 0, a1p);
     a0q = snap_mul<SF>(sr, sh, l1.slot, 1, a0q);
