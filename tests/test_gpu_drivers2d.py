@@ -164,6 +164,39 @@ def test_vepsdel_gn_wrapper(vti, ctx):
     assert np.abs(h[0]).max() > 0
 
 
+@pytest.mark.parametrize("dx,dz", [(10.0, 12.5), (1500.0, 1500.0), (2000.0, 10.0)])
+def test_spacings_bitexact(vti, orc, ctx, dx, dz):
+    """The 2D kernels divide by dx^2 and dz^2 through a reciprocal proven for
+    each divisor, or by the division when no proof is made (above 2^20:
+    1500^2 and 2000^2).  Unequal spacings, both routes and a mix of the two
+    stay bit-exact with the oracle's FP32 restatement: one-shot gradients,
+    the batched multi-shot route and batched modeling."""
+    import ctypes
+
+    vti.lib.vti_div_reciprocal.restype = ctypes.c_float
+    vti.lib.vti_div_reciprocal.argtypes = [ctypes.c_float]
+    for d in (dx, dz):
+        y = vti.lib.vti_div_reciprocal(np.float32(d) * np.float32(d))
+        assert (y != 0) == (d * d <= 2.0 ** 20)
+    P2 = problem2d(vti, ns=2, seed=8, dx=dx)
+    P2["dz"] = dz
+    a = args2d(P2)
+    _say("2d spacings", dx, dz)
+    data = orc.synthetic2d(orc.F64, *P2["ctrue"], P2["w"], P2["sloc"], P2["rloc"], P2["taper"], *a).astype(
+        np.float32) * np.float32(0.5)
+    for ns in (1, 2):
+        s = P2["sloc"][:ns]
+        f, g = vti.objfunc_gradient2d(ctx, data, *P2["c"], P2["w"], s, P2["rloc"], P2["taper"], *a)
+        fo, go = orc.fg2d(orc.F32, False, data, *P2["c"], None, P2["w"], s, P2["rloc"], P2["taper"], *a, ngpu=ns)
+        assert np.abs(go).max() > 0
+        assert np.array_equal(g, go)
+        assert np.float32(f) == np.float32(fo)
+    d = vti.synthetic2d(ctx, *P2["c"], P2["w"], P2["sloc"], P2["rloc"], P2["taper"], *a)
+    o = orc.synthetic2d(orc.F32, *P2["c"], P2["w"], P2["sloc"], P2["rloc"], P2["taper"], *a)
+    assert np.abs(o).max() > 0
+    assert np.array_equal(d, o)
+
+
 def test_multishot_call_runs_as_batch_bit_exact(vti, ctx):
     """A call with several shots and the standard taper runs them as one
     batch: its gradient equals the FP64 sum of one-shot calls rounded once,
