@@ -14,7 +14,6 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
-#include <deque>
 #include <functional>
 #include <map>
 #include <memory>
@@ -281,57 +280,13 @@ struct BatchWave {
     std::vector<int> pblk, poff;
 };
 
-// The host arrays a wave's uploads read from.  A caller that does not wait
-// for the uploads (so that the next wave is prepared while the device runs
-// this one) keeps them until `ev`, recorded after the uploads, has passed.
-struct HostStage {
-    std::vector<ShotB> sb, sbt;
-    std::vector<int> tile_shot, tile_shot_t, tsd, hmap;
-    std::vector<long long> fidx, tb;
-    std::vector<float> obs;
-    hipEvent_t ev = nullptr;
-    HostStage() = default;
-    HostStage(const HostStage &) = delete;
-    HostStage &operator=(const HostStage &) = delete;
-    ~HostStage()
-    {
-        if (ev) (void)hipEventDestroy(ev);
-    }
-};
-using Stages = std::deque<std::unique_ptr<HostStage>>;
-
-// release the oldest stages until at most `n` remain, each after its uploads
-int retire(Stages &pend, size_t n)
-{
-    while (pend.size() > n) {
-        VTI_HIP(hipEventSynchronize(pend.front()->ev));
-        pend.pop_front();
-    }
-    return vti::kOk;
-}
-
-// Waits for the stream before the stages it guards are destroyed, so that
-// an early return never frees host arrays an upload may still read.
-struct StageGuard {
-    Stages &pend;
-    hipStream_t st;
-    ~StageGuard()
-    {
-        if (!pend.empty()) (void)hipStreamSynchronize(st);
-    }
-};
-
 // the one-shot driver's residual grid for n trace samples
 int shot_residual_blocks(long long n)
 {
     return (int)std::min<long long>(2048, std::max<long long>(1, (n + 255) / 256));
 }
 
-// Allocate a wave's buffers and upload its tables and data.  With `keep`
-// the call returns without waiting for the uploads: the host arrays they
-// read go to *keep, which the caller holds until its event has passed.
-int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *data, const int *rloc, hipStream_t st,
-               std::unique_ptr<HostStage> *keep = nullptr)
+int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *data, const int *rloc, hipStream_t st)
 {
     const int nw = s1 - s0, nnz = a.nnz, npad = a.npad, nnt = a.nnt;
     const int ntz = (nnz - 8 + kBY2 - 1) / kBY2, ntz_t = (nnz - 8 + a.BYT - 1) / a.BYT;
@@ -375,13 +330,9 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
         W.nrec += q.nr;
     }
     const int nrec = W.nrec;
-    auto hs = std::make_unique<HostStage>();
-    std::vector<long long> &fidx = hs->fidx, &tb = hs->tb;
-    std::vector<int> &tsd = hs->tsd, &hmap = hs->hmap;
-    fidx.assign(nrec, 0);
-    tb.assign(nrec, 0);
-    tsd.assign(nrec, 0);
-    hmap.assign((size_t)W.f_total, -1);
+    std::vector<long long> fidx(nrec), tb(nrec);
+    std::vector<int> tsd(nrec);
+    std::vector<int> hmap((size_t)W.f_total, -1);
     for (int j = 0, k = 0; j < nw; ++j) {
         const ShotIn &q = a.in[s0 + j];
         for (int r = 0; r < q.nr; ++r, ++k) {
@@ -392,8 +343,7 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
             hmap[(size_t)fidx[k]] = r;
         }
     }
-    std::vector<float> &obs_h = hs->obs;
-    obs_h.assign((size_t)std::max<long long>(W.tr_total, 1), 0.f);
+    std::vector<float> obs_h((size_t)std::max<long long>(W.tr_total, 1));
     for (int j = 0; j < nw; ++j)
         if (a.in[s0 + j].nr)
             std::memcpy(obs_h.data() + W.sb[j].r_off, data + a.data_off[s0 + j],
@@ -442,16 +392,10 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
     }
     if (!rc) rc = W.part.alloc(std::max(kNblkR, ptot), st);
     if (rc) return rc;
-    // the uploads read the stage's own copies of the wave's tables
-    hs->sb = W.sb;
-    hs->sbt = W.sbt;
-    hs->tile_shot = W.tile_shot;
-    hs->tile_shot_t = W.tile_shot_t;
-    VTI_HIP(hipMemcpyAsync(W.sbd.p, hs->sb.data(), sizeof(ShotB) * nw, hipMemcpyHostToDevice, st));
-    VTI_HIP(hipMemcpyAsync(W.sbtd.p, hs->sbt.data(), sizeof(ShotB) * nw, hipMemcpyHostToDevice, st));
-    VTI_HIP(hipMemcpyAsync(W.tsh.p, hs->tile_shot.data(), sizeof(int) * hs->tile_shot.size(), hipMemcpyHostToDevice,
-                           st));
-    VTI_HIP(hipMemcpyAsync(W.tsht.p, hs->tile_shot_t.data(), sizeof(int) * hs->tile_shot_t.size(),
+    VTI_HIP(hipMemcpyAsync(W.sbd.p, W.sb.data(), sizeof(ShotB) * nw, hipMemcpyHostToDevice, st));
+    VTI_HIP(hipMemcpyAsync(W.sbtd.p, W.sbt.data(), sizeof(ShotB) * nw, hipMemcpyHostToDevice, st));
+    VTI_HIP(hipMemcpyAsync(W.tsh.p, W.tile_shot.data(), sizeof(int) * W.tile_shot.size(), hipMemcpyHostToDevice, st));
+    VTI_HIP(hipMemcpyAsync(W.tsht.p, W.tile_shot_t.data(), sizeof(int) * W.tile_shot_t.size(),
                            hipMemcpyHostToDevice, st));
     VTI_HIP(hipMemcpyAsync(W.recmap.p, hmap.data(), sizeof(int) * hmap.size(), hipMemcpyHostToDevice, st));
     if (nrec) {
@@ -461,12 +405,6 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
     }
     if (tr_total)
         VTI_HIP(hipMemcpyAsync(W.obs.p, obs_h.data(), sizeof(float) * tr_total, hipMemcpyHostToDevice, st));
-    if (keep) {
-        VTI_HIP(hipEventCreateWithFlags(&hs->ev, hipEventDisableTiming));
-        VTI_HIP(hipEventRecord(hs->ev, st));
-        *keep = std::move(hs);
-        return vti::kOk;
-    }
     VTI_HIP(hipStreamSynchronize(st));  // the host staging vectors end here
     return vti::kOk;
 }
@@ -809,41 +747,16 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
     const size_t mfree = vti::device_free_bytes();
     const long long budget = (long long)mfree - (long long)(2ULL << 30);
     VTI_CHECK(budget > a.shot_bytes(0), "not enough device memory for one shot");
-    // The waves follow one another on the stream with no wait in between:
-    // the next wave is prepared while the device runs this one, and each
-    // wave's residual partial sums are gathered on the device and summed in
-    // wave order at the end (the same FP64 sum as a wait after each wave).
-    const auto waves = split_waves(a, max_wave, budget);
-    DBuf<double> parts;
-    rc = parts.alloc((size_t)kNblkR * waves.size(), st);
-    if (rc) return rc;
-    std::vector<char> has(waves.size(), 0);
-    Stages pend;
-    StageGuard guard{pend, st};
-    for (size_t iw = 0; iw < waves.size(); ++iw) {
+    double obj = 0.0;
+    for (const auto &w : split_waves(a, max_wave, budget)) {
         BatchWave W;
-        std::unique_ptr<HostStage> hs;
-        rc = setup_wave(W, a, waves[iw].first, waves[iw].second, data, rloc, st, &hs);
-        if (hs) pend.push_back(std::move(hs));
+        rc = setup_wave(W, a, w.first, w.second, data, rloc, st);
         if (!rc) rc = enqueue_wave(W, S, a, wavelet, st);
-        if (rc) return rc;
-        if (W.tr_total) {
-            VTI_HIP(hipMemcpyAsync(parts.p + (size_t)kNblkR * iw, W.part.p, sizeof(double) * kNblkR,
-                                   hipMemcpyDeviceToDevice, st));
-            has[iw] = 1;
-        }
-        rc = retire(pend, 2);
+        if (!rc) rc = wave_objective(W, st, &obj);
         if (rc) return rc;
     }
     rc = download_gradient(S, a, gc11, gc13, gc33, st);
     if (rc) return rc;
-    std::vector<double> hp((size_t)kNblkR * waves.size());
-    VTI_HIP(hipMemcpyAsync(hp.data(), parts.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st));
-    VTI_HIP(hipStreamSynchronize(st));
-    double obj = 0.0;
-    for (size_t iw = 0; iw < waves.size(); ++iw)
-        if (has[iw])
-            for (int b = 0; b < kNblkR; ++b) obj += hp[(size_t)kNblkR * iw + b];
     *objfunc = 0.5 * obj;
     return vti::kOk;
 }
@@ -1011,41 +924,20 @@ int synth2d_batched(vti_ctx *ctx, float *out, const float *c11, const float *c13
     const std::vector<float> nodata((size_t)std::max<long long>((long long)nnt * (rl.size() / 2), 1), 0.f);
     const int cw = cache_wave(shots.data(), ns, npad, nz);
     const long long budget = (long long)vti::device_free_bytes() - (long long)(2ULL << 30);
-    // waves back to back on the stream; every wave's traces are gathered on
-    // the device and copied down once
-    const auto waves = split_waves(a, cw, budget);
-    long long ttotal = 0;
-    for (int k = 0; k < ns; ++k) ttotal += (long long)a.nnt_data * a.in[k].nr;
-    DBuf<float> alltr;
-    rc = alltr.alloc((size_t)std::max<long long>(ttotal, 1), st);
-    if (rc) return rc;
-    std::vector<long long> toff(ns, 0);
-    {
-        Stages pend;
-        StageGuard guard{pend, st};
-        long long woff = 0;
-        for (const auto &wv : waves) {
-            BatchWave W;
-            std::unique_ptr<HostStage> hs;
-            rc = setup_wave(W, a, wv.first, wv.second, nodata.data(), rl.data(), st, &hs);
-            if (hs) pend.push_back(std::move(hs));
-            if (!rc) rc = enqueue_model_wave(W, S, a, wavelet, st);
-            if (rc) return rc;
-            if (W.tr_total)
-                VTI_HIP(hipMemcpyAsync(alltr.p + woff, W.tr.p, W.tr_total * 4, hipMemcpyDeviceToDevice, st));
-            for (int j = 0; j < W.nw; ++j) toff[wv.first + j] = woff + W.sb[j].r_off;
-            woff += W.tr_total;
-            rc = retire(pend, 2);
-            if (rc) return rc;
+    for (const auto &wv : split_waves(a, cw, budget)) {
+        BatchWave W;
+        rc = setup_wave(W, a, wv.first, wv.second, nodata.data(), rl.data(), st);
+        if (!rc) rc = enqueue_model_wave(W, S, a, wavelet, st);
+        if (rc) return rc;
+        std::vector<float> h((size_t)std::max<long long>(W.tr_total, 1));
+        if (W.tr_total) VTI_HIP(hipMemcpyAsync(h.data(), W.tr.p, W.tr_total * 4, hipMemcpyDeviceToHost, st));
+        VTI_HIP(hipStreamSynchronize(st));
+        for (int j = 0; j < W.nw; ++j) {
+            const int k = wv.first + j, nri = sloc[4 * k + 2], r0 = sloc[4 * k + 3];
+            for (int row = 1; row < nnt; ++row)  // row 0 is never recorded
+                for (int ir = 0; ir < nri; ++ir)
+                    out[(size_t)row * nr + r0 + ir] = h[W.sb[j].r_off + (size_t)row * nri + ir];
         }
-    }
-    std::vector<float> h((size_t)std::max<long long>(ttotal, 1));
-    if (ttotal) VTI_HIP(hipMemcpyAsync(h.data(), alltr.p, ttotal * 4, hipMemcpyDeviceToHost, st));
-    VTI_HIP(hipStreamSynchronize(st));
-    for (int k = 0; k < ns; ++k) {
-        const int nri = sloc[4 * k + 2], r0 = sloc[4 * k + 3];
-        for (int row = 1; row < nnt; ++row)  // row 0 is never recorded
-            for (int ir = 0; ir < nri; ++ir) out[(size_t)row * nr + r0 + ir] = h[toff[k] + (size_t)row * nri + ir];
     }
     return vti::kOk;
 }
@@ -1136,50 +1028,17 @@ int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet
         // waves of at most max_wave shots, fewer when the cache model asks
         const int cw = cache_wave(shots.data(), ns, npad, nz);
         const int wave = max_wave > 0 ? std::min(max_wave, cw) : cw;
-        // waves back to back on the stream, as in vti2d_objfunc_gradient_batch;
-        // each shot's partial sums are gathered on the device and added to its
-        // request's objective in the same order afterwards
-        const auto waves = split_waves(a, wave, budget);
-        std::vector<long long> woff(waves.size() + 1, 0);
-        for (size_t iw = 0; iw < waves.size(); ++iw) {
-            long long pt = 0;
-            for (int k = waves[iw].first; k < waves[iw].second; ++k)
-                pt += shot_residual_blocks((long long)a.nnt_data * a.in[k].nr);
-            woff[iw + 1] = woff[iw] + std::max<long long>(kNblkR, pt);
-        }
-        DBuf<double> parts;
-        rc = parts.alloc((size_t)woff.back(), st);
-        if (rc) return rc;
-        struct ShotSum {
-            int owner, blocks;
-            long long off;
-        };
-        std::vector<ShotSum> sums;
-        {
-            Stages pend;
-            StageGuard guard{pend, st};
-            for (size_t iw = 0; iw < waves.size(); ++iw) {
-                BatchWave W;
-                std::unique_ptr<HostStage> hs;
-                rc = setup_wave(W, a, waves[iw].first, waves[iw].second, data.data(), rl.data(), st, &hs);
-                if (hs) pend.push_back(std::move(hs));
-                if (!rc) rc = enqueue_wave(W, S, a, wavelet, st, true);
-                if (rc) return rc;
-                VTI_CHECK((long long)W.part.n == woff[iw + 1] - woff[iw], "grad2d_merged: partial-sum layout");
-                VTI_HIP(hipMemcpyAsync(parts.p + woff[iw], W.part.p, sizeof(double) * W.part.n,
-                                       hipMemcpyDeviceToDevice, st));
-                for (int j = 0; j < W.nw; ++j)
-                    if (W.sb[j].nr) sums.push_back({owner[waves[iw].first + j], W.pblk[j], woff[iw] + W.poff[j]});
-                rc = retire(pend, 2);
-                if (rc) return rc;
-            }
-        }
-        {
-            std::vector<double> hp((size_t)woff.back());
-            VTI_HIP(hipMemcpyAsync(hp.data(), parts.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st));
+        for (const auto &wv : split_waves(a, wave, budget)) {
+            BatchWave W;
+            rc = setup_wave(W, a, wv.first, wv.second, data.data(), rl.data(), st);
+            if (!rc) rc = enqueue_wave(W, S, a, wavelet, st, true);
+            if (rc) return rc;
+            std::vector<double> hp(W.part.n);
+            VTI_HIP(hipMemcpyAsync(hp.data(), W.part.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st));
             VTI_HIP(hipStreamSynchronize(st));
-            for (const ShotSum &q : sums)
-                for (int b = 0; b < q.blocks; ++b) obj[q.owner] += hp[q.off + b];
+            for (int j = 0; j < W.nw; ++j)
+                if (W.sb[j].nr)
+                    for (int b = 0; b < W.pblk[j]; ++b) obj[owner[wv.first + j]] += hp[W.poff[j] + b];
         }
         // rounded to FP32 on the device
         const size_t g = (size_t)NNX * nnz;
