@@ -23,6 +23,7 @@ import json
 import os
 import sys
 import time
+import traceback
 
 import numpy as np
 
@@ -300,6 +301,26 @@ def synthetic_cfg1(vtiamd, ctx, ns=16):
             f"call_of_{ns}_shots_shots_per_s": round(many, 2), "speedup": round(many / one, 2)}
 
 
+def run_sections(extra, sections):
+    """Run the secondary workloads in order, each result under its name.
+    An error is recorded in place of that section's result (its traceback
+    goes to stderr) and the later sections are marked as not run, since the
+    device may be unusable after it; the headline measured before them is
+    still reported.  Returns the name of the failed section, or None."""
+    failed = None
+    for name, fn in sections:
+        if failed is not None:
+            extra[name] = {"not_run": f"after the error in {failed}"}
+            continue
+        try:
+            extra[name] = fn()
+        except Exception as e:  # reported in the output line
+            traceback.print_exc(file=sys.stderr)
+            extra[name] = {"error": f"{type(e).__name__}: {e}"}
+            failed = name
+    return failed
+
+
 def serve_trace(vtiamd, nreq=40, rate=5.0, frac3d=0.3, seed=5, surveys=False):
     """A seeded request trace: Poisson arrivals at `rate` req/s, a fraction
     frac3d of 3D modeling requests (each dimension 128-400, 1000-3000
@@ -541,15 +562,25 @@ def main():
     f32ms = prop.forward_steps(max(1, args.steps // 2), absorbing=True, timed=True)
     extra["fwd_step_fp32_arith"] = {"ms": round(f32ms, 4), "gpts_s": round(npts / (f32ms * 1e-3) / 1e9, 3)}
     prop.close()
+    failed = None
     if not args.small and not args.no_2d:
-        extra["2d_shot_gradient_512x1024_3000steps"] = shot_gradient_2d(vtiamd, ctx)
-        extra["2d_synthetic_256x256_2000steps"] = synthetic_cfg1(vtiamd, ctx)
-        extra["2d_batched_vs_plain_gradient"] = batch_gradient_2d(vtiamd, ctx)
-        extra["served_stream"] = serve_stream(vtiamd)
+        failed = run_sections(extra, [
+            ("2d_shot_gradient_512x1024_3000steps", lambda: shot_gradient_2d(vtiamd, ctx)),
+            ("2d_synthetic_256x256_2000steps", lambda: synthetic_cfg1(vtiamd, ctx)),
+            ("2d_batched_vs_plain_gradient", lambda: batch_gradient_2d(vtiamd, ctx)),
+            ("served_stream", lambda: serve_stream(vtiamd)),
+        ])
     out["extra"] = extra
+    if failed is not None:
+        out["errors"] = [f"{failed}: {extra[failed]['error']}"]
     if not args.no_cpu and not args.small:
         out["cpu_baseline"] = cpu_baseline(ved, npad, dx, dt)
-    ctx.close()
+    try:
+        ctx.close()
+    except Exception:
+        if failed is None:
+            raise
+        traceback.print_exc(file=sys.stderr)
     print(json.dumps(out), flush=True)
 
 
