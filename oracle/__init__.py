@@ -16,7 +16,9 @@ from ctypes import c_double, c_float, c_int, c_long, c_void_p
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# VTI_ORACLE_LIB names another build of this library (the mutation test's
+# deliberately broken builds); the default is the one built beside this file.
+LIB_PATH = os.environ.get("VTI_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 F32, REF, F64 = 0, 1, 2
 
@@ -26,8 +28,14 @@ PINS = {
     "orc3d_boundary": "bit-exact vs compiled reference boundary.cpp (tests/golden) + round trip",
     "orc3d_forward": "FP64 mode: plane-wave dispersion closed form, time reversibility, energy; "
     "REF/F32 modes within 1e-5 of FP64",
-    "orc3d_gradient": "FP64 mode: adjoint-state Taylor test of J; F32/REF within 1e-4 of FP64",
-    "orc3d_rtm": "FP64 mode: linearity in the data, F32/REF within 1e-4 of FP64",
+    "orc3d_gradient": "FP64 mode: equals a stored-history restatement (every source level kept, no "
+    "random-boundary reversal) to 1e-14, with and without the t^p gain; Taylor ratios per coefficient; "
+    "mutants of the receiver weights, gc13/gc33 terms, residual sign, t^p exponent and time index fail",
+    "orc3d_rtm": "FP64 mode: equals the stored-history image to 1e-14 (image weights mutant fails); "
+    "linear in the data",
+    "orc2d_rtm": "FP64 mode: rtm_f and odcig_f equal a stored-history restatement (every level kept, "
+    "closed-form slice and data interpolation) bit for bit, ratio 1/2/4, ot < 0; image, interpolation "
+    "and offset mutants fail",
 }
 
 _lib = None

@@ -399,7 +399,8 @@ static void FN(slice)(T *d, const T *p, const T *q, int nnx, int nnz)
 static void FN(image)(T *image, const T *ap, const T *aq, const T *s0, const T *s1, float f, int nnx, int nnz)
 {
     const T ff = (T)f;
-    INTERIOR(const T d = ((T)1 - ff) * s0[i] + ff * s1[i]; image[i] += (T)0.5 * (ap[i] + aq[i]) * d;)
+    INTERIOR(const T d = ((T)1 - SLICE_F(ff)) * s0[i] + SLICE_F(ff) * s1[i];
+             image[i] += (T)0.5 * (ap[i] + IMG2_AQ(aq[i])) * d;)
 }
 
 /* extendedImagingCrossCor, kernels.cu:425-436, over the points whose
@@ -414,8 +415,8 @@ static void FN(ximage)(T *image, const T *ap, const T *aq, const T *s0, const T 
         for (int ix = (nhx > 4 ? nhx : 4); ix < nnx - (nhx > 4 ? nhx : 4); ++ix) {
             const long i = ix + (long)iz * nnx;
             for (int h = -nhx; h <= nhx; ++h) {
-                const T d = ((T)1 - ff) * s0[i + h] + ff * s1[i + h];
-                image[i + (h + nhx) * nnxz] += (T)0.5 * (ap[i - h] + aq[i - h]) * d;
+                const T d = ((T)1 - SLICE_F(ff)) * s0[i + h] + SLICE_F(ff) * s1[i + h];
+                image[i + (h + nhx) * nnxz] += (T)0.5 * (ap[i - XIMG_SH(h)] + IMG2_AQ(aq[i - XIMG_SH(h)])) * d;
             }
         }
 }

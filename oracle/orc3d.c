@@ -408,7 +408,7 @@ static void tpower_f32(float *data, int nt, float ot, float dt, int nr, float p)
     for (int ir = 0; ir < nr; ir++)
         for (int it = 0; it < nt; it++) {
             float t = ot + it * dt;
-            t = powf(t, p);
+            t = powf(t, TPOW_EXP(p));
             data[it + (long)ir * nt] *= t;
         }
 }
@@ -417,7 +417,7 @@ static void tpower_f64(double *data, int nt, float ot, float dt, int nr, float p
 {
     for (int ir = 0; ir < nr; ir++)
         for (int it = 0; it < nt; it++) {
-            const double t = pow((double)(ot + it * dt), (double)p);
+            const double t = pow((double)(ot + it * dt), (double)TPOW_EXP(p));
             data[it + (long)ir * nt] *= t;
         }
 }
@@ -515,10 +515,10 @@ double orc3d_gradient(int mode, float *fgcij, double *fgcij64, const float *soul
         } else {
             for (long i = 0; i < (long)nnt * sh.nr; i++) {
                 if (mode == 2) {
-                    data64[i] = data64[i] - (double)ob[i];
+                    data64[i] = RESID(data64[i], (double)ob[i]);
                     objfunc += data64[i] * data64[i];
                 } else {
-                    data[i] = data[i] - ob[i];
+                    data[i] = RESID(data[i], ob[i]);
                     objfunc += (double)(data[i] * data[i]);
                 }
             }
@@ -543,8 +543,8 @@ double orc3d_gradient(int mode, float *fgcij, double *fgcij64, const float *soul
             t = cz; cz = pz; pz = t;
             for (int ir = 0; ir < sh.nr; ir++) {
                 double tt = (double)g.dt2 * data64[(nnt - 1) + (long)ir * nnt];
-                lcx[sh.rec_true[ir]] = TWOTHIRD_D * tt;
-                lcz[sh.rec_true[ir]] = ONETHIRD_D * tt;
+                lcx[sh.rec_true[ir]] = LAM0_WX * tt;
+                lcz[sh.rec_true[ir]] = LAM0_WZ * tt;
             }
             double *gc = (double *)gcv;
             backward_grad_pass_f64(&g, &sh, C64[0], C64[1], C64[2], damping, wavelet, lmax, nt, &cx, &px, &cz, &pz,
@@ -564,8 +564,8 @@ double orc3d_gradient(int mode, float *fgcij, double *fgcij64, const float *soul
             t = cz; cz = pz; pz = t;
             for (int ir = 0; ir < sh.nr; ir++) {
                 float tt = g.dt2 * data[(nnt - 1) + (long)ir * nnt];
-                lcx[sh.rec_true[ir]] = (float)(TWOTHIRD_D * tt);
-                lcz[sh.rec_true[ir]] = (float)(ONETHIRD_D * tt);
+                lcx[sh.rec_true[ir]] = (float)(LAM0_WX * tt);
+                lcz[sh.rec_true[ir]] = (float)(LAM0_WZ * tt);
             }
             float *gc = (float *)gcv;
             if (mode == 0)
@@ -644,8 +644,8 @@ int orc3d_rtm(int mode, void *image, const float *souloc, const int *shotid, int
             t = cz; cz = pz; pz = t;
             for (int ir = 0; ir < sh.nr; ir++) {
                 double tt = (double)g.dt2 * ob64[(nnt - 1) + (long)ir * nnt];
-                lcx[sh.rec_true[ir]] = TWOTHIRD_D * tt;
-                lcz[sh.rec_true[ir]] = ONETHIRD_D * tt;
+                lcx[sh.rec_true[ir]] = LAM0_WX * tt;
+                lcz[sh.rec_true[ir]] = LAM0_WZ * tt;
             }
             backward_rtm_pass_f64(&g, &sh, C64[0], C64[1], C64[2], damping, wavelet, lmax, nt, &cx, &px, &cz, &pz,
                                   &lcx, &lpx, &lcz, &lpz, (double *)image, ob64, sts, nnt);
@@ -665,8 +665,8 @@ int orc3d_rtm(int mode, void *image, const float *souloc, const int *shotid, int
             t = cz; cz = pz; pz = t;
             for (int ir = 0; ir < sh.nr; ir++) {
                 float tt = g.dt2 * ob[(nnt - 1) + (long)ir * nnt];
-                lcx[sh.rec_true[ir]] = (float)(TWOTHIRD_D * tt);
-                lcz[sh.rec_true[ir]] = (float)(ONETHIRD_D * tt);
+                lcx[sh.rec_true[ir]] = (float)(LAM0_WX * tt);
+                lcz[sh.rec_true[ir]] = (float)(LAM0_WZ * tt);
             }
             if (mode == 0)
                 backward_rtm_pass_f32(&g, &sh, c11, c13, c33, damping, wavelet, lmax, nt, &cx, &px, &cz, &pz, &lcx,

@@ -129,8 +129,8 @@ static void FN(grad_block)(const orc3d_geom *g, int ib, T *gc11, T *gc13, T *gc3
 #endif
                 const T lx = lam_x[j], lz = lam_z[j];
                 gc11[j] = gc11[j] + lx * ty;
-                gc33[j] = gc33[j] + lz * tz;
-                gc13[j] = gc13[j] + (lx * tz + lz * ty);
+                gc33[j] = gc33[j] + GC33_TERM(lz, ty, tz);
+                gc13[j] = gc13[j] + GC13_TERM(lx, lz, ty, tz);
             }
         }
 }
@@ -166,9 +166,10 @@ static void FN(image_block)(const orc3d_geom *g, int ib, T *image, const T *sx, 
                 const long j = ix + iy * nx + iz * nxy;
 #if defined(ARITH_REF)
                 image[j] = (float)((double)image[j] + (TWOTHIRD_D * sx[j] + ONETHIRD_D * sz[j]) *
-                                                          (TWOTHIRD_D * lx[j] + ONETHIRD_D * lz[j]));
+                                                          (IMG_LWX * lx[j] + IMG_LWZ * lz[j]));
 #else
-                image[j] = image[j] + (KTWOTHIRD * sx[j] + KONETHIRD * sz[j]) * (KTWOTHIRD * lx[j] + KONETHIRD * lz[j]);
+                image[j] = image[j] + (KTWOTHIRD * sx[j] + KONETHIRD * sz[j]) *
+                                          ((T)IMG_LWX * lx[j] + (T)IMG_LWZ * lz[j]);
 #endif
             }
 }
@@ -181,9 +182,9 @@ static void FN(record)(const orc3d_shot *sh, int itdata, int nnt, float *data, c
     for (int ir = 0; ir < sh->nr; ++ir) {
         long i = sh->rec_eff[ir];
 #if defined(ARITH_F64)
-        data[itdata + (long)ir * nnt] = (float)(TWOTHIRD_D * sx[i] + ONETHIRD_D * sz[i]);
+        data[itdata + (long)ir * nnt] = (float)(REC_WX * sx[i] + REC_WZ * sz[i]);
 #else
-        data[itdata + (long)ir * nnt] = (float)(TWOTHIRD_D * (double)sx[i] + ONETHIRD_D * (double)sz[i]);
+        data[itdata + (long)ir * nnt] = (float)(REC_WX * (double)sx[i] + REC_WZ * (double)sz[i]);
 #endif
     }
 }
@@ -192,7 +193,7 @@ static void FN(record64)(const orc3d_shot *sh, int itdata, int nnt, double *data
 {
     for (int ir = 0; ir < sh->nr; ++ir) {
         long i = sh->rec_eff[ir];
-        data[itdata + (long)ir * nnt] = TWOTHIRD_D * (double)sx[i] + ONETHIRD_D * (double)sz[i];
+        data[itdata + (long)ir * nnt] = REC_WX * (double)sx[i] + REC_WZ * (double)sz[i];
     }
 }
 
@@ -203,12 +204,12 @@ static void FN(inject_res)(const orc3d_shot *sh, const double *res, float dt2, T
         long i = sh->rec_eff[ir];
 #if defined(ARITH_F64)
         double s = (double)dt2 * res[ir];
-        lx[i] += TWOTHIRD_D * s;
-        lz[i] += ONETHIRD_D * s;
+        lx[i] += INJ_WX * s;
+        lz[i] += INJ_WZ * s;
 #else
         float s = dt2 * (float)res[ir];
-        lx[i] = (float)((double)lx[i] + TWOTHIRD_D * (double)s);
-        lz[i] = (float)((double)lz[i] + ONETHIRD_D * (double)s);
+        lx[i] = (float)((double)lx[i] + INJ_WX * (double)s);
+        lz[i] = (float)((double)lz[i] + INJ_WZ * (double)s);
 #endif
     }
 }
@@ -279,7 +280,7 @@ static void FN(backward_grad_pass)(const orc3d_geom *g, const orc3d_shot *sh, co
     double *fine = (double *)malloc(sizeof(double) * (sh->nr > 0 ? sh->nr : 1));
     for (int s = 2; s <= lmax; ++s) {
         int it = nt - 1 - s;
-        FN(interp_res)(sh, res, it + 1, nnt, sts, fine);
+        FN(interp_res)(sh, res, RES_TI(it), nnt, sts, fine);
         for (int ib = 0; ib < g->nb; ++ib) {
             FN(grad_block)(g, ib, gc11, gc13, gc33, *px, *cx, *px, *pz, *cz, *pz, *lcx, *lcz, c11, c13, c33);
             FN(fwd_block)(g, ib, *lpx, *lcx, *lpx, *lpz, *lcz, *lpz, c11, c13, c33);
@@ -309,7 +310,7 @@ static void FN(backward_rtm_pass)(const orc3d_geom *g, const orc3d_shot *sh, con
     double *fine = (double *)malloc(sizeof(double) * (sh->nr > 0 ? sh->nr : 1));
     for (int s = 2; s <= lmax; ++s) {
         int it = nt - 1 - s;
-        FN(interp_res)(sh, obs, it + 1, nnt, sts, fine);
+        FN(interp_res)(sh, obs, RES_TI(it), nnt, sts, fine);
         for (int ib = 0; ib < g->nb; ++ib) {
             FN(fwd_block)(g, ib, *px, *cx, *px, *pz, *cz, *pz, c11, c13, c33);
             FN(fwd_block)(g, ib, *lpx, *lcx, *lpx, *lpz, *lcz, *lpz, c11, c13, c33);
