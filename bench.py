@@ -582,6 +582,8 @@ def main():
             raise
         traceback.print_exc(file=sys.stderr)
     print(json.dumps(out), flush=True)
+    if failed is not None:
+        sys.exit(3)  # the headline is in the line; a section failed
 
 
 if __name__ == "__main__":

@@ -136,6 +136,46 @@ inline long long receivers_used(const float *souloc, const std::vector<int> &sho
     return n;
 }
 
+// The reference's split of a run over GPU groups (objFuncGradient3d_f.cu:311-335,
+// modelData3d_f.cu:396-420): group i gets GPUs [i*g, (i+1)*g), g =
+// |GPUs|/ngpugroup, and the shots in order, the first |shotid| % ngpugroup
+// groups one more than the others.  Here the groups run one after another on
+// the one device; g only sets each group's pipeline length.
+struct Group {
+    std::vector<int> shots, gpus;
+};
+inline std::vector<Group> groups(const std::vector<int> &shotid, const std::vector<int> &GPUs, int ngpugroup)
+{
+    std::vector<Group> out;
+    if (ngpugroup < 1) ngpugroup = 1;
+    const size_t g = GPUs.size() / (size_t)ngpugroup;
+    const size_t n1 = shotid.size() / (size_t)ngpugroup, r = shotid.size() % (size_t)ngpugroup;
+    size_t s0 = 0;
+    for (size_t i = 0; i < (size_t)ngpugroup; ++i) {
+        Group gr;
+        const size_t ns = n1 + (i < r ? 1 : 0);
+        gr.shots.assign(shotid.begin() + (long)s0, shotid.begin() + (long)(s0 + ns));
+        gr.gpus.assign(GPUs.begin() + (long)(i * g), GPUs.begin() + (long)((i + 1) * g));
+        s0 += ns;
+        out.push_back(gr);
+    }
+    return out;
+}
+
+// zeroBoundary on each of n fields, then the water layer: rows
+// [npad, npad + (int)((wbottom - oz)/dz + 1) - npad) of every field
+// (objFuncGradient3d_f.cu:276-282, rtm3d_f.cu:153-155)
+inline void mask_shell_and_water(float **f, int n, int nx, int ny, int nz, int npad, float oz, float dz, float wbottom)
+{
+    const long long nxy = (long long)nx * ny;
+    const int nw = (int)((wbottom - oz) / dz + 1) - npad;
+    for (int k = 0; k < n; ++k) {
+        report(vti_zero_boundary(f[k], nx, ny, nz, npad), "zeroBoundary");
+        if (nw > 0)
+            for (long long i = 0; i < (long long)nw * nxy && npad * nxy + i < nxy * nz; ++i) f[k][npad * nxy + i] = 0.f;
+    }
+}
+
 inline int nnt3d(int nt, float dt, float samplingRate)
 {
     return (nt - 1) / (int)std::round(samplingRate / dt) + 1;
@@ -214,6 +254,27 @@ void objFuncGradientCij3d_f(float *fgcij, float *souloc, int ns, std::vector<int
 void rtmCij3d_f(float *image, float *souloc, int ns, std::vector<int> &shotid, float *recloc, float *wavelet,
                 float *c11, float *c13, float *c33, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
                 float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate);
+void rtmVEpsDel(float *image, float *souloc, int ns, std::vector<int> &shotid, float *recloc, float *wavelet, float *v,
+                float *eps, float *del, float *randboundaryV, int nx, int ny, int nz, int nt, int npad, float ox,
+                float oy, float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate,
+                float wbottom);
+void objFuncGradientCij3d(float *fgcij, float *souloc, int ns, std::vector<int> &shotid, float *recloc,
+                          float *wavelet, float *cij, float *padboundaryCij, float *randboundaryCij, int nx, int ny,
+                          int nz, int nt, int npad, float ox, float oy, float oz, float ot, float dx, float dy,
+                          float dz, float dt, float samplingRate, std::vector<int> &GPUs, int ngpugroup);
+void objFuncCij3d(float *f, float *souloc, int ns, std::vector<int> &shotid, float *recloc, float *wavelet,
+                  float *cij, float *padboundaryCij, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
+                  float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate,
+                  std::vector<int> &GPUs, int ngpugroup);
+double objFuncGradientVEpsDel(float *gv, float *geps, float *gdel, float *souloc, int ns, std::vector<int> &shotid,
+                              float *recloc, float *wavelet, float *v, float *eps, float *del, float *padboundaryV,
+                              float *randboundaryV, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
+                              float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate,
+                              float v0, float eps0, float wbottom, std::vector<int> &GPUs, int ngpugroup);
+void modelData3d_f(float *ddata, float *souloc, int ns, std::vector<int> &shotid, float *recloc,
+                   const float *wavelet, float *c11, float *c13, float *c33, int nx, int ny, int nz, int nt, int npad,
+                   float ox, float oy, float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate,
+                   std::vector<int> &GPUs, int ngpugroup);
 
 // lib/lbfgs/lbfgs.h:32
 #ifndef LBFGS_H
@@ -417,6 +478,124 @@ void rtmCij3d_f(float *image, float *souloc, int ns, std::vector<int> &shotid, f
                                  wavelet, c11, c13, c33, nx, ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz, dt,
                                  samplingRate, vti_compat::ngpu(), obs),
                        "rtmCij3d_f");
+}
+
+// rtmVEpsDel (wave3d.h:90, rtm3d_f.cu:139): v/eps/del to Cij with unit
+// scales, rtmCij3d_f, then the boundary shell and the water layer of the
+// image zeroed.  randboundaryV is not read (the reference's putBoundary of it
+// is commented out, rtm3d_f.cu:148).
+void rtmVEpsDel(float *image, float *souloc, int ns, std::vector<int> &shotid, float *recloc, float *wavelet, float *v,
+                float *eps, float *del, float *randboundaryV, int nx, int ny, int nz, int nt, int npad, float ox,
+                float oy, float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate,
+                float wbottom)
+{
+    (void)randboundaryV;
+    const long long n = (long long)nx * ny * nz;
+    std::vector<float> c(3 * (size_t)n);
+    vti_compat::report(vti_vepsdel_to_cij(c.data(), c.data() + n, c.data() + 2 * n, v, eps, del, 1.f, 1.f, 1.f, n),
+                       "rtmVEpsDel");
+    rtmCij3d_f(image, souloc, ns, shotid, recloc, wavelet, c.data(), c.data() + n, c.data() + 2 * n, nx, ny, nz, nt,
+               npad, ox, oy, oz, ot, dx, dy, dz, dt, samplingRate);
+    float *f[1] = {image};
+    vti_compat::mask_shell_and_water(f, 1, nx, ny, nz, npad, oz, dz, wbottom);
+}
+
+// objFuncGradientCij3d (wave3d.h:94, objFuncGradient3d_f.cu:307): with one
+// group, objFuncGradientCij3d_f itself (cij's shell left holding the random
+// boundary); with more, each group on its own copy of cij (the caller's cij
+// untouched) and the groups' {f, g} summed in FP32, last group first.
+void objFuncGradientCij3d(float *fgcij, float *souloc, int ns, std::vector<int> &shotid, float *recloc,
+                          float *wavelet, float *cij, float *padboundaryCij, float *randboundaryCij, int nx, int ny,
+                          int nz, int nt, int npad, float ox, float oy, float oz, float ot, float dx, float dy,
+                          float dz, float dt, float samplingRate, std::vector<int> &GPUs, int ngpugroup)
+{
+    if (ngpugroup <= 1) {
+        objFuncGradientCij3d_f(fgcij, souloc, ns, shotid, recloc, wavelet, cij, padboundaryCij, randboundaryCij, nx,
+                               ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz, dt, samplingRate, GPUs);
+        return;
+    }
+    const size_t nn = 3 * (size_t)nx * ny * nz;
+    std::vector<vti_compat::Group> gs = vti_compat::groups(shotid, GPUs, ngpugroup);
+    std::vector<std::vector<float>> fg(gs.size());
+    for (size_t i = 0; i < gs.size(); ++i) {
+        std::vector<float> tc(cij, cij + nn);
+        fg[i].assign(nn + 1, 0.f);
+        objFuncGradientCij3d_f(fg[i].data(), souloc, ns, gs[i].shots, recloc, wavelet, tc.data(), padboundaryCij,
+                               randboundaryCij, nx, ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz, dt, samplingRate,
+                               gs[i].gpus);
+    }
+    for (size_t j = 0; j < nn + 1; ++j) fgcij[j] = 0.f;
+    for (size_t i = gs.size(); i-- > 0;)
+        for (size_t j = 0; j < nn + 1; ++j) fgcij[j] += fg[i][j];
+}
+
+// objFuncCij3d (wave3d.h:98, objFuncGradient3d_f.cu:1233): *f gains each
+// group's objective, first group first (*f is not cleared, as in the reference)
+void objFuncCij3d(float *f, float *souloc, int ns, std::vector<int> &shotid, float *recloc, float *wavelet,
+                  float *cij, float *padboundaryCij, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
+                  float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate,
+                  std::vector<int> &GPUs, int ngpugroup)
+{
+    std::vector<vti_compat::Group> gs = vti_compat::groups(shotid, GPUs, ngpugroup);
+    std::vector<float> objf(gs.size(), 0.f);
+    for (size_t i = 0; i < gs.size(); ++i)
+        objFuncCij3d_f(&objf[i], souloc, ns, gs[i].shots, recloc, wavelet, cij, padboundaryCij, nx, ny, nz, nt, npad,
+                       ox, oy, oz, ot, dx, dy, dz, dt, samplingRate, gs[i].gpus);
+    for (size_t i = 0; i < gs.size(); ++i) *f += objf[i];
+}
+
+// objFuncGradientVEpsDel, 3D (wave3d.h:102, objFuncGradient3d_f.cu:249): the
+// model and both boundary shells to Cij (v0, eps0 and unit delta scale), the
+// Cij gradient (grouped when ngpugroup > 1), its shell and water layer
+// zeroed, the chain rule to (v, eps, del).  Returns the objective f of fgcij.
+double objFuncGradientVEpsDel(float *gv, float *geps, float *gdel, float *souloc, int ns, std::vector<int> &shotid,
+                              float *recloc, float *wavelet, float *v, float *eps, float *del, float *padboundaryV,
+                              float *randboundaryV, int nx, int ny, int nz, int nt, int npad, float ox, float oy,
+                              float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate,
+                              float v0, float eps0, float wbottom, std::vector<int> &GPUs, int ngpugroup)
+{
+    const long long nxyz = (long long)nx * ny * nz;
+    const long long nb = vti_boundary_size(nx, ny, nz, npad);
+    std::vector<float> cij(3 * (size_t)nxyz), fgcij(3 * (size_t)nxyz + 1, 0.f);
+    std::vector<float> beps((size_t)nb), bdel((size_t)nb), padc(3 * (size_t)nb), randc(3 * (size_t)nb);
+    vti_compat::report(vti_get_boundary(beps.data(), eps, nx, ny, nz, npad), "objFuncGradientVEpsDel");
+    vti_compat::report(vti_get_boundary(bdel.data(), del, nx, ny, nz, npad), "objFuncGradientVEpsDel");
+    float *c = cij.data();
+    vti_compat::report(vti_vepsdel_to_cij(c, c + nxyz, c + 2 * nxyz, v, eps, del, v0, eps0, 1.f, nxyz),
+                       "objFuncGradientVEpsDel");
+    vti_compat::report(vti_vepsdel_to_cij(padc.data(), padc.data() + nb, padc.data() + 2 * nb, padboundaryV,
+                                          beps.data(), bdel.data(), v0, eps0, 1.f, nb),
+                       "objFuncGradientVEpsDel");
+    vti_compat::report(vti_vepsdel_to_cij(randc.data(), randc.data() + nb, randc.data() + 2 * nb, randboundaryV,
+                                          beps.data(), bdel.data(), v0, eps0, 1.f, nb),
+                       "objFuncGradientVEpsDel");
+    if (ngpugroup > 1)
+        objFuncGradientCij3d(fgcij.data(), souloc, ns, shotid, recloc, wavelet, c, padc.data(), randc.data(), nx, ny,
+                             nz, nt, npad, ox, oy, oz, ot, dx, dy, dz, dt, samplingRate, GPUs, ngpugroup);
+    else
+        objFuncGradientCij3d_f(fgcij.data(), souloc, ns, shotid, recloc, wavelet, c, padc.data(), randc.data(), nx,
+                               ny, nz, nt, npad, ox, oy, oz, ot, dx, dy, dz, dt, samplingRate, GPUs);
+    const double objfunc = fgcij[0];
+    float *g = fgcij.data() + 1;
+    float *gf[3] = {g, g + nxyz, g + 2 * nxyz};
+    vti_compat::mask_shell_and_water(gf, 3, nx, ny, nz, npad, oz, dz, wbottom);
+    vti_compat::report(vti_gradcij_to_gradvepsdel(gv, geps, gdel, gf[0], gf[1], gf[2], v, eps, del, v0, eps0, 1.f,
+                                                  nxyz),
+                       "objFuncGradientVEpsDel");
+    return objfunc;
+}
+
+// modelData3d_f, group form (wave3d.h:108, modelData3d_f.cu:395): each shot
+// group's modelDataCij3d_f with that group's GPUs
+void modelData3d_f(float *ddata, float *souloc, int ns, std::vector<int> &shotid, float *recloc,
+                   const float *wavelet, float *c11, float *c13, float *c33, int nx, int ny, int nz, int nt, int npad,
+                   float ox, float oy, float oz, float ot, float dx, float dy, float dz, float dt, float samplingRate,
+                   std::vector<int> &GPUs, int ngpugroup)
+{
+    std::vector<vti_compat::Group> gs = vti_compat::groups(shotid, GPUs, ngpugroup);
+    for (size_t i = gs.size(); i-- > 0;)
+        modelDataCij3d_f(ddata, souloc, ns, gs[i].shots, recloc, wavelet, c11, c13, c33, nx, ny, nz, nt, npad, ox, oy,
+                         oz, ot, dx, dy, dz, dt, samplingRate, gs[i].gpus);
 }
 
 #endif  // VTI_COMPAT_IMPLEMENTATION

@@ -154,3 +154,25 @@ def test_damping3d_closed_form(vti, orc):
     assert np.array_equal(d[: nx * ny], want)
     assert np.array_equal(d[nx * ny:], prof(nz))
     assert d[nx * ny + nz // 2] == 1.0  # exactly one in the z interior
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(8192, 9000, 16), (8192, 8200, 12), (9000, 9000, 40), (572, 572, 460),
+                                      (163, 140, 128), (2048, 2048, 8)])
+def test_zchunk_stays_within_32bit_offsets(vti, nx, ny, nz):
+    """The 3D kernels address a chunk's planes plus a 4-plane halo on each side
+    with 32-bit byte offsets from the chunk's first plane.  x-y planes of
+    67M-89.5M points allow only 4-plane chunks; the small-grid rule (at least
+    8 planes) must not override that."""
+    pitch = (nx + 63) // 64 * 64
+    plane_bytes = 4 * pitch * ny
+    for requested, fixed in ((64, 0), (8, 0), (64, 1), (32, 1)):
+        zc = vti.lib.vti3d_zchunk_for(nx, ny, nz, requested, fixed)
+        assert zc >= 4 and zc % 4 == 0
+        assert (zc + 8) * plane_bytes <= 0xFFFFFFFF, (nx, ny, zc)
+    if plane_bytes * 12 > 0xFFFFFFFF:
+        assert vti.lib.vti3d_zchunk_for(nx, ny, nz, 64, 0) == 4
+    assert vti.lib.vti3d_zchunk_for(572, 572, 460, 64, 0) == 64  # the full 460x572x572 grid keeps 64
+
+
+def test_zchunk_plane_too_large(vti):
+    assert vti.lib.vti3d_zchunk_for(16384, 16384, 16, 64, 0) == 0

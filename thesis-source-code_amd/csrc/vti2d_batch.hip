@@ -870,17 +870,21 @@ bool grad2d_batchable(const int *sloc, int ns, const int *rloc, int nr, const fl
     std::vector<float> t((size_t)nnx * nnz);
     vti::init_taper2d(t.data(), nx, nz, npad);
     if (std::memcmp(t.data(), taper, t.size() * sizeof(float)) != 0) return false;
-    std::vector<char> seen;
+    // one receiver per grid point per shot: the shot's linear receiver
+    // indices, sorted, have no repeats (O(nr log nr) per shot)
+    std::vector<long long> idx;
     for (int k = 0; k < ns; ++k) {
         const int ix = sloc[4 * k], iz = sloc[4 * k + 1], nri = sloc[4 * k + 2], r0 = sloc[4 * k + 3];
         if (nri < 0 || r0 < 0 || r0 + nri > nr) return false;
         if (ix < 4 || ix >= nnx - 4 || iz < 4 || iz >= nnz - 4) return false;
-        seen.assign((size_t)nnx * nnz, 0);
+        idx.clear();
         for (int ir = 0; ir < nri; ++ir) {
             const int x = rloc[2 * (r0 + ir)], z = rloc[2 * (r0 + ir) + 1];
-            if (x < 4 || x >= nnx - 4 || z < 4 || z >= nnz - 4 || seen[x + (size_t)z * nnx]) return false;
-            seen[x + (size_t)z * nnx] = 1;
+            if (x < 4 || x >= nnx - 4 || z < 4 || z >= nnz - 4) return false;
+            idx.push_back(x + (long long)z * nnx);
         }
+        std::sort(idx.begin(), idx.end());
+        if (std::adjacent_find(idx.begin(), idx.end()) != idx.end()) return false;
     }
     return true;
 }

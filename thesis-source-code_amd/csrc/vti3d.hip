@@ -184,6 +184,27 @@ int max_zchunk(const Geo3 &g)
     return (int)std::min<long long>(1 << 20, (planes - 8) / 4 * 4);
 }
 
+// z-chunk of a launch: `requested` (64 by default) on a large grid; on a
+// small grid a shorter chunk, so that a launch still has about 2048
+// workgroups (8 per CU) to spread over the 256 CUs -- each chunk re-reads a
+// 4-plane halo of sigma_z on either side, which the parallelism pays for
+// below 64 planes -- but never below 8 planes.  `fixed` keeps `requested`.
+// The result is always a multiple of 4 within [4, max_zchunk(g)], so a
+// chunk's planes and halo stay within 32-bit byte offsets; 0 when even 4
+// planes do not.
+int choose_zchunk(const Geo3 &g, int requested, bool fixed)
+{
+    const int cap = max_zchunk(g);
+    if (cap < 4) return 0;
+    long long zc = std::max(4, requested / 4 * 4);
+    if (!fixed) {
+        const long long cols = (long long)((g.nx - 4 + 63) / 64) * ((g.ny - 8 + kBY - 1) / kBY);
+        const long long fill = g.zmax * cols / 2048 / 4 * 4;
+        zc = std::max<long long>(8, std::min<long long>(zc, fill));
+    }
+    return (int)std::min<long long>(zc, cap);
+}
+
 void freep(void *p, hipStream_t s) { vti::dfree(p, s); }
 
 // stream-ordered allocation on the propagator's stream, returning on failure
@@ -604,20 +625,11 @@ int vti3d_prop_create(vti_ctx *ctx, vti3d_prop **out, int nx, int ny, int nz, in
     p->dt2dx2 = p->dt2 / p->dx2;
     p->dt2dy2 = p->dt2 / p->dy2;
     p->dt2dz2 = p->dt2 / p->dz2;
-    if (const char *s = getenv("VTI_ZCHUNK")) p->zc = std::max(4, atoi(s) / 4 * 4);
-    if (max_zchunk(g) < 4) {
+    const char *zenv = getenv("VTI_ZCHUNK");
+    p->zc = choose_zchunk(g, zenv ? atoi(zenv) : p->zc, zenv != nullptr);
+    if (p->zc == 0) {
         delete p;
         VTI_CHECK(false, "vti3d_prop_create: x-y plane %dx%d too large for 32-bit chunk offsets", nx, ny);
-    }
-    p->zc = std::min(p->zc, max_zchunk(g));
-    if (!getenv("VTI_ZCHUNK")) {
-        // a small grid gets a shorter z-chunk, so that a launch still has
-        // about 2048 workgroups (8 per CU) to spread over the 256 CUs; each
-        // chunk re-reads a 4-plane halo of sigma_z on either side, which the
-        // parallelism pays for below 64 planes
-        const long long cols = (long long)((g.nx - 4 + 63) / 64) * ((g.ny - 8 + kBY - 1) / kBY);
-        const long long zc = g.zmax * cols / 2048 / 4 * 4;
-        p->zc = (int)std::max<long long>(8, std::min<long long>(p->zc, zc));
     }
     const size_t nv = (size_t)vox(g);
     int rc = vti::kOk;
@@ -653,6 +665,19 @@ int vti3d_prop_set_arith(vti3d_prop *p, int arith)
     VTI_CHECK(p && (arith == 0 || arith == 1), "vti3d_prop_set_arith: arith must be 0 (fp32) or 1 (reference)");
     p->arith = arith;
     return vti::kOk;
+}
+
+// The z-chunk vti3d_prop_create picks for a grid (host only; for tests).
+int vti3d_zchunk_for(int nx, int ny, int nz, int requested, int fixed)
+{
+    Geo3 g{};
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.zmax = 4 * (nz / 4);
+    g.pitch = (nx + 63) / 64 * 64;
+    g.plane = (long long)g.pitch * ny;
+    return choose_zchunk(g, requested, fixed != 0);
 }
 
 int vti3d_prop_set_zchunk(vti3d_prop *p, int zc)

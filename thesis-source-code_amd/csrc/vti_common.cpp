@@ -10,6 +10,7 @@
 
 #include "vti_ctx.h"
 #include "vti_internal.h"
+#include "vti_pool.h"
 
 namespace vti {
 
@@ -29,44 +30,56 @@ namespace {
 std::mutex g_pool_mu;
 bool g_pool_ready[256];
 
-// Large blocks (>= kBig) are kept for reuse within the process instead of
-// going back to the runtime: a fresh multi-gigabyte allocation can stall for
-// hundreds of milliseconds once earlier frees have piled up (the freed memory
-// is cleared before it is handed out again), which made the drivers' calls
-// in an inversion loop three to seven times slower after a few seconds.
-// A freed block carries an event recorded on the stream that freed it; the
-// stream that takes it next waits for that event, so reuse stays stream-ordered.
+// Blocks of at least kBig bytes are kept for reuse (vti_pool.h); smaller
+// ones come from the device's stream-ordered pool.
 constexpr size_t kBig = size_t(64) << 20;
-struct Cached {
-    void *p;
-    size_t bytes;
-    int dev;
-    hipEvent_t ev;
-};
-std::vector<Cached> g_cached;                       // free large blocks
-std::unordered_map<void *, std::pair<size_t, int>> g_big;  // live large blocks: bytes, device
 
-// Release every cached block of `dev` (after its stream work is done).
-void flush_cached(int dev)
-{
-    for (size_t k = 0; k < g_cached.size();) {
-        if (g_cached[k].dev == dev) {
-            (void)hipEventSynchronize(g_cached[k].ev);
-            (void)hipEventDestroy(g_cached[k].ev);
-            (void)hipFree(g_cached[k].p);
-            g_cached[k] = g_cached.back();
-            g_cached.pop_back();
-        } else {
-            ++k;
-        }
+struct HipRT {
+    using Stream = hipStream_t;
+    using Event = hipEvent_t;
+    static bool malloc(void **p, size_t bytes)
+    {
+        if (hipMalloc(p, bytes) == hipSuccess) return true;
+        (void)hipGetLastError();
+        *p = nullptr;
+        return false;
     }
+    static void free(void *p) { (void)hipFree(p); }
+    static bool event_create(Event *e)
+    {
+        if (hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess) return true;
+        (void)hipGetLastError();
+        return false;
+    }
+    static bool event_record(Event e, Stream s)
+    {
+        if (hipEventRecord(e, s) == hipSuccess) return true;
+        (void)hipGetLastError();
+        return false;
+    }
+    static void event_sync(Event e) { (void)hipEventSynchronize(e); }
+    static void event_destroy(Event e) { (void)hipEventDestroy(e); }
+    static void stream_wait(Stream s, Event e) { (void)hipStreamWaitEvent(s, e, 0); }
+    static void stream_sync(Stream s) { (void)hipStreamSynchronize(s); }
+};
+
+BigPool<HipRT> &big_pool()
+{
+    static BigPool<HipRT> *pool = new BigPool<HipRT>();  // outlives static destructors
+    return *pool;
+}
+
+int current_device()
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    return dev;
 }
 }  // namespace
 
 int dmalloc(void **p, size_t bytes, hipStream_t s)
 {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
+    const int dev = current_device();
     if (dev >= 0 && dev < 256) {
         std::lock_guard<std::mutex> lk(g_pool_mu);
         if (!g_pool_ready[dev]) {
@@ -79,48 +92,15 @@ int dmalloc(void **p, size_t bytes, hipStream_t s)
         }
     }
     if (bytes >= kBig) {
-        std::lock_guard<std::mutex> lk(g_pool_mu);
-        // the smallest cached block that holds the request without wasting
-        // more than half of itself
-        size_t best = g_cached.size();
-        for (size_t k = 0; k < g_cached.size(); ++k) {
-            const Cached &c = g_cached[k];
-            if (c.dev == dev && c.bytes >= bytes && c.bytes / 2 <= bytes &&
-                (best == g_cached.size() || c.bytes < g_cached[best].bytes))
-                best = k;
-        }
-        if (best < g_cached.size()) {
-            const Cached c = g_cached[best];
-            g_cached[best] = g_cached.back();
-            g_cached.pop_back();
-            (void)hipStreamWaitEvent(s, c.ev, 0);
-            (void)hipEventDestroy(c.ev);
-            *p = c.p;
-            g_big[c.p] = {c.bytes, dev};
-            return kOk;
-        }
-        hipError_t e = hipMalloc(p, bytes);
-        if (e != hipSuccess) {  // give the cached blocks back and try once more
-            (void)hipGetLastError();
-            flush_cached(dev);
-            e = hipMalloc(p, bytes);
-        }
-        if (e != hipSuccess) {
-            *p = nullptr;
-            (void)hipGetLastError();
-            set_error("hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
-            return kErrNoMem;
-        }
-        g_big[*p] = {bytes, dev};
-        return kOk;
+        if (big_pool().take(p, bytes, dev, s) >= 0) return kOk;
+        *p = nullptr;
+        set_error("hipMalloc(%zu bytes): out of device memory", bytes);
+        return kErrNoMem;
     }
     hipError_t e = hipMallocAsync(p, bytes ? bytes : 1, s);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        {
-            std::lock_guard<std::mutex> lk(g_pool_mu);
-            flush_cached(dev);
-        }
+        big_pool().flush(dev);
         e = hipMallocAsync(p, bytes ? bytes : 1, s);
     }
     if (e != hipSuccess) {
@@ -134,39 +114,22 @@ int dmalloc(void **p, size_t bytes, hipStream_t s)
 
 size_t device_free_bytes()
 {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
+    const int dev = current_device();
     size_t mfree = 0, mtotal = 0;
     if (hipMemGetInfo(&mfree, &mtotal) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
-    std::lock_guard<std::mutex> lk(g_pool_mu);
-    for (const Cached &c : g_cached)
-        if (c.dev == dev) mfree += c.bytes;
-    return mfree;
+    return mfree + big_pool().cached_bytes(dev);
 }
 
 void dfree(void *p, hipStream_t s)
 {
     if (!p) return;
-    {
-        std::lock_guard<std::mutex> lk(g_pool_mu);
-        auto it = g_big.find(p);
-        if (it != g_big.end()) {
-            Cached c{p, it->second.first, it->second.second, nullptr};
-            g_big.erase(it);
-            if (hipEventCreateWithFlags(&c.ev, hipEventDisableTiming) == hipSuccess &&
-                hipEventRecord(c.ev, s) == hipSuccess) {
-                g_cached.push_back(c);
-            } else {
-                (void)hipGetLastError();
-                if (c.ev) (void)hipEventDestroy(c.ev);
-                (void)hipStreamSynchronize(s);
-                (void)hipFree(p);
-            }
-            return;
-        }
+    bool misuse = false;
+    if (big_pool().give(p, s, &misuse)) {
+        if (misuse) std::fprintf(stderr, "libvtiamd: device block %p released twice (ignored)\n", p);
+        return;
     }
     (void)hipFreeAsync(p, s);
 }

@@ -656,12 +656,17 @@ class Server:
         check(_lib.vti_serve_create(ctypes.byref(h), device, slots, float(budget_bytes)), "vti_serve_create")
         self.h = h
         self._models = {}
+        # the host arrays of every request not yet waited for: the server reads
+        # its inputs and writes its outputs until the request is done, so they
+        # stay alive here even if the caller drops the Ticket
+        self._pending = {}
         check(_lib.vti_serve_set_batching(self.h, batch), "vti_serve_set_batching")
 
     def close(self):
         if self.h:
-            check(_lib.vti_serve_destroy(self.h), "vti_serve_destroy")
+            check(_lib.vti_serve_destroy(self.h), "vti_serve_destroy")  # drains the queue
             self.h = None
+            self._pending.clear()
 
     def __enter__(self):
         return self
@@ -693,6 +698,7 @@ class Server:
         check(_lib.vti_serve_grad2d(self.h, ctypes.byref(rid), model, ptr(f), ptr(gc[0]), ptr(gc[1]), ptr(gc[2]),
                                     ptr(data), ptr(wavelet), ptr(sloc), sloc.size // 4, ptr(rloc), rloc.size // 2,
                                     ptr(taper), nx, nz, nt, npad, dx, dz, dt, rate), "vti_serve_grad2d")
+        self._pending[rid.value] = ((f, gc), (data, wavelet, taper, sloc, rloc))
         return Ticket(rid.value, "grad2d", (f, gc), (data, wavelet, taper, sloc, rloc))
 
     def model3d(self, model, souloc, shotid, recloc, wavelet, shape, nt, npad, o, d, dt, rate):
@@ -708,12 +714,15 @@ class Server:
         check(_lib.vti_serve_model3d(self.h, ctypes.byref(rid), model, ptr(dd), ptr(souloc), souloc.size // 5,
                                      ptr(shotid), shotid.size, ptr(recloc), nrt, ptr(wavelet), nx, ny, nz, nt, npad,
                                      *o, 0.0, *d, dt, rate), "vti_serve_model3d")
+        self._pending[rid.value] = (dd, (souloc, shotid, recloc, wavelet))
         return Ticket(rid.value, "model3d", dd, (souloc, shotid, recloc, wavelet))
 
     def wait(self, ticket):
         """Block until the request is done: (result, latency_s, service_s)."""
         lat, svc = ctypes.c_double(), ctypes.c_double()
-        check(_lib.vti_serve_wait(self.h, ticket.id, ctypes.byref(lat), ctypes.byref(svc)), "vti_serve_wait")
+        rc = _lib.vti_serve_wait(self.h, ticket.id, ctypes.byref(lat), ctypes.byref(svc))
+        self._pending.pop(ticket.id, None)  # the request is done, whatever its status
+        check(rc, "vti_serve_wait")
         if ticket.kind == "grad2d":
             f, gc = ticket.out
             res = (float(f[0]), gc)
