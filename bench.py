@@ -66,37 +66,106 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
-def cpu_baseline(ved, npad, dx, dt, budget_s=15.0):
-    """The reference's CPU path (laplacian3d.ispc forward, c_ij formed per point
-    from v/eps/del, in the loop of vtifd3d8o-modeling.cpp with its abc),
-    restated in C/OpenMP in the oracle, timed on the same grid for a bounded
-    number of steps on the host's cores."""
+def cpu_baseline(ved, npad, dx, dt, budget_s=6.0):
+    """The reference's CPU path restated in C/OpenMP (the oracle's cpu3d.c and
+    its 2D loop restatement), timed in this process on the host threads this
+    job is given (OMP_NUM_THREADS; the box gives one GPU's job 16), on the
+    same grids as the GPU legs:
+    - 3D forward step (laplacian3d.ispc forward + the modeling loop and abc
+      of vtifd3d8o-modeling.cpp) on the full 460x572x572 grid, >= 100 steps:
+      the headline CPU figure;
+    - 3D gradient step (gradientCij + forward on the adjoint + abc,
+      vtifd3d8o-objfuncGrad.cpp:322-367), a window on the full grid;
+    - 2D cfg1 modeling (synthetic_f, 256x256 + 40, 2000 steps) and cfg2 shot
+      gradient (objFuncGradient_f, 512x1024 + 40, 3000 steps), both in full;
+    - cfg4: the per-shot gradient at the mean aperture (a 500-step window,
+      scaled to 2500 steps) times 256 shots."""
     import oracle
+    from vtiamd import workloads as W
 
     v, e, d = ved
     nz, ny, nx = v.shape
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    legs = {}
     rng = np.random.default_rng(0)
     st = [np.zeros((nz, ny, nx), np.float32) for _ in range(4)]
     st[1][:] = rng.standard_normal((nz, ny, nx), dtype=np.float32) * 1e-3
-    t0 = time.perf_counter()
-    oracle.cpu3d_steps(0, st, v, e, d, npad, dx, dx, dx, dt, 1)
-    one = time.perf_counter() - t0
-    n = max(2, min(100, int(budget_s / max(one, 1e-3))))
+    oracle.cpu3d_steps(0, st, v, e, d, npad, dx, dx, dx, dt, 1)  # first touch
+    n = 100
     t0 = time.perf_counter()
     oracle.cpu3d_steps(0, st, v, e, d, npad, dx, dx, dx, dt, n)
     el = time.perf_counter() - t0
     gpts = nz * ny * nx * n / el / 1e9
+    legs["3d_forward_step"] = {"gpts_s": round(gpts, 4), "steps": n, "s": round(el, 2)}
+    lam = [np.zeros((nz, ny, nx), np.float32) for _ in range(4)]
+    lam[1][:] = rng.standard_normal((nz, ny, nx), dtype=np.float32) * 1e-6
+    gc = [np.zeros((nz, ny, nx), np.float32) for _ in range(3)]
+    oracle.cpu3d_backward_steps(st, lam, gc, v, e, d, npad, dx, dx, dx, dt, 1)
+    t0 = time.perf_counter()
+    nb = 20
+    oracle.cpu3d_backward_steps(st, lam, gc, v, e, d, npad, dx, dx, dx, dt, nb)
+    elb = time.perf_counter() - t0
+    legs["3d_gradient_step"] = {"gpts_s": round(nz * ny * nx * nb / elb / 1e9, 4), "steps": nb, "s": round(elb, 2)}
+    del st, lam, gc
+
+    # 2D cfg1: synthetic_f in full
+    nx1, nz1, np1, nt1 = 256, 256, 40, 2001
+    nnx1, nnz1 = nx1 + 2 * np1, nz1 + 2 * np1
+    v1 = 1500.0 + 2500.0 * np.arange(nnz1, dtype=np.float32)[:, None] / nnz1 * np.ones((1, nnx1), np.float32)
+    c1 = (v1 * v1).astype(np.float32)
+    taper1 = _taper2d(nx1, nz1, np1)
+    w1 = W.ricker(nt1, 1e-3, 15.0, 0.1)
+    r1 = np.array([[np1 + i, np1 + 1] for i in range(nx1)], np.int32)
+    s1 = np.array([[np1 + 128, np1 + 20, nx1, 0]], np.int32)
+    t0 = time.perf_counter()
+    oracle.synthetic2d(oracle.F32, c1 * np.float32(1.2), c1 * np.float32(1.05), c1, w1, s1, r1, taper1, nx1, nz1,
+                       nt1, np1, 10.0, 10.0, 1e-3, 4e-3)
+    el1 = time.perf_counter() - t0
+    legs["2d_cfg1_synthetic_full"] = {"s": round(el1, 2), "shots_per_s": round(1 / el1, 3),
+                                      "gpts_s": round(nnx1 * nnz1 * (nt1 - 1) / el1 / 1e9, 4)}
+    # 2D cfg2: one shot gradient in full
+    P = W.cfg2()
+    ci = W.cij(*P["init"])
+    taper2 = _taper2d(P["nx"], P["nz"], P["npad"])
+    sl = P["sloc"][16:17]
+    nnt = (P["nt"] - 1) // 4 + 1
+    obs = np.zeros((nnt, P["nx"]), np.float32)
+    t0 = time.perf_counter()
+    oracle.fg2d(oracle.F32, False, obs, *ci, None, P["w"], sl, P["rloc"], taper2, P["nx"], P["nz"], P["nt"],
+                P["npad"], P["dx"], P["dz"], P["dt"], P["rate"])
+    el2 = time.perf_counter() - t0
+    legs["2d_cfg2_shot_gradient_full"] = {"s": round(el2, 2), "shot_gradients_per_s": round(1 / el2, 4),
+                                          "call_of_32_shots_s": round(32 * el2, 1)}
+    # cfg4: one shot at the mean aperture (600 columns), 500-step window, x 256 shots
+    nx4, nt4w = 600, 501
+    c4 = [np.ascontiguousarray(a[:, :nx4 + 2 * P["npad"]]) for a in ci]
+    taper4 = _taper2d(nx4, P["nz"], P["npad"])
+    r4 = np.array([[P["npad"] + i, P["npad"] + 1] for i in range(nx4)], np.int32)
+    s4 = np.array([[P["npad"] + nx4 // 2, P["npad"] + 2, nx4, 0]], np.int32)
+    t0 = time.perf_counter()
+    oracle.fg2d(oracle.F32, False, np.zeros(((nt4w - 1) // 4 + 1, nx4), np.float32), *c4, None, P["w"][:nt4w], s4,
+                r4, taper4, nx4, P["nz"], nt4w, P["npad"], P["dx"], P["dz"], P["dt"], P["rate"])
+    el4 = (time.perf_counter() - t0) * 2500 / (nt4w - 1)
+    legs["cfg4_256_shots"] = {"per_shot_s": round(el4, 2), "s": round(256 * el4, 1),
+                              "shot_gradients_per_s": round(1 / el4, 4),
+                              "sample": "one 600-column aperture, 500-step window scaled to 2500 steps"}
     return {
         "value": round(gpts, 4),
         "unit": "Gpts/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n} absorbing forward steps of the full {nz}x{ny}x{nx} grid on {threads} threads, "
-        f"{el:.1f} s: the reference CPU path (laplacian3d.ispc forward + vtifd3d8o-modeling.cpp loop and abc) "
-        f"restated in C/OpenMP, FP32",
+        "sample": f"{n} absorbing forward steps of the full {nz}x{ny}x{nx} grid on {threads} threads "
+        f"(the job's host thread share), {el:.1f} s: the reference CPU path (laplacian3d.ispc forward + "
+        f"vtifd3d8o-modeling.cpp loop and abc) restated in C/OpenMP, FP32; other legs in 'legs'",
+        "legs": legs,
     }
+
+
+def _taper2d(nx, nz, npad):
+    import vtiamd
+
+    return vtiamd.init_taper2d(nx, nz, npad)
 
 
 def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7, variants=None, graph=True):

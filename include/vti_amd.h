@@ -108,6 +108,17 @@ int vti3d_prop_forward_shot(vti3d_prop *prop, int lmax, const float *wavelet, lo
  * forwardKernel on lambda + imaging, objFuncGradient3d_f.cu:1008-1028;
  * mode 1: RTM, rtm3d_f.cu:585-610), no source/receivers; for timing. */
 int vti3d_prop_backward_steps(vti3d_prop *prop, int mode, int nsteps, float *ms_per_step);
+/* The adjoint wavefield (lambda_x, lambda_z at the current and previous
+ * level) that the fused backward levels advance, and the FP32 accumulators
+ * they add into: gc11, gc13, gc33 for mode 0, the image for mode 1 -- the
+ * device-resident counterparts of curLambdaX/Z, prevLambdaX/Z and gc11/13/33
+ * (objFuncGradient3d_f.cu:916-925) and image (rtm3d_f.cu:178).  set_adjoint
+ * loads the state (reference layout; NULL leaves a field as is) and clears
+ * the accumulators; get_adjoint reads them back (NULL skips a field). */
+int vti3d_prop_set_adjoint(vti3d_prop *prop, int mode, const float *cur_x, const float *cur_z, const float *prev_x,
+                           const float *prev_z);
+int vti3d_prop_get_adjoint(vti3d_prop *prop, float *cur_x, float *cur_z, float *prev_x, float *prev_z, float *acc0,
+                           float *acc1, float *acc2);
 
 /* ---- 3D drivers with the reference's semantics ----------------------------
  * Geometry arrays as the reference takes them: souloc = 5 floats per shot

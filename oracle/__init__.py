@@ -124,6 +124,28 @@ def steps3d(mode, state, c11, c13, c33, npad, dx, dy, dz, dt, nsteps, absorbing=
         raise RuntimeError("orc3d_steps failed")
 
 
+def bwd_steps3d(mode, arith, sigma, lam, acc, c11, c13, c33, npad, dx, dy, dz, dt, nsteps):
+    """Backward levels with no source/receivers on float32 state, in place:
+    sigma = [cur_x, prev_x, cur_z, prev_z], lam likewise, acc = [gc11, gc13,
+    gc33] (mode 0) or [image] (mode 1).  Returns (sigma, lam) reordered so
+    that each is again [cur_x, prev_x, cur_z, prev_z] after the level swaps."""
+    L = lib()
+    if not getattr(L, "_has_bwd", False):
+        P = c_void_p
+        L.orc3d_bwd_steps.argtypes = [c_int, c_int] + [P] * 14 + [c_int] * 4 + [c_float] * 4 + [c_int]
+        L._has_bwd = True
+    nz, ny, nx = c11.shape
+    a = list(acc) + [None] * (3 - len(acc))
+    rc = L.orc3d_bwd_steps(mode, arith, *[_p(x) for x in sigma], *[_p(x) for x in lam], *[_p(x) for x in a],
+                           _p(c11), _p(c13), _p(c33), nx, ny, nz, npad, dx, dy, dz, dt, nsteps)
+    if rc != 0:
+        raise RuntimeError("orc3d_bwd_steps failed")
+    if nsteps % 2:
+        sigma = [sigma[1], sigma[0], sigma[3], sigma[2]]
+        lam = [lam[1], lam[0], lam[3], lam[2]]
+    return sigma, lam
+
+
 def steps3d_f64(state, c11, c13, c33, npad, dx, dy, dz, dt, nsteps, absorbing=True, gradient_form=False):
     """Advance float64 (cur_x, prev_x, cur_z, prev_z) in place with the FP64 oracle."""
     cx, px, cz, pz = state
@@ -296,6 +318,23 @@ def cpu3d_steps(mode, state, a, b, c, npad, dx, dy, dz, dt, nsteps, src_index=-1
                        int(src_index), _p(w))
     if rc != 0:
         raise RuntimeError("cpu3d_steps failed")
+
+
+def cpu3d_backward_steps(sigma, lam, gc, v, eps, dl, npad, dx, dy, dz, dt, nsteps):
+    """The reference CPU gradient's backward level (gradientCij + forward on
+    the adjoint + abc) on float32 state, in place: sigma = [prev_x, cur_x,
+    prev_z, cur_z], lam likewise, gc = [gc11, gc13, gc33]; no injections."""
+    L = lib()
+    if not getattr(L, "_has_cpub", False):
+        P = c_void_p
+        L.cpu3d_backward_steps.argtypes = [P] * 14 + [c_int] * 4 + [c_float] * 4 + [c_int]
+        L._has_cpub = True
+    nz, ny, nx = v.shape
+    v, eps, dl = (np.ascontiguousarray(x, np.float32) for x in (v, eps, dl))
+    rc = L.cpu3d_backward_steps(*[_p(s) for s in sigma], *[_p(s) for s in lam], *[_p(g) for g in gc], _p(v), _p(eps),
+                                _p(dl), nx, ny, nz, npad, dx, dy, dz, dt, nsteps)
+    if rc != 0:
+        raise RuntimeError("cpu3d_backward_steps failed")
 
 
 def cpu3d_damping(npad):

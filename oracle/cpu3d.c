@@ -11,6 +11,12 @@
  * damping[i] = 0.95 + 0.05*cos(pi*(npad-i)/npad), i < npad (:81).
  * The reference runs the x-rows through TBB over x/y/z blocks; here OpenMP
  * runs them over (z, y) -- the arithmetic per point is the same.
+ *
+ * The backward level of the CPU gradient (vtifd3d8o-objfuncGrad.cpp:322-367):
+ * gradientCij (chapter2/Src/laplacian3d.ispc:202-223, c_ij formed per point
+ * from v, eps, del; the source-side level reconstructed with /d2 stencils and
+ * the imaging gc11 += lx (dxx+dyy), gc33 += lz dzz, gc13 += lx dzz + lz
+ * (dxx+dyy)), forward on the adjoint, then abc on the adjoint's two levels.
  */
 #include <math.h>
 #include <stddef.h>
@@ -77,6 +83,26 @@ static void abc(float *s, const float *damping, int nx, int ny, int nz, int npad
             }
 }
 
+/* gradientCij on one x-row */
+static void row_gradient(long n, long nx, long nxy, float dx2, float dy2, float dz2, float dt2, const float *v,
+                         const float *eps, const float *del, float *px, const float *cx, float *pz, const float *cz,
+                         const float *lx, const float *lz, float *g11, float *g13, float *g33)
+{
+    for (long i = 0; i < n; ++i) {
+        const float c33 = v[i] * v[i];
+        const float c11 = c33 * (1.f + 2.f * eps[i]);
+        const float c13 = c33 * sqrtf(1.f + 2.f * del[i]);
+        const float dxx = d2(cx + i, 1) / dx2;
+        const float dyy = d2(cx + i, nx) / dy2;
+        const float dzz = d2(cz + i, nxy) / dz2;
+        px[i] = 2.f * cx[i] + dt2 * (c11 * (dxx + dyy) + c13 * dzz) - px[i];
+        pz[i] = 2.f * cz[i] + dt2 * (c13 * (dxx + dyy) + c33 * dzz) - pz[i];
+        g11[i] += lx[i] * (dxx + dyy);
+        g33[i] += lz[i] * dzz;
+        g13[i] += lx[i] * dzz + lz[i] * (dxx + dyy);
+    }
+}
+
 void cpu3d_damping(float *damping, int npad)
 {
     for (int i = 0; i < npad; i++) damping[i] = (float)(0.95 + (1. - 0.95) * cos(3.14159265359 * (npad - i) / npad));
@@ -130,6 +156,48 @@ int cpu3d_steps(int mode, float *prev_x, float *cur_x, float *prev_z, float *cur
             u = cur_z[i];
             cur_z[i] = prev_z[i];
             prev_z[i] = u;
+        }
+    }
+    return 0;
+}
+
+/* nsteps backward levels of the CPU gradient on caller-owned state (v, eps,
+ * del model; sigma and lambda as (prev, cur) pairs; gc accumulated), no
+ * source and no receivers: the loop body of vtifd3d8o-objfuncGrad.cpp:322-367
+ * without its point injections.  The pairs swap after each level, so after an
+ * odd count the newest levels sit in the prev arrays. */
+int cpu3d_backward_steps(float *sprev_x, float *scur_x, float *sprev_z, float *scur_z, float *lprev_x, float *lcur_x,
+                         float *lprev_z, float *lcur_z, float *g11, float *g13, float *g33, const float *v,
+                         const float *eps, const float *del, int nx, int ny, int nz, int npad, float dx, float dy,
+                         float dz, float dt, int nsteps)
+{
+    const long nxy = (long)nx * ny;
+    const float dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz, dt2 = dt * dt;
+    float damping[4096];
+    if (npad > 4096 || nx < 9 || ny < 9 || nz < 9) return -1;
+    cpu3d_damping(damping, npad);
+    float *px = sprev_x, *cx = scur_x, *pz = sprev_z, *cz = scur_z;
+    float *qx = lprev_x, *ax = lcur_x, *qz = lprev_z, *az = lcur_z;
+    for (int k = 0; k < nsteps; ++k) {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int iz = 4; iz < nz - 4; ++iz)
+            for (int iy = 4; iy < ny - 4; ++iy) {
+                const long i = 4 + iy * (long)nx + iz * nxy;
+                row_gradient(nx - 8, nx, nxy, dx2, dy2, dz2, dt2, v + i, eps + i, del + i, px + i, cx + i, pz + i,
+                             cz + i, ax + i, az + i, g11 + i, g13 + i, g33 + i);
+                row_forward(0, nx - 8, nx, nxy, dx2, dy2, dz2, dt2, v + i, eps + i, del + i, qx + i, ax + i, qz + i,
+                            az + i);
+            }
+        float *t;
+        t = px; px = cx; cx = t;
+        t = pz; pz = cz; cz = t;
+        t = qx; qx = ax; ax = t;
+        t = qz; qz = az; az = t;
+        if (npad > 0) {
+            abc(qx, damping, nx, ny, nz, npad);
+            abc(qz, damping, nx, ny, nz, npad);
+            abc(ax, damping, nx, ny, nz, npad);
+            abc(az, damping, nx, ny, nz, npad);
         }
     }
     return 0;

@@ -349,6 +349,53 @@ int orc3d_steps_f64(double *cur_x, double *prev_x, double *cur_z, double *prev_z
     return 0;
 }
 
+/* Backward levels on caller-owned float state with no source and no
+ * receivers (the fused device backward kernels' window tests): per level and
+ * z-block, mode 0 runs gradientKernel on sigma (undamped reconstruction and
+ * the imaging into acc0..2 = gc11, gc13, gc33) and forwardKernel + abc on
+ * lambda (objFuncGradient3d_f.cu:1008-1028); mode 1 runs forwardKernel on
+ * sigma and on lambda, abc on lambda and imagingKernel into acc0
+ * (rtm3d_f.cu:585-610).  arith 0 FP32, 1 the reference's arithmetic.  The
+ * level pairs swap roles after each level, as in orc3d_steps. */
+int orc3d_bwd_steps(int mode, int arith, float *cx, float *px, float *cz, float *pz, float *lcx, float *lpx,
+                    float *lcz, float *lpz, float *acc0, float *acc1, float *acc2, const float *c11, const float *c13,
+                    const float *c33, int nx, int ny, int nz, int npad, float dx, float dy, float dz, float dt,
+                    int nsteps)
+{
+    orc3d_geom g;
+    geom_setup(&g, nx, ny, nz, npad, dx, dy, dz, dt);
+    if (g.nb < 2 || (mode != 0 && mode != 1) || (arith != 0 && arith != 1)) return -1;
+    float *damping = (float *)malloc(sizeof(float) * ((long)nx * ny + nz));
+    orc3d_init_damping(damping, nx, ny, nz, npad);
+    for (int s = 0; s < nsteps; ++s) {
+        for (int ib = 0; ib < g.nb; ++ib) {
+            if (mode == 0) {
+                if (arith) grad_block_ref(&g, ib, acc0, acc1, acc2, px, cx, px, pz, cz, pz, lcx, lcz, c11, c13, c33);
+                else grad_block_f32(&g, ib, acc0, acc1, acc2, px, cx, px, pz, cz, pz, lcx, lcz, c11, c13, c33);
+            } else {
+                if (arith) fwd_block_ref(&g, ib, px, cx, px, pz, cz, pz, c11, c13, c33);
+                else fwd_block_f32(&g, ib, px, cx, px, pz, cz, pz, c11, c13, c33);
+            }
+            if (arith) {
+                fwd_block_ref(&g, ib, lpx, lcx, lpx, lpz, lcz, lpz, c11, c13, c33);
+                abc_block_ref(&g, ib, damping, lpx, lcx, lpz, lcz);
+                if (mode == 1) image_block_ref(&g, ib, acc0, cx, cz, lcx, lcz);
+            } else {
+                fwd_block_f32(&g, ib, lpx, lcx, lpx, lpz, lcz, lpz, c11, c13, c33);
+                abc_block_f32(&g, ib, damping, lpx, lcx, lpz, lcz);
+                if (mode == 1) image_block_f32(&g, ib, acc0, cx, cz, lcx, lcz);
+            }
+        }
+        float *t;
+        t = cx; cx = px; px = t;
+        t = cz; cz = pz; pz = t;
+        t = lcx; lcx = lpx; lpx = t;
+        t = lcz; lcz = lpz; lpz = t;
+    }
+    free(damping);
+    return 0;
+}
+
 /* ---- objective variants of the 3D drivers (lib/vti-fwi-3d/gpu/agc.cpp) ---- */
 #define AGC_EPS 1e-6f
 

@@ -56,3 +56,43 @@ def test_cpu_damping_profile_closed_form(orc):
     exp = (0.95 + 0.05 * np.cos(3.14159265359 * (npad - i) / npad)).astype(np.float32)
     assert np.array_equal(dmp, exp)
     assert dmp[0] == np.float32(0.9) and np.all(np.diff(dmp) > 0)
+
+
+def test_cpu_gradient_step_matches_fp64_formulas(orc):
+    """One backward level of the CPU gradient (gradientCij + forward on the
+    adjoint, no damping) against the same level written out in FP64 numpy:
+    sigma' = 2 sigma + dt2 (c11 (dxx+dyy) + c13 dzz) - sigma_prev (the
+    stencils divided by d2), lambda' by forward, and the imaging gc11 = lx
+    (dxx+dyy), gc33 = lz dzz, gc13 = lx dzz + lz (dxx+dyy) at the updated
+    interior."""
+    import brute_history as bh
+
+    nz, ny, nx = 20, 18, 22
+    v, e, d = layered_model(nz, ny, nx, seed=7)
+    c33 = v.astype(np.float64) ** 2
+    c11 = c33 * (1 + 2 * e.astype(np.float64))
+    c13 = c33 * np.sqrt(1 + 2 * d.astype(np.float64))
+    sp, sc, zp, zc = _fields(nz, ny, nx, 3)
+    lp, lc, mp, mc = _fields(nz, ny, nx, 4)
+    sig, lam = [sp.copy(), sc.copy(), zp.copy(), zc.copy()], [lp.copy(), lc.copy(), mp.copy(), mc.copy()]
+    gc = [np.zeros((nz, ny, nx), np.float32) for _ in range(3)]
+    dx, dt = 10.0, 1e-3
+    orc.cpu3d_backward_steps(sig, lam, gc, v, e, d, 0, dx, dx, dx, dt, 1)
+    f64 = lambda a: a.astype(np.float64)  # noqa: E731
+    h = dx * dx
+
+    def terms(ux, uz):
+        return (bh._d2(f64(ux), 2) + bh._d2(f64(ux), 1)) / h, bh._d2(f64(uz), 0) / h
+
+    txy, tz = terms(sc, zc)
+    dt2 = dt * dt
+    want_sx = 2 * f64(sc) + dt2 * (c11 * txy + c13 * tz) - f64(sp)
+    lxy, lz = terms(lc, mc)
+    want_lx = 2 * f64(lc) + dt2 * (c11 * lxy + c13 * lz) - f64(lp)
+    inner = (slice(4, nz - 4), slice(4, ny - 4), slice(4, nx - 4))
+    # after one level the newest levels sit in the arrays passed as prev
+    assert rel(sig[0][inner], want_sx[inner]) < 1e-5
+    assert rel(lam[0][inner], want_lx[inner]) < 1e-5
+    assert rel(gc[0][inner], (f64(lc) * txy)[inner]) < 1e-5
+    assert rel(gc[2][inner], (f64(mc) * tz)[inner]) < 1e-5
+    assert rel(gc[1][inner], (f64(lc) * tz + f64(mc) * txy)[inner]) < 1e-5

@@ -768,6 +768,50 @@ int vti3d_prop_get_state(vti3d_prop *p, float *cur_x, float *cur_z, float *prev_
     return vti::kOk;
 }
 
+// Load the backward passes' adjoint state (reference layout; NULL leaves a
+// field as is) and clear the FP32 accumulators the fused backward kernels
+// add into (gc11/gc13/gc33 for mode 0, the image for mode 1).
+int vti3d_prop_set_adjoint(vti3d_prop *p, int mode, const float *cur_x, const float *cur_z, const float *prev_x,
+                           const float *prev_z)
+{
+    VTI_CHECK(p && (mode == 0 || mode == 1), "vti3d_prop_set_adjoint: bad argument");
+    VTI_HIP(hipSetDevice(p->ctx->device));
+    (void)hipGetLastError();  // drop a stale error left by an unrelated earlier call
+    int rc = ensure_backward(p, mode == 0, false);
+    if (rc) return rc;
+    const float *src[4] = {cur_x, cur_z, prev_x, prev_z};
+    float *dst[4] = {p->lx[0], p->lz[0], p->lx[1], p->lz[1]};
+    for (int k = 0; k < 4; ++k)
+        if (src[k]) {
+            rc = upload_volume(p, dst[k], src[k]);
+            if (rc) return rc;
+        }
+    for (int k = 0; k < (mode == 0 ? 3 : 1); ++k)
+        VTI_HIP(hipMemsetAsync(p->acc[k], 0, sizeof(float) * (size_t)vox(p->g), p->ctx->stream));
+    VTI_HIP(hipStreamSynchronize(p->ctx->stream));
+    return vti::kOk;
+}
+
+// Read the adjoint state and the FP32 accumulators back (NULL skips a field;
+// acc1/acc2 exist after a gradient-mode set_adjoint only).
+int vti3d_prop_get_adjoint(vti3d_prop *p, float *cur_x, float *cur_z, float *prev_x, float *prev_z, float *acc0,
+                           float *acc1, float *acc2)
+{
+    VTI_CHECK(p && p->lx[0], "vti3d_prop_get_adjoint: no adjoint state (call vti3d_prop_set_adjoint first)");
+    VTI_CHECK(!(acc1 || acc2) || p->acc[2], "vti3d_prop_get_adjoint: no gradient accumulators on this propagator");
+    VTI_HIP(hipSetDevice(p->ctx->device));
+    (void)hipGetLastError();
+    float *dst[7] = {cur_x, cur_z, prev_x, prev_z, acc0, acc1, acc2};
+    const float *src[7] = {p->lx[0], p->lz[0], p->lx[1], p->lz[1], p->acc[0], p->acc[1], p->acc[2]};
+    for (int k = 0; k < 7; ++k)
+        if (dst[k]) {
+            int rc = download_volume(p, dst[k], src[k]);
+            if (rc) return rc;
+        }
+    VTI_HIP(hipStreamSynchronize(p->ctx->stream));
+    return vti::kOk;
+}
+
 // Advance `nsteps` forward levels with no source.  When elapsed_ms is not
 // NULL the launches are bracketed by HIP events on the propagator's stream
 // and the average time per step is returned there.

@@ -69,6 +69,8 @@ _SIGS = {
         [P, c_int, P, c_longlong, c_int, P, c_int, c_int, c_int, P],
     ),
     "vti3d_prop_backward_steps": (c_int, [P, c_int, c_int, POINTER(c_float)]),
+    "vti3d_prop_set_adjoint": (c_int, [P, c_int, P, P, P, P]),
+    "vti3d_prop_get_adjoint": (c_int, [P, P, P, P, P, P, P, P]),
     "vti3d_model_data": (
         c_int,
         [P, P, P, c_int, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int],
@@ -305,6 +307,20 @@ class Prop3D:
             "vti3d_prop_forward_shot",
         )
         return np.ascontiguousarray(tr.T)
+
+    def set_adjoint(self, mode, cur_x=None, cur_z=None, prev_x=None, prev_z=None):
+        """Load lambda (reference layout) and clear the accumulators of `mode`."""
+        a = [None if x is None else f32(x) for x in (cur_x, cur_z, prev_x, prev_z)]
+        check(_lib.vti3d_prop_set_adjoint(self.h, mode, *[ptr(x) for x in a]), "vti3d_prop_set_adjoint")
+
+    def get_adjoint(self, mode):
+        """(lambda cur_x, cur_z, prev_x, prev_z, accumulators): the
+        accumulators are (gc11, gc13, gc33) for mode 0, (image,) for mode 1."""
+        out = [np.zeros(self.shape, np.float32) for _ in range(4 + (3 if mode == 0 else 1))]
+        acc = out[4:] + [None] * (3 - len(out[4:]))
+        check(_lib.vti3d_prop_get_adjoint(self.h, *[ptr(x) for x in out[:4]], *[ptr(x) for x in acc]),
+              "vti3d_prop_get_adjoint")
+        return out[0], out[1], out[2], out[3], tuple(out[4:])
 
     def backward_steps(self, nsteps: int, mode: int = 0, timed: bool = False):
         ms = c_float(0.0)
