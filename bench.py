@@ -38,22 +38,9 @@ BYTES_RTM = 68  # fused RTM step: 12 reads + 5 writes
 
 
 def layered(nz, ny, nx, seed=2024):
-    rng = np.random.default_rng(seed)
-    nl = 6
-    z = np.arange(nz, dtype=np.float32)[:, None, None]
-    y = np.arange(ny, dtype=np.float32)[None, :, None]
-    x = np.arange(nx, dtype=np.float32)[None, None, :]
-    depth = z + 0.12 * (x - nx / 2) + 0.06 * (y - ny / 2)
-    bounds = np.sort(rng.uniform(0.1, 0.9, nl - 1)) * nz
-    layer = np.searchsorted(bounds, depth).astype(np.int8)
-    vl = np.sort(rng.uniform(1500.0, 4500.0, nl)).astype(np.float32)
-    el = rng.uniform(0.0, 0.25, nl).astype(np.float32)
-    dl = np.minimum(rng.uniform(0.0, 0.12, nl), el).astype(np.float32)
-    v, e, d = vl[layer], el[layer], dl[layer]
-    c33 = v * v
-    c11 = c33 * (1 + 2 * e)
-    c13 = c33 * np.sqrt(1 + 2 * d)
-    return c11.astype(np.float32), c13.astype(np.float32), c33.astype(np.float32), (v, e, d)
+    from vtiamd import workloads as W
+
+    return W.layered3d(nz, ny, nx, seed)
 
 
 def load_traffic(kernel: str, workload: str):
@@ -304,39 +291,37 @@ def batch_small_2d(vtiamd, ctx, ns=32, seed=5):
 
 
 def shot_gradient_2d(vtiamd, ctx, nshots=2, ns_call=32):
-    """2D gradients on the 512x1024 (+40 boundary) grid, 3000 steps, 1024
-    receivers, rate 4 ms: one shot per call (the one-shot loop), and one call
-    with 32 shots as the reference's configuration makes it (run as one batch
-    inside objfunc_gradient2d): shot-gradients/s of both."""
-    nx, nz, npad, nt, dx, dt, rate = 1024, 512, 40, 3001, 10.0, 1e-3, 4e-3
-    nnx, nnz = nx + 2 * npad, nz + 2 * npad
-    z = np.arange(nnz, dtype=np.float32)[:, None] * np.ones((1, nnx), np.float32)
-    v = 1500.0 + 3000.0 * z / nnz
-    c33 = (v * v).astype(np.float32)
-    c11 = (c33 * 1.2).astype(np.float32)
-    c13 = (c33 * 1.05).astype(np.float32)
+    """2D gradients of the cfg2 inversion (512x1024 + 40 boundary, 3000 steps,
+    1024 receivers, rate 4 ms; initial model against data of the true model
+    with its 64 anomalies): one shot per call (the one-shot loop), and one call
+    with all 32 shots as the reference's configuration makes it (run as one
+    batch inside objfunc_gradient2d): shot-gradients/s of both."""
+    from vtiamd import workloads as W
+
+    P = W.cfg2(nshots=ns_call)
+    ct, ci = W.cij(*P["true"]), W.cij(*P["init"])
+    nx, nz, npad, nt = P["nx"], P["nz"], P["npad"], P["nt"]
     taper = vtiamd.init_taper2d(nx, nz, npad)
-    t = np.arange(nt) * dt - 0.12
-    a = (np.pi * 10.0 * t) ** 2
-    w = ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
-    rloc = np.array([[npad + i, npad + 1] for i in range(nx)], np.int32)
-    nnt = (nt - 1) // int(rate / dt + 0.5) + 1
-    data = np.zeros((nnt, nx), np.float32)
-    args = (taper, nx, nz, nt, npad, dx, dx, dt, rate)
-
-    def sources(n):
-        return np.array([[npad + 8 + (k * (nx - 16)) // max(n - 1, 1), npad + 2, nx, 0] for k in range(n)], np.int32)
-
-    sl = sources(nshots)
-    vtiamd.objfunc_gradient2d(ctx, data, c11, c13, c33, w, sl[:1], rloc, *args)
+    args = (taper, nx, nz, nt, npad, P["dx"], P["dz"], P["dt"], P["rate"])
+    sl, rloc = P["sloc"], P["rloc"]
+    nnt = (nt - 1) // 4 + 1
+    # every shot records all 1024 receivers: rloc repeated per shot, sloc pointing at its block
+    sl_all = sl.copy()
+    sl_all[:, 3] = np.arange(ns_call) * nx
+    rl_all = np.tile(rloc, (ns_call, 1))
+    data = vtiamd.synthetic2d(ctx, *ct, P["w"], sl_all, rl_all, *args)
+    assert data.shape == (nnt, ns_call * nx)
+    one = [np.ascontiguousarray(data[:, k * nx:(k + 1) * nx]) for k in range(nshots)]
+    vtiamd.objfunc_gradient2d(ctx, one[0], *ci, P["w"], sl[:1], rloc, *args)
     t0 = time.perf_counter()
     for k in range(nshots):
-        vtiamd.objfunc_gradient2d(ctx, data, c11, c13, c33, w, sl[k:k + 1], rloc, *args)
+        vtiamd.objfunc_gradient2d(ctx, one[k], *ci, P["w"], sl[k:k + 1], rloc, *args)
     el = time.perf_counter() - t0
     t0 = time.perf_counter()
-    vtiamd.objfunc_gradient2d(ctx, data, c11, c13, c33, w, sources(ns_call), rloc, *args)
+    vtiamd.objfunc_gradient2d(ctx, data, *ci, P["w"], sl_all, rl_all, *args)
     el32 = time.perf_counter() - t0
-    return {"shots": nshots, "grid_padded": [nnz, nnx], "steps": nt - 1, "receivers": nx,
+    return {"shots": nshots, "grid_padded": [nz + 2 * npad, nx + 2 * npad], "steps": nt - 1, "receivers": nx,
+            "model": "cfg2 initial model (smoothed 1D gradient + 64 anomalies), data of the true model",
             "ms_per_shot": round(el * 1e3 / nshots, 2), "shot_gradients_per_s": round(nshots / el, 3),
             f"call_of_{ns_call}_shots": {"s": round(el32, 3), "shot_gradients_per_s": round(ns_call / el32, 3),
                                           "speedup_over_one_shot_calls": round((ns_call / el32) / (nshots / el), 2)}}
@@ -368,6 +353,47 @@ def synthetic_cfg1(vtiamd, ctx, ns=16):
     many = ns / (time.perf_counter() - t0)
     return {"grid_padded": [nnz, nnx], "steps": nt - 1, "receivers": nx, "one_shot_per_call_shots_per_s": round(one, 2),
             f"call_of_{ns}_shots_shots_per_s": round(many, 2), "speedup": round(many / one, 2)}
+
+
+def cfg3_jobs(vtiamd, ctx, c11, c13, c33, npad, dx, dt, nt=3010):
+    """The whole 3D jobs of the 460x572x572 configuration, one shot: forward
+    modeling (modelDataCij3d_f) recording a 512 x 512 receiver grid two
+    points below the boundary layer, and RTM (rtmCij3d_f): random-boundary
+    forward, then the fused backward pass with the imaging condition.  nt =
+    3010 makes the reference pipeline advance 3009 levels ((nt-2) % 64 == 0).
+    The observed data of the RTM are the modeled traces."""
+    nz, ny, nx = c11.shape
+    rate = 4e-3
+    w = ricker_np(nt, dt, 12.0, 0.1)
+    i0 = (nx - 512) // 2
+    zr = (npad + 2) * dx + 0.1
+    ix, iy = np.meshgrid(np.arange(512) + i0, np.arange(512) + i0, indexing="xy")
+    recloc = np.stack([ix.ravel() * dx + 0.1, iy.ravel() * dx + 0.1, np.full(ix.size, zr), np.ones(ix.size)],
+                      axis=1).astype(np.float32)
+    souloc = np.array([[nx // 2 * dx + 0.1, ny // 2 * dx + 0.1, (npad + 4) * dx + 0.1, ix.size, 0]], np.float32)
+    o, d = (0.0, 0.0, 0.0), (dx, dx, dx)
+    lmax = ((nt - 2) // 64) * 64 + 1
+    npts = nz * ny * nx
+    t0 = time.perf_counter()
+    obs = vtiamd.model_data3d(ctx, souloc, np.arange(1), recloc, w, c11, c13, c33, nt, npad, o, d, dt, rate)
+    tm = time.perf_counter() - t0
+    rec3 = np.ascontiguousarray(recloc[:, :3])
+    t0 = time.perf_counter()
+    img = vtiamd.rtm3d(ctx, souloc, np.arange(1), rec3, w, c11, c13, c33, nt, npad, o, d, dt, rate, obs)
+    tr = time.perf_counter() - t0
+    assert np.isfinite(img).all() and np.abs(img).max() > 0
+    # algorithmic bytes: forward 36 B/pt-level; RTM 36 (random-boundary forward) + 68 (fused backward)
+    return {"grid_padded": [nz, ny, nx], "levels": lmax, "receivers": int(ix.size), "samples": int(obs.size // ix.size),
+            "modeling_s": round(tm, 3), "modeling_gpts_s": round(npts * lmax / tm / 1e9, 2),
+            "rtm_s": round(tr, 3), "rtm_floor_s_at_8tbs": round((36 + 68) * npts * lmax / 8e12, 3),
+            "rtm_level_gpts_s": round(npts * lmax / tr / 1e9, 2),
+            "note": "host-side times of the whole calls: uploads, the 790 MB trace transfer and setup included"}
+
+
+def ricker_np(nt, dt, f, t0):
+    t = np.arange(nt) * dt - t0
+    a = (np.pi * f * t) ** 2
+    return ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
 
 
 def run_sections(extra, sections):
@@ -529,7 +555,7 @@ def main():
     ap.add_argument("--no-2d", action="store_true", help="skip the 2D and serving workloads (profiling runs)")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the graph replays of the batched workload (kernel-trace profiling runs)")
-    ap.add_argument("--only", choices=["shot2d", "batch2d", "serve"],
+    ap.add_argument("--only", choices=["shot2d", "batch2d", "serve", "jobs3d"],
                     help="run one secondary workload alone and print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -549,6 +575,9 @@ def main():
                 r["synthetic_256x256_2000steps"] = synthetic_cfg1(vtiamd, ctx)
             elif args.only == "batch2d":
                 r = batch_gradient_2d(vtiamd, ctx, graph=not args.no_graph)
+            elif args.only == "jobs3d":
+                c11, c13, c33, _ = layered(460, 572, 572)
+                r = cfg3_jobs(vtiamd, ctx, c11, c13, c33, 30, 12.5, 1e-3)
             else:
                 r = serve_stream(vtiamd)
         print(json.dumps({"workload": args.only, **r}), flush=True)
@@ -634,6 +663,7 @@ def main():
     failed = None
     if not args.small and not args.no_2d:
         failed = run_sections(extra, [
+            ("3d_jobs_460x572x572_3009_levels", lambda: cfg3_jobs(vtiamd, ctx, c11, c13, c33, npad, dx, dt)),
             ("2d_shot_gradient_512x1024_3000steps", lambda: shot_gradient_2d(vtiamd, ctx)),
             ("2d_synthetic_256x256_2000steps", lambda: synthetic_cfg1(vtiamd, ctx)),
             ("2d_batched_vs_plain_gradient", lambda: batch_gradient_2d(vtiamd, ctx)),

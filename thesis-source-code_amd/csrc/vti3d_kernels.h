@@ -730,6 +730,262 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_t2_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Two forward levels per launch on 64 x 16 tiles (1024 threads).
+//
+// The same scheme and arithmetic as fwd3d_t2_kernel, arranged for fewer
+// registers per thread so that four waves per SIMD stay resident without
+// spills:
+//  - a 16-row tile: E = 72 x 24 points, the tile's 1024 plus a 704-point rim,
+//    so each thread owns its tile point and at most one rim point (the level
+//    L+1 recompute on the rim is 0.69 of the tile's work, against 1.25 on
+//    8-row tiles);
+//  - a rim point needs only sigma_x(L+1) (level L+2 reads sigma_z(L+1) on
+//    the tile's own columns), so it computes sigma_x alone, and the 9-deep
+//    sigma_z(L) queue of its column lives in LDS, private to its thread (no
+//    barrier: one lane writes and reads it);
+//  - the operands of level L+2 at plane z-4 (sigma_x(L) there and the
+//    coefficients) are read again from memory -- the same values, from L2 --
+//    instead of being held in registers for four planes.
+// Launch: blocks of (64, 16), grid ntx * nty * ntz with nty = ceil((ny-8)/16).
+// ---------------------------------------------------------------------------
+template <int ARITH, bool ABC>
+__global__ __launch_bounds__(1024) void fwd3d_t2b_kernel(
+    float *__restrict__ cx, float *__restrict__ cz, float *__restrict__ dx_, float *__restrict__ dz_,
+    const float *__restrict__ ax, const float *__restrict__ az, const float *__restrict__ bx,
+    const float *__restrict__ bz, const float *__restrict__ c11, const float *__restrict__ c13,
+    const float *__restrict__ c33, const float *__restrict__ dxy_plane, const float *__restrict__ dzv, Geo3 g,
+    int zc, int ntx, int nty, float dt2dx2, float dt2dy2, float dt2dz2, long long src_idx, float src1, float src2)
+{
+    constexpr int BY = 16;
+    using S = T2Shape<BY>;
+    constexpr int TW = S::TW, TH = S::TH, EW = S::EW, EH = S::EH, NRIM = S::NRIM;
+    constexpr int TL = (TW * TH + S::NT - 1) / S::NT;  // tile elements loaded per thread
+    __shared__ float tile[2][TH * TW];
+    __shared__ float ring[5][EH * EW];
+    __shared__ float rq[9][NRIM];   // sigma_z(L) queues of the rim columns, slot = plane mod 9
+    __shared__ float q1s[9][S::NT];  // sigma_z(L+1) queues of the tile columns, slot = plane mod 9
+
+    const int lb = xcd_logical_block(gridDim.x * gridDim.y * gridDim.z);
+    const int bxi = lb % ntx, byi = (lb / ntx) % nty, bzi = lb / (ntx * nty);
+    const int tid = threadIdx.x + 64 * threadIdx.y;
+    const int x0 = bxi * 64, y0 = 4 + byi * BY;
+    const int zs = bzi * zc, ze = min(zs + zc, g.zmax);
+    const int z1s = max(zs - 4, 0), z1e = min(ze + 4, g.zmax);  // planes of level L+1
+
+    // the tile point (slot 0) and the rim point (slot 1) of this thread
+    int ex0, ey0, ex1 = 0, ey1 = 0;
+    t2_point<BY>(tid, 0, ex0, ey0);
+    const bool rim = t2_point<BY>(tid, 1, ex1, ey1);
+    const int gx0 = x0 - 4 + ex0, gy0 = y0 - 4 + ey0, gx1 = x0 - 4 + ex1, gy1 = y0 - 4 + ey1;
+    const bool inr0 = gx0 < g.nx && gy0 < g.ny;
+    const bool act0 = inr0 && gx0 >= 4 && gx0 < g.nx - 4 && gy0 < g.ny - 4;
+    const bool inr1 = rim && gx1 >= 0 && gx1 < g.nx && gy1 >= 0 && gy1 < g.ny;
+    const bool act1 = inr1 && gx1 >= 4 && gx1 < g.nx - 4 && gy1 >= 4 && gy1 < g.ny - 4;
+    const unsigned off0 = inr0 ? (unsigned)(gy0 * g.pitch + gx0) : 0u;
+    const unsigned off1 = inr1 ? (unsigned)(gy1 * g.pitch + gx1) : 0u;
+    const float dxy0 = (ABC && inr0) ? dxy_plane[off0] : 1.f;
+    const float dxy1 = (ABC && inr1) ? dxy_plane[off1] : 1.f;
+    auto P = [&](const float *base, int z) { return base + (size_t)z * (size_t)g.plane; };
+
+    // sigma_x(L) tile elements this thread loads: offsets within a plane
+    unsigned toff[TL];
+    bool tok[TL];
+#pragma unroll
+    for (int k = 0; k < TL; ++k) {
+        const int e = tid + k * S::NT;
+        const int x = x0 - 8 + e % TW, y = y0 - 8 + e / TW;
+        tok[k] = e < TW * TH && x >= 0 && x < g.nx && y >= 0 && y < g.ny;
+        toff[k] = tok[k] ? (unsigned)(y * g.pitch + x) : 0u;
+    }
+    float treg[TL];
+    auto tile_regs = [&](int z) {
+        const float *B = P(bx, z);
+#pragma unroll
+        for (int k = 0; k < TL; ++k) treg[k] = tok[k] ? B[toff[k]] : 0.f;
+    };
+    auto tile_store = [&](float *dst) {
+#pragma unroll
+        for (int k = 0; k < TL; ++k)
+            if (tid + k * S::NT < TW * TH) dst[tid + k * S::NT] = treg[k];
+    };
+
+    // sigma_z(L) queue of the tile column (q[k] = plane z-4+k) and of the rim
+    // column (rq, slot (z-4+k) mod 9); sigma_z(L+1) of the tile column (q1s,
+    // slot (z-8+k) mod 9).  The LDS queues are private to their thread.
+    float q[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int zz = z1s - 4 + k;
+        const bool in = zz >= 0 && zz < g.zmax;
+        q[k] = (inr0 && in) ? P(bz, zz)[off0] : 0.f;
+        q1s[(z1s - 8 + k + 18) % 9][tid] = 0.f;
+        if (rim) rq[(zz + 9) % 9][tid] = (inr1 && in) ? P(bz, zz)[off1] : 0.f;
+    }
+
+    // operands of level L+1 at one plane: the previous level and the
+    // coefficients (the rim point needs what sigma_x takes)
+    float pax0, paz0, k11_0, k13_0, k33_0, pax1, k11_1, k13_1;
+    auto load_pts = [&](int z, float &oax0, float &oaz0, float &o11_0, float &o13_0, float &o33_0, float &oax1,
+                        float &o11_1, float &o13_1) {
+        const bool l0 = z < z1e && inr0, l1 = z < z1e && inr1;
+        oax0 = l0 ? P(ax, z)[off0] : 0.f;
+        oaz0 = l0 ? P(az, z)[off0] : 0.f;
+        o11_0 = l0 && act0 ? P(c11, z)[off0] : 0.f;
+        o13_0 = l0 && act0 ? P(c13, z)[off0] : 0.f;
+        o33_0 = l0 && act0 ? P(c33, z)[off0] : 0.f;
+        oax1 = l1 ? P(ax, z)[off1] : 0.f;
+        o11_1 = l1 && act1 ? P(c11, z)[off1] : 0.f;
+        o13_1 = l1 && act1 ? P(c13, z)[off1] : 0.f;
+    };
+
+    if (z1s < z1e) {
+        tile_regs(z1s);
+        tile_store(tile[z1s & 1]);
+    }
+    if (z1s + 1 < z1e) tile_regs(z1s + 1);
+    load_pts(z1s, pax0, paz0, k11_0, k13_0, k33_0, pax1, k11_1, k13_1);
+    __syncthreads();
+
+    for (int z = z1s; z < ze + 4; ++z) {
+        const bool step1 = z < z1e;
+        const int zp = z - 4;
+        const bool step2 = zp >= zs && zp < ze && act0;
+        // requests for later planes, in flight while this one is computed:
+        // level L+2's operands at zp, the next plane's point operands and
+        // sigma_z(L) planes, then the sigma_x(L) tile two planes ahead
+        float hx = 0.f, s11 = 0.f, s13 = 0.f, s33 = 0.f;
+        if (step2) {
+            hx = P(bx, zp)[off0];
+            s11 = P(c11, zp)[off0];
+            s13 = P(c13, zp)[off0];
+            s33 = P(c33, zp)[off0];
+        }
+        float nax0, naz0, n11_0, n13_0, n33_0, nax1, n11_1, n13_1;
+        load_pts(z + 1, nax0, naz0, n11_0, n13_0, n33_0, nax1, n11_1, n13_1);
+        const bool qz = z + 5 < g.zmax;
+        const float qn0 = (inr0 && qz) ? P(bz, z + 5)[off0] : 0.f;
+        const float qn1 = (inr1 && qz) ? P(bz, z + 5)[off1] : 0.f;
+        if (z + 1 < z1e) tile_store(tile[(z + 1) & 1]);
+        if (z + 2 < z1e) tile_regs(z + 2);
+
+        // ---- level L+1 at plane z on E
+        if (step1) {
+            const float *Tp = tile[z & 1];
+            const float d1 = ABC ? dzv[z] : 1.f;
+            const long long pz = (long long)z * g.plane;
+            // tile point: sigma_x and sigma_z
+            float vx = 0.f, vz = 0.f;
+            if (act0) {
+                const float *T = &Tp[(ey0 + 4) * TW + ex0 + 4];
+                const float s0 = T[0];
+                const float txy = sx_scaled_w<ARITH, TW>(T, dt2dx2, dt2dy2);
+                float e9[9];
+                effective_column_rt<ABC>(e9, q, dxy0, dzv, z, z & 3);
+                const float tz = sz_scaled<ARITH>(e9, dt2dz2);
+                float px = pax0, pzv = paz0, d = 1.f;
+                if (ABC) {
+                    d = dxy0 * d1;
+                    px = px * d;
+                    pzv = pzv * d;
+                }
+                vx = leap<ARITH>(k11_0 * txy + k13_0 * tz, s0, px);
+                vz = leap<ARITH>(k13_0 * txy + k33_0 * tz, q[4], pzv);
+                const long long gi = pz + off0;
+                if (gi == src_idx) {
+                    vx = vx + src1;
+                    vz = vz + src1;
+                }
+                if (ABC) {
+                    vx = vx * d;
+                    vz = vz * d;
+                }
+                if (z >= zs && z < ze) {
+                    cx[gi] = vx;
+                    cz[gi] = vz;
+                }
+            } else if (inr0) {
+                vx = pax0;
+                vz = paz0;
+            }
+            ring[z % 5][ey0 * EW + ex0] = vx;
+            q1s[z % 9][tid] = vz;
+            // rim point: sigma_x only
+            if (rim) {
+                float vr = 0.f;
+                if (act1) {
+                    const float *T = &Tp[(ey1 + 4) * TW + ex1 + 4];
+                    const float s0 = T[0];
+                    const float txy = sx_scaled_w<ARITH, TW>(T, dt2dx2, dt2dy2);
+                    float e9[9];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) e9[k] = rq[(z - 4 + k + 9) % 9][tid];
+                    effective_column_rt<ABC>(e9, e9, dxy1, dzv, z, z & 3);
+                    const float tz = sz_scaled<ARITH>(e9, dt2dz2);
+                    float px = pax1, d = 1.f;
+                    if (ABC) {
+                        d = dxy1 * d1;
+                        px = px * d;
+                    }
+                    vr = leap<ARITH>(k11_1 * txy + k13_1 * tz, s0, px);
+                    if (pz + off1 == src_idx) vr = vr + src1;
+                    if (ABC) vr = vr * d;
+                } else if (inr1) {
+                    vr = pax1;
+                }
+                ring[z % 5][ey1 * EW + ex1] = vr;
+            }
+        } else {
+            q1s[z % 9][tid] = 0.f;
+        }
+        __syncthreads();
+        // ---- level L+2 at plane z-4 on the tile
+        if (step2) {
+            const float *T = &ring[zp % 5][ey0 * EW + ex0];
+            const float s0 = T[0];
+            const float txy = sx_scaled_w<ARITH, EW>(T, dt2dx2, dt2dy2);
+            float e9[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) e9[k] = q1s[(zp - 4 + k + 9) % 9][tid];
+            const float q1c = e9[4];
+            effective_column_rt<ABC>(e9, e9, dxy0, dzv, zp, zp & 3);
+            const float tz = sz_scaled<ARITH>(e9, dt2dz2);
+            const long long gi = (long long)zp * g.plane + off0;
+            float px = hx, pzv = q[0], d = 1.f;
+            if (ABC) {
+                d = dxy0 * dzv[zp];
+                px = px * d;
+                pzv = pzv * d;
+            }
+            float vx = leap<ARITH>(s11 * txy + s13 * tz, s0, px);
+            float vz = leap<ARITH>(s13 * txy + s33 * tz, q1c, pzv);
+            if (gi == src_idx) {
+                vx = vx + src2;
+                vz = vz + src2;
+            }
+            if (ABC) {
+                vx = vx * d;
+                vz = vz * d;
+            }
+            dx_[gi] = vx;
+            dz_[gi] = vz;
+        }
+        // advance to plane z+1: queues, operands
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
+        q[8] = qn0;
+        if (rim) rq[(z + 5) % 9][tid] = qn1;
+        pax0 = nax0;
+        paz0 = naz0;
+        k11_0 = n11_0;
+        k13_0 = n13_0;
+        k33_0 = n33_0;
+        pax1 = nax1;
+        k11_1 = n11_1;
+        k13_1 = n13_1;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Fused backward step of the 3D gradient and RTM passes.
 //
 // MODE 0 (gradient, objFuncGradient3d_f.cu:1008-1028): per point
