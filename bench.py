@@ -43,6 +43,12 @@ def layered(nz, ny, nx, seed=2024):
     return W.layered3d(nz, ny, nx, seed)
 
 
+def progress(msg: str) -> None:
+    """One line on stderr as each part of the run ends (the JSON result line
+    goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def load_traffic(kernel: str, workload: str):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -85,6 +91,7 @@ def cpu_baseline(ved, npad, dx, dt, budget_s=6.0):
     el = time.perf_counter() - t0
     gpts = nz * ny * nx * n / el / 1e9
     legs["3d_forward_step"] = {"gpts_s": round(gpts, 4), "steps": n, "s": round(el, 2)}
+    progress(f"cpu 3d forward leg: {el:.1f} s")
     lam = [np.zeros((nz, ny, nx), np.float32) for _ in range(4)]
     lam[1][:] = rng.standard_normal((nz, ny, nx), dtype=np.float32) * 1e-6
     gc = [np.zeros((nz, ny, nx), np.float32) for _ in range(3)]
@@ -94,6 +101,7 @@ def cpu_baseline(ved, npad, dx, dt, budget_s=6.0):
     oracle.cpu3d_backward_steps(st, lam, gc, v, e, d, npad, dx, dx, dx, dt, nb)
     elb = time.perf_counter() - t0
     legs["3d_gradient_step"] = {"gpts_s": round(nz * ny * nx * nb / elb / 1e9, 4), "steps": nb, "s": round(elb, 2)}
+    progress(f"cpu 3d gradient leg: {elb:.1f} s")
     del st, lam, gc
 
     # 2D cfg1: synthetic_f in full
@@ -122,6 +130,7 @@ def cpu_baseline(ved, npad, dx, dt, budget_s=6.0):
     oracle.fg2d(oracle.F32, False, obs, *ci, None, P["w"], sl, P["rloc"], taper2, P["nx"], P["nz"], P["nt"],
                 P["npad"], P["dx"], P["dz"], P["dt"], P["rate"])
     el2 = time.perf_counter() - t0
+    progress(f"cpu 2d legs: {el1:.1f} s, {el2:.1f} s")
     legs["2d_cfg2_shot_gradient_full"] = {"s": round(el2, 2), "shot_gradients_per_s": round(1 / el2, 4),
                                           "call_of_32_shots_s": round(32 * el2, 1)}
     # cfg4: one shot at the mean aperture (600 columns), 500-step window, x 256 shots
@@ -407,8 +416,10 @@ def run_sections(extra, sections):
         if failed is not None:
             extra[name] = {"not_run": f"after the error in {failed}"}
             continue
+        progress(f"section {name}")
         try:
             extra[name] = fn()
+            progress(f"section {name} done")
         except Exception as e:  # reported in the output line
             traceback.print_exc(file=sys.stderr)
             extra[name] = {"error": f"{type(e).__name__}: {e}"}
@@ -603,6 +614,7 @@ def main():
     ctx.sync()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / max(args.steps, 1)
+    progress(f"headline: {args.steps} forward steps, {ms_per_step:.4f} ms/step")
     value = npts / (ms_per_step * 1e-3) / 1e9
     achieved = BYTES_FWD * npts / (kernel_ms * 1e-3) / 1e9
     traffic = load_traffic("fwd3d_kernel", workload)
@@ -656,6 +668,7 @@ def main():
         "ms": round(br, 4), "gpts_s": round(npts / (br * 1e-3) / 1e9, 3), "bytes_per_point": BYTES_RTM,
         "achieved_gbs": round(BYTES_RTM * npts / (br * 1e-3) / 1e9, 1),
     }
+    progress("backward steps timed")
     prop.set_arith(0)
     f32ms = prop.forward_steps(max(1, args.steps // 2), absorbing=True, timed=True)
     extra["fwd_step_fp32_arith"] = {"ms": round(f32ms, 4), "gpts_s": round(npts / (f32ms * 1e-3) / 1e9, 3)}
@@ -673,6 +686,7 @@ def main():
     if failed is not None:
         out["errors"] = [f"{failed}: {extra[failed]['error']}"]
     if not args.no_cpu and not args.small:
+        progress("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(ved, npad, dx, dt)
     try:
         ctx.close()
