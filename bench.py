@@ -336,6 +336,53 @@ def shot_gradient_2d(vtiamd, ctx, nshots=2, ns_call=32):
                                           "speedup_over_one_shot_calls": round((ns_call / el32) / (nshots / el), 2)}}
 
 
+def fwi_lbfgs_2d(vtiamd, ctx, iters=5, nfg=12):
+    """The cfg2 inversion as fwi-lbfgs-c-param.cu runs it (parameter vepsdel):
+    32 shots of data of the true model, the initial model (v, eps, del scaled
+    by v0 = 100 m/s, eps0 = del0 = 0.01, so that the first trial step, of
+    length 1 along -g, moves v by at most 100 m/s), objFuncGradientVEpsDel on the GPU
+    (one call with the 32 shots, run as one batch; wbottom 0) and the FP64
+    L-BFGS (m = 5) with its vectors resident in HBM, until `iters`
+    iterations are complete or `nfg` evaluations are spent.  Wall time of the
+    whole loop; the objective of every evaluation."""
+    from vtiamd import workloads as W
+
+    ns = 32
+    P = W.cfg2(nshots=ns)
+    nx, nz, npad, nt = P["nx"], P["nz"], P["npad"], P["nt"]
+    taper = vtiamd.init_taper2d(nx, nz, npad)
+    geo = (nx, nz, nt, npad, P["dx"], P["dz"], P["dt"], P["rate"])
+    sl_all = P["sloc"].copy()
+    sl_all[:, 3] = np.arange(ns) * nx
+    rl_all = np.tile(P["rloc"], (ns, 1))
+    data = vtiamd.synthetic2d(ctx, *W.cij(*P["true"]), P["w"], sl_all, rl_all, taper, *geo)
+    v0, eps0, del0 = 100.0, 0.01, 0.01
+    v, e, d = P["init"]
+    nn = v.size
+    x = np.concatenate([(v / np.float32(v0)).ravel(), (e / np.float32(eps0)).ravel(),
+                        (d / np.float32(del0)).ravel()]).astype(np.float32)
+    opt = vtiamd.LBFGS(3 * nn, 5, ctx=ctx)
+    fs, iflag, t_grad = [], 0, 0.0
+    t0 = time.perf_counter()
+    while True:
+        t1 = time.perf_counter()
+        f, g, e2, d2, _ = vtiamd.objfunc_gradient2d_vepsdel(ctx, data, x[:nn], x[nn:2 * nn], x[2 * nn:], v0, eps0,
+                                                             del0, P["w"], sl_all, rl_all, taper, *geo)
+        t_grad += time.perf_counter() - t1
+        x[nn:2 * nn], x[2 * nn:] = e2, d2  # checkEpsDel projects eps/del in place, as in the reference
+        fs.append(float(f))
+        iflag = opt.step(x, f, g.ravel(), iflag)
+        if iflag != 1 or opt.iter > iters or len(fs) >= nfg:
+            break
+    wall = time.perf_counter() - t0
+    done = min(opt.iter - 1, iters) if iflag == 1 else opt.iter
+    opt.close()
+    return {"shots": ns, "grid_padded": [nz + 2 * npad, nx + 2 * npad], "steps": nt - 1, "unknowns": 3 * nn,
+            "lbfgs": "FP64, m=5, vectors in HBM (vti_lbfgs_create_device)", "iterations_completed": int(done),
+            "evaluations": len(fs), "objective": [round(q, 6) for q in fs], "wall_s": round(wall, 2),
+            "gradient_s": round(t_grad, 2), "shot_gradients_per_s": round(ns * len(fs) / t_grad, 2)}
+
+
 def synthetic_cfg1(vtiamd, ctx, ns=16):
     """synthetic_f on the 256x256 (+40 boundary) grid, 2000 steps, 256
     receivers, rate 4 ms: 16 shots in one call (run as one batch) against
@@ -566,7 +613,7 @@ def main():
     ap.add_argument("--no-2d", action="store_true", help="skip the 2D and serving workloads (profiling runs)")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip the graph replays of the batched workload (kernel-trace profiling runs)")
-    ap.add_argument("--only", choices=["shot2d", "batch2d", "serve", "jobs3d"],
+    ap.add_argument("--only", choices=["shot2d", "batch2d", "serve", "jobs3d", "fwi2d"],
                     help="run one secondary workload alone and print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -586,6 +633,8 @@ def main():
                 r["synthetic_256x256_2000steps"] = synthetic_cfg1(vtiamd, ctx)
             elif args.only == "batch2d":
                 r = batch_gradient_2d(vtiamd, ctx, graph=not args.no_graph)
+            elif args.only == "fwi2d":
+                r = fwi_lbfgs_2d(vtiamd, ctx)
             elif args.only == "jobs3d":
                 c11, c13, c33, _ = layered(460, 572, 572)
                 r = cfg3_jobs(vtiamd, ctx, c11, c13, c33, 30, 12.5, 1e-3)
@@ -678,6 +727,7 @@ def main():
         failed = run_sections(extra, [
             ("3d_jobs_460x572x572_3009_levels", lambda: cfg3_jobs(vtiamd, ctx, c11, c13, c33, npad, dx, dt)),
             ("2d_shot_gradient_512x1024_3000steps", lambda: shot_gradient_2d(vtiamd, ctx)),
+            ("2d_fwi_lbfgs_512x1024_32shots", lambda: fwi_lbfgs_2d(vtiamd, ctx)),
             ("2d_synthetic_256x256_2000steps", lambda: synthetic_cfg1(vtiamd, ctx)),
             ("2d_batched_vs_plain_gradient", lambda: batch_gradient_2d(vtiamd, ctx)),
             ("served_stream", lambda: serve_stream(vtiamd)),

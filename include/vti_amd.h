@@ -98,6 +98,12 @@ int vti3d_prop_set_arith(vti3d_prop *prop, int arith);
 int vti3d_prop_set_levels(vti3d_prop *p, int levels);
 int vti3d_prop_set_model(vti3d_prop *prop, const float *c11, const float *c13, const float *c33);
 int vti3d_prop_reset(vti3d_prop *prop, long long sou_index, float value);
+/* The two time levels as the propagator holds them.  In absorbing runs the
+ * reference damps the current level in place after each block's step
+ * (abc3d.cu:82-125), so the level it reads as prev carries the factor d =
+ * dxy*dz twice; the propagator stores every level damped once and applies
+ * the second factor when it reads prev.  So its prev equals the reference's
+ * prev divided by d at the points the steps update (cur is the same). */
 int vti3d_prop_set_state(vti3d_prop *prop, const float *cur_x, const float *cur_z, const float *prev_x,
                          const float *prev_z);
 int vti3d_prop_get_state(vti3d_prop *prop, float *cur_x, float *cur_z, float *prev_x, float *prev_z);
@@ -321,6 +327,12 @@ int vti2d_batch_tune(vti_ctx *ctx, const int *shots, int ns, const int *rloc, in
  * 1 restarts at stp = 1 as in Liu & Nocedal's method. */
 typedef struct vti_lbfgs vti_lbfgs;
 int vti_lbfgs_create(vti_lbfgs **state, long long n, int m, int unit_step);
+/* The same method with its vectors resident in device memory (FP64: the m
+ * stored steps and gradient differences, the direction, the base point, the
+ * gradients), every vector operation a kernel on ctx's stream and the dot
+ * products reduced there in a fixed order (lbfgs_c.cpp:226-240,350-428,
+ * la.cpp:7-31); vti_lbfgs_step then takes x and g in host or device memory. */
+int vti_lbfgs_create_device(vti_lbfgs **state, vti_ctx *ctx, long long n, int m, int unit_step);
 int vti_lbfgs_destroy(vti_lbfgs *state);
 int vti_lbfgs_step(vti_lbfgs *state, float *x, double f, const float *g, int *iflag, int *iter, int *nfun);
 

@@ -151,9 +151,10 @@ struct BatchArgs {
     {
         // 8 wavefield planes, 3 gradient planes, the receiver map, the
         // snapshots (and for 16-bit ones two staging planes and the slice
-        // scales/maxima) and two trace copies
+        // scales/maxima, or the segment scales of S16) and two trace copies
         const long long nxz = (long long)(in[s].nx + 2 * npad) * nnz;
-        const long long snapb = half ? 2LL * nxz * 2 * nnt + 4LL * nxz * 2 + 16LL * nnt : 4LL * nxz * 2 * nnt;
+        const long long scaleb = snap_fmt == 3 ? 4LL * 2 * nnt * ((nxz + 63) / 64) : 16LL * nnt;
+        const long long snapb = half ? 2LL * nxz * 2 * nnt + 4LL * nxz * 2 + scaleb : 4LL * nxz * 2 * nnt;
         return 4LL * nxz * (8 + 3 + 1) + snapb + 8LL * nnt_data * in[s].nr + 64;
     }
 };
@@ -167,8 +168,8 @@ int parse_args(BatchArgs &a, const int *shots, int ns, const int *rloc, int nr, 
     if (tile_rows == 0) tile_rows = 16;
     VTI_CHECK(levels == 1 || levels == 2, "levels per launch must be 1 or 2, got %d", levels);
     VTI_CHECK(tile_rows == 4 || tile_rows == 8 || tile_rows == 16, "tile rows must be 4, 8 or 16, got %d", tile_rows);
-    VTI_CHECK(snap_fmt >= 0 && snap_fmt <= 2, "snapshot format must be 0 (FP32), 1 (FP16) or 2 (BF16), got %d",
-              snap_fmt);
+    VTI_CHECK(snap_fmt >= 0 && snap_fmt <= 3,
+              "snapshot format must be 0 (FP32), 1 (FP16), 2 (BF16) or 3 (S16), got %d", snap_fmt);
     a.ns = ns;
     a.nr = nr;
     a.NX = NX;
@@ -265,7 +266,7 @@ int upload_model(BatchShared &S, const BatchArgs &a, const float *c11, const flo
 // traces, the observed data and the residual's partial sums.
 struct BatchWave {
     int nw = 0, nrec = 0;
-    long long f_total = 0, snap_total = 0, tr_total = 0, max_plane = 0;
+    long long f_total = 0, snap_total = 0, tr_total = 0, max_plane = 0, scale_total = 0;
     std::vector<ShotB> sb, sbt;
     std::vector<int> tile_shot, tile_shot_t;
     DBuf<ShotB> sbd, sbtd;
@@ -295,7 +296,7 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
     W.sbt.assign(nw, ShotB{});
     W.tile_shot.clear();
     W.tile_shot_t.clear();
-    W.f_total = W.snap_total = W.tr_total = W.max_plane = 0;
+    W.f_total = W.snap_total = W.tr_total = W.max_plane = W.scale_total = 0;
     W.nrec = 0;
     for (int j = 0; j < nw; ++j) {
         const ShotIn &q = a.in[s0 + j];
@@ -318,7 +319,8 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
             b.rz0 = std::min(b.rz0, z);
             b.rz1 = std::max(b.rz1, z);
         }
-        b.scale_off = 2 * nnt * j;
+        // FP16/BF16: 2 slice scales per slot; S16: 2 * nseg segment scales per slot
+        b.scale_off = a.snap_fmt == 3 ? W.scale_total : 2LL * nnt * j;
         W.sbt[j] = b;
         W.sbt[j].tile0 = (int)W.tile_shot_t.size();
         for (int t = 0; t < b.ntx * ntz_t; ++t) W.tile_shot_t.push_back(j);
@@ -326,6 +328,7 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
         W.max_plane = std::max(W.max_plane, plane);
         W.f_total += plane;
         W.snap_total += (long long)nnt * 2 * plane;
+        W.scale_total += (long long)nnt * 2 * ((plane + 63) / 64);
         W.tr_total += (long long)a.nnt_data * q.nr;
         W.nrec += q.nr;
     }
@@ -368,7 +371,7 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
     if (a.half) {
         if (!rc) rc = W.snaph.alloc((size_t)W.snap_total, st, false);
         if (!rc) rc = W.stage.alloc((size_t)(2 * f_total), st);
-        if (!rc) rc = W.scale.alloc((size_t)(2 * nnt * nw), st);
+        if (!rc) rc = W.scale.alloc((size_t)(a.snap_fmt == 3 ? W.scale_total : 2 * nnt * nw), st);
         if (!rc) rc = W.amax.alloc((size_t)(2 * nnt * nw), st);
     } else {
         if (!rc) rc = W.snapf.alloc((size_t)W.snap_total, st, false);
@@ -430,7 +433,7 @@ int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *
     }
     for (int k = 0; k < 3; ++k) zero_words(G[k].p, (size_t)f_total, st);
     zero_words(W.tr.p, W.tr.n, st);  // row 0 is never recorded
-    if (half) zero_words(W.amax.p, W.amax.n, st);
+    if (half && snap_fmt != 3) zero_words(W.amax.p, W.amax.n, st);
 
     BatchGeo bg;
     bg.nnz = nnz;
@@ -442,11 +445,17 @@ int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *
     const dim3 grid((unsigned)W.tile_shot.size()), block(64, kBY2);
     const dim3 gridt((unsigned)W.tile_shot_t.size()), blockt(64, BYT);
     const int rb = (nrec + 255) / 256;
-    const vti::SnapW sw{half ? W.stage.p : W.snapf.p, half ? W.amax.p : nullptr};
+    const vti::SnapW sw{half ? W.stage.p : W.snapf.p, half && snap_fmt != 3 ? W.amax.p : nullptr, half};
     const vti::SnapR sr{W.snapf.p, W.snaph.p, W.scale.p};
     const dim3 gpack((unsigned)std::min<long long>((2 * W.max_plane + 255) / 256, 1024), (unsigned)nw);
+    // S16: one wavefront per 64-point segment of either component
+    const dim3 gpack16((unsigned)std::min<long long>((2 * ((W.max_plane + 63) / 64) + 3) / 4, 2048), (unsigned)nw);
     auto pack = [&](int slot) {
-        if (half && slot >= 0)
+        if (!half || slot < 0) return;
+        if (snap_fmt == 3)
+            hipLaunchKernelGGL(vti::pack_s16_kernel, gpack16, dim3(256), 0, st, W.snaph.p, W.scale.p, W.stage.p,
+                               W.sbd.p, nnz, slot);
+        else
             hipLaunchKernelGGL(vti::pack_snap_kernel, gpack, dim3(256), 0, st, W.snaph.p, W.scale.p, W.stage.p,
                                W.amax.p, W.sbd.p, nnz, slot, snap_fmt == 2 ? 1 : 0);
     };
@@ -530,7 +539,8 @@ int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *
                        G[0].p, G[1].p, G[2].p)
         if (snap_fmt == 0) XCOR(0);
         else if (snap_fmt == 1) XCOR(1);
-        else XCOR(2);
+        else if (snap_fmt == 2) XCOR(2);
+        else XCOR(3);
 #undef XCOR
     }
     // trace rows and snapshot slots bracketing adjoint level it
@@ -557,7 +567,8 @@ int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *
 #define BWD_T2_SF(BYV)                      \
     if (snap_fmt == 0) BWD_T2(BYV, 0);      \
     else if (snap_fmt == 1) BWD_T2(BYV, 1); \
-    else BWD_T2(BYV, 2)
+    else if (snap_fmt == 2) BWD_T2(BYV, 2); \
+    else BWD_T2(BYV, 3)
             if (BYT == 4) {
                 BWD_T2_SF(4);
             } else if (BYT == 8) {
@@ -578,7 +589,8 @@ int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *
                        G[2].p, dx2, dz2, dt2)
             if (snap_fmt == 0) BWD1(0);
             else if (snap_fmt == 1) BWD1(1);
-            else BWD1(2);
+            else if (snap_fmt == 2) BWD1(2);
+            else BWD1(3);
 #undef BWD1
             std::swap(ip, ic);
             it -= 1;

@@ -20,6 +20,11 @@
 // write the FP32 slice into a staging area and fold |D| into a per-slice
 // maximum (one atomic per block); pack_snap_kernel rounds the staged slice to
 // 16 bits after the launch.  The adjoint reads h * s.
+// 3 = S16: 16-bit signed integers with a power-of-two scale per segment of 64
+// consecutive points of a slice (the scale puts the segment's largest
+// magnitude in [2^14, 2^15)), so every value keeps 15 bits relative to its
+// neighbourhood's largest instead of FP16's 11 bits relative to itself;
+// pack_s16_kernel finds each segment's maximum itself (no atomics).
 //
 // Every step kernel issues the loads of its points' own operands together
 // with the LDS tile loads, before the barrier, so that a workgroup waits for
@@ -46,7 +51,7 @@ struct ShotB {
     long long stage_off;  // offset of the shot's staging planes (16-bit snapshots)
     int src;              // source point within the sub-grid (ix + iz * nnx)
     int rz0, rz1;         // rows holding receivers (rz0 > rz1 when there are none)
-    int scale_off;        // first of the shot's 2*nnt slice scales / maxima
+    long long scale_off;  // first of the shot's slice scales / maxima (2 per slot), or segment scales (S16)
 };
 
 // The tile a workgroup works on.  Workgroups are dealt to the 8 XCDs in
@@ -69,16 +74,25 @@ struct BatchGeo {
 
 // Where a forward kernel puts the D snapshot of the level it reads.
 struct SnapW {
-    float *d;        // FP32 slots (amax == nullptr) or the staging planes
-    unsigned *amax;  // per-slice maxima of |D| as float bits, or nullptr
+    float *d;        // FP32 slots (staged == false) or the staging planes
+    unsigned *amax;  // per-slice maxima of |D| as float bits (FP16/BF16), or nullptr
+    bool staged;     // 16-bit formats: D goes to the staging planes
 };
 
 // Where the adjoint kernels read them.
 struct SnapR {
     const float *f;           // FP32 slots
     const unsigned short *h;  // 16-bit slots
-    const float *scale;       // per-slice scale of the 16-bit slots
+    const float *scale;       // per-slice (FP16/BF16) or per-segment (S16) scales of the 16-bit slots
 };
+
+// S16: segments of a component plane of `plane` points, and the index of the
+// scale of point i's segment in SnapR::scale
+__device__ __forceinline__ long long s16_nseg(long long plane) { return (plane + 63) >> 6; }
+__device__ __forceinline__ long long s16_scale_idx(const ShotB &sh, long long plane, int slot, int c, long long i)
+{
+    return sh.scale_off + (2LL * slot + c) * s16_nseg(plane) + (i >> 6);
+}
 
 // taper of the point (ix, iz) of a sub-grid nnx wide: t0 along x, then along z
 // The taper of point (ix, iz) as two loaded factors whose product it is:
@@ -128,12 +142,13 @@ __device__ __forceinline__ float snap_raw32(const SnapR &r, const ShotB &sh, lon
     if (SF == 0) return ld32(r.f + o, i);
     const unsigned short h = ld32(r.h + o, i);
     if (SF == 1) return __half2float(__ushort_as_half(h));
+    if (SF == 3) return (float)(short)h;
     return __uint_as_float((unsigned)h << 16);
 }
 __device__ __forceinline__ void snap_put32(const SnapW &w, const ShotB &sh, long long plane, int slot, unsigned i,
                                            float tp, float tq)
 {
-    float *sp = w.amax ? w.d + sh.stage_off : w.d + sh.snap_off + (long long)slot * 2 * plane;
+    float *sp = w.staged ? w.d + sh.stage_off : w.d + sh.snap_off + (long long)slot * 2 * plane;
     st32(sp, i, tp);
     st32(sp + plane, i, tq);
 }
@@ -142,7 +157,7 @@ __device__ __forceinline__ void snap_put32(const SnapW &w, const ShotB &sh, long
 __device__ __forceinline__ void snap_put(const SnapW &w, const ShotB &sh, long long plane, int slot, long long i,
                                          float tp, float tq)
 {
-    float *sp = w.amax ? w.d + sh.stage_off : w.d + sh.snap_off + (long long)slot * 2 * plane;
+    float *sp = w.staged ? w.d + sh.stage_off : w.d + sh.snap_off + (long long)slot * 2 * plane;
     sp[i] = tp;
     sp[i + plane] = tq;
 }
@@ -182,12 +197,15 @@ __device__ __forceinline__ float snap_raw(const SnapR &r, const ShotB &sh, long 
     const long long k = sh.snap_off + (long long)slot * 2 * plane + c * plane + i;
     if (SF == 0) return r.f[k];
     if (SF == 1) return __half2float(__ushort_as_half(r.h[k]));
+    if (SF == 3) return (float)(short)r.h[k];
     return __uint_as_float((unsigned)r.h[k] << 16);
 }
 template <int SF>
-__device__ __forceinline__ float snap_mul(const SnapR &r, const ShotB &sh, int slot, int c, float raw)
+__device__ __forceinline__ float snap_mul(const SnapR &r, const ShotB &sh, long long plane, int slot, int c,
+                                          long long i, float raw)
 {
     if (SF == 0) return raw;
+    if (SF == 3) return raw * r.scale[s16_scale_idx(sh, plane, slot, c, i)];
     return raw * r.scale[sh.scale_off + 2 * slot + c];
 }
 
@@ -198,6 +216,7 @@ __device__ __forceinline__ float snap_get(const SnapR &r, const ShotB &sh, long 
 {
     const long long k = sh.snap_off + (long long)slot * 2 * plane + c * plane + i;
     if (SF == 0) return r.f[k];
+    if (SF == 3) return (float)(short)r.h[k] * r.scale[s16_scale_idx(sh, plane, slot, c, i)];
     const float s = r.scale[sh.scale_off + 2 * slot + c];
     if (SF == 1) return __half2float(__ushort_as_half(r.h[k])) * s;
     return __uint_as_float((unsigned)r.h[k] << 16) * s;
@@ -237,6 +256,43 @@ static __global__ void pack_snap_kernel(unsigned short *__restrict__ snap, float
             out[k] = (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
         } else {
             out[k] = __half_as_ushort(__float2half_rn(v));
+        }
+    }
+}
+
+// S16: round the staged slices of slot `slot` of every shot (blockIdx.y) to
+// 16-bit integers, one wavefront per segment of 64 consecutive points of a
+// component plane.  The segment's scale 2^e puts its largest magnitude m in
+// [2^14, 2^15) (1 for an all-zero segment); values are v / 2^e rounded to
+// nearest even, clamped to +-32767 (m itself can round to 2^15).
+static __global__ __launch_bounds__(256) void pack_s16_kernel(unsigned short *__restrict__ snap,
+                                                              float *__restrict__ scale,
+                                                              const float *__restrict__ stage,
+                                                              const ShotB *__restrict__ shots, int nnz, int slot)
+{
+    const ShotB sh = shots[blockIdx.y];
+    const long long plane = (long long)sh.nnx * nnz, nseg = s16_nseg(plane);
+    const int lane = threadIdx.x & 63;
+    const long long nwaves = (long long)gridDim.x * (blockDim.x >> 6);
+    unsigned short *out = snap + sh.snap_off + (long long)slot * 2 * plane;
+    const float *in = stage + sh.stage_off;
+    // segments of both components: g < nseg is component 0, else 1
+    for (long long g = blockIdx.x * (long long)(blockDim.x >> 6) + (threadIdx.x >> 6); g < 2 * nseg; g += nwaves) {
+        const int c = g < nseg ? 0 : 1;
+        const long long i = ((c ? g - nseg : g) << 6) + lane;  // point within the component plane
+        const bool ok = i < plane;
+        const float v = ok ? in[c * plane + i] : 0.f;
+        float m = fabsf(v);
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        int e = 0;
+        if (m > 0.f) {
+            (void)frexpf(m, &e);  // m in [2^(e-1), 2^e)
+            e = max(e - 15, -126);
+        }
+        if (lane == 0) scale[sh.scale_off + (2LL * slot) * nseg + g] = ldexpf(1.f, e);
+        if (ok) {
+            const float q = fminf(fmaxf(rintf(ldexpf(v, -e)), -32767.f), 32767.f);
+            out[c * plane + i] = (unsigned short)(short)q;
         }
     }
 }
@@ -428,10 +484,10 @@ static __global__ __launch_bounds__(64 * kBY2) void bwd2db_kernel(
     __syncthreads();
     if (!act) return;
     const float tap = taper_of(tw);
-    d0p = snap_mul<SF>(sr, sh, slot0, 0, d0p);
-    d1p = snap_mul<SF>(sr, sh, slot0 + 1, 0, d1p);
-    d0q = snap_mul<SF>(sr, sh, slot0, 1, d0q);
-    d1q = snap_mul<SF>(sr, sh, slot0 + 1, 1, d1q);
+    d0p = snap_mul<SF>(sr, sh, plane, slot0, 0, i, d0p);
+    d1p = snap_mul<SF>(sr, sh, plane, slot0 + 1, 0, i, d1p);
+    d0q = snap_mul<SF>(sr, sh, plane, slot0, 1, i, d0q);
+    d1q = snap_mul<SF>(sr, sh, plane, slot0 + 1, 1, i, d1q);
     const float tp0 = dxx(t, dx2, bg.rx2), tq0 = dzz(t, dz2, bg.rz2);
     float pn = 2.f * pc + dt2 * tp0 - pp * tap;
     float qn = 2.f * qc + dt2 * tq0 - qp * tap;
@@ -925,14 +981,14 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
         b0q = a0q;
         b1q = a1q;
     }
-    a0p = snap_mul<SF>(sr, sh, l1.slot, 0, a0p);
-    a1p = snap_mul<SF>(sr, sh, l1.slot + 1, 0, a1p);
-    a0q = snap_mul<SF>(sr, sh, l1.slot, 1, a0q);
-    a1q = snap_mul<SF>(sr, sh, l1.slot + 1, 1, a1q);
-    b0p = snap_mul<SF>(sr, sh, l2.slot, 0, b0p);
-    b1p = snap_mul<SF>(sr, sh, l2.slot + 1, 0, b1p);
-    b0q = snap_mul<SF>(sr, sh, l2.slot, 1, b0q);
-    b1q = snap_mul<SF>(sr, sh, l2.slot + 1, 1, b1q);
+    a0p = snap_mul<SF>(sr, sh, plane, l1.slot, 0, i, a0p);
+    a1p = snap_mul<SF>(sr, sh, plane, l1.slot + 1, 0, i, a1p);
+    a0q = snap_mul<SF>(sr, sh, plane, l1.slot, 1, i, a0q);
+    a1q = snap_mul<SF>(sr, sh, plane, l1.slot + 1, 1, i, a1q);
+    b0p = snap_mul<SF>(sr, sh, plane, l2.slot, 0, i, b0p);
+    b1p = snap_mul<SF>(sr, sh, plane, l2.slot + 1, 0, i, b1p);
+    b0q = snap_mul<SF>(sr, sh, plane, l2.slot, 1, i, b0q);
+    b1q = snap_mul<SF>(sr, sh, plane, l2.slot + 1, 1, i, b1q);
     const float Dp1 = (1.f - l1.f) * a0p + l1.f * a1p;
     const float Dq1 = (1.f - l1.f) * a0q + l1.f * a1q;
     const float Dp2 = (1.f - l2.f) * b0p + l2.f * b1p;

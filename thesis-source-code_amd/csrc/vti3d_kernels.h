@@ -937,8 +937,8 @@ __global__ __launch_bounds__(1024) void fwd3d_t2b_kernel(
         } else {
             q1s[z % 9][tid] = 0.f;
         }
-        __syncthreads();
-        // ---- level L+2 at plane z-4 on the tile
+        // ---- level L+2 at plane z-4 on the tile (its ring plane was
+        // written four iterations ago)
         if (step2) {
             const float *T = &ring[zp % 5][ey0 * EW + ex0];
             const float s0 = T[0];
@@ -969,6 +969,9 @@ __global__ __launch_bounds__(1024) void fwd3d_t2b_kernel(
             dx_[gi] = vx;
             dz_[gi] = vz;
         }
+        // the next iteration overwrites the tile buffer read above and the
+        // ring slot of plane z-4
+        __syncthreads();
         // advance to plane z+1: queues, operands
 #pragma unroll
         for (int k = 0; k < 8; ++k) q[k] = q[k + 1];

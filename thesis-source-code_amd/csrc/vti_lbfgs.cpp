@@ -8,7 +8,8 @@
 //  - m corrections, diagonal H0 = (y's)/(y'y) of the newest pair;
 //  - first trial step 1/|g0|; every later line search starts from the step
 //    the previous one accepted (lbfgs_c.cpp:433-436: stp is not reset);
-//  - a function value above 1e10 halves the trial step (lsearch.cpp:367-380);
+//  - a function value above 1e10 (or a NaN) halves the trial step
+//    (lsearch.cpp:367-380);
 //  - stop when |g| / max(1, |x|) <= 1e-5 (lbfgs_c.cpp:336-340);
 //  - ftol 1e-4, gtol 0.9, xtol 1e-16, step bounds [1e-10, 1e10], 20
 //    evaluations per search, extrapolation factor 4 (lbfgs.h:8-15).
@@ -19,24 +20,19 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <vector>
 
 #include "vti_amd.h"
 #include "vti_internal.h"
+#include "vti_lbfgs_ops.h"
 
 namespace {
 
 constexpr double kEps = 1e-5, kXtol = 1e-16, kFtol = 1e-4, kGtol = 0.9, kStpmin = 1e-10, kStpmax = 1e10;
 constexpr double kXtrapf = 4.0;
 constexpr int kMaxfev = 20;
-
-double dot(size_t n, const double *a, const double *b)
-{
-    double s = 0.0;
-    for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
-    return s;
-}
 
 // MCSTEP (lsearch.cpp:9-201): safeguarded cubic/quadratic step and update of
 // the interval of uncertainty [stx, sty].
@@ -125,15 +121,90 @@ void mcstep(double &stx, double &fx, double &dx, double &sty, double &fy, double
     }
 }
 
+// Host vectors: every loop and sum in index order.
+class HostOps final : public vti::LbfgsOps {
+public:
+    explicit HostOps(size_t n, int m) : n_(n), v_((size_t)(kPairs + 2 * m), std::vector<double>(n, 0.0)) {}
+    void load_g(const float *g) override
+    {
+        double *G = v_[kG].data();
+        for (size_t i = 0; i < n_; ++i) G[i] = g[i];
+    }
+    double dot(int a, int b) override
+    {
+        const double *x = v_[a].data(), *y = v_[b].data();
+        double s = 0.0;
+        for (size_t i = 0; i < n_; ++i) s += x[i] * y[i];
+        return s;
+    }
+    void scale_copy(int dst, int src, double a) override
+    {
+        double *d = v_[dst].data();
+        const double *x = v_[src].data();
+        for (size_t i = 0; i < n_; ++i) d[i] = x[i] * a;
+    }
+    void axpy(int dst, double a, int src) override
+    {
+        double *d = v_[dst].data();
+        const double *x = v_[src].data();
+        for (size_t i = 0; i < n_; ++i) d[i] += a * x[i];
+    }
+    void scale(int dst, double a) override
+    {
+        double *d = v_[dst].data();
+        for (size_t i = 0; i < n_; ++i) d[i] = a * d[i];
+    }
+    void copy(int dst, int src) override { v_[dst] = v_[src]; }
+    void store_pair(int slot, double stp) override
+    {
+        const int m = (int)(v_.size() - kPairs) / 2;
+        double *S = v_[kPairs + slot].data(), *Y = v_[kPairs + m + slot].data();
+        const double *s = v_[kS].data(), *g = v_[kG].data(), *w = v_[kW].data();
+        for (size_t i = 0; i < n_; ++i) {
+            S[i] = stp * s[i];
+            Y[i] = g[i] - w[i];
+        }
+    }
+    void save_x0(const float *x) override
+    {
+        double *X0 = v_[kX0].data();
+        for (size_t i = 0; i < n_; ++i) X0[i] = x[i];
+    }
+    void set_trial(float *x, double stp) override
+    {
+        const double *X0 = v_[kX0].data(), *s = v_[kS].data();
+        for (size_t i = 0; i < n_; ++i) x[i] = (float)(X0[i] + stp * s[i]);
+    }
+    double sumsq_x(const float *x) override
+    {
+        double s = 0.0;
+        for (size_t i = 0; i < n_; ++i) s += (double)x[i] * (double)x[i];
+        return s;
+    }
+
+private:
+    size_t n_;
+    std::vector<std::vector<double>> v_;
+};
+
 }  // namespace
 
+vti::LbfgsOps *vti::lbfgs_host_ops(size_t n, int m)
+{
+    try {
+        return new HostOps(n, m);
+    } catch (...) {
+        return nullptr;
+    }
+}
+
 struct vti_lbfgs {
-    size_t n = 0;
     int m = 0;
-    // the method's vectors: g0 (the gradient at the line search's base point,
-    // then the two-loop work vector), rho[m], alpha[m], the last m steps and
-    // gradient differences in circular order (lbfgs_c.cpp:226-240)
-    std::vector<double> w, rho, alpha, S, Y, diag, x0, s;
+    // the method's vectors, on the host or in device memory (vti::LbfgsOps:
+    // G, W, S, X0 and the last m steps and gradient differences in circular
+    // order, lbfgs_c.cpp:226-240); rho[m], alpha[m] and the scalars here
+    std::unique_ptr<vti::LbfgsOps> v;
+    std::vector<double> rho, alpha;
     int iter = 0, point = 0, nfun = 0, bound = 0, nfev = 0, info = 0;
     double stp = 0.0;
     // line-search state (MCSRCH's SAVE variables)
@@ -144,19 +215,21 @@ struct vti_lbfgs {
     bool active = false;
     bool unit_step = false;  // restart each line search at stp = 1 (Liu & Nocedal) instead
 
-    // x = x0 + stp*s in FP64, rounded to the caller's float x
-    void set_trial(float *x)
-    {
-        for (size_t i = 0; i < n; ++i) x[i] = (float)(x0[i] + stp * s[i]);
-    }
+    using Ops = vti::LbfgsOps;
+    int Sk(int k) const { return Ops::kPairs + k; }
+    int Yk(int k) const { return Ops::kPairs + m + k; }
 
-    // MCSRCH entry with a new direction s (lsearch.cpp:493-549); returns
-    // -1 (evaluate f, g at x) or 0 (bad input / not a descent direction)
-    int search_start(float *x, double f, const double *g)
+    // x = x0 + stp*s in FP64, rounded to the caller's float x
+    void set_trial(float *x) { v->set_trial(x, stp); }
+
+    // MCSRCH entry with a new direction s (lsearch.cpp:493-549), the gradient
+    // at x in G; returns -1 (evaluate f, g at x) or 0 (bad input / not a
+    // descent direction)
+    int search_start(float *x, double f)
     {
         infoc = 1;
         if (stp <= 0.0) return 0;
-        dginit = dot(n, g, s.data());
+        dginit = v->dot(Ops::kG, Ops::kS);
         if (dginit >= 0.0) return 0;
         brackt = false;
         stage1 = true;
@@ -165,7 +238,7 @@ struct vti_lbfgs {
         dgtest = kFtol * dginit;
         width = kStpmax - kStpmin;
         width1 = 2 * width;
-        for (size_t i = 0; i < n; ++i) x0[i] = x[i];
+        v->save_x0(x);
         stx = 0.0;
         fx = finit;
         dgx = dginit;
@@ -193,14 +266,17 @@ struct vti_lbfgs {
         return -1;
     }
 
-    // MCSRCH re-entry with f, g at the trial point (lsearch.cpp:344-491):
+    // MCSRCH re-entry with f, g (in G) at the trial point (lsearch.cpp:344-491):
     // -1 = evaluate again, 1 = step accepted, 2..6 = the reference's failures
-    int search_continue(float *x, double f, const double *g)
+    int search_continue(float *x, double f)
     {
         ++nfev;
-        const double dg = dot(n, g, s.data());
+        const double dg = v->dot(Ops::kG, Ops::kS);
         const double ftest1 = finit + stp * dgtest;
-        if (f > 1e10) {
+        // a function value above 1e10 halves the step (lsearch.cpp:367-380);
+        // so does a NaN (an unstable trial model), which the reference's
+        // comparison lets through
+        if (!(f <= 1e10)) {
             stp = stp / 2;
             set_trial(x);
             return -1;
@@ -233,34 +309,46 @@ struct vti_lbfgs {
         return next_trial(x);
     }
 
-    // -H*g by the two-loop recursion (lbfgs_c.cpp:371-428) into s
-    void direction(const double *g)
+    // -H*g by the two-loop recursion (lbfgs_c.cpp:371-428) into S; the
+    // diagonal H0 = (y's)/(y'y) of the newest pair is one scalar
+    void direction()
     {
-        const double ys = dot(n, &Y[(size_t)((point + m - 1) % m) * n], &S[(size_t)((point + m - 1) % m) * n]);
-        const double yy = dot(n, &Y[(size_t)((point + m - 1) % m) * n], &Y[(size_t)((point + m - 1) % m) * n]);
-        for (size_t i = 0; i < n; ++i) diag[i] = ys / yy;
-        rho[(point + m - 1) % m] = 1.0 / ys;
-        for (size_t i = 0; i < n; ++i) w[i] = -g[i];
+        const int last = (point + m - 1) % m;
+        const double ys = v->dot(Yk(last), Sk(last));
+        const double yy = v->dot(Yk(last), Yk(last));
+        const double h0 = ys / yy;
+        rho[last] = 1.0 / ys;
+        v->scale_copy(Ops::kW, Ops::kG, -1.0);
         int cp = point;
         for (int k = 0; k < bound; ++k) {
             cp = (cp + m - 1) % m;
-            const double sq = dot(n, &S[(size_t)cp * n], w.data());
+            const double sq = v->dot(Sk(cp), Ops::kW);
             alpha[cp] = rho[cp] * sq;
-            const double a = -alpha[cp];
-            const double *y = &Y[(size_t)cp * n];
-            for (size_t i = 0; i < n; ++i) w[i] += a * y[i];
+            v->axpy(Ops::kW, -alpha[cp], Yk(cp));
         }
-        for (size_t i = 0; i < n; ++i) w[i] = diag[i] * w[i];
+        v->scale(Ops::kW, h0);
         for (int k = 0; k < bound; ++k) {
-            const double yr = dot(n, &Y[(size_t)cp * n], w.data());
+            const double yr = v->dot(Yk(cp), Ops::kW);
             const double beta = alpha[cp] - rho[cp] * yr;
-            const double *sv = &S[(size_t)cp * n];
-            for (size_t i = 0; i < n; ++i) w[i] += beta * sv[i];
+            v->axpy(Ops::kW, beta, Sk(cp));
             cp = (cp + 1) % m;
         }
-        for (size_t i = 0; i < n; ++i) s[i] = w[i];
+        v->copy(Ops::kS, Ops::kW);
     }
 };
+
+namespace {
+
+int init_state(vti_lbfgs *s, int m, int unit_step)
+{
+    s->m = m;
+    s->unit_step = unit_step != 0;
+    s->rho.assign(m, 0.0);
+    s->alpha.assign(m, 0.0);
+    return vti::kOk;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -269,22 +357,29 @@ int vti_lbfgs_create(vti_lbfgs **out, long long n, int m, int unit_step)
     VTI_CHECK(out && n > 0 && m > 0, "vti_lbfgs_create: n and m must be positive");
     vti_lbfgs *s = new (std::nothrow) vti_lbfgs();
     VTI_CHECK(s, "vti_lbfgs_create: out of host memory");
-    try {
-        s->n = (size_t)n;
-        s->m = m;
-        s->unit_step = unit_step != 0;
-        s->w.assign(s->n, 0.0);
-        s->rho.assign(m, 0.0);
-        s->alpha.assign(m, 0.0);
-        s->S.assign((size_t)m * s->n, 0.0);
-        s->Y.assign((size_t)m * s->n, 0.0);
-        s->diag.assign(s->n, 1.0);
-        s->x0.assign(s->n, 0.0);
-        s->s.assign(s->n, 0.0);
-    } catch (...) {
+    s->v.reset(vti::lbfgs_host_ops((size_t)n, m));
+    if (!s->v) {
         delete s;
         VTI_CHECK(false, "vti_lbfgs_create: out of host memory for n=%lld, m=%d", n, m);
     }
+    init_state(s, m, unit_step);
+    *out = s;
+    return vti::kOk;
+}
+
+int vti_lbfgs_create_device(vti_lbfgs **out, vti_ctx *ctx, long long n, int m, int unit_step)
+{
+    VTI_CHECK(out && ctx && n > 0 && m > 0, "vti_lbfgs_create_device: null context or non-positive n, m");
+    vti_lbfgs *s = new (std::nothrow) vti_lbfgs();
+    VTI_CHECK(s, "vti_lbfgs_create_device: out of host memory");
+    vti::LbfgsOps *ops = nullptr;
+    const int rc = vti::lbfgs_device_ops(ctx, (size_t)n, m, &ops);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    s->v.reset(ops);
+    init_state(s, m, unit_step);
     *out = s;
     return vti::kOk;
 }
@@ -299,23 +394,24 @@ int vti_lbfgs_destroy(vti_lbfgs *s)
 int vti_lbfgs_step(vti_lbfgs *s, float *x, double f, const float *gf, int *iflag, int *iter, int *nfun)
 {
     VTI_CHECK(s && x && gf && iflag, "vti_lbfgs_step: null argument");
-    const size_t n = s->n;
-    std::vector<double> g(gf, gf + n);
+    using Ops = vti::LbfgsOps;
+    vti::LbfgsOps *v = s->v.get();
+    VTI_CHECK(!v->failed(), "vti_lbfgs_step: an earlier call failed; the state is not usable");
+    v->load_g(gf);
     int res;
     if (*iflag == 0) {
-        // initial call (lbfgs_c.cpp:202-262)
+        // initial call (lbfgs_c.cpp:202-262): H0 = I
         s->iter = 1;
         s->nfun = 1;
         s->point = 0;
         s->bound = 0;
         s->info = 0;
-        for (size_t i = 0; i < n; ++i) s->diag[i] = 1.0;
-        for (size_t i = 0; i < n; ++i) s->s[i] = -g[i] * s->diag[i];
-        const double gnorm = std::sqrt(dot(n, g.data(), g.data()));
+        v->scale_copy(Ops::kS, Ops::kG, -1.0);
+        const double gnorm = std::sqrt(v->dot(Ops::kG, Ops::kG));
         s->stp = 1.0 / gnorm;
         s->active = true;
-        res = s->search_start(x, f, g.data());
-        if (res == 0) {
+        res = s->search_start(x, f);
+        if (res == 0 && !v->failed()) {
             // a zero gradient (or no descent direction) at the start: the
             // reference's first call then returns with iflag still 0
             // (lbfgs_c.cpp:262-276), i.e. done
@@ -325,21 +421,15 @@ int vti_lbfgs_step(vti_lbfgs *s, float *x, double f, const float *gf, int *iflag
             return vti::kOk;
         }
     } else if (*iflag == 1 && s->active) {
-        res = s->search_continue(x, f, g.data());
+        res = s->search_continue(x, f);
         if (res == 1) {
             // accepted: store the step and gradient change (lbfgs_c.cpp:314-325)
             s->nfun += s->nfev;
-            double *S = &s->S[(size_t)s->point * n], *Y = &s->Y[(size_t)s->point * n];
-            for (size_t i = 0; i < n; ++i) {
-                S[i] = s->stp * s->s[i];
-                Y[i] = g[i] - s->w[i];
-            }
+            v->store_pair(s->point, s->stp);
             s->point = (s->point + 1) % s->m;
-            const double gnorm = std::sqrt(dot(n, g.data(), g.data()));
-            double xnorm = 0.0;
-            for (size_t i = 0; i < n; ++i) xnorm += (double)x[i] * (double)x[i];
-            xnorm = std::fmax(1.0, std::sqrt(xnorm));
-            if (gnorm / xnorm <= kEps) {
+            const double gnorm = std::sqrt(v->dot(Ops::kG, Ops::kG));
+            const double xnorm = std::fmax(1.0, std::sqrt(v->sumsq_x(x)));
+            if (gnorm / xnorm <= kEps && !v->failed()) {
                 s->active = false;
                 *iflag = 0;
                 if (iter) *iter = s->iter;
@@ -348,19 +438,25 @@ int vti_lbfgs_step(vti_lbfgs *s, float *x, double f, const float *gf, int *iflag
             }
             ++s->iter;
             s->bound = std::min(s->iter - 1, s->m);
-            s->direction(g.data());
+            s->direction();
             if (s->unit_step) s->stp = 1.0;  // else: the step just accepted, as the reference does
-            res = s->search_start(x, f, g.data());
+            res = s->search_start(x, f);
         }
     } else {
         VTI_CHECK(false, "vti_lbfgs_step: iflag must be 0 (start) or 1 (f, g evaluated at x), got %d", *iflag);
     }
-    // the gradient at the base point of the search just started (w, lbfgs_c.cpp:246,436)
-    if (res == -1 && s->nfev == 0) {
-        for (size_t i = 0; i < n; ++i) s->w[i] = g[i];
+    if (v->failed()) {
+        s->active = false;
+        return vti::kErrHip;  // the failing operation set the message
     }
+    // the gradient at the base point of the search just started (w, lbfgs_c.cpp:246,436)
+    if (res == -1 && s->nfev == 0) v->copy(Ops::kW, Ops::kG);
     if (iter) *iter = s->iter;
     if (nfun) *nfun = s->nfun;
+    if (v->failed()) {
+        s->active = false;
+        return vti::kErrHip;
+    }
     if (res == -1) {
         *iflag = 1;
         return vti::kOk;

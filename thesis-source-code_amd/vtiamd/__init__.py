@@ -111,6 +111,7 @@ _SIGS = {
     "vti2d_batch_plan_run": (c_int, [P] * 8),
     "vti2d_batch_plan_destroy": (c_int, [P]),
     "vti_lbfgs_create": (c_int, [POINTER(c_void_p), c_longlong, c_int, c_int]),
+    "vti_lbfgs_create_device": (c_int, [POINTER(c_void_p), P, c_longlong, c_int, c_int]),
     "vti_lbfgs_destroy": (c_int, [P]),
     "vti_lbfgs_step": (c_int, [P, P, ctypes.c_double, P, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
     "vti_serve_create": (c_int, [P, c_int, c_int, ctypes.c_double]),
@@ -547,9 +548,15 @@ class LBFGS:
     ...     if iflag != 1: break
     """
 
-    def __init__(self, n: int, m: int = 5, unit_step: bool = False):
+    def __init__(self, n: int, m: int = 5, unit_step: bool = False, ctx=None):
+        """ctx: keep the method's FP64 vectors in that context's device memory
+        (vti_lbfgs_create_device); x and g stay host arrays here."""
         h = c_void_p()
-        check(_lib.vti_lbfgs_create(ctypes.byref(h), n, m, int(unit_step)), "vti_lbfgs_create")
+        if ctx is None:
+            check(_lib.vti_lbfgs_create(ctypes.byref(h), n, m, int(unit_step)), "vti_lbfgs_create")
+        else:
+            check(_lib.vti_lbfgs_create_device(ctypes.byref(h), ctx.h, n, m, int(unit_step)),
+                  "vti_lbfgs_create_device")
         self.h = h
         self.iter = 0
         self.nfun = 0
@@ -578,7 +585,7 @@ class LBFGS:
             pass
 
 
-SNAP_FORMATS = {"fp32": 0, "fp16": 1, "bf16": 2}
+SNAP_FORMATS = {"fp32": 0, "fp16": 1, "bf16": 2, "s16": 3}
 
 
 def tune_batch2d(ctx, shots, rloc, NX, nz, nt, npad, dx, dz, dt, rate):
