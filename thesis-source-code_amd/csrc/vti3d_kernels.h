@@ -67,6 +67,20 @@ __device__ __forceinline__ void bstore(rsrc_t r, unsigned lane_byte, unsigned un
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)lane_byte, (int)uni_byte, 0);
 }
 
+// The same with the non-temporal cache policy (the SLC/NT bit, 2), for the
+// streams a step touches once per point -- the previous level it overwrites,
+// the coefficients, the accumulators -- so that they do not displace the
+// sigma planes whose halo rows neighbouring workgroups read again from L2.
+constexpr int kNonTemporal = 2;
+__device__ __forceinline__ float bload_nt(rsrc_t r, unsigned lane_byte, unsigned uni_byte)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)lane_byte, (int)uni_byte, kNonTemporal));
+}
+__device__ __forceinline__ void bstore_nt(rsrc_t r, unsigned lane_byte, unsigned uni_byte, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)lane_byte, (int)uni_byte, kNonTemporal);
+}
+
 template <int BY>
 struct Tile {
     // LDS row width: 64 columns + 4 halo on each side
@@ -341,11 +355,11 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
     // point data of plane z (prev and coefficients)
     float px = 0.f, pz = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f;
     if (inrange) {
-        px = bload(Rnx, col4, so(zs));
-        pz = bload(Rnz, col4, so(zs));
-        a11 = bload(R11, col4, so(zs));
-        a13 = bload(R13, col4, so(zs));
-        a33 = bload(R33, col4, so(zs));
+        px = bload_nt(Rnx, col4, so(zs));
+        pz = bload_nt(Rnz, col4, so(zs));
+        a11 = bload_nt(R11, col4, so(zs));
+        a13 = bload_nt(R13, col4, so(zs));
+        a33 = bload_nt(R33, col4, so(zs));
     }
     PlaneRegs<BY> P;
     P.loadb(Rcx, so(zs), L);
@@ -364,11 +378,11 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
             float npx = 0.f, npz = 0.f, n11 = 0.f, n13 = 0.f, n33 = 0.f, qn = 0.f;
             if (nx1) {
                 const unsigned o1 = so(z + 1);
-                npx = bload(Rnx, col4, o1);
-                npz = bload(Rnz, col4, o1);
-                n11 = bload(R11, col4, o1);
-                n13 = bload(R13, col4, o1);
-                n33 = bload(R33, col4, o1);
+                npx = bload_nt(Rnx, col4, o1);
+                npz = bload_nt(Rnz, col4, o1);
+                n11 = bload_nt(R11, col4, o1);
+                n13 = bload_nt(R13, col4, o1);
+                n33 = bload_nt(R33, col4, o1);
             }
             if (inrange && z + 5 < g.zmax) qn = bload(Rcz, col4, so(z + 5));
 
@@ -396,8 +410,8 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
                 vz = vz * d;
             }
             if (active) {
-                bstore(Rnx, col4, so(z), vx);
-                bstore(Rnz, col4, so(z), vz);
+                bstore_nt(Rnx, col4, so(z), vx);
+                bstore_nt(Rnz, col4, so(z), vz);
             }
             __syncthreads();
 #pragma unroll
@@ -1059,13 +1073,13 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
     float spx = 0.f, spz = 0.f, lpx = 0.f, lpz = 0.f, a11 = 0.f, a13 = 0.f, a33 = 0.f;
     if (inrange) {
         const unsigned o0 = so(zs);
-        spx = bload(Rsnx, col4, o0);
-        spz = bload(Rsnz, col4, o0);
-        lpx = bload(Rlnx, col4, o0);
-        lpz = bload(Rlnz, col4, o0);
-        a11 = bload(R11, col4, o0);
-        a13 = bload(R13, col4, o0);
-        a33 = bload(R33, col4, o0);
+        spx = bload_nt(Rsnx, col4, o0);
+        spz = bload_nt(Rsnz, col4, o0);
+        lpx = bload_nt(Rlnx, col4, o0);
+        lpz = bload_nt(Rlnz, col4, o0);
+        a11 = bload_nt(R11, col4, o0);
+        a13 = bload_nt(R13, col4, o0);
+        a33 = bload_nt(R33, col4, o0);
     }
     PlaneRegs<BY> PS, PL;
     PS.loadb(Rscx, so(zs), L);
@@ -1094,23 +1108,23 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
             float o11 = 0.f, o13 = 0.f, o33 = 0.f;
             if (inrange) {
                 const unsigned oz = so(z);
-                o11 = bload(G11, col4, oz);
+                o11 = bload_nt(G11, col4, oz);
                 if (MODE == 0) {
-                    o13 = bload(G13, col4, oz);
-                    o33 = bload(G33, col4, oz);
+                    o13 = bload_nt(G13, col4, oz);
+                    o33 = bload_nt(G33, col4, oz);
                 }
             }
             float n_spx = 0.f, n_spz = 0.f, n_lpx = 0.f, n_lpz = 0.f, n11 = 0.f, n13 = 0.f, n33 = 0.f;
             float qsn = 0.f, qln = 0.f;
             if (inrange && z + 1 < ze) {
                 const unsigned o1 = so(z + 1);
-                n_spx = bload(Rsnx, col4, o1);
-                n_spz = bload(Rsnz, col4, o1);
-                n_lpx = bload(Rlnx, col4, o1);
-                n_lpz = bload(Rlnz, col4, o1);
-                n11 = bload(R11, col4, o1);
-                n13 = bload(R13, col4, o1);
-                n33 = bload(R33, col4, o1);
+                n_spx = bload_nt(Rsnx, col4, o1);
+                n_spz = bload_nt(Rsnz, col4, o1);
+                n_lpx = bload_nt(Rlnx, col4, o1);
+                n_lpz = bload_nt(Rlnz, col4, o1);
+                n11 = bload_nt(R11, col4, o1);
+                n13 = bload_nt(R13, col4, o1);
+                n33 = bload_nt(R33, col4, o1);
             }
             if (inrange && z + 5 < g.zmax) {
                 qsn = bload(Rscz, col4, so(z + 5));
@@ -1166,14 +1180,14 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(4))) vo
             }
             if (active) {
                 const unsigned oz = so(z);
-                bstore(Rsnx, col4, oz, svx);
-                bstore(Rsnz, col4, oz, svz);
-                bstore(Rlnx, col4, oz, lvx);
-                bstore(Rlnz, col4, oz, lvz);
-                bstore(G11, col4, oz, o11);
+                bstore_nt(Rsnx, col4, oz, svx);
+                bstore_nt(Rsnz, col4, oz, svz);
+                bstore_nt(Rlnx, col4, oz, lvx);
+                bstore_nt(Rlnz, col4, oz, lvz);
+                bstore_nt(G11, col4, oz, o11);
                 if (MODE == 0) {
-                    bstore(G13, col4, oz, o13);
-                    bstore(G33, col4, oz, o33);
+                    bstore_nt(G13, col4, oz, o13);
+                    bstore_nt(G33, col4, oz, o33);
                 }
             }
             __syncthreads();
