@@ -93,49 +93,70 @@ def test_cfg2_gn_hessian_bitexact(vti, orc, ctx, cfg2):
 
 
 def test_cfg2_lbfgs_two_iterations(vti, orc, ctx, cfg2):
-    """Two iterations of the FP64 L-BFGS (m = 5) on x = c / 1e6 of one shot,
-    600-step record: the GPU-driven run and the oracle-driven run visit the
-    same models (bit for bit) and decrease the objective."""
+    """Two iterations of the FP64 L-BFGS (m = 5) as fwi-lbfgs-c-param.cu runs
+    them (parameter vepsdel: v/v0, eps/eps0, del/del0; checkEpsDel projects
+    eps and del in place each evaluation), one shot, 600-step record, both
+    with the host L-BFGS and with its device-resident form.  The objective
+    is divided by its initial value (so the reference's stopping test
+    |g|/max(1,|x|) <= 1e-5 does not end the run on the first step).  The
+    GPU-driven run (objFuncGradientVEpsDel on the GPU) and the oracle-driven
+    run (checkEpsDel, VEpsDel2Cij, the oracle's FP32 gradient,
+    GradCij2GradVEpsDel, each pinned on its own) visit the same models bit
+    for bit and decrease the objective."""
     P = cfg2
     nt = 601
     w = P["w"][:nt]
     _say("cfg2 L-BFGS: observed data, 600 steps")
     obs = vti.synthetic2d(ctx, *P["ctrue"], w, P["sl"], P["rloc"], P["taper"], *_a2(P, nt))
-    # x = c / 1e6: the first trial step (length 1 along -g, lbfgs_c.cpp:262)
-    # then moves a coefficient by at most 1e6, well inside stability
-    scale = np.float32(1e6)
+    v0, eps0, del0 = 100.0, 0.01, 0.01
+    v, e, d = P["init"]
+    nn = v.size
+    shape = v.shape
 
-    def run(grad):
-        x = (np.stack(P["cinit"]) / scale).astype(np.float32).ravel()
-        opt = vti.LBFGS(x.size, m=5)
-        fs, xs, iflag = [], [], 0
+    def run(grad, device):
+        x = np.concatenate([(v / np.float32(v0)).ravel(), (e / np.float32(eps0)).ravel(),
+                            (d / np.float32(del0)).ravel()]).astype(np.float32)
+        opt = vti.LBFGS(x.size, m=5, ctx=ctx if device else None)
+        fs, xs, iflag, f0 = [], [], 0, None
         while True:
-            c = (x.reshape(3, *P["cinit"][0].shape) * scale).astype(np.float32)
-            f, g = grad(c)
+            f, g, e2, d2 = grad(x[:nn], x[nn:2 * nn], x[2 * nn:])
+            x[nn:2 * nn], x[2 * nn:] = e2.ravel(), d2.ravel()
+            f0 = f if f0 is None else f0
             fs.append(f)
-            iflag = opt.step(x, f, (np.asarray(g, np.float64) * float(scale)).astype(np.float32), iflag)
+            iflag = opt.step(x, f / f0, (np.asarray(g, np.float64) / f0).astype(np.float32).ravel(), iflag)
             xs.append(x.copy())
             if iflag != 1 or opt.iter >= 3 or len(fs) >= 24:
                 break
-        return fs, xs, opt.iter
+        it = opt.iter
+        opt.close()
+        return fs, xs, it
 
-    def gpu(c):
-        return vti.objfunc_gradient2d(ctx, obs, *c, w, P["sl"], P["rloc"], P["taper"], *_a2(P, nt))
+    def gpu(xv, xe, xd):
+        f, g, e2, d2, _ = vti.objfunc_gradient2d_vepsdel(ctx, obs, xv, xe, xd, v0, eps0, del0, w, P["sl"],
+                                                         P["rloc"], P["taper"], *_a2(P, nt))
+        return f, g, e2, d2
 
-    def oracle(c):
-        f, g = orc.fg2d(orc.F32, False, obs, *c, None, w, P["sl"], P["rloc"], P["taper"], *_a2(P, nt))
-        return float(np.float32(f)), g
+    def oracle(xv, xe, xd):
+        ce, cd, _, _ = vti.check_eps_del(xe, xd, eps0, del0)
+        c = [a.reshape(shape) for a in vti.vepsdel_to_cij(xv, ce, cd, v0, eps0, del0)]
+        f, gc = orc.fg2d(orc.F32, False, obs, *c, None, w, P["sl"], P["rloc"], P["taper"], *_a2(P, nt))
+        gg = vti.gradcij_to_gradvepsdel(gc[0], gc[1], gc[2], xv, ce, cd, v0, eps0, del0)
+        rows = P["npad"] + 1  # wbottom 0: the reference zeroes npad + 1 rows
+        g = np.stack([a.reshape(shape) for a in gg])
+        g[:, :rows] = 0
+        return float(np.float32(f)), g, ce, cd
 
-    _say("cfg2 L-BFGS: GPU-driven")
-    fg, xg, itg = run(gpu)
-    _say(f"  objectives {fg}, iterations {itg}")
-    _say("cfg2 L-BFGS: oracle-driven")
-    fo, xo, ito = run(oracle)
-    assert itg >= 2 and itg == ito
-    assert fg == fo
-    for a, b in zip(xg, xo):
-        assert np.array_equal(a, b)
-    assert fg[-1] < fg[0]
+    for device in (False, True):
+        _say(f"cfg2 L-BFGS ({'device' if device else 'host'} vectors): GPU-driven")
+        fg, xg, itg = run(gpu, device)
+        _say(f"  objectives {fg}, iterations {itg}")
+        _say("cfg2 L-BFGS: oracle-driven")
+        fo, xo, ito = run(oracle, device)
+        assert itg >= 2 and itg == ito
+        assert fg == fo
+        for a, b in zip(xg, xo):
+            assert np.array_equal(a, b)
+        assert fg[-1] < fg[0]
 
 
 # ---------------------------------------------------------------------------
