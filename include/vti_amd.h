@@ -180,6 +180,22 @@ int vti2d_synthetic(vti_ctx *ctx, float *data, const float *c11, const float *c1
                     const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
                     int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot);
 
+/* vti2d_synthetic with no allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and the same data as vti2d_synthetic (its one-shot schedule, bit
+ * for bit); every device buffer lives in a fixed device array of about
+ * 52 MiB per GPU, held until the process ends.  Limits (kErrArg beyond
+ * them): nnx*nnz <= 2^20 grid points, <= 65536 receivers per shot,
+ * receivers x recorded rows <= 2^22 per shot.  Calls on one GPU share the
+ * array: calls that overlap on one GPU (two contexts, two threads) race.
+ * The only host allocation is the process-wide divisor memo the first call
+ * per distinct dx, dz fills. */
+int vti2d_synthetic_noalloc(vti_ctx *ctx, float *data, const float *c11, const float *c13, const float *c33,
+                            const float *wavelet, const int *sloc, int ns, const int *rloc, int nr,
+                            const float *taper, int nx, int nz, int nt, int npad, float dx, float dz, float dt,
+                            float rate, float ot);
+
 /* objFuncGradient_f, lib/vti-fwi-2d/objFuncGradient_f.cu:15 (wave.h:12).
  * *objfunc receives the value the reference returns (0.5*sum r^2).
  * Requires rate/dt >= 2, (nt-1) % (rate/dt) == 0 and ot == 0, the conditions

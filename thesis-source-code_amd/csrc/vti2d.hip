@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "vti2d_kernels.h"
@@ -376,6 +377,159 @@ int vti2d_synthetic(vti_ctx *ctx, float *data, const float *c11, const float *c1
         for (int row = 0; row < c.nnt_data; ++row)
             if (rec_row[row])
                 for (int ir = 0; ir < s.nri; ++ir) data[(size_t)row * nr + s.r0 + ir] = h[(size_t)row * s.nri + ir];
+    }
+    return vti::kOk;
+}
+
+}  // extern "C"
+
+// ---- synthetic_f without allocation ----------------------------------------
+//
+// vti2d_synthetic_noalloc runs the one-shot schedule of vti2d_synthetic with
+// every device buffer carved out of `scratch`, a file-scope device array of
+// fixed size, so that a call makes no device (or host) allocation.  Each GPU
+// holds its own copy of the array until the process ends; calls on one GPU
+// share it, so calls that overlap on one GPU race.
+namespace {
+
+constexpr long long kNaPoints = 1LL << 20;  // padded grid points (nnx * nnz)
+constexpr long long kNaRec = 1LL << 16;     // receivers of one shot
+constexpr long long kNaTrace = 1LL << 22;   // floats of one shot's traces (receivers x rows)
+// layout in floats: c11 c13 c33 taper p0 p1 q0 q1 (kNaPoints each), the
+// shot's receiver (ix, iz) pairs (2 kNaRec ints), its traces (kNaTrace)
+constexpr long long kNaFloats = 8 * kNaPoints + 2 * kNaRec + kNaTrace;
+constexpr int kNaMaxDevices = 64;
+
+__device__ __align__(256) float scratch[kNaFloats];
+
+// Device address of this device's `scratch` (the host-side symbol is only a
+// handle), looked up once per device with that device current.
+std::mutex g_na_mu;
+float *g_na_ptr[kNaMaxDevices] = {};
+
+int scratch_ptr(int device, float **out)
+{
+    VTI_CHECK(device >= 0 && device < kNaMaxDevices, "vti2d_synthetic_noalloc: device %d beyond %d", device,
+              kNaMaxDevices);
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[device]) {
+        void *p = nullptr;
+        VTI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        VTI_CHECK(p != nullptr, "vti2d_synthetic_noalloc: no device address for the scratch array");
+        g_na_ptr[device] = static_cast<float *>(p);
+    }
+    *out = g_na_ptr[device];
+    return vti::kOk;
+}
+
+__global__ void fill2_kernel(float *__restrict__ a, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        a[i] = 0.f;
+}
+
+// recordData (kernels.cu:317-324) at receivers given as (ix, iz) pairs
+__global__ void record2d_xz_kernel(float *__restrict__ row, const float *__restrict__ p, const float *__restrict__ q,
+                                   const int *__restrict__ rxz, int nr, int nnx)
+{
+    const int ir = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ir < nr) {
+        const long long i = rxz[2 * ir] + (long long)rxz[2 * ir + 1] * nnx;
+        row[ir] = 0.5f * (p[i] + q[i]);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// vti2d_synthetic's results (the one-shot schedule, bit for bit) with no
+// allocation: see vti_amd.h.
+int vti2d_synthetic_noalloc(vti_ctx *ctx, float *data, const float *c11, const float *c13, const float *c33,
+                            const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper,
+                            int nx, int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    Call2 c;
+    int rc = setup(c, ctx, nx, nz, nt, npad, dx, dz, dt, rate, ot, false);
+    if (rc) return rc;
+    VTI_CHECK(data && c11 && c13 && c33 && wavelet && sloc && rloc && taper && ns >= 0 && nr >= 0,
+              "vti2d_synthetic_noalloc: null argument");
+    VTI_CHECK(c.nnxz <= kNaPoints, "vti2d_synthetic_noalloc: %lld grid points above the scratch limit %lld", c.nnxz,
+              kNaPoints);
+    // every shot is checked before any work is queued
+    for (int is = 0; is < ns; ++is) {
+        const int ix = sloc[4 * is], iz = sloc[4 * is + 1], nri = sloc[4 * is + 2], r0 = sloc[4 * is + 3];
+        VTI_CHECK(nri >= 0 && r0 >= 0 && r0 + (long long)nri <= nr, "shot %d: receiver range outside rloc", is);
+        VTI_CHECK(interior(c, ix, iz), "shot %d: source (%d,%d) outside the updated interior", is, ix, iz);
+        VTI_CHECK(nri <= kNaRec, "vti2d_synthetic_noalloc: shot %d has %d receivers, above the scratch limit %lld",
+                  is, nri, kNaRec);
+        VTI_CHECK((long long)nri * c.nnt_data <= kNaTrace,
+                  "vti2d_synthetic_noalloc: shot %d needs %lld trace samples, above the scratch limit %lld", is,
+                  (long long)nri * c.nnt_data, kNaTrace);
+        for (int ir = 0; ir < nri; ++ir) {
+            const int x = rloc[2 * (r0 + ir)], z = rloc[2 * (r0 + ir) + 1];
+            VTI_CHECK(x >= 0 && x < c.g.nnx && z >= 0 && z < c.g.nnz, "receiver %d outside the grid", r0 + ir);
+        }
+    }
+    float *base = nullptr;
+    rc = scratch_ptr(ctx->device, &base);
+    if (rc) return rc;
+    const size_t n = (size_t)c.nnxz;
+    float *m11 = base, *m13 = base + kNaPoints, *m33 = base + 2 * kNaPoints, *tap = base + 3 * kNaPoints;
+    float *p[2] = {base + 4 * kNaPoints, base + 5 * kNaPoints};
+    float *q[2] = {base + 6 * kNaPoints, base + 7 * kNaPoints};
+    int *rxz = reinterpret_cast<int *>(base + 8 * kNaPoints);
+    float *tr = base + 8 * kNaPoints + 2 * kNaRec;
+    hipStream_t st = ctx->stream;
+    rc = upload(c, m11, c11, n * 4);
+    if (!rc) rc = upload(c, m13, c13, n * 4);
+    if (!rc) rc = upload(c, m33, c33, n * 4);
+    if (!rc) rc = upload(c, tap, taper, n * 4);
+    if (rc) return rc;
+    const int fill_blocks = launch_blocks((long long)n);
+    for (int is = 0; is < ns; ++is) {
+        const int ix = sloc[4 * is], iz = sloc[4 * is + 1], nri = sloc[4 * is + 2], r0 = sloc[4 * is + 3];
+        const long long src = ix + (long long)iz * c.g.nnx;
+        if (nri) rc = upload(c, rxz, rloc + 2 * (size_t)r0, sizeof(int) * 2 * (size_t)nri);
+        if (rc) return rc;
+        for (int k = 0; k < 2; ++k) {
+            hipLaunchKernelGGL(fill2_kernel, dim3(fill_blocks), dim3(256), 0, st, p[k], (long long)n);
+            hipLaunchKernelGGL(fill2_kernel, dim3(fill_blocks), dim3(256), 0, st, q[k], (long long)n);
+        }
+        // level 1: dt2*w[0] at the source, then abc (synthetic_f.cu:86-89)
+        hipLaunchKernelGGL(set2_kernel, dim3(1), dim3(1), 0, st, p[0], q[0], src, c.dt2 * wavelet[0] * taper[src]);
+        int row_lo = c.nnt_data, row_hi = -1, nrows = 0;
+        auto record = [&](int row) {
+            hipLaunchKernelGGL(record2d_xz_kernel, dim3((nri + 255) / 256), dim3(256), 0, st,
+                               tr + (size_t)row * nri, p[0], q[0], rxz, nri, c.g.nnx);
+            row_lo = std::min(row_lo, row);
+            row_hi = std::max(row_hi, row);
+            ++nrows;
+        };
+        if (c.ratio == 1 && ot == 0.f && nri && 1 < c.nnt_data) record(1);
+        for (int it = 2; it < nt; ++it) {
+            const float t = it * dt + ot;
+            hipLaunchKernelGGL((vti::fwd2d_kernel<false>), c.grid, c.block, 0, st, p[1], q[1], p[0], q[0], nullptr,
+                               nullptr, nullptr, nullptr, m11, m13, m33, nullptr, nullptr, nullptr, tap, c.g, c.dx2,
+                               c.dz2, c.dt2, src, c.dt2 * wavelet[it - 1], nullptr, nullptr, -1LL, nullptr, nullptr);
+            std::swap(p[0], p[1]);
+            std::swap(q[0], q[1]);
+            if (t >= 0. && (it - c.ntNeg) % c.ratio == 0 && nri) {
+                const int row = (it - c.ntNeg) / c.ratio;
+                if (row < c.nnt_data) record(row);
+            }
+        }
+        VTI_HIP(hipGetLastError());
+        // each row is recorded at most once, so the recorded rows are the
+        // whole range [row_lo, row_hi] when their count fills it
+        VTI_CHECK(nrows == 0 || nrows == row_hi - row_lo + 1,
+                  "vti2d_synthetic_noalloc: recorded rows %d..%d not contiguous", row_lo, row_hi);
+        if (nrows)
+            VTI_HIP(hipMemcpy2DAsync(data + (size_t)row_lo * nr + r0, sizeof(float) * (size_t)nr,
+                                     tr + (size_t)row_lo * nri, sizeof(float) * (size_t)nri,
+                                     sizeof(float) * (size_t)nri, (size_t)nrows, hipMemcpyDeviceToHost, st));
+        // the next shot's uploads and fills reuse the buffers
+        VTI_HIP(hipStreamSynchronize(st));
     }
     return vti::kOk;
 }

@@ -88,6 +88,7 @@ _SIGS = {
         [P, P, P, c_int, P, c_int, P, P, P, P, P] + [c_int] * 5 + [c_float] * 9 + [c_int, P],
     ),
     "vti2d_synthetic": (c_int, [P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
+    "vti2d_synthetic_noalloc": (c_int, [P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
     "vti2d_hessian": (c_int, [P] * 13 + [c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
     "vti2d_rtm": (c_int, [P] * 7 + [P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
     "vti2d_odcig": (c_int, [P] * 7 + [P, c_int, P, c_int, P] + [c_int] * 5 + [c_float] * 5),
@@ -399,10 +400,12 @@ def _i32(a):
 
 
 def synthetic2d(ctx, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0,
-                data=None):
+                data=None, noalloc=False):
     """synthetic_f: returns data (nnt_data, nr) time-major.  Rows the
     reference never records keep the values of `data` (written in place when
-    given as a C-contiguous float32 array of that shape), else zeros."""
+    given as a C-contiguous float32 array of that shape), else zeros.
+    noalloc: call vti2d_synthetic_noalloc (no device allocation; grids up to
+    2^20 points)."""
     c11, c13, c33, wavelet, taper = (f32(a).ravel() for a in (c11, c13, c33, wavelet, taper))
     sloc, rloc = _i32(sloc), _i32(rloc)
     ns, nr = sloc.size // 4, rloc.size // 2
@@ -413,8 +416,9 @@ def synthetic2d(ctx, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad
         data = np.zeros((nnt_data, nr), np.float32)
     elif data.dtype != np.float32 or data.shape != (nnt_data, nr) or not data.flags["C_CONTIGUOUS"]:
         raise ValueError(f"data must be a C-contiguous float32 array of shape {(nnt_data, nr)}")
-    check(_lib.vti2d_synthetic(ctx.h, ptr(data), ptr(c11), ptr(c13), ptr(c33), ptr(wavelet), ptr(sloc), ns, ptr(rloc),
-                               nr, ptr(taper), nx, nz, nt, npad, dx, dz, dt, rate, ot), "vti2d_synthetic")
+    name = "vti2d_synthetic_noalloc" if noalloc else "vti2d_synthetic"
+    check(getattr(_lib, name)(ctx.h, ptr(data), ptr(c11), ptr(c13), ptr(c33), ptr(wavelet), ptr(sloc), ns, ptr(rloc),
+                              nr, ptr(taper), nx, nz, nt, npad, dx, dz, dt, rate, ot), name)
     return data
 
 
