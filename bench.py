@@ -193,7 +193,10 @@ def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7, variants=None, graph
     data = np.zeros(int(sum(nnt * s[4] for s in shots)), np.float32)
     tuned = vtiamd.tune_batch2d(ctx, shots, rloc, NX, nz, nt, npad, dx, dx, dt, rate)
     if variants is None:
-        variants = {"autotuned": {k: tuned[k] for k in ("levels", "tile_rows", "max_wave")}}
+        shape = {k: tuned[k] for k in ("levels", "tile_rows", "max_wave")}
+        # FP32 snapshots (bit-exact with the one-shot driver), then S16
+        # snapshots (16 bits, within 1e-4 of the FP64 oracle)
+        variants = {"autotuned": shape, "autotuned_s16_snapshots": {**shape, "snap": "s16"}}
     timed = {}
     for name, opts in variants.items():
         t0 = time.perf_counter()
@@ -228,7 +231,8 @@ def batch_gradient_2d(vtiamd, ctx, ns=32, nplain=4, seed=7, variants=None, graph
             **({"graph_plan": plan,
                 "graph_plan_speedup_over_plain": round(plan["shot_gradients_per_s"] / (nplain / tp), 2),
                 "small_grid": batch_small_2d(vtiamd, ctx)} if graph else {}),
-            **({"variants_shot_gradients_per_s": {k: round(ns / v, 3) for k, v in timed.items()}}
+            **({"variants_shot_gradients_per_s": {k: round(ns / v, 3) for k, v in timed.items()},
+                "variants_speedup_over_plain": {k: round((ns / v) / (nplain / tp), 2) for k, v in timed.items()}}
                if variants else {})}
 
 
@@ -336,14 +340,16 @@ def shot_gradient_2d(vtiamd, ctx, nshots=2, ns_call=32):
                                           "speedup_over_one_shot_calls": round((ns_call / el32) / (nshots / el), 2)}}
 
 
-def fwi_lbfgs_2d(vtiamd, ctx, iters=5, nfg=12):
+def fwi_lbfgs_2d(vtiamd, ctx, iters=5, nfg=20):
     """The cfg2 inversion as fwi-lbfgs-c-param.cu runs it (parameter vepsdel):
     32 shots of data of the true model, the initial model (v, eps, del scaled
     by v0 = 100 m/s, eps0 = del0 = 0.01, so that the first trial step, of
     length 1 along -g, moves v by at most 100 m/s), objFuncGradientVEpsDel on the GPU
     (one call with the 32 shots, run as one batch; wbottom 0) and the FP64
     L-BFGS (m = 5) with its vectors resident in HBM, until `iters`
-    iterations are complete or `nfg` evaluations are spent.  Wall time of the
+    iterations are complete or `nfg` evaluations are spent.  The objective
+    reaches L-BFGS scaled to 1e6 at the initial model, since the reference's
+    stopping test |g|/max(1,|x|) <= 1e-5 is absolute.  Wall time of the
     whole loop; the objective of every evaluation."""
     from vtiamd import workloads as W
 
@@ -371,7 +377,8 @@ def fwi_lbfgs_2d(vtiamd, ctx, iters=5, nfg=12):
         t_grad += time.perf_counter() - t1
         x[nn:2 * nn], x[2 * nn:] = e2, d2  # checkEpsDel projects eps/del in place, as in the reference
         fs.append(float(f))
-        iflag = opt.step(x, f, g.ravel(), iflag)
+        f0 = fs[0]  # the objective goes to L-BFGS scaled to 1e6 at the initial model (its stopping test is absolute)
+        iflag = opt.step(x, f / f0 * 1e6, (np.asarray(g, np.float64) / f0 * 1e6).astype(np.float32).ravel(), iflag)
         if iflag != 1 or opt.iter > iters or len(fs) >= nfg:
             break
     wall = time.perf_counter() - t0
@@ -379,7 +386,8 @@ def fwi_lbfgs_2d(vtiamd, ctx, iters=5, nfg=12):
     opt.close()
     return {"shots": ns, "grid_padded": [nz + 2 * npad, nx + 2 * npad], "steps": nt - 1, "unknowns": 3 * nn,
             "lbfgs": "FP64, m=5, vectors in HBM (vti_lbfgs_create_device)", "iterations_completed": int(done),
-            "evaluations": len(fs), "objective": [round(q, 6) for q in fs], "wall_s": round(wall, 2),
+            "evaluations": len(fs), "objective_initial": float(f"{fs[0]:.6e}"),
+            "objective_relative": [round(q / fs[0], 6) for q in fs], "wall_s": round(wall, 2),
             "gradient_s": round(t_grad, 2), "shot_gradients_per_s": round(ns * len(fs) / t_grad, 2)}
 
 
@@ -559,45 +567,51 @@ def serve_run(vtiamd, tr, slots, realtime, batch=8):
             "peak_gb": round(st["peak_bytes"] / 2**30, 2), "merged_requests": st["merged_requests"]}
 
 
-def serve_stream(vtiamd, nreq=40):
-    """The served stream (70% 2D gradients, 30% 3D modeling) at 5 req/s and
-    at 12 req/s through a 4-slot server and through one slot without merging
-    (each request alone, in arrival order: the plain per-request launch), and
-    the saturation throughput (all requests at once) of both, also for a
-    stream of 2D gradient requests only; then the same for a stream whose 2D
-    requests are single shots of two surveys, which the server merges into
-    batched runs."""
-    out = {"requests": nreq, "mix": "70% 2D gradient, 30% 3D modeling"}
-    for rate in (5.0, 12.0):
-        tr = serve_trace(vtiamd, nreq, rate)
-        out.setdefault("grids_2d", [list(g) for g in tr["grids2"]])
-        out.setdefault("grids_3d", [list(g) for g in tr["grids3"]])
-        served, plain = serve_run(vtiamd, tr, 4, True), serve_run(vtiamd, tr, 1, True, batch=1)
-        out[f"poisson_{rate:g}rps"] = {"served_4slots": served, "plain_1slot": plain,
-                                       "p99_ratio_plain_over_served": round(plain["p99_s"] / max(served["p99_s"],
-                                                                                                 1e-9), 2)}
+def serve_stream(vtiamd, nreq=300, nsat=150):
+    """The served stream of the cfg5 mix (70% 2D gradients, 30% 3D modeling)
+    at its Poisson rate of 5 req/s, shortened from 10 minutes to `nreq`
+    requests (about a minute) so that p99 is the 3rd-slowest of 300 rather
+    than the maximum, through a 4-slot server and through one slot without
+    merging (each request alone, in arrival order: the plain per-request
+    launch); the saturation throughput (all requests at once) of both on
+    `nsat` of them, also for a stream of 2D gradient requests only; then the
+    same for a stream whose 2D requests are single shots of two surveys,
+    which the server merges into batched runs (12 req/s, `nreq` requests)."""
+    out = {"requests": nreq, "mix": "70% 2D gradient, 30% 3D modeling",
+           "duration_note": f"{nreq} Poisson arrivals at 5 req/s (about {nreq / 5:.0f} s) in place of 10 minutes"}
+    tr = serve_trace(vtiamd, nreq, 5.0)
+    out["grids_2d"] = [list(g) for g in tr["grids2"]]
+    out["grids_3d"] = [list(g) for g in tr["grids3"]]
+    served, plain = serve_run(vtiamd, tr, 4, True), serve_run(vtiamd, tr, 1, True, batch=1)
+    out["poisson_5rps"] = {"served_4slots": served, "plain_1slot": plain,
+                           "p99_ratio_plain_over_served": round(plain["p99_s"] / max(served["p99_s"], 1e-9), 2)}
+    progress("served stream: cfg5 mix at 5 req/s")
+
     def saturation(tr):
         sat = {"served_4slots": serve_run(vtiamd, tr, 4, False)["req_per_s"],
                "plain_1slot": serve_run(vtiamd, tr, 1, False, batch=1)["req_per_s"]}
         sat["ratio"] = round(sat["served_4slots"] / max(sat["plain_1slot"], 1e-9), 2)
         return sat
 
-    out["saturation_req_s"] = saturation(tr)
-    out["saturation_2d_only_shot_gradients_per_s"] = saturation(serve_trace(vtiamd, nreq, 5.0, frac3d=0.0, seed=6))
+    out["saturation_req_s"] = saturation(serve_trace(vtiamd, nsat, 5.0))
+    out["saturation_requests"] = nsat
+    out["saturation_2d_only_shot_gradients_per_s"] = saturation(serve_trace(vtiamd, nsat, 5.0, frac3d=0.0, seed=6))
+    progress("served stream: saturation")
     sv = {"mix": "70% single-shot 2D gradients of two surveys (grids_2d[0] at 2000 steps, grids_2d[1] at 1500), "
-                 "30% 3D modeling"}
+                 "30% 3D modeling", "requests": nreq}
     tr = serve_trace(vtiamd, nreq, 12.0, surveys=True, seed=7)
     served, plain = serve_run(vtiamd, tr, 4, True), serve_run(vtiamd, tr, 1, True, batch=1)
     sv["poisson_12rps"] = {"served_4slots": served, "plain_1slot": plain,
                            "p99_ratio_plain_over_served": round(plain["p99_s"] / max(served["p99_s"], 1e-9), 2)}
-    sv["saturation_req_s"] = saturation(tr)
-    tr = serve_trace(vtiamd, 2 * nreq, 5.0, frac3d=0.0, seed=8, surveys=True)
+    sv["saturation_req_s"] = saturation(serve_trace(vtiamd, nsat, 12.0, surveys=True, seed=9))
+    tr = serve_trace(vtiamd, nsat, 5.0, frac3d=0.0, seed=8, surveys=True)
     sat = saturation(tr)
     sat["served_2slots"] = serve_run(vtiamd, tr, 2, False)["req_per_s"]
     sat["ratio_2slots"] = round(sat["served_2slots"] / max(sat["plain_1slot"], 1e-9), 2)
     sv["saturation_2d_only_shot_gradients_per_s"] = sat
-    sv["saturation_2d_only_requests"] = 2 * nreq
+    sv["saturation_2d_only_requests"] = nsat
     out["surveys"] = sv
+    progress("served stream: surveys")
     return out
 
 

@@ -97,8 +97,9 @@ def test_cfg2_lbfgs_two_iterations(vti, orc, ctx, cfg2):
     them (parameter vepsdel: v/v0, eps/eps0, del/del0; checkEpsDel projects
     eps and del in place each evaluation), one shot, 600-step record, both
     with the host L-BFGS and with its device-resident form.  The objective
-    is divided by its initial value (so the reference's stopping test
-    |g|/max(1,|x|) <= 1e-5 does not end the run on the first step).  The
+    is scaled to 1e6 at the initial model (the reference's stopping test
+    |g|/max(1,|x|) <= 1e-5 is absolute, and the raw misfit of this record,
+    about 1e-9, would end the run after its first step).  The
     GPU-driven run (objFuncGradientVEpsDel on the GPU) and the oracle-driven
     run (checkEpsDel, VEpsDel2Cij, the oracle's FP32 gradient,
     GradCij2GradVEpsDel, each pinned on its own) visit the same models bit
@@ -123,7 +124,8 @@ def test_cfg2_lbfgs_two_iterations(vti, orc, ctx, cfg2):
             x[nn:2 * nn], x[2 * nn:] = e2.ravel(), d2.ravel()
             f0 = f if f0 is None else f0
             fs.append(f)
-            iflag = opt.step(x, f / f0, (np.asarray(g, np.float64) / f0).astype(np.float32).ravel(), iflag)
+            iflag = opt.step(x, f / f0 * 1e6, (np.asarray(g, np.float64) / f0 * 1e6).astype(np.float32).ravel(),
+                             iflag)
             xs.append(x.copy())
             if iflag != 1 or opt.iter >= 3 or len(fs) >= 24:
                 break

@@ -779,7 +779,7 @@ static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
         }
     }
     if (sw.amax && slot1 >= 0) snap_amax(sw, sh, slot1, mp, mq, red);
-    __syncthreads();
+    lds_barrier();  // level L+1 reaches level L+2 through LDS only
     // level L+2 on the tile
     mp = 0.f;
     mq = 0.f;
@@ -956,7 +956,9 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
             t.qq1[rz][rx] = qq;
         }
     }
-    __syncthreads();
+    // level L+1 reaches level L+2 through LDS only: the snapshot and gradient
+    // loads stay in flight across this barrier
+    lds_barrier();
     if (!act) return;
     const float tap = taper_of(tw);
     const int rx = threadIdx.x + 4, rz = threadIdx.y + 4;

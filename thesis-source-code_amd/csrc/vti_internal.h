@@ -35,6 +35,16 @@ enum Status : int {
     kErrState = -4,     // call sequence error (e.g. no model uploaded)
 };
 
+// A workgroup barrier that orders LDS only: global loads and stores issued
+// before it stay in flight (__syncthreads also waits for every outstanding
+// global access).  For kernels whose threads exchange data through LDS only.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 void set_error(const char *fmt, ...);
 const char *get_error();
 
