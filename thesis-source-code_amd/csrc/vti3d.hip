@@ -45,6 +45,30 @@ __global__ void record3d_kernel(float *__restrict__ row, const float *__restrict
     }
 }
 
+// out[c * rows + r] = in[r * cols + c]: traces between the time-major rows
+// the kernels record ([it][ir]) and the trace-major layout of the drivers'
+// arguments ([ir][it]), through a 32 x 32 LDS tile (padded against bank
+// conflicts); blocks of 32 x 8 threads, one block per tile, tiles numbered
+// column tile fastest (transpose_tiles).
+__global__ __launch_bounds__(256) void transpose_kernel(float *__restrict__ out, const float *__restrict__ in,
+                                                        int rows, int cols)
+{
+    __shared__ float t[32][33];
+    const int ntc = (cols + 31) / 32;
+    const int c0 = (int)(blockIdx.x % ntc) * 32, r0 = (int)(blockIdx.x / ntc) * 32;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + threadIdx.x;
+        if (r < rows && c < cols) t[k][threadIdx.x] = in[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + threadIdx.x;
+        if (r < rows && c < cols) out[(size_t)c * rows + r] = t[threadIdx.x][k];
+    }
+}
+
+dim3 transpose_tiles(int rows, int cols) { return dim3((unsigned)(((cols + 31) / 32) * (long long)((rows + 31) / 32))); }
+
 // First adjoint sample (objFuncGradient3d_f.cu:921-927): lambda_cur at the
 // true receiver index = (2/3, 1/3) * dt2 * r[nnt-1] (assignment).
 __global__ void lambda_init3d_kernel(float *__restrict__ lx, float *__restrict__ lz, const long long *__restrict__ rec,
@@ -270,21 +294,54 @@ int ensure_backward(vti3d_prop *p, bool gradient, bool random_model)
 }
 
 // host (reference layout, pitch nx) -> device (pitched)
-// src in host memory or, for a model kept resident by the caller, in device
-// memory (the direction is taken from the pointers)
+bool device_ptr(const void *q)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, q) != hipSuccess) {
+        (void)hipGetLastError();  // pageable host memory is not known to the runtime
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// host (reference layout, pitch nx) -> device (pitched).  src in host memory
+// goes over as one contiguous copy into a stream-ordered staging block and is
+// pitched on the device (a 2D copy from pageable memory moves row by row and
+// took seconds for a 600 MB volume); a model kept resident by the caller in
+// device memory is pitched directly.
 int upload_volume(vti3d_prop *p, float *dst, const float *src)
 {
     const Geo3 &g = p->g;
-    VTI_HIP(hipMemcpy2DAsync(dst, (size_t)g.pitch * 4, src, (size_t)g.nx * 4, (size_t)g.nx * 4,
-                             (size_t)g.ny * g.nz, hipMemcpyDefault, p->ctx->stream));
+    const size_t row = (size_t)g.nx * 4, rows = (size_t)g.ny * g.nz;
+    hipStream_t s = p->ctx->stream;
+    if (device_ptr(src)) {
+        VTI_HIP(hipMemcpy2DAsync(dst, (size_t)g.pitch * 4, src, row, row, rows, hipMemcpyDeviceToDevice, s));
+        return vti::kOk;
+    }
+    float *stage = nullptr;
+    DMALLOC(&stage, row * rows);
+    hipError_t e = hipMemcpyAsync(stage, src, row * rows, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(dst, (size_t)g.pitch * 4, stage, row, row, rows, hipMemcpyDeviceToDevice, s);
+    freep(stage, s);
+    VTI_HIP(e);
     return vti::kOk;
 }
 
+// device (pitched) -> host (reference layout): unpitched on the device into a
+// staging block, then one contiguous copy
 int download_volume(vti3d_prop *p, float *dst, const float *src)
 {
     const Geo3 &g = p->g;
-    VTI_HIP(hipMemcpy2DAsync(dst, (size_t)g.nx * 4, src, (size_t)g.pitch * 4, (size_t)g.nx * 4,
-                             (size_t)g.ny * g.nz, hipMemcpyDeviceToHost, p->ctx->stream));
+    const size_t row = (size_t)g.nx * 4, rows = (size_t)g.ny * g.nz;
+    hipStream_t s = p->ctx->stream;
+    float *stage = nullptr;
+    DMALLOC(&stage, row * rows);
+    hipError_t e = hipMemcpy2DAsync(stage, row, src, (size_t)g.pitch * 4, row, rows, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, stage, row * rows, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // dst is complete when the call returns
+    freep(stage, s);
+    VTI_HIP(e);
     return vti::kOk;
 }
 
@@ -338,10 +395,20 @@ int fwd_launch(vti3d_prop *p, bool absorbing, long long src_dev, float src_val)
     return vti::kOk;
 }
 
+// 64 x 16 tiles of the two-level kernel
+Launch launch_shape_t2(const vti3d_prop *p)
+{
+    Launch L = launch_shape(p);
+    L.nty = (p->g.ny - 8 + 15) / 16;
+    L.block = dim3(64, 16);
+    L.grid = dim3(L.ntx * L.nty * L.ntz);
+    return L;
+}
+
 template <int ARITH, bool ABC>
 void fwd_t2_kernel(vti3d_prop *p, const Launch &L, long long src_dev, float v1, float v2)
 {
-    hipLaunchKernelGGL((vti::fwd3d_t2_kernel<ARITH, ABC, kBY>), L.grid, L.block, 0, p->ctx->stream, p->sx2[0],
+    hipLaunchKernelGGL((vti::fwd3d_t2b_kernel<ARITH, ABC>), L.grid, L.block, 0, p->ctx->stream, p->sx2[0],
                        p->sz2[0], p->sx2[1], p->sz2[1], p->sx[1], p->sz[1], p->sx[0], p->sz[0], p->cm[p->active][0],
                        p->cm[p->active][1], p->cm[p->active][2], p->dxy, p->dzv, p->g, p->zc, L.ntx, L.nty,
                        p->dt2dx2, p->dt2dy2, p->dt2dz2, src_dev, v1, v2);
@@ -365,7 +432,7 @@ int fwd2_launch(vti3d_prop *p, bool absorbing, long long src_dev, float v1, floa
         VTI_HIP(hipMemcpyAsync(p->sx2[1], p->sx[0], nv * 4, hipMemcpyDeviceToDevice, p->ctx->stream));
         VTI_HIP(hipMemcpyAsync(p->sz2[1], p->sz[0], nv * 4, hipMemcpyDeviceToDevice, p->ctx->stream));
     }
-    const Launch L = launch_shape(p);
+    const Launch L = launch_shape_t2(p);
     if (p->arith == vti::kArithRef) {
         if (absorbing) fwd_t2_kernel<vti::kArithRef, true>(p, L, src_dev, v1, v2);
         else fwd_t2_kernel<vti::kArithRef, false>(p, L, src_dev, v1, v2);
@@ -1099,13 +1166,25 @@ int vti3d_model_data(vti_ctx *ctx, float *ddata, const float *souloc, int ns, co
         rc = shot_setup(sh, souloc, shotid[k], recloc, 4, nx, ny, nz, ox, oy, oz, dx, dy, dz);
         if (rc) return rc;
         ShotDev sd(ctx->stream);
-        std::vector<float> tr;
         double ss = 0;
-        rc = modeled_shot(G.p, a, sh, sd, lmax, sts, nnt, nullptr, 0.f, 0, 0, &ss, &tr);
+        rc = modeled_shot(G.p, a, sh, sd, lmax, sts, nnt, nullptr, 0.f, 0, 0, &ss, nullptr);
         if (rc) return rc;
-        float *dst = ddata + (size_t)nnt * sh.irbegin;
-        for (int ir = 0; ir < sh.nr; ++ir)
-            for (int it = 0; it < nnt; ++it) dst[(size_t)ir * nnt + it] = tr[(size_t)it * sh.nr + ir];
+        if (sh.nr > 0) {
+            // time-major rows -> the trace-major block of this shot, on the device
+            float *tm = nullptr;
+            const size_t nn = (size_t)sh.nr * nnt;
+            rc = vti::dmalloc((void **)&tm, sizeof(float) * nn, ctx->stream);
+            if (rc) return rc;
+            hipLaunchKernelGGL(transpose_kernel, transpose_tiles(nnt, sh.nr), dim3(32, 8), 0, ctx->stream, tm, sd.tr,
+                               nnt, sh.nr);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(ddata + (size_t)nnt * sh.irbegin, tm, sizeof(float) * nn, hipMemcpyDeviceToHost,
+                                   ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            vti::dfree(tm, ctx->stream);
+            VTI_HIP(e);
+        }
     }
     return vti::kOk;
 }
@@ -1252,14 +1331,25 @@ int vti3d_rtm(vti_ctx *ctx, float *image, const float *souloc, int ns, const int
         if (rc) return rc;
         rc = to_dev_idx(p, &sd.rec_true, sh.rec_true);
         if (rc) return rc;
-        // observed traces, time-major on the device
+        // observed traces, time-major on the device: the shot's trace-major
+        // block is uploaded as it is and transposed there
         const size_t nn = (size_t)std::max(1, sh.nr) * nnt;
-        std::vector<float> tm(nn, 0.f);
-        const float *ob = obs + (size_t)nnt * sh.irbegin;
-        for (int ir = 0; ir < sh.nr; ++ir)
-            for (int it = 0; it < nnt; ++it) tm[(size_t)it * sh.nr + ir] = ob[(size_t)ir * nnt + it];
         DMALLOC(&sd.tr, sizeof(float) * nn);
-        VTI_HIP(hipMemcpyAsync(sd.tr, tm.data(), sizeof(float) * nn, hipMemcpyHostToDevice, p->ctx->stream));
+        if (sh.nr > 0) {
+            float *trm = nullptr;
+            DMALLOC(&trm, sizeof(float) * nn);
+            hipError_t e = hipMemcpyAsync(trm, obs + (size_t)nnt * sh.irbegin, sizeof(float) * nn,
+                                          hipMemcpyHostToDevice, p->ctx->stream);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(transpose_kernel, transpose_tiles(sh.nr, nnt), dim3(32, 8), 0, p->ctx->stream,
+                                   sd.tr, trm, sh.nr, nnt);
+                e = hipGetLastError();
+            }
+            freep(trm, p->ctx->stream);
+            VTI_HIP(e);
+        } else {
+            VTI_HIP(hipMemsetAsync(sd.tr, 0, sizeof(float) * nn, p->ctx->stream));
+        }
         rc = zero_fields(p, true, true);
         if (rc) return rc;
         hipLaunchKernelGGL(set_point_kernel, dim3(1), dim3(1), 0, p->ctx->stream, p->sx[0], p->sz[0],

@@ -445,12 +445,9 @@ __global__ __launch_bounds__(64 * BY) void fwd3d_kernel(
 // values outside the interior of (cx, cz) equal to those of (ax, az), and of
 // (dx_, dz_) equal to (bx, bz).
 //
-// Thread t owns the tile point (t % 64, t / 64) of E's centre and up to
-// NE - 1 points of E's 4-wide rim; each owned point keeps a 9-deep register
-// queue of its sigma_z(L) column.  sigma_x(L) planes sit in an 80-wide LDS
-// tile (double buffered); sigma_x(L+1) planes of E in a 5-slot LDS ring, so
-// that level L+2 at plane z-4 finds its plane there; sigma_z(L+1) of the
-// centre column stays in a 9-deep register queue.
+// sigma_x(L) planes sit in an 80-wide LDS tile (double buffered) and
+// sigma_x(L+1) planes of E in a 5-slot LDS ring, so that level L+2 at plane
+// z-4 finds its plane there (fwd3d_t2b_kernel below).
 // ---------------------------------------------------------------------------
 
 template <int ARITH, int W>
@@ -478,7 +475,6 @@ struct T2Shape {
     static constexpr int NT = 64 * BY;            // threads
     static constexpr int EW = 72, EH = BY + 8;    // E: x0-4 .. x0+68, y0-4 .. y0+BY+4
     static constexpr int NRIM = EW * EH - 64 * BY;
-    static constexpr int NE = 1 + (NRIM + NT - 1) / NT;  // owned points of E per thread
     static constexpr int TW = 80, TH = BY + 16;   // sigma_x(L) tile: x0-8 .., y0-8 ..
 };
 
@@ -511,242 +507,10 @@ __device__ __forceinline__ bool t2_point(int tid, int j, int &ex, int &ey)
     return true;
 }
 
-template <int ARITH, bool ABC, int BY>
-__global__ __launch_bounds__(64 * BY) void fwd3d_t2_kernel(
-    float *__restrict__ cx, float *__restrict__ cz, float *__restrict__ dx_, float *__restrict__ dz_,
-    const float *__restrict__ ax, const float *__restrict__ az, const float *__restrict__ bx,
-    const float *__restrict__ bz, const float *__restrict__ c11, const float *__restrict__ c13,
-    const float *__restrict__ c33, const float *__restrict__ dxy_plane, const float *__restrict__ dzv, Geo3 g,
-    int zc, int ntx, int nty, float dt2dx2, float dt2dy2, float dt2dz2, long long src_idx, float src1, float src2)
-{
-    using S = T2Shape<BY>;
-    constexpr int NE = S::NE, TW = S::TW, TH = S::TH, EW = S::EW, EH = S::EH;
-    constexpr int TL = (TW * TH + S::NT - 1) / S::NT;  // tile elements loaded per thread
-    __shared__ float tile[2][TH * TW];
-    __shared__ float ring[5][EH * EW];
-
-    const int lb = xcd_logical_block(gridDim.x * gridDim.y * gridDim.z);
-    const int bxi = lb % ntx, byi = (lb / ntx) % nty, bzi = lb / (ntx * nty);
-    const int tid = threadIdx.x + 64 * threadIdx.y;
-    const int x0 = bxi * 64, y0 = 4 + byi * BY;
-    const int zs = bzi * zc, ze = min(zs + zc, g.zmax);
-    const int z1s = max(zs - 4, 0), z1e = min(ze + 4, g.zmax);  // planes of level L+1
-
-    // owned points of E: position, class, offset within a plane, damping
-    int ex[NE], ey[NE];
-    bool own[NE], inr[NE], act[NE];
-    unsigned off[NE];
-    float dxy[NE];
-#pragma unroll
-    for (int j = 0; j < NE; ++j) {
-        own[j] = t2_point<BY>(tid, j, ex[j], ey[j]);
-        const int x = x0 - 4 + ex[j], y = y0 - 4 + ey[j];
-        inr[j] = own[j] && x >= 0 && x < g.nx && y >= 0 && y < g.ny;
-        act[j] = inr[j] && x >= 4 && x < g.nx - 4 && y >= 4 && y < g.ny - 4;
-        off[j] = inr[j] ? (unsigned)(y * g.pitch + x) : 0u;
-        dxy[j] = (ABC && inr[j]) ? dxy_plane[off[j]] : 1.f;
-    }
-    // sigma_x(L) tile elements this thread loads: offsets within a plane
-    unsigned toff[TL];
-    bool tok[TL], thas[TL];
-#pragma unroll
-    for (int k = 0; k < TL; ++k) {
-        const int e = tid + k * S::NT;
-        const int x = x0 - 8 + e % TW, y = y0 - 8 + e / TW;
-        thas[k] = e < TW * TH;
-        tok[k] = thas[k] && x >= 0 && x < g.nx && y >= 0 && y < g.ny;
-        toff[k] = tok[k] ? (unsigned)(y * g.pitch + x) : 0u;
-    }
-    auto P = [&](const float *base, int z) { return base + (size_t)z * (size_t)g.plane; };
-
-    // sigma_z(L) queues of the owned columns: q[j][k] = sigma_z(L)(z-4+k)
-    float q[NE][9];
-#pragma unroll
-    for (int j = 0; j < NE; ++j)
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const int zz = z1s - 4 + k;
-            q[j][k] = (inr[j] && zz >= 0 && zz < g.zmax) ? P(bz, zz)[off[j]] : 0.f;
-        }
-    // sigma_z(L+1) of the centre column (planes z-8 .. z) and sigma_x(L) of the
-    // centre (planes z-4 .. z)
-    float q1[9], hx[5];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) q1[k] = 0.f;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) hx[k] = 0.f;
-
-    // per-point operands of level L+1 at one plane: the previous level and the
-    // coefficients (rim points only need what sigma_x takes)
-    float pax[NE], paz, k11[NE], k13[NE], k33;
-    auto load_pts = [&](int z, float *oax, float &oaz, float *o11, float *o13, float &o33) {
-        const float *Ax = P(ax, z), *Az = P(az, z), *C11 = P(c11, z), *C13 = P(c13, z), *C33 = P(c33, z);
-#pragma unroll
-        for (int j = 0; j < NE; ++j) {
-            const bool ld = (z < z1e) && (j == 0 ? inr[0] : act[j] || inr[j]);
-            oax[j] = ld ? Ax[off[j]] : 0.f;
-            o11[j] = ld && act[j] ? C11[off[j]] : 0.f;
-            o13[j] = ld && act[j] ? C13[off[j]] : 0.f;
-        }
-        const bool l0 = (z < z1e) && inr[0];
-        oaz = l0 ? Az[off[0]] : 0.f;
-        o33 = l0 && act[0] ? C33[off[0]] : 0.f;
-    };
-    // the centre's coefficients for level L+2 at one plane
-    float s11 = 0.f, s13 = 0.f, s33 = 0.f;
-    auto load_s2 = [&](int zp, float &o11, float &o13, float &o33) {
-        const bool ld = act[0] && zp >= zs && zp < ze;
-        o11 = ld ? P(c11, zp)[off[0]] : 0.f;
-        o13 = ld ? P(c13, zp)[off[0]] : 0.f;
-        o33 = ld ? P(c33, zp)[off[0]] : 0.f;
-    };
-    float treg[TL];
-    auto tile_regs = [&](int z) {
-        const float *B = P(bx, z);
-#pragma unroll
-        for (int k = 0; k < TL; ++k) treg[k] = tok[k] ? B[toff[k]] : 0.f;
-    };
-    auto tile_store = [&](float *dst) {
-#pragma unroll
-        for (int k = 0; k < TL; ++k)
-            if (thas[k]) dst[tid + k * S::NT] = treg[k];
-    };
-
-    if (z1s < z1e) {
-        tile_regs(z1s);
-        tile_store(tile[z1s & 1]);
-    }
-    if (z1s + 1 < z1e) tile_regs(z1s + 1);
-    load_pts(z1s, pax, paz, k11, k13, k33);
-    load_s2(z1s - 4, s11, s13, s33);
-    __syncthreads();
-
-    const int cxl = (tid & 63), cyl = (tid >> 6);  // centre point in tile coordinates
-    for (int z = z1s; z < ze + 4; ++z) {
-        const bool step1 = z < z1e;
-        // requests for the next plane, in flight while this one is computed
-        if (z + 1 < z1e) tile_store(tile[(z + 1) & 1]);
-        if (z + 2 < z1e) tile_regs(z + 2);
-        float nax[NE], naz, n11[NE], n13[NE], n33;
-        load_pts(z + 1, nax, naz, n11, n13, n33);
-        float t11, t13, t33;
-        load_s2(z - 3, t11, t13, t33);
-        float qn[NE];
-#pragma unroll
-        for (int j = 0; j < NE; ++j) qn[j] = (inr[j] && z + 5 < g.zmax) ? P(bz, z + 5)[off[j]] : 0.f;
-
-        // ---- level L+1 at plane z on E
-        if (step1) {
-            const float *Tp = tile[z & 1];
-            const float d1 = ABC ? dzv[z] : 1.f;
-            const long long pz = (long long)z * g.plane;
-#pragma unroll
-            for (int j = 0; j < NE; ++j) {
-                if (!own[j]) continue;
-                float vx = 0.f, vz = 0.f;
-                if (act[j]) {
-                    const float *T = &Tp[(ey[j] + 4) * TW + ex[j] + 4];
-                    const float s0 = T[0];
-                    const float txy = sx_scaled_w<ARITH, TW>(T, dt2dx2, dt2dy2);
-                    float e9[9];
-                    effective_column_rt<ABC>(e9, q[j], dxy[j], dzv, z, z & 3);
-                    const float tz = sz_scaled<ARITH>(e9, dt2dz2);
-                    float px = pax[j];
-                    float d = 1.f;
-                    if (ABC) {
-                        d = dxy[j] * d1;
-                        px = px * d;
-                    }
-                    vx = leap<ARITH>(k11[j] * txy + k13[j] * tz, s0, px);
-                    const long long gi = pz + off[j];
-                    if (gi == src_idx) vx = vx + src1;
-                    if (ABC) vx = vx * d;
-                    if (j == 0) {
-                        float pzv = paz;
-                        if (ABC) pzv = pzv * d;
-                        vz = leap<ARITH>(k13[0] * txy + k33 * tz, q[0][4], pzv);
-                        if (gi == src_idx) vz = vz + src1;
-                        if (ABC) vz = vz * d;
-                    }
-                } else if (inr[j]) {
-                    vx = pax[j];
-                    if (j == 0) vz = paz;
-                }
-                ring[z % 5][ey[j] * EW + ex[j]] = vx;
-                if (j == 0) {
-                    if (act[0] && z >= zs && z < ze) {
-                        cx[pz + off[0]] = vx;
-                        cz[pz + off[0]] = vz;
-                    }
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) q1[k] = q1[k + 1];
-                    q1[8] = vz;
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) hx[k] = hx[k + 1];
-                    hx[4] = Tp[(cyl + 8) * TW + cxl + 8];
-                }
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) q1[k] = q1[k + 1];
-            q1[8] = 0.f;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) hx[k] = hx[k + 1];
-            hx[4] = 0.f;
-        }
-        // ---- level L+2 at plane z-4 on the tile
-        const int zp = z - 4;
-        if (zp >= zs && zp < ze && act[0]) {
-            const float *T = &ring[zp % 5][ey[0] * EW + ex[0]];
-            const float s0 = T[0];
-            const float txy = sx_scaled_w<ARITH, EW>(T, dt2dx2, dt2dy2);
-            float e9[9];
-            effective_column_rt<ABC>(e9, q1, dxy[0], dzv, zp, zp & 3);
-            const float tz = sz_scaled<ARITH>(e9, dt2dz2);
-            const long long gi = (long long)zp * g.plane + off[0];
-            float px = hx[0], pzv = q[0][0];
-            float d = 1.f;
-            if (ABC) {
-                d = dxy[0] * dzv[zp];
-                px = px * d;
-                pzv = pzv * d;
-            }
-            float vx = leap<ARITH>(s11 * txy + s13 * tz, s0, px);
-            float vz = leap<ARITH>(s13 * txy + s33 * tz, q1[4], pzv);
-            if (gi == src_idx) {
-                vx = vx + src2;
-                vz = vz + src2;
-            }
-            if (ABC) {
-                vx = vx * d;
-                vz = vz * d;
-            }
-            dx_[gi] = vx;
-            dz_[gi] = vz;
-        }
-        __syncthreads();
-        // advance to plane z+1
-#pragma unroll
-        for (int j = 0; j < NE; ++j) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) q[j][k] = q[j][k + 1];
-            q[j][8] = qn[j];
-            pax[j] = nax[j];
-            k11[j] = n11[j];
-            k13[j] = n13[j];
-        }
-        paz = naz;
-        k33 = n33;
-        s11 = t11;
-        s13 = t13;
-        s33 = t33;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Two forward levels per launch on 64 x 16 tiles (1024 threads).
 //
-// The same scheme and arithmetic as fwd3d_t2_kernel, arranged for fewer
+// The scheme above with the arithmetic of fwd3d_kernel, arranged for few
 // registers per thread so that four waves per SIMD stay resident without
 // spills:
 //  - a 16-row tile: E = 72 x 24 points, the tile's 1024 plus a 704-point rim,
