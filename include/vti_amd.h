@@ -299,10 +299,12 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
 /* The batched gradient captured once and replayed: for the iterations of an
  * inversion, which evaluate the same shots (geometry, observed data, wavelet)
  * on a new model each time.  vti2d_batch_plan_create checks the arguments as
- * vti2d_objfunc_gradient_batch does, allocates the device state of every
- * wave (waves of at most max_wave shots when > 0, else one; all held at once,
- * an error when they do not fit the free device memory), uploads the shot
- * layout, receiver maps and observed data, and records the call's whole
+ * vti2d_objfunc_gradient_batch does, splits the shots into waves (of at
+ * most max_wave shots when > 0, and within the free device memory, as the
+ * direct call does), allocates one set of wavefields, snapshots and trace
+ * buffers sized for the largest wave, which the waves use one after another,
+ * uploads each wave's shot layout, receiver maps and observed data (an error
+ * when the whole does not fit the free device memory), and records the call's whole
  * launch sequence -- forward pass, residual, adjoint pass, FP64 accumulation,
  * every buffer clearing done by a kernel -- into a hipGraph on the context's
  * stream.  vti2d_batch_plan_run uploads c11/c13/c33, replays the graph and

@@ -16,7 +16,9 @@ def _say(*a):
     print(*a, flush=True)
 
 
-def batch_problem(vti, NX=160, nz=40, npad=10, nt=161, ns=5, seed=8):
+def batch_problem(vti, NX=160, nz=40, npad=10, nt=161, ns=5, seed=8, nx_range=(40, 90), sz=2, rz=1):
+    """ns shots on random apertures nx_range wide of an NX-wide model; source
+    sz and receivers rz rows below the top padding."""
     rng = np.random.default_rng(seed)
     NNX, nnz = NX + 2 * npad, nz + 2 * npad
     v, e, d = layered2d(nnz, NNX, seed=seed)
@@ -26,13 +28,13 @@ def batch_problem(vti, NX=160, nz=40, npad=10, nt=161, ns=5, seed=8):
     ct = cij_from(vt, e, d)
     shots, rloc = [], []
     for s in range(ns):
-        nx = int(rng.integers(40, 90))
+        nx = int(rng.integers(*nx_range))
         c0 = int(rng.integers(0, NX - nx + 1))
         w = nx + 2 * npad
         step = int(rng.integers(1, 4))
         rx = list(range(npad, npad + nx, step))
-        shots.append((c0, nx, npad + int(rng.integers(4, nx - 4)), npad + 2, len(rx), len(rloc)))
-        rloc += [(x, npad + 1) for x in rx]
+        shots.append((c0, nx, npad + int(rng.integers(4, nx - 4)), npad + sz, len(rx), len(rloc)))
+        rloc += [(x, npad + rz) for x in rx]
     return dict(NX=NX, nz=nz, npad=npad, NNX=NNX, nnz=nnz, nt=nt, dx=10.0, dz=10.0, dt=1e-3, rate=4e-3, c=c, ct=ct,
                 w=ricker(nt, 1e-3, 20.0, 0.05), shots=np.array(shots, np.int32), rloc=np.array(rloc, np.int32))
 
@@ -112,6 +114,31 @@ def test_launch_shapes_bit_exact(vti, ctx, levels, rows):
     assert abs(fb - fp) <= 1e-6 * fp
 
 
+@pytest.mark.parametrize("rows,snap", [(4, "fp32"), (8, "fp32"), (16, "fp32"), (16, "s16"), (16, "bf16")])
+def test_inner_tiles_bit_exact(vti, ctx, rows, snap):
+    """Apertures wide and deep enough that most two-level tiles lie where the
+    taper is 1 (the kernels' inner-tile form: no clamps, bounds tests or
+    taper factors), with the source and the receiver row inside such tiles:
+    the one-shot driver's bits with FP32 snapshots, and with 16-bit ones the
+    bits of single-level launches, which have no inner-tile form."""
+    B = batch_problem(vti, NX=600, nz=120, npad=20, nt=121, ns=4, seed=15, nx_range=(150, 300), sz=40, rz=30)
+    _say("batch2d inner tiles rows", rows, snap)
+    # half the observed data: a residual that is not zero within the short run
+    data = observed(vti, ctx, B) * np.float32(0.5)
+    run = lambda **kw: vti.objfunc_gradient2d_batch(ctx, data, *B["c"], B["w"], B["shots"], B["rloc"], B["NX"],
+                                                    B["nz"], B["nt"], B["npad"], B["dx"], B["dz"], B["dt"], B["rate"],
+                                                    snap=snap, **kw)
+    fb, gb = run(levels=2, tile_rows=rows, max_wave=3)
+    assert np.abs(gb).max() > 0
+    if snap == "fp32":
+        fp, gp = plain(vti, ctx, B, data)
+        assert np.array_equal(gb, gp)
+        assert abs(fb - fp) <= 1e-6 * fp
+    else:
+        f1, g1 = run(levels=1, tile_rows=8)
+        assert np.array_equal(gb, g1) and fb == f1
+
+
 def oracle_f64(orc, vti, B, data):
     """The FP64 oracle of objFuncGradient_f on each shot's cropped model, summed."""
     g = np.zeros((3, B["nnz"], B["NNX"]), np.float64)
@@ -176,12 +203,15 @@ def test_autotuned_launch_shape(vti, ctx):
     assert np.array_equal(g0, g1) and f0 == f1
 
 
-@pytest.mark.parametrize("snap,levels,max_wave", [("fp32", 2, 0), ("fp32", 1, 3), ("bf16", 2, 2)])
+@pytest.mark.parametrize("snap,levels,max_wave",
+                         [("fp32", 2, 0), ("fp32", 1, 3), ("bf16", 2, 2), ("s16", 2, 1), ("fp16", 1, 3)])
 def test_graph_plan_replays_equal_direct_calls(vti, ctx, snap, levels, max_wave):
     """A captured plan (hipGraph of the whole launch sequence) replayed on
     three models, with other drivers' work and synchronisations between the
     replays, gives each time the direct batched call's bits on that model --
-    also the third replay, back on the first model."""
+    also the third replay, back on the first model.  With max_wave > 0 the
+    plan's waves run one after another through one shared set of wavefield,
+    snapshot and trace buffers."""
     B = batch_problem(vti, seed=14, ns=4)
     _say("batch2d plan", snap, "levels", levels, "max_wave", max_wave)
     data = observed(vti, ctx, B)

@@ -136,6 +136,26 @@ void zero_words(void *p, size_t nwords, hipStream_t st)
     hipLaunchKernelGGL(fill0_kernel, dim3(nb), dim3(256), 0, st, (unsigned *)p, (long long)nwords);
 }
 
+// Slot 0 of each shot's snapshots, D of the zero level 0, which the forward
+// pass never writes, and its 16-bit scales (two per slot for FP16/BF16, two
+// per 64-point segment for S16): cleared in each wave's sequence, because
+// the waves of a captured batch share the snapshot buffers.  Grid: y = shot.
+__global__ void clear_slot0_kernel(float *__restrict__ snapf, unsigned short *__restrict__ snaph,
+                                   float *__restrict__ scale, const vti::ShotB *__restrict__ sb, int nnz, int fmt)
+{
+    const vti::ShotB sh = sb[blockIdx.y];
+    const long long plane = (long long)sh.nnx * nnz;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (fmt == 0) {
+        for (long long k = t; k < 2 * plane; k += stride) snapf[sh.snap_off + k] = 0.f;
+        return;
+    }
+    for (long long k = t; k < 2 * plane; k += stride) snaph[sh.snap_off + k] = 0;
+    const long long nsc = fmt == 3 ? 2 * ((plane + 63) >> 6) : 2;
+    for (long long k = t; k < nsc; k += stride) scale[sh.scale_off + k] = 0.f;
+}
+
 struct ShotIn {
     int c0, nx, ix, iz, nr, r0;
 };
@@ -265,7 +285,7 @@ int upload_model(BatchShared &S, const BatchArgs &a, const float *c11, const flo
 // two-level ones' 64 x tile_rows), receiver maps, wavefields, snapshots,
 // traces, the observed data and the residual's partial sums.
 struct BatchWave {
-    int nw = 0, nrec = 0;
+    int s0 = 0, nw = 0, nrec = 0;
     long long f_total = 0, snap_total = 0, tr_total = 0, max_plane = 0, scale_total = 0;
     std::vector<ShotB> sb, sbt;
     std::vector<int> tile_shot, tile_shot_t;
@@ -287,10 +307,12 @@ int shot_residual_blocks(long long n)
     return (int)std::min<long long>(2048, std::max<long long>(1, (n + 255) / 256));
 }
 
-int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *data, const int *rloc, hipStream_t st)
+// The wave's shot descriptors, tile maps and buffer sizes (host only).
+void size_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const int *rloc)
 {
     const int nw = s1 - s0, nnz = a.nnz, npad = a.npad, nnt = a.nnt;
     const int ntz = (nnz - 8 + kBY2 - 1) / kBY2, ntz_t = (nnz - 8 + a.BYT - 1) / a.BYT;
+    W.s0 = s0;
     W.nw = nw;
     W.sb.assign(nw, ShotB{});
     W.sbt.assign(nw, ShotB{});
@@ -332,6 +354,66 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
         W.tr_total += (long long)a.nnt_data * q.nr;
         W.nrec += q.nr;
     }
+}
+
+// Device buffers that waves run one after another can share: wavefields,
+// gradients, snapshots (with their staging planes, scales and maxima) and
+// recorded traces, sized for the largest wave.  enqueue_wave clears
+// whatever a wave reads before writing it, so a captured sequence of waves
+// runs them all through one set.
+struct WavePool {
+    long long f = 0, snap = 0, scale = 0, amax = 0, tr = 1;
+    DBuf<float> P[4], Q[4], G[3], snapf, stage, scale_b, tr_b;
+    DBuf<unsigned short> snaph;
+    DBuf<unsigned> amax_b;
+};
+
+long long wave_scale_count(const BatchWave &W, const BatchArgs &a)
+{
+    return a.snap_fmt == 3 ? W.scale_total : 2LL * a.nnt * W.nw;
+}
+
+void pool_fit(WavePool &pool, const BatchWave &W, const BatchArgs &a)
+{
+    pool.f = std::max(pool.f, W.f_total);
+    pool.snap = std::max(pool.snap, W.snap_total);
+    pool.scale = std::max(pool.scale, wave_scale_count(W, a));
+    pool.amax = std::max(pool.amax, 2LL * a.nnt * W.nw);
+    pool.tr = std::max(pool.tr, std::max<long long>(W.tr_total, 1));
+}
+
+long long pool_bytes(const WavePool &pool, const BatchArgs &a)
+{
+    const long long snapb = a.half ? 2 * pool.snap + 4 * 2 * pool.f + 4 * pool.scale + 4 * pool.amax : 4 * pool.snap;
+    return 4 * pool.f * 11 + snapb + 4 * pool.tr;
+}
+
+int pool_alloc(WavePool &pool, const BatchArgs &a, hipStream_t st)
+{
+    int rc = vti::kOk;
+    for (int k = 0; k < 4 && !rc; ++k) {
+        rc = pool.P[k].alloc((size_t)pool.f, st, false);
+        if (!rc) rc = pool.Q[k].alloc((size_t)pool.f, st, false);
+    }
+    for (int k = 0; k < 3 && !rc; ++k) rc = pool.G[k].alloc((size_t)pool.f, st, false);
+    if (a.half) {
+        if (!rc) rc = pool.snaph.alloc((size_t)pool.snap, st, false);
+        if (!rc) rc = pool.stage.alloc((size_t)(2 * pool.f), st, false);
+        if (!rc) rc = pool.scale_b.alloc((size_t)pool.scale, st, false);
+        if (!rc) rc = pool.amax_b.alloc((size_t)pool.amax, st, false);
+    } else {
+        if (!rc) rc = pool.snapf.alloc((size_t)pool.snap, st, false);
+    }
+    if (!rc) rc = pool.tr_b.alloc((size_t)pool.tr, st, false);
+    return rc;
+}
+
+// Allocate (or, with a pool, view) the wave's device buffers and upload its
+// tables, receiver map and observed data.
+int fill_wave(BatchWave &W, const BatchArgs &a, const float *data, const int *rloc, hipStream_t st,
+              const WavePool *pool)
+{
+    const int s0 = W.s0, nw = W.nw, nnt = a.nnt;
     const int nrec = W.nrec;
     std::vector<long long> fidx(nrec), tb(nrec);
     std::vector<int> tsd(nrec);
@@ -360,30 +442,43 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
     if (!rc) rc = W.tsdd.alloc(nrec, st);
     if (!rc) rc = W.fidxd.alloc(nrec, st);
     if (!rc) rc = W.tbd.alloc(nrec, st);
-    for (int k = 0; k < 4 && !rc; ++k) {
-        rc = W.P[k].alloc((size_t)f_total, st);
-        if (!rc) rc = W.Q[k].alloc((size_t)f_total, st);
-    }
-    for (int k = 0; k < 3 && !rc; ++k) rc = W.G[k].alloc((size_t)f_total, st);
     // snapshot slots 1.. are written by the forward pass before the adjoint
     // reads them; slot 0 (D of the zero level 0) is never written and is
-    // cleared below, so the slots themselves need no clearing
-    if (a.half) {
-        if (!rc) rc = W.snaph.alloc((size_t)W.snap_total, st, false);
-        if (!rc) rc = W.stage.alloc((size_t)(2 * f_total), st);
-        if (!rc) rc = W.scale.alloc((size_t)(a.snap_fmt == 3 ? W.scale_total : 2 * nnt * nw), st);
-        if (!rc) rc = W.amax.alloc((size_t)(2 * nnt * nw), st);
+    // cleared by enqueue_wave, so the slots themselves need no clearing
+    if (pool) {
+        VTI_CHECK(pool->f >= f_total && pool->snap >= W.snap_total && pool->scale >= wave_scale_count(W, a) &&
+                      pool->amax >= 2LL * nnt * nw && pool->tr >= tr_total,
+                  "batch wave larger than its shared buffers");
+        for (int k = 0; k < 4; ++k) {
+            W.P[k].borrow(pool->P[k], (size_t)f_total);
+            W.Q[k].borrow(pool->Q[k], (size_t)f_total);
+        }
+        for (int k = 0; k < 3; ++k) W.G[k].borrow(pool->G[k], (size_t)f_total);
+        if (a.half) {
+            W.snaph.borrow(pool->snaph, (size_t)W.snap_total);
+            W.stage.borrow(pool->stage, (size_t)(2 * f_total));
+            W.scale.borrow(pool->scale_b, (size_t)wave_scale_count(W, a));
+            W.amax.borrow(pool->amax_b, (size_t)(2 * nnt * nw));
+        } else {
+            W.snapf.borrow(pool->snapf, (size_t)W.snap_total);
+        }
+        W.tr.borrow(pool->tr_b, (size_t)std::max<long long>(tr_total, 1));
     } else {
-        if (!rc) rc = W.snapf.alloc((size_t)W.snap_total, st, false);
+        for (int k = 0; k < 4 && !rc; ++k) {
+            rc = W.P[k].alloc((size_t)f_total, st);
+            if (!rc) rc = W.Q[k].alloc((size_t)f_total, st);
+        }
+        for (int k = 0; k < 3 && !rc; ++k) rc = W.G[k].alloc((size_t)f_total, st);
+        if (a.half) {
+            if (!rc) rc = W.snaph.alloc((size_t)W.snap_total, st, false);
+            if (!rc) rc = W.stage.alloc((size_t)(2 * f_total), st);
+            if (!rc) rc = W.scale.alloc((size_t)wave_scale_count(W, a), st);
+            if (!rc) rc = W.amax.alloc((size_t)(2 * nnt * nw), st);
+        } else {
+            if (!rc) rc = W.snapf.alloc((size_t)W.snap_total, st, false);
+        }
+        if (!rc) rc = W.tr.alloc((size_t)std::max<long long>(tr_total, 1), st);
     }
-    for (int j = 0; j < nw && !rc; ++j) {
-        const size_t plane2 = (size_t)2 * W.sb[j].nnx * nnz;
-        if (a.half)
-            VTI_HIP(hipMemsetAsync(W.snaph.p + W.sb[j].snap_off, 0, plane2 * sizeof(unsigned short), st));
-        else
-            VTI_HIP(hipMemsetAsync(W.snapf.p + W.sb[j].snap_off, 0, plane2 * sizeof(float), st));
-    }
-    if (!rc) rc = W.tr.alloc((size_t)std::max<long long>(tr_total, 1), st);
     if (!rc) rc = W.obs.alloc((size_t)std::max<long long>(tr_total, 1), st);
     W.pblk.assign(nw, 0);
     W.poff.assign(nw, 0);
@@ -412,6 +507,13 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
     return vti::kOk;
 }
 
+int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *data, const int *rloc, hipStream_t st,
+               const WavePool *pool = nullptr)
+{
+    size_wave(W, a, s0, s1, rloc);
+    return fill_wave(W, a, data, rloc, st, pool);
+}
+
 // Enqueue one wave's whole objective + gradient on `st`: the forward pass
 // with snapshots and traces, the residual (its partial sums into W.part),
 // the adjoint pass with the imaging condition and the shots' gradients
@@ -434,6 +536,13 @@ int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *
     for (int k = 0; k < 3; ++k) zero_words(G[k].p, (size_t)f_total, st);
     zero_words(W.tr.p, W.tr.n, st);  // row 0 is never recorded
     if (half && snap_fmt != 3) zero_words(W.amax.p, W.amax.n, st);
+    // the forward kernels stage D at interior points only; the rest of the
+    // staging planes stays 0, so that an S16 segment's scale comes from the
+    // shot's own values (waves of a captured batch share these planes)
+    if (half) zero_words(W.stage.p, (size_t)(2 * f_total), st);
+    hipLaunchKernelGGL(clear_slot0_kernel, dim3((unsigned)std::min<long long>((2 * W.max_plane + 255) / 256, 1024),
+                                                (unsigned)nw),
+                       dim3(256), 0, st, W.snapf.p, W.snaph.p, W.scale.p, W.sbd.p, nnz, snap_fmt);
 
     BatchGeo bg;
     bg.nnz = nnz;
@@ -722,12 +831,15 @@ int cache_wave(const int *shots, int ns, int npad, int nz)
 
 }  // namespace
 
-// A captured batch: the device state of every wave (all held at once) and
-// the graph of the whole launch sequence, wave after wave.
+// A captured batch: each wave's own tables, receiver map and observed data,
+// one set of wavefield/snapshot/trace buffers that the waves share (sized
+// for the largest), and the graph of the whole launch sequence, wave after
+// wave.
 struct vti2d_batch_plan {
     vti_ctx *ctx = nullptr;
     BatchArgs a;
     BatchShared S;
+    WavePool pool;
     std::vector<std::unique_ptr<BatchWave>> W;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -793,22 +905,28 @@ int vti2d_batch_plan_create(vti_ctx *ctx, vti2d_batch_plan **plan, const float *
         }
     }
     if (!rc) {
-        const size_t mfree = vti::device_free_bytes();
-        long long need = 0;
-        for (int s = 0; s < ns; ++s) need += pl->a.shot_bytes(s);
-        if (need > (long long)mfree - (long long)(2ULL << 30)) {
-            vti::set_error("vti2d_batch_plan_create: the %d shots need %.1f GB held at once, %.1f GB free", ns,
-                           need / 1e9, mfree / 1e9);
+        // waves within the memory budget, sized on the host; the shared set
+        // fits the largest, and each wave adds its own tables and data
+        const long long budget = (long long)vti::device_free_bytes() - (long long)(2ULL << 30);
+        for (const auto &w : split_waves(pl->a, max_wave, budget)) {
+            pl->W.emplace_back(new BatchWave);
+            size_wave(*pl->W.back(), pl->a, w.first, w.second, rloc);
+            pool_fit(pl->pool, *pl->W.back(), pl->a);
+        }
+        long long need = pool_bytes(pl->pool, pl->a);
+        for (const auto &W : pl->W)
+            need += 4 * W->f_total + 4 * std::max<long long>(W->tr_total, 1) + 20LL * W->nrec +
+                    4LL * (long long)(W->tile_shot.size() + W->tile_shot_t.size()) + (1LL << 16);
+        if (need > budget) {
+            vti::set_error("vti2d_batch_plan_create: the %d shots in %d waves need %.1f GB, %.1f GB free", ns,
+                           (int)pl->W.size(), need / 1e9, (budget + (2LL << 30)) / 1e9);
             rc = vti::kErrArg;
         }
     }
     if (!rc) rc = setup_shared(pl->S, pl->a, st);
-    if (!rc)
-        for (const auto &w : split_waves(pl->a, max_wave, 1LL << 62)) {
-            pl->W.emplace_back(new BatchWave);
-            rc = setup_wave(*pl->W.back(), pl->a, w.first, w.second, data, rloc, st);
-            if (rc) break;
-        }
+    if (!rc) rc = pool_alloc(pl->pool, pl->a, st);
+    for (auto &W : pl->W)
+        if (!rc) rc = fill_wave(*W, pl->a, data, rloc, st, &pl->pool);
     if (!rc) {
         hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {

@@ -614,7 +614,7 @@ struct HaloV {
     float p, q, u, w;
 };
 
-template <int BY, bool ADJ>
+template <int BY, bool ADJ, bool IN>
 __device__ __forceinline__ void halo_load_t2(HaloV<ADJ> (&hv)[HaloT2<BY>::NH], const float *__restrict__ P,
                                              const float *__restrict__ Q, const float *__restrict__ c11,
                                              const float *__restrict__ c13, const float *__restrict__ c33, int nnx,
@@ -625,8 +625,10 @@ __device__ __forceinline__ void halo_load_t2(HaloV<ADJ> (&hv)[HaloT2<BY>::NH], c
         bool in_a;
         int r, c, x, z;
         (void)HaloT2<BY>::at(tid + j * HaloT2<BY>::NT, x0, z0, in_a, r, c, x, z);
-        x = min(max(x, 0), nnx - 1);
-        z = min(max(z, 0), nnz - 1);
+        if (!IN) {
+            x = min(max(x, 0), nnx - 1);
+            z = min(max(z, 0), nnz - 1);
+        }
         const unsigned i = x + z * nnx;
         // the wavefront's tile, uniform across it by the slot layout
         const bool wa = __builtin_amdgcn_readfirstlane((tid + j * HaloT2<BY>::NT) < HaloT2<BY>::NAW);
@@ -642,7 +644,7 @@ __device__ __forceinline__ void halo_load_t2(HaloV<ADJ> (&hv)[HaloT2<BY>::NH], c
     }
 }
 
-template <int BY, bool ADJ>
+template <int BY, bool ADJ, bool IN>
 __device__ __forceinline__ void halo_store_t2(TB2<BY, ADJ> &t, const HaloV<ADJ> (&hv)[HaloT2<BY>::NH], int nnx,
                                               int nnz, int x0, int z0, int tid)
 {
@@ -653,7 +655,7 @@ __device__ __forceinline__ void halo_store_t2(TB2<BY, ADJ> &t, const HaloV<ADJ> 
         if (HaloT2<BY>::at(tid + j * HaloT2<BY>::NT, x0, z0, in_a, r, c, x, z)) {
             // x tile: p (adjoint c11 p + c13 q); z tile: q (adjoint c13 p + c33 q)
             float v = ADJ ? hv[j].u * hv[j].p + hv[j].w * hv[j].q : hv[j].p;
-            v = (x >= 0 && x < nnx && z >= 0 && z < nnz) ? v : 0.f;
+            v = (IN || (x >= 0 && x < nnx && z >= 0 && z < nnz)) ? v : 0.f;
             if (in_a)
                 t.a[r][c] = v;
             else
@@ -675,31 +677,31 @@ __device__ __forceinline__ float st9_col(const float (*a)[W], int r, int c)
                a[r + 4][c]);
 }
 
+// Whether a tile's R1 (and so its tile and halo) lies where the taper is 1:
+// every point then has the taper factor t0[npad] = 1 in both directions, is
+// inside the grid and in the interior, so the clamps, the bounds tests and
+// the taper loads and products drop out (a product by 1 is exact).
+template <int BY>
+__device__ __forceinline__ bool t2_inner(int x0, int z0, int nnx, int nnz, int npad)
+{
+    return x0 - 4 >= npad && x0 + 68 <= nnx - npad && z0 - 4 >= npad && z0 + BY + 4 <= nnz - npad;
+}
+
 // Forward: two forwardC steps with injectSource, abc and the D snapshots of
 // the two levels read (slot1 for level L, slot2 for level L+1; < 0: none).
-template <int BY>
-static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
-    float *__restrict__ Pc, float *__restrict__ Qc, float *__restrict__ Pd, float *__restrict__ Qd,
-    const float *__restrict__ Pa, const float *__restrict__ Qa, const float *__restrict__ Pb,
-    const float *__restrict__ Qb, const float *__restrict__ c11, const float *__restrict__ c13,
-    const float *__restrict__ c33, const float *__restrict__ t0, const ShotB *__restrict__ shots,
-    const int *__restrict__ tile_shot, BatchGeo bg, float dx2, float dz2, float dt2, float src1, float src2,
-    SnapW sw, int slot1, int slot2)
+// IN: the tile is inner (t2_inner).
+template <int BY, bool IN>
+__device__ __forceinline__ void fwd2db_t2_body(
+    TB2<BY, false> &t, float2 *red, const ShotB &sh, int x0, int z0, float *__restrict__ Pc,
+    float *__restrict__ Qc, float *__restrict__ Pd, float *__restrict__ Qd, const float *__restrict__ Pa,
+    const float *__restrict__ Qa, const float *__restrict__ Pb, const float *__restrict__ Qb,
+    const float *__restrict__ cc11, const float *__restrict__ cc13, const float *__restrict__ cc33,
+    const float *__restrict__ t0, const BatchGeo &bg, float dx2, float dz2, float dt2, float src1, float src2,
+    const SnapW &sw, int slot1, int slot2)
 {
     constexpr int NT = 64 * BY, N1 = (BY + 8) * 72, NK = (N1 + NT - 1) / NT;
-    __shared__ TB2<BY, false> t;
-    __shared__ float2 red[BY];
-    const int tile = xcd_tile();
-    const ShotB sh = shots[tile_shot[tile]];
-    const int tl = tile - sh.tile0;
-    const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * BY;
     const int nnx = sh.nnx, nnz = bg.nnz, tid = threadIdx.x + 64 * threadIdx.y;
     const long long plane = (long long)nnx * nnz;
-    const float *cc11 = c11 + sh.c_off, *cc13 = c13 + sh.c_off, *cc33 = c33 + sh.c_off;
-    Pa += sh.f_off;
-    Qa += sh.f_off;
-    Pb += sh.f_off;
-    Qb += sh.f_off;
     // operands of this thread's points of R1 (levels L and L-1, coefficients,
     // taper) and of its own point of the tile.  Every load is made, at the
     // point clamped into the grid, without a branch: values of points outside
@@ -709,7 +711,11 @@ static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
         const int k = min(tid + j * NT, N1 - 1);
-        const int x = min(x0 - 4 + k % 72, nnx - 1), z = min(z0 - 4 + k / 72, nnz - 1);
+        int x = x0 - 4 + k % 72, z = z0 - 4 + k / 72;
+        if (!IN) {
+            x = min(x, nnx - 1);
+            z = min(z, nnz - 1);
+        }
         const unsigned i = x + z * nnx, c = x + z * bg.cpitch;
         pb[j] = ld32(Pb, i);
         qb[j] = ld32(Qb, i);
@@ -718,33 +724,33 @@ static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
         k11[j] = ld32(cc11, c);
         k13[j] = ld32(cc13, c);
         k33[j] = ld32(cc33, c);
-        tk[j] = taper_ld32(t0, bg.npad, nnx, nnz, x, z);
+        tk[j] = IN ? TapF{1.f, 1.f} : taper_ld32(t0, bg.npad, nnx, nnz, x, z);
     }
     HaloV<false> hv[HaloT2<BY>::NH];
-    halo_load_t2<BY, false>(hv, Pb, Qb, nullptr, nullptr, nullptr, nnx, nnz, 0, x0, z0, tid);
+    halo_load_t2<BY, false, IN>(hv, Pb, Qb, nullptr, nullptr, nullptr, nnx, nnz, 0, x0, z0, tid);
     const int x = x0 + threadIdx.x, z = z0 + threadIdx.y;
-    const bool act = x < nnx - 4 && z < nnz - 4;
+    const bool act = IN || (x < nnx - 4 && z < nnz - 4);
     float a11, a13, a33;
     TapF tw;
     {
-        const int xo = min(x, nnx - 1), zo = min(z, nnz - 1);
+        const int xo = IN ? x : min(x, nnx - 1), zo = IN ? z : min(z, nnz - 1);
         const unsigned c = xo + zo * bg.cpitch;
         a11 = ld32(cc11, c);
         a13 = ld32(cc13, c);
         a33 = ld32(cc33, c);
-        tw = taper_ld32(t0, bg.npad, nnx, nnz, xo, zo);
+        tw = IN ? TapF{1.f, 1.f} : taper_ld32(t0, bg.npad, nnx, nnz, xo, zo);
     }
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
         const int k = tid + j * NT;
         if (k < N1) {
             const int rx = k % 72, rz = k / 72;
-            const bool in = x0 - 4 + rx < nnx && z0 - 4 + rz < nnz;
+            const bool in = IN || (x0 - 4 + rx < nnx && z0 - 4 + rz < nnz);
             t.a[rz][rx + 4] = in ? pb[j] : 0.f;
             t.b[rz + 4][rx] = in ? qb[j] : 0.f;
         }
     }
-    halo_store_t2<BY, false>(t, hv, nnx, nnz, x0, z0, tid);
+    halo_store_t2<BY, false, IN>(t, hv, nnx, nnz, x0, z0, tid);
     __syncthreads();
     float mp = 0.f, mq = 0.f;
     // level L+1 on R1
@@ -755,7 +761,7 @@ static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
             const int rx = k % 72, rz = k / 72;
             const int xx = x0 - 4 + rx, zz = z0 - 4 + rz;
             float p1 = 0.f, q1 = 0.f;
-            if (xx >= 4 && xx < nnx - 4 && zz >= 4 && zz < nnz - 4) {
+            if (IN || (xx >= 4 && xx < nnx - 4 && zz >= 4 && zz < nnz - 4)) {
                 const int i = xx + zz * nnx;
                 const float tp = qdiv(st9_row(t.a[rz], rx + 4), dx2, bg.rx2);
                 const float tq = qdiv(st9_col<72>(t.b, rz + 4, rx), dz2, bg.rz2);
@@ -801,39 +807,60 @@ static __global__ __launch_bounds__(64 * BY) void fwd2db_t2_kernel(
             pn += src2;
             qn += src2;
         }
-        st32(Pc + sh.f_off, i, p1);
-        st32(Qc + sh.f_off, i, q1);
-        st32(Pd + sh.f_off, i, pn * tap);
-        st32(Qd + sh.f_off, i, qn * tap);
+        st32(Pc, i, p1);
+        st32(Qc, i, q1);
+        st32(Pd, i, pn * tap);
+        st32(Qd, i, qn * tap);
     }
     if (sw.amax && slot2 >= 0) snap_amax(sw, sh, slot2, mp, mq, red);
 }
 
-// Adjoint: two backwardDC steps with the residual injection, abc and
-// gradientCrossCor of both levels, the gradient read and written once.
-template <int BY, int SF>
-static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
+// A 16-row workgroup is 16 waves; two of them per CU need 8 waves per SIMD,
+// which gfx950 grants only to kernels of at most 80 SGPRs.  The inner and
+// edge bodies together would take about 93, so the occupancy is pinned and
+// the compiler keeps a few uniform values in VGPR lanes instead.
+template <int BY>
+static __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(BY == 16 ? 8 : 1, BY == 16 ? 8 : 10))) void fwd2db_t2_kernel(
     float *__restrict__ Pc, float *__restrict__ Qc, float *__restrict__ Pd, float *__restrict__ Qd,
     const float *__restrict__ Pa, const float *__restrict__ Qa, const float *__restrict__ Pb,
     const float *__restrict__ Qb, const float *__restrict__ c11, const float *__restrict__ c13,
     const float *__restrict__ c33, const float *__restrict__ t0, const ShotB *__restrict__ shots,
-    const int *__restrict__ tile_shot, BatchGeo bg, const int *__restrict__ recmap, const float *__restrict__ traces,
-    LevelB l1, LevelB l2, SnapR sr, float *__restrict__ G11, float *__restrict__ G13, float *__restrict__ G33,
-    float dx2, float dz2, float dt2)
+    const int *__restrict__ tile_shot, BatchGeo bg, float dx2, float dz2, float dt2, float src1, float src2,
+    SnapW sw, int slot1, int slot2)
 {
-    constexpr int NT = 64 * BY, N1 = (BY + 8) * 72, NK = (N1 + NT - 1) / NT;
-    __shared__ TB2<BY, true> t;
+    __shared__ TB2<BY, false> t;
+    __shared__ float2 red[BY];
     const int tile = xcd_tile();
     const ShotB sh = shots[tile_shot[tile]];
     const int tl = tile - sh.tile0;
     const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * BY;
+    const long long fo = sh.f_off, co = sh.c_off;
+    if (t2_inner<BY>(x0, z0, sh.nnx, bg.nnz, bg.npad))
+        fwd2db_t2_body<BY, true>(t, red, sh, x0, z0, Pc + fo, Qc + fo, Pd + fo, Qd + fo, Pa + fo, Qa + fo, Pb + fo,
+                                 Qb + fo, c11 + co, c13 + co, c33 + co, t0, bg, dx2, dz2, dt2, src1, src2, sw, slot1,
+                                 slot2);
+    else
+        fwd2db_t2_body<BY, false>(t, red, sh, x0, z0, Pc + fo, Qc + fo, Pd + fo, Qd + fo, Pa + fo, Qa + fo, Pb + fo,
+                                  Qb + fo, c11 + co, c13 + co, c33 + co, t0, bg, dx2, dz2, dt2, src1, src2, sw, slot1,
+                                  slot2);
+}
+
+// Adjoint: two backwardDC steps with the residual injection, abc and
+// gradientCrossCor of both levels, the gradient read and written once.
+// IN: the tile is inner (t2_inner).
+template <int BY, int SF, bool IN>
+__device__ __forceinline__ void bwd2db_t2_body(
+    TB2<BY, true> &t, const ShotB &sh, int x0, int z0, float *__restrict__ Pc, float *__restrict__ Qc,
+    float *__restrict__ Pd, float *__restrict__ Qd, const float *__restrict__ Pa, const float *__restrict__ Qa,
+    const float *__restrict__ Pb, const float *__restrict__ Qb, const float *__restrict__ cc11,
+    const float *__restrict__ cc13, const float *__restrict__ cc33, const float *__restrict__ t0, const BatchGeo &bg,
+    const int *__restrict__ rmap, const float *__restrict__ traces, const LevelB &l1, const LevelB &l2,
+    const SnapR &sr, float *__restrict__ g11p, float *__restrict__ g13p, float *__restrict__ g33p, float dx2,
+    float dz2, float dt2)
+{
+    constexpr int NT = 64 * BY, N1 = (BY + 8) * 72, NK = (N1 + NT - 1) / NT;
     const int nnx = sh.nnx, nnz = bg.nnz, tid = threadIdx.x + 64 * threadIdx.y;
     const long long plane = (long long)nnx * nnz;
-    const float *cc11 = c11 + sh.c_off, *cc13 = c13 + sh.c_off, *cc33 = c33 + sh.c_off;
-    Pa += sh.f_off;
-    Qa += sh.f_off;
-    Pb += sh.f_off;
-    Qb += sh.f_off;
     // operands of this thread's points of R1 (levels L and L-1, coefficients,
     // taper, receiver index) and of its own point of the tile (level L, taper,
     // receiver index, the snapshots of both levels, the gradient).  Every load
@@ -848,7 +875,11 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
         const int k = min(tid + j * NT, N1 - 1);
-        const int x = min(x0 - 4 + k % 72, nnx - 1), z = min(z0 - 4 + k / 72, nnz - 1);
+        int x = x0 - 4 + k % 72, z = z0 - 4 + k / 72;
+        if (!IN) {
+            x = min(x, nnx - 1);
+            z = min(z, nnz - 1);
+        }
         const unsigned i = x + z * nnx, c = x + z * bg.cpitch;
         pb[j] = ld32(Pb, i);
         qb[j] = ld32(Qb, i);
@@ -857,19 +888,18 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
         k11[j] = ld32(cc11, c);
         k13[j] = ld32(cc13, c);
         k33[j] = ld32(cc33, c);
-        tk[j] = taper_ld32(t0, bg.npad, nnx, nnz, x, z);
+        tk[j] = IN ? TapF{1.f, 1.f} : taper_ld32(t0, bg.npad, nnx, nnz, x, z);
         irj[j] = -1;
         inj[j] = 0.f;
     }
     const int x = x0 + threadIdx.x, z = z0 + threadIdx.y;
-    const bool act = x < nnx - 4 && z < nnz - 4;
-    const unsigned i = min(x, nnx - 1) + min(z, nnz - 1) * nnx;
+    const bool act = IN || (x < nnx - 4 && z < nnz - 4);
+    const int xo = IN ? x : min(x, nnx - 1), zo = IN ? z : min(z, nnz - 1);
+    const unsigned i = xo + zo * nnx;
     const float pbc = ld32(Pb, i), qbc = ld32(Qb, i);
-    const TapF tw = taper_ld32(t0, bg.npad, nnx, nnz, min(x, nnx - 1), min(z, nnz - 1));
-    const int *rmap = recmap + sh.f_off;
-    float *g11p = G11 + sh.f_off, *g13p = G13 + sh.f_off, *g33p = G33 + sh.f_off;
+    const TapF tw = IN ? TapF{1.f, 1.f} : taper_ld32(t0, bg.npad, nnx, nnz, xo, zo);
     HaloV<true> hv[HaloT2<BY>::NH];
-    halo_load_t2<BY, true>(hv, Pb, Qb, cc11, cc13, cc33, nnx, nnz, bg.cpitch, x0, z0, tid);
+    halo_load_t2<BY, true, IN>(hv, Pb, Qb, cc11, cc13, cc33, nnx, nnz, bg.cpitch, x0, z0, tid);
     // the injected residual: recmap, then the trace reads that depend on it
     float inj2 = 0.f;
     int ir2 = -1;
@@ -903,12 +933,12 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
         const int k = tid + j * NT;
         if (k < N1) {
             const int rx = k % 72, rz = k / 72;
-            const bool in = x0 - 4 + rx < nnx && z0 - 4 + rz < nnz;
+            const bool in = IN || (x0 - 4 + rx < nnx && z0 - 4 + rz < nnz);
             t.a[rz][rx + 4] = in ? k11[j] * pb[j] + k13[j] * qb[j] : 0.f;
             t.b[rz + 4][rx] = in ? k13[j] * pb[j] + k33[j] * qb[j] : 0.f;
         }
     }
-    halo_store_t2<BY, true>(t, hv, nnx, nnz, x0, z0, tid);
+    halo_store_t2<BY, true, IN>(t, hv, nnx, nnz, x0, z0, tid);
     __syncthreads();
     // the snapshots of both levels and the gradient at this thread's point
     // (clamped into the grid), issued here so that they arrive while level
@@ -935,7 +965,7 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
             const int rx = k % 72, rz = k / 72;
             const int xx = x0 - 4 + rx, zz = z0 - 4 + rz;
             float p1 = 0.f, q1 = 0.f, pp = 0.f, qq = 0.f;
-            if (xx >= 4 && xx < nnx - 4 && zz >= 4 && zz < nnz - 4) {
+            if (IN || (xx >= 4 && xx < nnx - 4 && zz >= 4 && zz < nnz - 4)) {
                 const float tp0 = qdiv(st9_row(t.a[rz], rx + 4), dx2, bg.rx2);
                 const float tq0 = qdiv(st9_col<72>(t.b, rz + 4, rx), dz2, bg.rz2);
                 const float tkj = taper_of(tk[j]);
@@ -973,10 +1003,10 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
     }
     pn = pn * tap;
     qn = qn * tap;
-    st32(Pc + sh.f_off, i, p1);
-    st32(Qc + sh.f_off, i, q1);
-    st32(Pd + sh.f_off, i, pn);
-    st32(Qd + sh.f_off, i, qn);
+    st32(Pc, i, p1);
+    st32(Qc, i, q1);
+    st32(Pd, i, pn);
+    st32(Qd, i, qn);
     if (same) {  // the raw values, scaled below like the others
         b0p = a0p;
         b1p = a1p;
@@ -1004,6 +1034,33 @@ static __global__ __launch_bounds__(64 * BY) void bwd2db_t2_kernel(
     st32(g11p, i, g11);
     st32(g13p, i, g13);
     st32(g33p, i, g33);
+}
+
+// (16 rows: 8 waves per SIMD pinned, as for the forward)
+template <int BY, int SF>
+static __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(BY == 16 ? 8 : 1, BY == 16 ? 8 : 10))) void bwd2db_t2_kernel(
+    float *__restrict__ Pc, float *__restrict__ Qc, float *__restrict__ Pd, float *__restrict__ Qd,
+    const float *__restrict__ Pa, const float *__restrict__ Qa, const float *__restrict__ Pb,
+    const float *__restrict__ Qb, const float *__restrict__ c11, const float *__restrict__ c13,
+    const float *__restrict__ c33, const float *__restrict__ t0, const ShotB *__restrict__ shots,
+    const int *__restrict__ tile_shot, BatchGeo bg, const int *__restrict__ recmap, const float *__restrict__ traces,
+    LevelB l1, LevelB l2, SnapR sr, float *__restrict__ G11, float *__restrict__ G13, float *__restrict__ G33,
+    float dx2, float dz2, float dt2)
+{
+    __shared__ TB2<BY, true> t;
+    const int tile = xcd_tile();
+    const ShotB sh = shots[tile_shot[tile]];
+    const int tl = tile - sh.tile0;
+    const int x0 = 4 + (tl % sh.ntx) * 64, z0 = 4 + (tl / sh.ntx) * BY;
+    const long long fo = sh.f_off, co = sh.c_off;
+    if (t2_inner<BY>(x0, z0, sh.nnx, bg.nnz, bg.npad))
+        bwd2db_t2_body<BY, SF, true>(t, sh, x0, z0, Pc + fo, Qc + fo, Pd + fo, Qd + fo, Pa + fo, Qa + fo, Pb + fo,
+                                     Qb + fo, c11 + co, c13 + co, c33 + co, t0, bg, recmap + fo, traces, l1, l2, sr,
+                                     G11 + fo, G13 + fo, G33 + fo, dx2, dz2, dt2);
+    else
+        bwd2db_t2_body<BY, SF, false>(t, sh, x0, z0, Pc + fo, Qc + fo, Pd + fo, Qd + fo, Pa + fo, Qa + fo, Pb + fo,
+                                      Qb + fo, c11 + co, c13 + co, c33 + co, t0, bg, recmap + fo, traces, l1, l2, sr,
+                                      G11 + fo, G13 + fo, G33 + fo, dx2, dz2, dt2);
 }
 
 }  // namespace vti
