@@ -40,6 +40,18 @@ int vti_ctx_synchronize(vti_ctx *ctx);
  * as lib/vti-fwi-3d/gpu/kernel3d.cu:54-69 does -- double weights, products
  * and sums over float storage; 0 evaluates them in FP32. */
 int vti_ctx_set_arith(vti_ctx *ctx, int arith);
+/* Snapshot storage of the 2D objective + gradient (vti2d_objfunc_gradient
+ * and its v/eps/del wrapper) on ctx: 0 = FP32 (default, the reference's
+ * storage, objFuncGradient_f.cu:188-202), 1 = FP16, 2 = BF16 with a
+ * power-of-two scale per slice, 3 = S16 (16-bit integers with a power-of-two
+ * scale per 64 consecutive points).  A 16-bit format applies to calls whose
+ * arguments the batched driver takes (the init_abc taper, ot = 0, rate/dt an
+ * integer >= 2 dividing nt-1, sources and receivers in the interior); those
+ * run through it, also for one shot, with half the snapshot memory.  Other
+ * calls keep FP32.  The 3D drivers store no snapshots (they rebuild the
+ * source wavefield from a random boundary), so the setting does not reach
+ * them.  No counterpart in the reference. */
+int vti_ctx_set_snapshots(vti_ctx *ctx, int snap_fmt);
 
 /* ---- host helpers (bit-exact restatements of the reference's host code) - */
 
@@ -395,6 +407,11 @@ int vti_serve_model3d(vti_server *s, long long *id, int model, float *ddata, con
  * were and how many requests they held.  No counterpart in the reference. */
 int vti_serve_set_batching(vti_server *s, int max_requests);
 int vti_serve_batch_stats(vti_server *s, long long *merged_runs, long long *merged_requests);
+/* Snapshot storage of the 2D gradient requests that workers start from now
+ * on, as vti_ctx_set_snapshots sets it for a context (0 = FP32, the default;
+ * 1 = FP16, 2 = BF16, 3 = S16); merged runs and single requests alike.  No
+ * counterpart in the reference. */
+int vti_serve_set_snapshots(vti_server *s, int snap_fmt);
 
 /* blocks until request `id` is done; returns its status (vti_last_error()
  * then holds its message) and its latency and service time in seconds */

@@ -229,3 +229,45 @@ def test_graph_plan_replays_equal_direct_calls(vti, ctx, snap, levels, max_wave)
             # eager work of another driver between replays
             vti.objfunc_gradient2d_batch(ctx, data[::-1].copy(), *B["ct"], B["w"], B["shots"], B["rloc"], *a,
                                          max_wave=2)
+
+
+@pytest.mark.parametrize("snap,tol", [("s16", 1e-4), ("bf16", 3e-3)])
+def test_context_snapshot_format_one_shot_and_served(vti, orc, ctx, snap, tol):
+    """A context set to 16-bit snapshots runs a one-shot objFuncGradient_f
+    through the batched driver with that storage: its gradient is the batched
+    call's on the same shot bit for bit, within the format's bar of the FP64
+    oracle, and a server set to the same format returns the same bits; the
+    default context keeps FP32 (the one-shot loop's bits)."""
+    B = batch_problem(vti, seed=16, ns=1)
+    data = observed(vti, ctx, B) * np.float32(0.5)
+    nnt = (B["nt"] - 1) // 4 + 1
+    c0, nx, ix, iz, nr, r0 = (int(q) for q in B["shots"][0])
+    w = nx + 2 * B["npad"]
+    cc = [crop(c, c0, w) for c in B["c"]]
+    taper = vti.init_taper2d(nx, B["nz"], B["npad"])
+    sl = np.array([[ix, iz, nr, 0]], np.int32)
+    rl = B["rloc"][r0:r0 + nr]
+    d = data[:nnt * nr].reshape(nnt, nr)
+    geo = (nx, B["nz"], B["nt"], B["npad"], B["dx"], B["dz"], B["dt"], B["rate"])
+    _say("context snapshots", snap, "shot width", w)
+    f32, g32 = vti.objfunc_gradient2d(ctx, d, *cc, B["w"], sl, rl, taper, *geo)
+    with vti.Context(0) as c16:
+        c16.set_snapshots(snap)
+        f16, g16 = vti.objfunc_gradient2d(c16, d, *cc, B["w"], sl, rl, taper, *geo)
+        c16.set_snapshots("fp32")
+        f32b, g32b = vti.objfunc_gradient2d(c16, d, *cc, B["w"], sl, rl, taper, *geo)
+    assert np.array_equal(g32, g32b) and f32 == f32b
+    assert not np.array_equal(g16, g32)
+    shot1 = np.array([[0, nx, ix, iz, nr, 0]], np.int32)
+    fb, gb = vti.objfunc_gradient2d_batch(ctx, d.ravel(), *cc, B["w"], shot1, rl, *geo, snap=snap)
+    assert np.array_equal(g16, gb.astype(np.float32))
+    assert abs(float(f16) - fb) <= 1e-6 * fb
+    _, g64 = orc.fg2d(orc.F64, False, d.ravel(), *cc, None, B["w"], sl, rl, taper, *geo)
+    e16, e32 = rel(g16, g64), rel(g32, g64)
+    _say(f"  rel vs FP64: {snap} {e16:.3e}, fp32 {e32:.3e}")
+    assert e16 < tol
+    with vti.Server(0, slots=2, snap=snap) as srv:
+        h = srv.put_model(*cc)
+        t = srv.grad2d(h, d, B["w"], sl, rl, taper, *geo)
+        (fs, gs), _, _ = srv.wait(t)
+    assert np.array_equal(gs, g16) and np.float32(fs) == np.float32(f16)

@@ -1067,8 +1067,10 @@ int vti2d_objfunc_gradient(vti_ctx *ctx, float *objfunc, float *gc11, float *gc1
     // launch for all of a wave's shots); each shot's gradient and objective
     // are the one-shot loop's, summed in the same order, so the result is the
     // same bit for bit.
-    if (ctx && ns >= 2 && objfunc && gc11 && gc13 && gc33 && data && c11 && c13 && c33 && wavelet && dx > 0 &&
-        dz > 0 && vti::grad2d_batchable(sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dt, rate, ot)) {
+    // A context set to 16-bit snapshots sends single shots there too.
+    if (ctx && (ns >= 2 || (ns == 1 && ctx->snap_fmt != 0)) && objfunc && gc11 && gc13 && gc33 && data && c11 &&
+        c13 && c33 && wavelet && dx > 0 && dz > 0 &&
+        vti::grad2d_batchable(sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dt, rate, ot)) {
         vti::Grad2dPart part;
         part.c[0] = c11;
         part.c[1] = c13;
@@ -1083,7 +1085,7 @@ int vti2d_objfunc_gradient(vti_ctx *ctx, float *objfunc, float *gc11, float *gc1
         part.g[0] = gc11;
         part.g[1] = gc13;
         part.g[2] = gc33;
-        return vti::grad2d_merged(ctx, &part, 1, wavelet, nz, nt, npad, dx, dz, dt, rate, 8);
+        return vti::grad2d_merged(ctx, &part, 1, wavelet, nz, nt, npad, dx, dz, dt, rate, 8, ctx->snap_fmt);
     }
     return fg2d(ctx, false, objfunc, gc11, gc13, gc33, data, c11, c13, c33, nullptr, nullptr, nullptr, wavelet, sloc,
                 ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot);

@@ -1104,9 +1104,10 @@ long long grad2d_merged_bytes(const Grad2dPart *p, int n, int nz, int nt, int np
 // stay within their request's columns, in shot order, and each shot's
 // residual runs with the one-shot driver's grid, so every request gets the
 // objective and gradient vti2d_objfunc_gradient returns for it alone, bit for
-// bit (FP32 outputs, as that driver's).
+// bit (FP32 outputs, as that driver's), with the snapshots stored in
+// snap_fmt (0 = FP32 as the one-shot loop stores them; 1-3 the 16-bit forms).
 int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet, int nz, int nt, int npad,
-                  float dx, float dz, float dt, float rate, int max_wave)
+                  float dx, float dz, float dt, float rate, int max_wave, int snap_fmt)
 {
     VTI_CHECK(ctx && p && n >= 1 && wavelet, "grad2d_merged: bad arguments");
     const int nnz = nz + 2 * npad;
@@ -1141,7 +1142,7 @@ int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet
     if (ns) {
         BatchArgs a;
         int rc = parse_args(a, shots.data(), ns, rl.data(), (int)(rl.size() / 2), NNX - 2 * npad, nz, nt, npad, dx,
-                            dz, dt, rate, 2, 16, 0);
+                            dz, dt, rate, 2, 16, snap_fmt);
         if (rc) return rc;
         VTI_HIP(hipSetDevice(ctx->device));
         hipStream_t st = ctx->stream;

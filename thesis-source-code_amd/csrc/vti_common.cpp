@@ -221,6 +221,14 @@ int vti_ctx_set_arith(vti_ctx *c, int arith)
     return vti::kOk;
 }
 
+int vti_ctx_set_snapshots(vti_ctx *c, int snap_fmt)
+{
+    VTI_CHECK(c != nullptr && snap_fmt >= 0 && snap_fmt <= 3,
+              "vti_ctx_set_snapshots: format must be 0 (FP32), 1 (FP16), 2 (BF16) or 3 (S16)");
+    c->snap_fmt = snap_fmt;
+    return vti::kOk;
+}
+
 int vti_ctx_synchronize(vti_ctx *c)
 {
     VTI_CHECK(c != nullptr, "null context");

@@ -97,6 +97,7 @@ struct vti_server {
     bool any = false;
     vti_ctx *upload_ctx = nullptr;
     int batch = 8;                        // most requests merged into one batched run
+    int snap_fmt = 0;                     // snapshot storage of 2D gradient requests
     int idle = 0;                         // workers waiting for work
     long long groups = 0, grouped = 0;    // merged runs and the requests in them
 };
@@ -237,6 +238,7 @@ void worker(vti_server *s)
             ms.push_back(s->models.at(q->model));
         }
         const int wave = s->batch;
+        if (ctx) ctx->snap_fmt = s->snap_fmt;  // this worker's context only
         lk.unlock();
         std::vector<int> rcs(grp.size(), crc);
         std::vector<std::string> errs(grp.size());
@@ -246,7 +248,7 @@ void worker(vti_server *s)
                 std::vector<vti::Grad2dPart> parts;
                 for (size_t i = 0; i < grp.size(); ++i) parts.push_back(part_of(*grp[i], ms[i]));
                 alone = vti::grad2d_merged(ctx, parts.data(), (int)parts.size(), r->wavelet, r->nz, r->nt, r->npad,
-                                           r->dx, r->dz, r->dt, r->rate, wave) != 0;
+                                           r->dx, r->dz, r->dt, r->rate, wave, ctx->snap_fmt) != 0;
                 if (!alone) std::fill(rcs.begin(), rcs.end(), 0);
             }
             if (alone)
@@ -498,6 +500,15 @@ int vti_serve_set_batching(vti_server *s, int max_requests)
     VTI_CHECK(s && max_requests >= 1 && max_requests <= 64, "vti_serve_set_batching: max_requests in [1, 64]");
     std::lock_guard<std::mutex> lk(s->mu);
     s->batch = max_requests;
+    return vti::kOk;
+}
+
+int vti_serve_set_snapshots(vti_server *s, int snap_fmt)
+{
+    VTI_CHECK(s && snap_fmt >= 0 && snap_fmt <= 3,
+              "vti_serve_set_snapshots: format must be 0 (FP32), 1 (FP16), 2 (BF16) or 3 (S16)");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->snap_fmt = snap_fmt;
     return vti::kOk;
 }
 

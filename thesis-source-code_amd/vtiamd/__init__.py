@@ -35,6 +35,7 @@ _SIGS = {
     "vti_ctx_destroy": (c_int, [P]),
     "vti_ctx_synchronize": (c_int, [P]),
     "vti_ctx_set_arith": (c_int, [P, c_int]),
+    "vti_ctx_set_snapshots": (c_int, [P, c_int]),
     "vti_init_damping3d": (c_int, [P, c_int, c_int, c_int, c_int]),
     "vti_init_taper2d": (c_int, [P, c_int, c_int, c_int]),
     "vti_vepsdel_to_cij": (c_int, [P, P, P, P, P, P, c_float, c_float, c_float, c_longlong]),
@@ -126,6 +127,7 @@ _SIGS = {
     "vti_serve_wait": (c_int, [P, c_longlong, POINTER(ctypes.c_double), POINTER(ctypes.c_double)]),
     "vti_serve_stats": (c_int, [P] + [POINTER(ctypes.c_double)] * 3 + [POINTER(c_longlong), POINTER(ctypes.c_double)]),
     "vti_serve_set_batching": (c_int, [P, c_int]),
+    "vti_serve_set_snapshots": (c_int, [P, c_int]),
     "vti_serve_batch_stats": (c_int, [P, POINTER(c_longlong), POINTER(c_longlong)]),
     "vti2d_objfunc_gradient_vepsdel": (
         c_int,
@@ -226,6 +228,12 @@ class Context:
     def set_arith(self, arith: int):
         """1: the reference's mixed FP64 stencil arithmetic (default); 0: FP32."""
         check(_lib.vti_ctx_set_arith(self.h, arith), "vti_ctx_set_arith")
+
+    def set_snapshots(self, snap: str):
+        """Snapshot storage of 2D gradients on this context: "fp32" (default),
+        "fp16", "bf16" or "s16"; 16-bit forms apply to calls the batched
+        driver takes (see vti_ctx_set_snapshots)."""
+        check(_lib.vti_ctx_set_snapshots(self.h, SNAP_FORMATS[snap]), "vti_ctx_set_snapshots")
 
     def close(self):
         if self.h:
@@ -678,7 +686,7 @@ class Server:
     `batch` per run; 1 = off).  Submit with grad2d / model3d, collect with
     wait."""
 
-    def __init__(self, device=0, slots=4, budget_bytes=0.0, batch=8):
+    def __init__(self, device=0, slots=4, budget_bytes=0.0, batch=8, snap="fp32"):
         h = ctypes.c_void_p()
         check(_lib.vti_serve_create(ctypes.byref(h), device, slots, float(budget_bytes)), "vti_serve_create")
         self.h = h
@@ -688,6 +696,7 @@ class Server:
         # stay alive here even if the caller drops the Ticket
         self._pending = {}
         check(_lib.vti_serve_set_batching(self.h, batch), "vti_serve_set_batching")
+        check(_lib.vti_serve_set_snapshots(self.h, SNAP_FORMATS[snap]), "vti_serve_set_snapshots")
 
     def close(self):
         if self.h:
