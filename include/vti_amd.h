@@ -412,6 +412,13 @@ int vti_serve_batch_stats(vti_server *s, long long *merged_runs, long long *merg
  * 1 = FP16, 2 = BF16, 3 = S16); merged runs and single requests alike.  No
  * counterpart in the reference. */
 int vti_serve_set_snapshots(vti_server *s, int snap_fmt);
+/* on = 1 (default): a 3D modeling request starts only when nothing else
+ * runs, and nothing starts beside it.  The 3D step is bound by HBM
+ * bandwidth, so work beside it only stretches it; alone it keeps the
+ * latency the one-request-at-a-time path gives it, while 2D requests still
+ * overlap and merge with each other.  on = 0: 3D requests share the device
+ * like the rest.  No counterpart in the reference. */
+int vti_serve_set_exclusive3d(vti_server *s, int on);
 
 /* blocks until request `id` is done; returns its status (vti_last_error()
  * then holds its message) and its latency and service time in seconds */

@@ -98,6 +98,9 @@ struct vti_server {
     vti_ctx *upload_ctx = nullptr;
     int batch = 8;                        // most requests merged into one batched run
     int snap_fmt = 0;                     // snapshot storage of 2D gradient requests
+    bool exclusive3d = true;              // 3D modeling requests run with nothing beside them
+    int running = 0;                      // runs in flight
+    bool excl_running = false;            // one of them is an exclusive 3D request
     int idle = 0;                         // workers waiting for work
     long long groups = 0, grouped = 0;    // merged runs and the requests in them
 };
@@ -188,17 +191,25 @@ void worker(vti_server *s)
     for (;;) {
         std::unique_lock<std::mutex> lk(s->mu);
         // the oldest request, once it fits beside those in flight (or runs
-        // alone); on shutdown the queue is drained first
+        // alone); on shutdown the queue is drained first.  A 3D modeling
+        // request (bound by HBM bandwidth, so sharing only stretches it)
+        // waits until nothing runs and keeps the device to itself, as the
+        // one-request-at-a-time path would give it
         s->idle += 1;
         s->work_cv.wait(lk, [&] {
             if (s->queue.empty()) return s->stop;
-            const long long b = s->queue.front()->bytes;
-            return s->inflight == 0 || s->inflight + b <= s->budget - s->resident;
+            if (s->excl_running) return false;
+            const Request *f = s->queue.front();
+            if (s->exclusive3d && f->kind == kModel3d && s->running > 0) return false;
+            return s->inflight == 0 || s->inflight + f->bytes <= s->budget - s->resident;
         });
         s->idle -= 1;
         if (s->queue.empty()) break;
         Request *r = s->queue.front();
         s->queue.pop_front();
+        const bool excl = s->exclusive3d && r->kind == kModel3d;
+        s->running += 1;
+        if (excl) s->excl_running = true;
         // queued requests that merge with it, oldest first, while the merged
         // run fits beside those in flight
         std::vector<Request *> grp{r};
@@ -275,6 +286,8 @@ void worker(vti_server *s)
             s->grouped += (long long)grp.size();
         }
         s->inflight -= gbytes;
+        s->running -= 1;
+        if (excl) s->excl_running = false;
         s->last_done = std::max(s->last_done, t_done);
         s->done_cv.notify_all();
         s->work_cv.notify_all();
@@ -509,6 +522,15 @@ int vti_serve_set_snapshots(vti_server *s, int snap_fmt)
               "vti_serve_set_snapshots: format must be 0 (FP32), 1 (FP16), 2 (BF16) or 3 (S16)");
     std::lock_guard<std::mutex> lk(s->mu);
     s->snap_fmt = snap_fmt;
+    return vti::kOk;
+}
+
+int vti_serve_set_exclusive3d(vti_server *s, int on)
+{
+    VTI_CHECK(s && (on == 0 || on == 1), "vti_serve_set_exclusive3d: on must be 0 or 1");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->exclusive3d = on != 0;
+    s->work_cv.notify_all();
     return vti::kOk;
 }
 

@@ -128,6 +128,7 @@ _SIGS = {
     "vti_serve_stats": (c_int, [P] + [POINTER(ctypes.c_double)] * 3 + [POINTER(c_longlong), POINTER(ctypes.c_double)]),
     "vti_serve_set_batching": (c_int, [P, c_int]),
     "vti_serve_set_snapshots": (c_int, [P, c_int]),
+    "vti_serve_set_exclusive3d": (c_int, [P, c_int]),
     "vti_serve_batch_stats": (c_int, [P, POINTER(c_longlong), POINTER(c_longlong)]),
     "vti2d_objfunc_gradient_vepsdel": (
         c_int,
@@ -686,7 +687,7 @@ class Server:
     `batch` per run; 1 = off).  Submit with grad2d / model3d, collect with
     wait."""
 
-    def __init__(self, device=0, slots=4, budget_bytes=0.0, batch=8, snap="fp32"):
+    def __init__(self, device=0, slots=4, budget_bytes=0.0, batch=8, snap="fp32", exclusive3d=True):
         h = ctypes.c_void_p()
         check(_lib.vti_serve_create(ctypes.byref(h), device, slots, float(budget_bytes)), "vti_serve_create")
         self.h = h
@@ -697,6 +698,7 @@ class Server:
         self._pending = {}
         check(_lib.vti_serve_set_batching(self.h, batch), "vti_serve_set_batching")
         check(_lib.vti_serve_set_snapshots(self.h, SNAP_FORMATS[snap]), "vti_serve_set_snapshots")
+        check(_lib.vti_serve_set_exclusive3d(self.h, int(bool(exclusive3d))), "vti_serve_set_exclusive3d")
 
     def close(self):
         if self.h:
