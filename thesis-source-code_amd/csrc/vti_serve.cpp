@@ -254,7 +254,12 @@ void worker(vti_server *s)
         std::vector<int> rcs(grp.size(), crc);
         std::vector<std::string> errs(grp.size());
         if (crc == 0) {
-            bool alone = grp.size() == 1;
+            // With batching on, a mergeable 2D request takes the batched
+            // driver's two-level kernels even when no other request joins it
+            // (10-27% less time than the one-shot driver's per-level
+            // launches for one shot on 261x877 to 512x1024 grids, the same
+            // bits); the rest, and everything with batching off, run alone.
+            bool alone = grp.size() == 1 && !(wave > 1 && r->kind == kGrad2d && r->mergeable);
             if (!alone) {
                 std::vector<vti::Grad2dPart> parts;
                 for (size_t i = 0; i < grp.size(); ++i) parts.push_back(part_of(*grp[i], ms[i]));
@@ -435,6 +440,8 @@ int vti_serve_grad2d(vti_server *s, long long *id, int model, float *objfunc, fl
     r->rate = rate;
     r->bytes = est_grad2d(*r);
     r->mergeable = mergeable_grad2d(*r);
+    // a mergeable request runs through the batched driver even alone
+    if (r->mergeable) r->bytes = std::max(r->bytes, merged_bytes({r.get()}, 1));
     {
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->models.find(model);
