@@ -192,6 +192,16 @@ int vti2d_synthetic(vti_ctx *ctx, float *data, const float *c11, const float *c1
                     const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx,
                     int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot);
 
+/* modeling_f, lib/vti-fwi-2d/modeling_f.cu:10-86 (wave.h:14): one source at
+ * the padded-grid index slocxz (inside the updated interior); wavefield holds
+ * (nt - ntNeg - 1)/ratio + 1 slots of nnx*nnz floats (ratio = round(rate/dt),
+ * ntNeg = round(|ot|/dt)); slot (it-ntNeg)/ratio receives the p field after
+ * step it (the source adds w[it-1], with no dt^2, as there), slot 1 level 1
+ * when ratio = 1 and ot = 0; slots never recorded keep their input values. */
+int vti2d_modeling(vti_ctx *ctx, float *wavefield, const float *c11, const float *c13, const float *c33,
+                   const float *wavelet, int slocxz, const float *taper, int nx, int nz, int nt, int npad, float dx,
+                   float dz, float dt, float rate, float ot);
+
 /* vti2d_synthetic with no allocation.  No counterpart in the reference: it is
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Same

@@ -226,6 +226,9 @@ void rtm_f(float *image, const float *data, const float *c11, const float *c13, 
 void odcig_f(float *image, const float *data, const float *c11, const float *c13, const float *c33,
              const float *wavelet, const int *sloc, int ns, const int *rloc, int nr, const float *taper, int nx, int nz,
              int nhx, int nt, int npad, float dx, float dz, float dt, float rate, float ot = 0.);
+void modeling_f(float *d, const float *c11, const float *c13, const float *c33, const float *wavelet, int slocxz,
+                const float *taper, int nx, int nz, int nt, int npad, float dx, float dz, float dt, float rate,
+                float ot = 0.);
 float objFuncGradientVEpsDel(float *gv, float *geps, float *gdel, const float *data, float *v, float *eps, float *del,
                              float v0, float eps0, float del0, const float *wavelet, const int *sloc, int ns,
                              const int *rloc, int nr, const float *taper, int nx, int nz, int nt, int npad, float dx,
@@ -338,6 +341,15 @@ void rtm_f(float *image, const float *data, const float *c11, const float *c13, 
     vti_compat::report(vti2d_rtm(vti_compat::context(), image, data, c11, c13, c33, wavelet, sloc, ns, rloc, nr, taper,
                                  nx, nz, nt, npad, dx, dz, dt, rate, ot),
                        "rtm_f");
+}
+
+// modeling_f (wave.h:14, modeling_f.cu:10)
+void modeling_f(float *d, const float *c11, const float *c13, const float *c33, const float *wavelet, int slocxz,
+                const float *taper, int nx, int nz, int nt, int npad, float dx, float dz, float dt, float rate, float ot)
+{
+    vti_compat::report(vti2d_modeling(vti_compat::context(), d, c11, c13, c33, wavelet, slocxz, taper, nx, nz, nt, npad,
+                                      dx, dz, dt, rate, ot),
+                       "modeling_f");
 }
 
 // odcig_f (wave.h:16, odcig_f.cu:12)

@@ -213,6 +213,7 @@ def _lib2():
     if not getattr(L, "_has2", False):
         P = c_void_p
         L.orc2d_synthetic.argtypes = [c_int, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5
+        L.orc2d_modeling.argtypes = [c_int, P, P, P, P, P, c_int, P] + [c_int] * 4 + [c_float] * 5
         L.orc2d_fg.restype = c_double
         L.orc2d_fg.argtypes = [c_int, c_int, c_int, P, P, P, P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + \
             [c_float] * 5
@@ -239,6 +240,24 @@ def synthetic2d(mode, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npa
     if rc != 0:
         raise RuntimeError("orc2d_synthetic failed")
     return data
+
+
+def modeling2d(mode, c11, c13, c33, wavelet, slocxz, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0, init=None):
+    """modeling_f (modeling_f.cu:10-86): p snapshots (nnt_data, nnz, nnx); slots
+    it never records keep `init` (zeros by default)."""
+    c11, c13, c33, wavelet, taper = (np.ascontiguousarray(a, np.float32).ravel() for a in (c11, c13, c33, wavelet, taper))
+    q = float(np.float32(rate) / np.float32(dt))
+    ratio = int(np.floor(q + 0.5))  # roundf: half away from zero
+    ntneg = int(round(abs(ot) / dt))
+    nnt_data = (nt - ntneg - 1) // ratio + 1
+    shape = (nnt_data, nz + 2 * npad, nx + 2 * npad)
+    dtype = np.float64 if mode == F64 else np.float32
+    wf = np.zeros(shape, dtype) if init is None else np.array(init, dtype).reshape(shape)
+    rc = _lib2().orc2d_modeling(mode, _p(wf), _p(c11), _p(c13), _p(c33), _p(wavelet), int(slocxz), _p(taper), nx, nz,
+                                nt, npad, dx, dz, dt, rate, ot)
+    if rc != 0:
+        raise RuntimeError("orc2d_modeling failed")
+    return wf
 
 
 def fg2d(mode, born, data, c11, c13, c33, dc, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0,

@@ -5,6 +5,7 @@
  *                                 scattering, gradientCrossCor, recordData,
  *                                 injectData, injectSource
  *   lib/vti-fwi-2d/synthetic_f.cu:13-152        synthetic_f
+ *   lib/vti-fwi-2d/modeling_f.cu:10-86          modeling_f
  *   lib/vti-fwi-2d/objFuncGradient_f.cu:15-371  objFuncGradient_f
  *   lib/vti-fwi-2d/hessian_f.cu:575-1010        GNhessian_f
  *   lib/vti-fwi-2d/hessian_f.cu:15-573          hessian_f
@@ -72,6 +73,27 @@ int orc2d_synthetic(int mode, void *data, const float *c11, const float *c13, co
         double *a = widen(c11, n), *b = widen(c13, n), *c = widen(c33, n);
         synthetic_f64((double *)data, a, b, c, wavelet, sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt,
                       rate, ot);
+        free(a);
+        free(b);
+        free(c);
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+/* modeling_f; wavefield: float (mode 0) or double (mode 2), nnxz samples per
+ * slot, (nt - ntNeg - 1) / ratio + 1 slots. */
+int orc2d_modeling(int mode, void *wavefield, const float *c11, const float *c13, const float *c33,
+                   const float *wavelet, int slocxz, const float *taper, int nx, int nz, int nt, int npad, float dx,
+                   float dz, float dt, float rate, float ot)
+{
+    const long n = (long)(nx + 2 * npad) * (nz + 2 * npad);
+    if (mode == 0) {
+        modeling_f32((float *)wavefield, c11, c13, c33, wavelet, slocxz, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot);
+    } else if (mode == 2) {
+        double *a = widen(c11, n), *b = widen(c13, n), *c = widen(c33, n);
+        modeling_f64((double *)wavefield, a, b, c, wavelet, slocxz, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot);
         free(a);
         free(b);
         free(c);

@@ -166,6 +166,47 @@ static void FN(synthetic)(T *data, const T *c11, const T *c13, const T *c33, con
     free(d);
 }
 
+/* modeling_f, lib/vti-fwi-2d/modeling_f.cu:10-86, one GPU: one source at the
+ * grid index slocxz; level 1 is dt2*w[0] there, level it >= 2 adds w[it-1]
+ * with no dt2 (modeling_f.cu:66); after each recorded step the whole p field
+ * (after abc) goes to slot (it-ntNeg)/ratio of wavefield (nnxz floats per
+ * slot; the reference copies the half-sum slice first and then p over it,
+ * modeling_f.cu:71-72), and slot 1 gets level 1 when ratio == 1 and ot == 0.
+ * Slots never recorded keep their input values. */
+static void FN(modeling)(T *wavefield, const T *c11, const T *c13, const T *c33, const float *wavelet, int slocxz,
+                         const float *taper, int nx, int nz, int nt, int npad, float dx, float dz, float dt, float rate,
+                         float ot)
+{
+    const int ratio = (int)roundf(rate / dt);
+    const int ntNeg = (int)round(fabs(ot) / dt);
+    const int nnx = nx + 2 * npad, nnz = nz + 2 * npad;
+    const long nnxz = (long)nnx * nnz;
+    const float dx2f = dx * dx, dz2f = dz * dz, dt2f = dt * dt;
+    const T dx2 = (T)dx2f, dz2 = (T)dz2f, dt2 = (T)dt2f;
+    T *p0 = (T *)calloc(nnxz, sizeof(T)), *q0 = (T *)calloc(nnxz, sizeof(T));
+    T *p1 = (T *)calloc(nnxz, sizeof(T)), *q1 = (T *)calloc(nnxz, sizeof(T));
+    const float v0 = dt2f * wavelet[0];
+    p1[slocxz] += (T)v0;
+    q1[slocxz] += (T)v0;
+    FN(abc2)(p1, q1, taper, nnx, nnz);
+    if (ratio == 1 && ot == 0.f) memcpy(wavefield + nnxz, p1, sizeof(T) * nnxz);
+    for (int it = 2; it < nt; ++it) {
+        const float t = it * dt + ot;
+        FN(forwardCD)(p0, q0, p1, q1, c11, c13, c33, dx2, dz2, dt2, nnx, nnz);
+        p0[slocxz] += (T)wavelet[it - 1];
+        q0[slocxz] += (T)wavelet[it - 1];
+        FN(abc4)(p1, q1, p0, q0, taper, nnx, nnz);
+        if (t >= 0.f && (it - ntNeg) % ratio == 0)
+            memcpy(wavefield + (long)((it - ntNeg) / ratio) * nnxz, p0, sizeof(T) * nnxz);
+        SWAP(p0, p1);
+        SWAP(q0, q1);
+    }
+    free(p0);
+    free(q0);
+    free(p1);
+    free(q1);
+}
+
 /*
  * objFuncGradient_f (objFuncGradient_f.cu:15-371) and GNhessian_f
  * (hessian_f.cu:575-1010), one GPU, restated literally: the device buffers

@@ -29,6 +29,12 @@ void h_synthetic_f(float *d, const float *c11, const float *c13, const float *c3
     synthetic_f(d, c11, c13, c33, w, sloc, ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate);
 }
 
+void h_modeling_f(float *d, const float *c11, const float *c13, const float *c33, const float *w, int slocxz,
+                  const float *taper, int nx, int nz, int nt, int npad, float dx, float dz, float dt, float rate)
+{
+    modeling_f(d, c11, c13, c33, w, slocxz, taper, nx, nz, nt, npad, dx, dz, dt, rate);
+}
+
 float h_objFuncGradient_f(float *g11, float *g13, float *g33, const float *data, const float *c11, const float *c13,
                           const float *c33, const float *w, const int *sloc, int ns, const int *rloc, int nr,
                           const float *taper, int nx, int nz, int nt, int npad, float dx, float dz, float dt,

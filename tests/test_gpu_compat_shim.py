@@ -79,6 +79,14 @@ def test_shim_2d_bitexact(vti, ctx, H):
                               F(a[7]))
     assert f == np.float32(f_abi)
     assert np.array_equal(g, g_abi)
+    print("shim 2D: modeling_f", flush=True)
+    sxz = int(sloc[0, 0] + sloc[0, 1] * P["nnx"])
+    m_abi = vti.modeling2d(ctx, *c, w, sxz, taper, *a)
+    m_shim = np.zeros_like(m_abi)
+    H.h_modeling_f(P_(m_shim), *[P_(x) for x in c], P_(w), sxz, P_(taper), a[0], a[1], a[2], a[3], F(a[4]), F(a[5]),
+                   F(a[6]), F(a[7]))
+    assert np.abs(m_abi).max() > 0
+    assert np.array_equal(m_shim, m_abi)
 
 
 def _g3(P):

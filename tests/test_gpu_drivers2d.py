@@ -50,6 +50,27 @@ def test_synthetic_ratio1_and_negative_time(vti, orc, ctx, P):
         assert np.array_equal(d, o)
 
 
+@pytest.mark.parametrize("rate,ot", [(4e-3, 0.0), (1e-3, 0.0), (4e-3, -0.02)])
+def test_modeling_bitexact(vti, orc, ctx, P, rate, ot):
+    """modeling_f: the p field of every recorded slot equals the oracle's FP32
+    restatement bit for bit, slots never recorded keep their input values,
+    and the FP64 oracle states the error."""
+    _say("2d modeling rate", rate, "ot", ot)
+    a = list(args2d(P))
+    a[-1] = rate
+    sxz = int(P["sloc"][0, 0] + P["sloc"][0, 1] * P["nnx"])
+    o = orc.modeling2d(orc.F32, *P["c"], P["w"], sxz, P["taper"], *a, ot=ot)
+    init = np.full(o.shape, 3.0, np.float32)
+    o = orc.modeling2d(orc.F32, *P["c"], P["w"], sxz, P["taper"], *a, ot=ot, init=init)
+    m = vti.modeling2d(ctx, *P["c"], P["w"], sxz, P["taper"], *a, ot=ot, wavefield=init.copy())
+    o64 = orc.modeling2d(orc.F64, *P["c"], P["w"], sxz, P["taper"], *a, ot=ot, init=init)
+    assert np.abs(o[o != 3.0]).max() > 0
+    assert np.array_equal(m, o)
+    e = rel(m[m != 3.0], o64[m != 3.0].astype(np.float32))
+    _say(f"  rel vs FP64 {e:.3e}")
+    assert e < 1e-4
+
+
 @pytest.mark.parametrize("nshot", [1, 2])
 @pytest.mark.parametrize("kind", ["zero", "model"])
 def test_objfunc_gradient(vti, orc, ctx, P, obs, nshot, kind):

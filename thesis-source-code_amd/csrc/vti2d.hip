@@ -383,6 +383,81 @@ int vti2d_synthetic(vti_ctx *ctx, float *data, const float *c11, const float *c1
 
 }  // extern "C"
 
+extern "C" {
+
+// modeling_f, lib/vti-fwi-2d/modeling_f.cu:10-86 (wave.h:14): one source at
+// the grid index slocxz; level 1 is dt2*w[0] there, level it >= 2 adds
+// w[it-1] with no dt2 (modeling_f.cu:66); after each recorded step the
+// whole p field, tapered, goes to slot (it-ntNeg)/ratio of wavefield (nnxz
+// floats per slot, (nt-ntNeg-1)/ratio+1 slots; the reference's second copy
+// of p over the half-sum slice, modeling_f.cu:71-72, leaves p), and slot 1
+// gets level 1 when ratio == 1 and ot == 0.  ratio = round(rate/dt) in
+// float, as there.  Slots never recorded keep their input values.  The
+// fields go to a device array of the recorded slots and come down in one
+// copy at the end.
+int vti2d_modeling(vti_ctx *ctx, float *wavefield, const float *c11, const float *c13, const float *c33,
+                   const float *wavelet, int slocxz, const float *taper, int nx, int nz, int nt, int npad, float dx,
+                   float dz, float dt, float rate, float ot)
+{
+    Call2 c;
+    int rc = setup(c, ctx, nx, nz, nt, npad, dx, dz, dt, rate, ot, false);
+    if (rc) return rc;
+    VTI_CHECK(wavefield && c11 && c13 && c33 && wavelet && taper, "vti2d_modeling: null argument");
+    c.ratio = (int)std::round(rate / dt);
+    VTI_CHECK(c.ratio >= 1, "vti2d_modeling: sampling rate below the time step");
+    c.nnt_data = (nt - c.ntNeg - 1) / c.ratio + 1;
+    VTI_CHECK(slocxz >= 0 && slocxz < c.nnxz && interior(c, slocxz % c.g.nnx, slocxz / c.g.nnx),
+              "vti2d_modeling: source index %d outside the updated interior", slocxz);
+    Fields2 F;
+    rc = load_model(c, F, c11, c13, c33, taper);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    rc = zero2(c, F.p, F.q);
+    if (rc) return rc;
+    // the recorded slots form one run [lo, hi]
+    std::vector<int> rows;
+    if (c.ratio == 1 && ot == 0.f && 1 < c.nnt_data) rows.push_back(1);
+    for (int it = 2; it < nt; ++it) {
+        const float t = it * dt + ot;
+        if (t >= 0.f && (it - c.ntNeg) % c.ratio == 0 && (it - c.ntNeg) / c.ratio < c.nnt_data)
+            rows.push_back((it - c.ntNeg) / c.ratio);
+    }
+    const int lo = rows.empty() ? 0 : rows.front(), hi = rows.empty() ? -1 : rows.back();
+    VTI_CHECK(rows.empty() || hi - lo + 1 == (int)rows.size(), "vti2d_modeling: recorded slots %d..%d not contiguous",
+              lo, hi);
+    const size_t n = (size_t)c.nnxz;
+    DBuf<float> slots;
+    rc = slots.alloc(std::max<size_t>(1, (size_t)(hi - lo + 1) * n), st, false);
+    if (rc) return rc;
+    auto keep = [&](int row) {
+        return hipMemcpyAsync(slots.p + (size_t)(row - lo) * n, F.p[0].p, n * 4, hipMemcpyDeviceToDevice, st);
+    };
+    const long long src = slocxz;
+    hipLaunchKernelGGL(set2_kernel, dim3(1), dim3(1), 0, st, F.p[0].p, F.q[0].p, src, c.dt2 * wavelet[0] * taper[src]);
+    if (c.ratio == 1 && ot == 0.f && 1 < c.nnt_data) VTI_HIP(keep(1));
+    for (int it = 2; it < nt; ++it) {
+        const float t = it * dt + ot;
+        hipLaunchKernelGGL((vti::fwd2d_kernel<false>), c.grid, c.block, 0, st, F.p[1].p, F.q[1].p, F.p[0].p,
+                           F.q[0].p, nullptr, nullptr, nullptr, nullptr, F.c11.p, F.c13.p, F.c33.p, nullptr, nullptr,
+                           nullptr, F.taper.p, c.g, c.dx2, c.dz2, c.dt2, src, wavelet[it - 1], nullptr, nullptr, -1LL,
+                           nullptr, nullptr);
+        std::swap(F.p[0].p, F.p[1].p);
+        std::swap(F.q[0].p, F.q[1].p);
+        if (t >= 0.f && (it - c.ntNeg) % c.ratio == 0) {
+            const int row = (it - c.ntNeg) / c.ratio;
+            if (row < c.nnt_data) VTI_HIP(keep(row));
+        }
+    }
+    VTI_HIP(hipGetLastError());
+    if (!rows.empty())
+        VTI_HIP(hipMemcpyAsync(wavefield + (size_t)lo * n, slots.p, (size_t)(hi - lo + 1) * n * 4,
+                               hipMemcpyDeviceToHost, st));
+    VTI_HIP(hipStreamSynchronize(st));
+    return vti::kOk;
+}
+
+}  // extern "C"
+
 // ---- synthetic_f without allocation ----------------------------------------
 //
 // vti2d_synthetic_noalloc runs the one-shot schedule of vti2d_synthetic with

@@ -90,6 +90,7 @@ _SIGS = {
     ),
     "vti2d_synthetic": (c_int, [P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
     "vti2d_synthetic_noalloc": (c_int, [P, P, P, P, P, P, P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
+    "vti2d_modeling": (c_int, [P, P, P, P, P, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
     "vti2d_hessian": (c_int, [P] * 13 + [c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
     "vti2d_rtm": (c_int, [P] * 7 + [P, c_int, P, c_int, P] + [c_int] * 4 + [c_float] * 5),
     "vti2d_odcig": (c_int, [P] * 7 + [P, c_int, P, c_int, P] + [c_int] * 5 + [c_float] * 5),
@@ -429,6 +430,25 @@ def synthetic2d(ctx, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad
     check(getattr(_lib, name)(ctx.h, ptr(data), ptr(c11), ptr(c13), ptr(c33), ptr(wavelet), ptr(sloc), ns, ptr(rloc),
                               nr, ptr(taper), nx, nz, nt, npad, dx, dz, dt, rate, ot), name)
     return data
+
+
+def modeling2d(ctx, c11, c13, c33, wavelet, slocxz, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot=0.0,
+               wavefield=None):
+    """modeling_f: returns the p snapshots (nnt_data, nnz, nnx).  Slots the
+    reference never records keep the values of `wavefield` (written in place
+    when given as a C-contiguous float32 array of that shape), else zeros."""
+    c11, c13, c33, wavelet, taper = (f32(a).ravel() for a in (c11, c13, c33, wavelet, taper))
+    q = float(np.float32(rate) / np.float32(dt))
+    ratio = int(np.floor(q + 0.5))  # round(rate/dt) in float, half away from zero
+    ntneg = int(round(abs(ot) / dt))
+    shape = ((nt - ntneg - 1) // ratio + 1, nz + 2 * npad, nx + 2 * npad)
+    if wavefield is None:
+        wavefield = np.zeros(shape, np.float32)
+    elif wavefield.dtype != np.float32 or wavefield.shape != shape or not wavefield.flags["C_CONTIGUOUS"]:
+        raise ValueError(f"wavefield must be a C-contiguous float32 array of shape {shape}")
+    check(_lib.vti2d_modeling(ctx.h, ptr(wavefield), ptr(c11), ptr(c13), ptr(c33), ptr(wavelet), int(slocxz), ptr(taper),
+                              nx, nz, nt, npad, dx, dz, dt, rate, ot), "vti2d_modeling")
+    return wavefield
 
 
 def objfunc_gradient2d(ctx, data, c11, c13, c33, wavelet, sloc, rloc, taper, nx, nz, nt, npad, dx, dz, dt, rate,
