@@ -39,6 +39,7 @@
 #include "vti_ctx.h"
 #include "vti_host.h"
 #include "vti_internal.h"
+#include "vti_serve_policy.h"
 
 namespace {
 
@@ -198,10 +199,15 @@ void worker(vti_server *s)
         s->idle += 1;
         s->work_cv.wait(lk, [&] {
             if (s->queue.empty()) return s->stop;
-            if (s->excl_running) return false;
+            vti::ServeState st;
+            st.budget = s->budget;
+            st.resident = s->resident;
+            st.inflight = s->inflight;
+            st.running = s->running;
+            st.excl_running = s->excl_running;
+            st.exclusive3d = s->exclusive3d;
             const Request *f = s->queue.front();
-            if (s->exclusive3d && f->kind == kModel3d && s->running > 0) return false;
-            return s->inflight == 0 || s->inflight + f->bytes <= s->budget - s->resident;
+            return vti::may_start(st, f->kind == kModel3d, f->bytes);
         });
         s->idle -= 1;
         if (s->queue.empty()) break;
