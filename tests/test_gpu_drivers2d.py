@@ -127,6 +127,19 @@ def test_synthetic_cfg1_size(vti, orc, ctx):
     assert np.array_equal(d, o)
 
 
+def test_modeling_cfg1_size(vti, orc, ctx):
+    """modeling_f at the 336 x 336, 2000-step size, source at (z=20, x=128),
+    snapshots every 4 ms: every slot bit-exact with the oracle."""
+    P = problem2d(vti, nx=256, nz=256, npad=40, nt=2001, ns=1, rate=4e-3, rec_step=1, seed=9)
+    sxz = (40 + 128) + (40 + 20) * P["nnx"]
+    _say("2d modeling 336x336x2000")
+    m = vti.modeling2d(ctx, *P["c"], P["w"], sxz, P["taper"], *args2d(P))
+    o = orc.modeling2d(orc.F32, *P["c"], P["w"], sxz, P["taper"], *args2d(P))
+    assert m.shape == (501, 336, 336)
+    assert np.abs(o).max() > 0
+    assert np.array_equal(m, o)
+
+
 def _vepsdel_problem(vti):
     from common2d import layered2d
 
