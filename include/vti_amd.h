@@ -293,9 +293,12 @@ int vti2d_gn_hessian_vepsdel(vti_ctx *ctx, float *gv, float *geps, float *gdel, 
 /* ---- batched 2D gradient --------------------------------------------------------
  * Many shots, each on its own aperture sub-grid of one global model
  * (NNX = NX + 2*npad columns, nnz = nz + 2*npad rows, the reference layout),
- * advanced in lockstep: one launch per time level for all shots of a wave
- * (waves are sized to the free device memory, or at most max_wave shots when
- * max_wave > 0).  Per shot the result is objFuncGradient_f
+ * advanced in lockstep: one launch per time level for all shots of a wave.
+ * Two waves are in flight at once, each starting its forward pass when the
+ * previous one's is done, so that a wave's forward pass overlaps the
+ * previous wave's adjoint: waves of (max_wave+1)/2 shots when max_wave > 0
+ * (about max_wave shots held at once), else sized to half the free device
+ * memory.  Per shot the result is objFuncGradient_f
  * (lib/vti-fwi-2d/objFuncGradient_f.cu:15) on the model cropped to the shot's
  * columns [c0, c0 + nx + 2*npad), with the standard init_abc taper of that
  * sub-grid; the shots' gradients are summed in FP64 into gc11/gc13/gc33
@@ -309,8 +312,10 @@ int vti2d_gn_hessian_vepsdel(vti_ctx *ctx, float *gv, float *geps, float *gdel, 
  * of a two-level workgroup tile, 4, 8 or 16 (0: 16).  Neither changes a bit of
  * the result.  snap_fmt: 0 keeps the imaging snapshots in FP32 (the result
  * then equals the one-shot driver's bit for bit); 1 (FP16) and 2 (BF16)
- * store each snapshot slice scaled by a power of two, halving their memory;
- * the gradient then differs at the level the format's rounding sets.
+ * store each snapshot slice scaled by a power of two, 3 (S16) 16-bit
+ * integers with a power-of-two scale per 64 points, halving their memory;
+ * the gradient then differs at the level the format's rounding sets.  The
+ * wave split changes no bit of the result.
  * No counterpart in the reference, which runs one shot per call. */
 int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, double *gc13, double *gc33,
                                  const float *data, const float *c11, const float *c13, const float *c33,
