@@ -282,3 +282,26 @@ def test_multishot_synthetic_runs_as_batch_bit_exact(vti, ctx):
     t2.flat[0] = np.float32(0.5)
     d2 = vti.synthetic2d(ctx, *P3["ctrue"], P3["w"], P3["sloc"], P3["rloc"], t2, *a, data=fill.copy())
     assert np.array_equal(d2, d)
+
+
+def test_multishot_call_waves_in_flight_match_batched_driver(vti, ctx):
+    """A 20-shot call runs as waves of 4 shots, two in flight on two streams
+    (the forward pass of one beside the adjoint of the other).  Its gradient
+    and objective equal the batched driver's on the same shots, rounded to
+    FP32, and those do not depend on how the batched driver splits the
+    shots: one wave of all 20, or 20 waves of one, two in flight, give the
+    same FP64 bits."""
+    P = problem2d(vti, ns=20, seed=11)
+    a = args2d(P)
+    data = vti.synthetic2d(ctx, *P["ctrue"], P["w"], P["sloc"], P["rloc"], P["taper"], *a)
+    data = (data * np.float32(0.9)).astype(np.float32)
+    f, g = vti.objfunc_gradient2d(ctx, data, *P["c"], P["w"], P["sloc"], P["rloc"], P["taper"], *a)
+    shots = np.array([(0, P["nx"], ix, iz, nr, r0) for ix, iz, nr, r0 in P["sloc"]], np.int32)
+    cat = np.concatenate([data[:, r0:r0 + nr].ravel() for _, _, nr, r0 in P["sloc"]])
+    out = [vti.objfunc_gradient2d_batch(ctx, cat, *P["c"], P["w"], shots, P["rloc"], *a, max_wave=m)
+           for m in (0, 1)]
+    (f0, g0), (f1, g1) = out
+    assert np.abs(g0).max() > 0
+    assert np.array_equal(g0, g1) and f0 == f1
+    assert np.array_equal(g, g0.astype(np.float32))
+    assert np.float32(f) == np.float32(f0)

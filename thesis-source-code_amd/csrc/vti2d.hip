@@ -1160,7 +1160,7 @@ int vti2d_objfunc_gradient(vti_ctx *ctx, float *objfunc, float *gc11, float *gc1
         part.g[0] = gc11;
         part.g[1] = gc13;
         part.g[2] = gc33;
-        return vti::grad2d_merged(ctx, &part, 1, wavelet, nz, nt, npad, dx, dz, dt, rate, 8, ctx->snap_fmt);
+        return vti::grad2d_merged(ctx, &part, 1, wavelet, nz, nt, npad, dx, dz, dt, rate, 8, ctx->snap_fmt, true);
     }
     return fg2d(ctx, false, objfunc, gc11, gc13, gc33, data, c11, c13, c33, nullptr, nullptr, nullptr, wavelet, sloc,
                 ns, rloc, nr, taper, nx, nz, nt, npad, dx, dz, dt, rate, ot);

@@ -817,18 +817,21 @@ int enqueue_model_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const f
 }
 
 // the objective's share of a wave: the residual's partial sums, in order
-// Adds a wave's residual partial sums to *obj, shot by shot and block by
-// block (enqueue_wave with per_shot_obj), so that the objective does not
-// depend on how the shots were split into waves.
-int wave_objective(const BatchWave &W, hipStream_t st, double *obj)
+// Adds a wave's residual partial sums, shot by shot and block by block
+// (enqueue_wave with per_shot_obj), to obj[owner[s]] for batch shot s
+// (obj[0] when owner is null), so that the objective does not depend on how
+// the shots were split into waves.
+int wave_objective(const BatchWave &W, hipStream_t st, const int *owner, double *obj)
 {
     if (!W.tr_total) return vti::kOk;
     std::vector<double> hp(W.part.n);
     VTI_HIP(hipMemcpyAsync(hp.data(), W.part.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st));
     VTI_HIP(hipStreamSynchronize(st));
     for (int j = 0; j < W.nw; ++j)
-        if (W.sb[j].nr)
-            for (int b = 0; b < W.pblk[j]; ++b) *obj += hp[W.poff[j] + b];
+        if (W.sb[j].nr) {
+            double &o = obj[owner ? owner[W.s0 + j] : 0];
+            for (int b = 0; b < W.pblk[j]; ++b) o += hp[W.poff[j] + b];
+        }
     return vti::kOk;
 }
 
@@ -857,6 +860,49 @@ std::vector<std::pair<int, int>> split_waves(const BatchArgs &a, int max_wave, l
         s0 = s1;
     }
     return w;
+}
+
+// Runs the waves of a batch, two in flight when there are several.  Wave k
+// runs on stream k % 2 (st, or a second one made for the call) and starts its
+// forward pass when wave k-1's is done, so that it overlaps wave k-1's
+// adjoint (the forward is bound by VALU work, the adjoint waits on memory),
+// and the host sets wave k+1 up meanwhile.  The FP64 accumulations and the
+// objective keep wave order, so the result is that of one wave after the
+// other, bit for bit.  Objective partial sums go to obj as wave_objective
+// adds them.
+int run_waves(BatchShared &S, const BatchArgs &a, const std::vector<std::pair<int, int>> &waves,
+              const float *data, const int *rloc, const float *wavelet, hipStream_t st, const int *owner,
+              double *obj)
+{
+    int rc = vti::kOk;
+    StreamPair sp;
+    if (waves.size() > 1) rc = sp.init(st);
+    std::unique_ptr<BatchWave> prev;
+    hipStream_t prev_st = st;
+    for (size_t k = 0; k < waves.size() && !rc; ++k) {
+        hipStream_t sk = k % 2 ? sp.side : st;
+        auto W = std::make_unique<BatchWave>();
+        // (on sk: waits for wave k-2, which ran there)
+        rc = setup_wave(*W, a, waves[k].first, waves[k].second, data, rloc, sk);
+        WaveSync sync;
+        if (waves.size() > 1) {
+            sync.fwd_done = sp.fwd[k % 2];
+            sync.acc_done = sp.acc[k % 2];
+            if (k >= 1) {
+                sync.acc_wait = sp.acc[(k - 1) % 2];
+                if (!rc && hipStreamWaitEvent(sk, sp.fwd[(k - 1) % 2], 0) != hipSuccess) {
+                    vti::set_error("batched 2D waves: hipStreamWaitEvent failed");
+                    rc = vti::kErrHip;
+                }
+            }
+        }
+        if (!rc) rc = enqueue_wave(*W, S, a, wavelet, sk, true, waves.size() > 1 ? &sync : nullptr);
+        if (!rc && prev) rc = wave_objective(*prev, prev_st, owner, obj);  // wave k-1 done
+        prev = std::move(W);
+        prev_st = sk;
+    }
+    if (!rc && prev) rc = wave_objective(*prev, prev_st, owner, obj);
+    return rc;  // (sp's destructor waits for both streams before it releases them)
 }
 
 // Shots per wave whose adjoint working set stays in the 256 MB Infinity
@@ -912,52 +958,18 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
     if (!rc) rc = upload_model(S, a, c11, c13, c33, st);
     if (rc) return rc;
 
-    // Waves within the memory budget, two held at once.  Wave k runs on
-    // stream k % 2 (the context's, or a second one) and starts its forward
-    // pass when wave k-1's is done, so that it overlaps wave k-1's adjoint
-    // (the forward is bound by VALU work, the adjoint waits on memory), and
-    // the host sets wave k+1 up meanwhile.  The FP64 accumulations and the
-    // objective keep wave order, so the result is that of one wave after
-    // the other, bit for bit.  Each wave takes half of max_wave (rounded
-    // up), so the two in flight hold about max_wave shots, the number whose
-    // working set the cache model sizes: at cfg4's 256 shots, waves of 4
-    // overlapped gave 69.4 shot-gradients/s against 64.7-65.5 for waves of 7
-    // one after the other (waves of 7 overlapped: 43.6-60.2).
+    // Waves within the memory budget, two in flight (run_waves).  Each takes
+    // half of max_wave (rounded up), so the two in flight hold about
+    // max_wave shots, the number whose working set the cache model sizes: at
+    // cfg4's 256 shots, waves of 4 overlapped gave 69.4 shot-gradients/s
+    // against 64.7-65.5 for waves of 7 one after the other (waves of 7
+    // overlapped: 43.6-60.2).
     const size_t mfree = vti::device_free_bytes();
     const long long budget = ((long long)mfree - (long long)(2ULL << 30)) / 2;
     VTI_CHECK(budget > a.shot_bytes(0), "not enough device memory for one shot");
     const auto waves = split_waves(a, max_wave > 0 ? (max_wave + 1) / 2 : 0, budget);
     double obj = 0.0;
-    {
-        StreamPair sp;
-        if (waves.size() > 1) rc = sp.init(st);
-        std::unique_ptr<BatchWave> prev;
-        hipStream_t prev_st = st;
-        for (size_t k = 0; k < waves.size() && !rc; ++k) {
-            hipStream_t sk = k % 2 ? sp.side : st;
-            auto W = std::make_unique<BatchWave>();
-            // (on sk: waits for wave k-2, which ran there)
-            rc = setup_wave(*W, a, waves[k].first, waves[k].second, data, rloc, sk);
-            WaveSync sync;
-            if (waves.size() > 1) {
-                sync.fwd_done = sp.fwd[k % 2];
-                sync.acc_done = sp.acc[k % 2];
-                if (k >= 1) {
-                    sync.acc_wait = sp.acc[(k - 1) % 2];
-                    if (!rc && hipStreamWaitEvent(sk, sp.fwd[(k - 1) % 2], 0) != hipSuccess) {
-                        vti::set_error("vti2d_objfunc_gradient_batch: hipStreamWaitEvent failed");
-                        rc = vti::kErrHip;
-                    }
-                }
-            }
-            if (!rc) rc = enqueue_wave(*W, S, a, wavelet, sk, true, waves.size() > 1 ? &sync : nullptr);
-            if (!rc && prev) rc = wave_objective(*prev, prev_st, &obj);  // wave k-1 done
-            prev = std::move(W);
-            prev_st = sk;
-        }
-        if (!rc && prev) rc = wave_objective(*prev, prev_st, &obj);
-        // (sp's destructor waits for both streams before it releases them)
-    }
+    rc = run_waves(S, a, waves, data, rloc, wavelet, st, nullptr, &obj);
     if (rc) return rc;
     rc = download_gradient(S, a, gc11, gc13, gc33, st);
     if (rc) return rc;
@@ -1041,7 +1053,7 @@ int vti2d_batch_plan_run(vti2d_batch_plan *plan, double *objfunc, double *gc11, 
     VTI_HIP(hipGraphLaunch(plan->exec, st));
     double obj = 0.0;
     for (const auto &W : plan->W)
-        if (!rc) rc = wave_objective(*W, st, &obj);
+        if (!rc) rc = wave_objective(*W, st, nullptr, &obj);
     if (!rc) rc = download_gradient(plan->S, plan->a, gc11, gc13, gc33, st);
     if (rc) return rc;
     *objfunc = 0.5 * obj;
@@ -1186,8 +1198,11 @@ long long grad2d_merged_bytes(const Grad2dPart *p, int n, int nz, int nt, int np
 // objective and gradient vti2d_objfunc_gradient returns for it alone, bit for
 // bit (FP32 outputs, as that driver's), with the snapshots stored in
 // snap_fmt (0 = FP32 as the one-shot loop stores them; 1-3 the 16-bit forms).
+// With overlap, two waves of half the size run at once (run_waves), in
+// memory the estimate of grad2d_merged_bytes does not cover; the request
+// server runs merged requests without it.
 int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet, int nz, int nt, int npad,
-                  float dx, float dz, float dt, float rate, int max_wave, int snap_fmt)
+                  float dx, float dz, float dt, float rate, int max_wave, int snap_fmt, bool overlap)
 {
     VTI_CHECK(ctx && p && n >= 1 && wavelet, "grad2d_merged: bad arguments");
     const int nnz = nz + 2 * npad;
@@ -1238,23 +1253,24 @@ int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet
                     VTI_HIP(hipMemcpy2DAsync(S.C[k].p + off[r], (size_t)NNX * 4, p[r].c[k], (size_t)w * 4,
                                              (size_t)w * 4, nnz, hipMemcpyDefault, st));
         }
-        const size_t mfree = vti::device_free_bytes();
-        const long long budget = (long long)mfree - (long long)(2ULL << 30);
-        // waves of at most max_wave shots, fewer when the cache model asks
+        // at most max_wave shots, fewer when the cache model asks; with
+        // overlap, two waves of half as many in flight (run_waves)
         const int cw = cache_wave(shots.data(), ns, npad, nz);
         const int wave = max_wave > 0 ? std::min(max_wave, cw) : cw;
-        for (const auto &wv : split_waves(a, wave, budget)) {
-            BatchWave W;
-            rc = setup_wave(W, a, wv.first, wv.second, data.data(), rl.data(), st);
-            if (!rc) rc = enqueue_wave(W, S, a, wavelet, st, true);
-            if (rc) return rc;
-            std::vector<double> hp(W.part.n);
-            VTI_HIP(hipMemcpyAsync(hp.data(), W.part.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, st));
-            VTI_HIP(hipStreamSynchronize(st));
-            for (int j = 0; j < W.nw; ++j)
-                if (W.sb[j].nr)
-                    for (int b = 0; b < W.pblk[j]; ++b) obj[owner[wv.first + j]] += hp[W.poff[j] + b];
-        }
+        const size_t mfree = vti::device_free_bytes();
+        const long long budget = ((long long)mfree - (long long)(2ULL << 30)) / (overlap ? 2 : 1);
+        if (overlap)
+            rc = run_waves(S, a, split_waves(a, (wave + 1) / 2, budget), data.data(), rl.data(), wavelet, st,
+                           owner.data(), obj.data());
+        else
+            for (const auto &wv : split_waves(a, wave, budget)) {
+                BatchWave W;
+                rc = setup_wave(W, a, wv.first, wv.second, data.data(), rl.data(), st);
+                if (!rc) rc = enqueue_wave(W, S, a, wavelet, st, true);
+                if (!rc) rc = wave_objective(W, st, owner.data(), obj.data());
+                if (rc) break;
+            }
+        if (rc) return rc;
         // rounded to FP32 on the device
         const size_t g = (size_t)NNX * nnz;
         DBuf<float> G32;

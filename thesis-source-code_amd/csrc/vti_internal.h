@@ -76,7 +76,7 @@ int synth2d_batched(vti_ctx *ctx, float *out, const float *c11, const float *c13
 long long grad2d_merged_bytes(const Grad2dPart *p, int n, int nz, int nt, int npad, float dt, float rate,
                               int max_wave);
 int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet, int nz, int nt, int npad,
-                  float dx, float dz, float dt, float rate, int max_wave, int snap_fmt);
+                  float dx, float dz, float dt, float rate, int max_wave, int snap_fmt, bool overlap = false);
 
 int dmalloc(void **p, size_t bytes, hipStream_t s);
 // free device memory of the current device, counting the large blocks
