@@ -44,7 +44,7 @@ int vti_ctx_set_arith(vti_ctx *ctx, int arith);
  * and its v/eps/del wrapper) on ctx: 0 = FP32 (default, the reference's
  * storage, objFuncGradient_f.cu:188-202), 1 = FP16, 2 = BF16 with a
  * power-of-two scale per slice, 3 = S16 (16-bit integers with a power-of-two
- * scale per 64 consecutive points).  A 16-bit format applies to calls whose
+ * scale per 64-point piece of a grid row).  A 16-bit format applies to calls whose
  * arguments the batched driver takes (the init_abc taper, ot = 0, rate/dt an
  * integer >= 2 dividing nt-1, sources and receivers in the interior); those
  * run through it, also for one shot, with half the snapshot memory.  Other
@@ -313,7 +313,7 @@ int vti2d_gn_hessian_vepsdel(vti_ctx *ctx, float *gv, float *geps, float *gdel, 
  * the result.  snap_fmt: 0 keeps the imaging snapshots in FP32 (the result
  * then equals the one-shot driver's bit for bit); 1 (FP16) and 2 (BF16)
  * store each snapshot slice scaled by a power of two, 3 (S16) 16-bit
- * integers with a power-of-two scale per 64 points, halving their memory;
+ * integers with a power-of-two scale per 64-point row piece, halving their memory;
  * the gradient then differs at the level the format's rounding sets.  The
  * wave split changes no bit of the result.
  * No counterpart in the reference, which runs one shot per call. */

@@ -152,7 +152,7 @@ __global__ void clear_slot0_kernel(float *__restrict__ snapf, unsigned short *__
         return;
     }
     for (long long k = t; k < 2 * plane; k += stride) snaph[sh.snap_off + k] = 0;
-    const long long nsc = fmt == 3 ? 2 * ((plane + 63) >> 6) : 2;
+    const long long nsc = fmt == 3 ? 2 * vti::s16_nseg(sh.nnx, nnz) : 2;
     for (long long k = t; k < nsc; k += stride) scale[sh.scale_off + k] = 0.f;
 }
 
@@ -170,11 +170,13 @@ struct BatchArgs {
     long long shot_bytes(int s) const
     {
         // 8 wavefield planes, 3 gradient planes, the receiver map, the
-        // snapshots (and for 16-bit ones two staging planes and the slice
-        // scales/maxima, or the segment scales of S16) and two trace copies
+        // snapshots (and for FP16/BF16 two staging planes and the slice
+        // scales/maxima, for S16 the segment scales) and two trace copies
         const long long nxz = (long long)(in[s].nx + 2 * npad) * nnz;
-        const long long scaleb = snap_fmt == 3 ? 4LL * 2 * nnt * ((nxz + 63) / 64) : 16LL * nnt;
-        const long long snapb = half ? 2LL * nxz * 2 * nnt + 4LL * nxz * 2 + scaleb : 4LL * nxz * 2 * nnt;
+        const long long snapb = !half ? 4LL * nxz * 2 * nnt
+                                : snap_fmt == 3
+                                    ? 2LL * nxz * 2 * nnt + 4LL * 2 * nnt * vti::s16_nseg(in[s].nx + 2 * npad, nnz)
+                                    : 2LL * nxz * 2 * nnt + 4LL * nxz * 2 + 16LL * nnt;
         return 4LL * nxz * (8 + 3 + 1) + snapb + 8LL * nnt_data * in[s].nr + 64;
     }
 };
@@ -350,14 +352,15 @@ void size_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const int *rloc
         W.max_plane = std::max(W.max_plane, plane);
         W.f_total += plane;
         W.snap_total += (long long)nnt * 2 * plane;
-        W.scale_total += (long long)nnt * 2 * ((plane + 63) / 64);
+        W.scale_total += (long long)nnt * 2 * vti::s16_nseg(b.nnx, nnz);
         W.tr_total += (long long)a.nnt_data * q.nr;
         W.nrec += q.nr;
     }
 }
 
 // Device buffers that waves run one after another can share: wavefields,
-// gradients, snapshots (with their staging planes, scales and maxima) and
+// gradients, snapshots (with their scales, and for FP16/BF16 the staging
+// planes and slice maxima) and
 // recorded traces, sized for the largest wave.  enqueue_wave clears
 // whatever a wave reads before writing it, so a captured sequence of waves
 // runs them all through one set.
@@ -384,7 +387,8 @@ void pool_fit(WavePool &pool, const BatchWave &W, const BatchArgs &a)
 
 long long pool_bytes(const WavePool &pool, const BatchArgs &a)
 {
-    const long long snapb = a.half ? 2 * pool.snap + 4 * 2 * pool.f + 4 * pool.scale + 4 * pool.amax : 4 * pool.snap;
+    const long long stage = a.snap_fmt == 3 ? 0 : 4 * 2 * pool.f;
+    const long long snapb = a.half ? 2 * pool.snap + stage + 4 * pool.scale + 4 * pool.amax : 4 * pool.snap;
     return 4 * pool.f * 11 + snapb + 4 * pool.tr;
 }
 
@@ -398,7 +402,7 @@ int pool_alloc(WavePool &pool, const BatchArgs &a, hipStream_t st)
     for (int k = 0; k < 3 && !rc; ++k) rc = pool.G[k].alloc((size_t)pool.f, st, false);
     if (a.half) {
         if (!rc) rc = pool.snaph.alloc((size_t)pool.snap, st, false);
-        if (!rc) rc = pool.stage.alloc((size_t)(2 * pool.f), st, false);
+        if (!rc && a.snap_fmt != 3) rc = pool.stage.alloc((size_t)(2 * pool.f), st, false);
         if (!rc) rc = pool.scale_b.alloc((size_t)pool.scale, st, false);
         if (!rc) rc = pool.amax_b.alloc((size_t)pool.amax, st, false);
     } else {
@@ -456,7 +460,7 @@ int fill_wave(BatchWave &W, const BatchArgs &a, const float *data, const int *rl
         for (int k = 0; k < 3; ++k) W.G[k].borrow(pool->G[k], (size_t)f_total);
         if (a.half) {
             W.snaph.borrow(pool->snaph, (size_t)W.snap_total);
-            W.stage.borrow(pool->stage, (size_t)(2 * f_total));
+            if (a.snap_fmt != 3) W.stage.borrow(pool->stage, (size_t)(2 * f_total));
             W.scale.borrow(pool->scale_b, (size_t)wave_scale_count(W, a));
             W.amax.borrow(pool->amax_b, (size_t)(2 * nnt * nw));
         } else {
@@ -471,7 +475,7 @@ int fill_wave(BatchWave &W, const BatchArgs &a, const float *data, const int *rl
         for (int k = 0; k < 3 && !rc; ++k) rc = W.G[k].alloc((size_t)f_total, st);
         if (a.half) {
             if (!rc) rc = W.snaph.alloc((size_t)W.snap_total, st, false);
-            if (!rc) rc = W.stage.alloc((size_t)(2 * f_total), st);
+            if (!rc && a.snap_fmt != 3) rc = W.stage.alloc((size_t)(2 * f_total), st);
             if (!rc) rc = W.scale.alloc((size_t)wave_scale_count(W, a), st);
             if (!rc) rc = W.amax.alloc((size_t)(2 * nnt * nw), st);
         } else {
@@ -572,10 +576,11 @@ int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *
     for (int k = 0; k < 3; ++k) zero_words(G[k].p, (size_t)f_total, st);
     zero_words(W.tr.p, W.tr.n, st);  // row 0 is never recorded
     if (half && snap_fmt != 3) zero_words(W.amax.p, W.amax.n, st);
-    // the forward kernels stage D at interior points only; the rest of the
-    // staging planes stays 0, so that an S16 segment's scale comes from the
-    // shot's own values (waves of a captured batch share these planes)
-    if (half) zero_words(W.stage.p, (size_t)(2 * f_total), st);
+    // FP16/BF16: the forward kernels stage D at interior points only; the
+    // rest of the staging planes stays 0 (waves of a captured batch share
+    // these planes).  S16 has no staging: the kernels write its 16-bit
+    // values and segment scales themselves.
+    if (half && snap_fmt != 3) zero_words(W.stage.p, (size_t)(2 * f_total), st);
     hipLaunchKernelGGL(clear_slot0_kernel, dim3((unsigned)std::min<long long>((2 * W.max_plane + 255) / 256, 1024),
                                                 (unsigned)nw),
                        dim3(256), 0, st, W.snapf.p, W.snaph.p, W.scale.p, W.sbd.p, nnz, snap_fmt);
@@ -590,19 +595,14 @@ int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *
     const dim3 grid((unsigned)W.tile_shot.size()), block(64, kBY2);
     const dim3 gridt((unsigned)W.tile_shot_t.size()), blockt(64, BYT);
     const int rb = (nrec + 255) / 256;
-    const vti::SnapW sw{half ? W.stage.p : W.snapf.p, half && snap_fmt != 3 ? W.amax.p : nullptr, half};
+    vti::SnapW sw{half ? W.stage.p : W.snapf.p, half && snap_fmt != 3 ? W.amax.p : nullptr, half};
+    if (snap_fmt == 3) sw = vti::SnapW{nullptr, nullptr, false, W.snaph.p, W.scale.p, true};
     const vti::SnapR sr{W.snapf.p, W.snaph.p, W.scale.p};
     const dim3 gpack((unsigned)std::min<long long>((2 * W.max_plane + 255) / 256, 1024), (unsigned)nw);
-    // S16: one wavefront per 64-point segment of either component
-    const dim3 gpack16((unsigned)std::min<long long>((2 * ((W.max_plane + 63) / 64) + 3) / 4, 2048), (unsigned)nw);
-    auto pack = [&](int slot) {
-        if (!half || slot < 0) return;
-        if (snap_fmt == 3)
-            hipLaunchKernelGGL(vti::pack_s16_kernel, gpack16, dim3(256), 0, st, W.snaph.p, W.scale.p, W.stage.p,
-                               W.sbd.p, nnz, slot);
-        else
-            hipLaunchKernelGGL(vti::pack_snap_kernel, gpack, dim3(256), 0, st, W.snaph.p, W.scale.p, W.stage.p,
-                               W.amax.p, W.sbd.p, nnz, slot, snap_fmt == 2 ? 1 : 0);
+    auto pack = [&](int slot) {  // FP16/BF16: the staged slot, once its slice maxima are known
+        if (!half || snap_fmt == 3 || slot < 0) return;
+        hipLaunchKernelGGL(vti::pack_snap_kernel, gpack, dim3(256), 0, st, W.snaph.p, W.scale.p, W.stage.p,
+                           W.amax.p, W.sbd.p, nnz, slot, snap_fmt == 2 ? 1 : 0);
     };
     auto record = [&](int it, int buf) {
         if (it % ratio == 0 && nrec)

@@ -76,7 +76,11 @@ struct BatchGeo {
 struct SnapW {
     float *d;        // FP32 slots (staged == false) or the staging planes
     unsigned *amax;  // per-slice maxima of |D| as float bits (FP16/BF16), or nullptr
-    bool staged;     // 16-bit formats: D goes to the staging planes
+    bool staged;     // FP16/BF16: D goes to the staging planes
+    // S16: the kernels round D to 16 bits themselves (snap_put_s16)
+    unsigned short *h = nullptr;
+    float *scale = nullptr;
+    bool s16 = false;
 };
 
 // Where the adjoint kernels read them.
@@ -86,13 +90,41 @@ struct SnapR {
     const float *scale;       // per-slice (FP16/BF16) or per-segment (S16) scales of the 16-bit slots
 };
 
-// S16: segments of a component plane of `plane` points, and the index of the
-// scale of point i's segment in SnapR::scale
-__device__ __forceinline__ long long s16_nseg(long long plane) { return (plane + 63) >> 6; }
-__device__ __forceinline__ long long s16_scale_idx(const ShotB &sh, long long plane, int slot, int c, long long i)
+// S16 segments: each row z of a component plane nnx wide is cut into the
+// 64-point pieces [4 + 64 s, 68 + 64 s), the rows of the kernels' 64-wide
+// tiles, so that one wavefront holds a whole segment.  s16_seg is the
+// segment of point (x, z) (x >= 4) within its plane, s16_nseg the segments
+// of a plane (every row counted; rows outside the interior are not used).
+__host__ __device__ __forceinline__ long long s16_nseg(int nnx, int nnz)
 {
-    return sh.scale_off + (2LL * slot + c) * s16_nseg(plane) + (i >> 6);
+    return (long long)nnz * ((nnx + 59) >> 6);
 }
+__device__ __forceinline__ long long s16_seg(int nnx, int x, int z)
+{
+    return (long long)z * ((nnx + 59) >> 6) + ((x - 4) >> 6);
+}
+__device__ __forceinline__ long long s16_scale_idx(const ShotB &sh, int nnz, int slot, int c, long long seg)
+{
+    return sh.scale_off + (2LL * slot + c) * s16_nseg(sh.nnx, nnz) + seg;
+}
+
+// S16 scale exponent of a segment whose largest magnitude is m: the segment
+// / 2^e lies in [2^14, 2^15); 0 (scale 1) for an all-zero segment
+__device__ __forceinline__ int s16_exp(float m)
+{
+    int e = 0;
+    if (m > 0.f) {
+        (void)frexpf(m, &e);  // m in [2^(e-1), 2^e)
+        e = max(e - 15, -126);
+    }
+    return e;
+}
+// v / 2^e rounded to nearest even, clamped to +-32767 (m itself can round to 2^15)
+__device__ __forceinline__ unsigned short s16_q(float v, int e)
+{
+    return (unsigned short)(short)fminf(fmaxf(rintf(ldexpf(v, -e)), -32767.f), 32767.f);
+}
+
 
 // taper of the point (ix, iz) of a sub-grid nnx wide: t0 along x, then along z
 // The taper of point (ix, iz) as two loaded factors whose product it is:
@@ -133,6 +165,35 @@ __device__ __forceinline__ TapF taper_ld32(const float *__restrict__ t0, int npa
     const int dz = iz < npad ? iz : (iz >= nnz - npad ? nnz - 1 - iz : npad);
     return {ld32(t0, (unsigned)dx), ld32(t0, (unsigned)dz)};
 }
+// S16: D of the wavefront's 64 points of row z, x = x0 + lane (one
+// segment), into slot `slot`: the segment's scales from the wave's largest
+// magnitudes, then the 16-bit values of the interior points (on).  Every
+// lane of the wave calls this, with tp = tq = 0 where !on; z is the same for
+// the whole wave, and a row outside the interior writes nothing.
+__device__ __forceinline__ void snap_put_s16(const SnapW &w, const ShotB &sh, int nnz, int slot, int x, int z,
+                                             bool on, float tp, float tq)
+{
+    float mp = on ? fabsf(tp) : 0.f, mq = on ? fabsf(tq) : 0.f;
+    for (int o = 32; o > 0; o >>= 1) {
+        mp = fmaxf(mp, __shfl_xor(mp, o, 64));
+        mq = fmaxf(mq, __shfl_xor(mq, o, 64));
+    }
+    if (z >= nnz - 4) return;
+    const int ep = s16_exp(mp), eq = s16_exp(mq);
+    const long long sg = s16_seg(sh.nnx, x, z);
+    if ((threadIdx.x & 63) == 0) {
+        w.scale[s16_scale_idx(sh, nnz, slot, 0, sg)] = ldexpf(1.f, ep);
+        w.scale[s16_scale_idx(sh, nnz, slot, 1, sg)] = ldexpf(1.f, eq);
+    }
+    if (on) {
+        const long long plane = (long long)sh.nnx * nnz;
+        unsigned short *o = w.h + sh.snap_off + (long long)slot * 2 * plane;
+        const unsigned i = (unsigned)(x + z * sh.nnx);
+        st32(o, i, s16_q(tp, ep));
+        st32(o + plane, i, s16_q(tq, eq));
+    }
+}
+
 // the snapshot plane of component c at a slot, and its element i widened
 template <int SF>
 __device__ __forceinline__ float snap_raw32(const SnapR &r, const ShotB &sh, long long plane, int slot, int c,
@@ -200,23 +261,34 @@ __device__ __forceinline__ float snap_raw(const SnapR &r, const ShotB &sh, long 
     if (SF == 3) return (float)(short)r.h[k];
     return __uint_as_float((unsigned)r.h[k] << 16);
 }
-template <int SF>
-__device__ __forceinline__ float snap_mul(const SnapR &r, const ShotB &sh, long long plane, int slot, int c,
-                                          long long i, float raw)
+// A scale read through the constant address space: the scales are not
+// written while an adjoint kernel runs, so a wave-uniform index becomes one
+// scalar load instead of a vector load per lane.
+__device__ __forceinline__ float scale_ld(const float *scale, long long k)
 {
-    if (SF == 0) return raw;
-    if (SF == 3) return raw * r.scale[s16_scale_idx(sh, plane, slot, c, i)];
-    return raw * r.scale[sh.scale_off + 2 * slot + c];
+    return ((const __attribute__((address_space(4))) float *)scale)[k];
 }
 
-// D component c of point i at a slot
+// (sg: the point's S16 segment, s16_seg, uniform across the wave for a
+// scalar load of its scale)
 template <int SF>
-__device__ __forceinline__ float snap_get(const SnapR &r, const ShotB &sh, long long plane, int slot, int c,
-                                          long long i)
+__device__ __forceinline__ float snap_mul(const SnapR &r, const ShotB &sh, int nnz, int slot, int c, long long sg,
+                                          float raw)
 {
+    if (SF == 0) return raw;
+    if (SF == 3) return raw * scale_ld(r.scale, s16_scale_idx(sh, nnz, slot, c, sg));
+    return raw * scale_ld(r.scale, sh.scale_off + 2 * slot + c);
+}
+
+// D component c of point i (S16 segment sg) at a slot
+template <int SF>
+__device__ __forceinline__ float snap_get(const SnapR &r, const ShotB &sh, int nnz, int slot, int c, long long i,
+                                          long long sg)
+{
+    const long long plane = (long long)sh.nnx * nnz;
     const long long k = sh.snap_off + (long long)slot * 2 * plane + c * plane + i;
     if (SF == 0) return r.f[k];
-    if (SF == 3) return (float)(short)r.h[k] * r.scale[s16_scale_idx(sh, plane, slot, c, i)];
+    if (SF == 3) return (float)(short)r.h[k] * r.scale[s16_scale_idx(sh, nnz, slot, c, sg)];
     const float s = r.scale[sh.scale_off + 2 * slot + c];
     if (SF == 1) return __half2float(__ushort_as_half(r.h[k])) * s;
     return __uint_as_float((unsigned)r.h[k] << 16) * s;
@@ -256,43 +328,6 @@ static __global__ void pack_snap_kernel(unsigned short *__restrict__ snap, float
             out[k] = (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
         } else {
             out[k] = __half_as_ushort(__float2half_rn(v));
-        }
-    }
-}
-
-// S16: round the staged slices of slot `slot` of every shot (blockIdx.y) to
-// 16-bit integers, one wavefront per segment of 64 consecutive points of a
-// component plane.  The segment's scale 2^e puts its largest magnitude m in
-// [2^14, 2^15) (1 for an all-zero segment); values are v / 2^e rounded to
-// nearest even, clamped to +-32767 (m itself can round to 2^15).
-static __global__ __launch_bounds__(256) void pack_s16_kernel(unsigned short *__restrict__ snap,
-                                                              float *__restrict__ scale,
-                                                              const float *__restrict__ stage,
-                                                              const ShotB *__restrict__ shots, int nnz, int slot)
-{
-    const ShotB sh = shots[blockIdx.y];
-    const long long plane = (long long)sh.nnx * nnz, nseg = s16_nseg(plane);
-    const int lane = threadIdx.x & 63;
-    const long long nwaves = (long long)gridDim.x * (blockDim.x >> 6);
-    unsigned short *out = snap + sh.snap_off + (long long)slot * 2 * plane;
-    const float *in = stage + sh.stage_off;
-    // segments of both components: g < nseg is component 0, else 1
-    for (long long g = blockIdx.x * (long long)(blockDim.x >> 6) + (threadIdx.x >> 6); g < 2 * nseg; g += nwaves) {
-        const int c = g < nseg ? 0 : 1;
-        const long long i = ((c ? g - nseg : g) << 6) + lane;  // point within the component plane
-        const bool ok = i < plane;
-        const float v = ok ? in[c * plane + i] : 0.f;
-        float m = fabsf(v);
-        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-        int e = 0;
-        if (m > 0.f) {
-            (void)frexpf(m, &e);  // m in [2^(e-1), 2^e)
-            e = max(e - 15, -126);
-        }
-        if (lane == 0) scale[sh.scale_off + (2LL * slot) * nseg + g] = ldexpf(1.f, e);
-        if (ok) {
-            const float q = fminf(fmaxf(rintf(ldexpf(v, -e)), -32767.f), 32767.f);
-            out[c * plane + i] = (unsigned short)(short)q;
         }
     }
 }
@@ -383,11 +418,13 @@ static __global__ __launch_bounds__(64 * kBY2) void fwd2db_kernel(
     }
     load_tiles_b<0>(t, P_c + sh.f_off, Q_c + sh.f_off, nullptr, nullptr, nullptr, nnx, nnz, 0, x0, z0, pc, qc);
     __syncthreads();
-    float mp = 0.f, mq = 0.f;
+    float mp = 0.f, mq = 0.f, dp = 0.f, dq = 0.f;
     if (act) {
         const float tap = taper_of(tw);
         const float tp = dxx(t, dx2, bg.rx2), tq = dzz(t, dz2, bg.rz2);
-        if (slot >= 0) {
+        dp = tp;
+        dq = tq;
+        if (slot >= 0 && !sw.s16) {
             snap_put(sw, sh, (long long)nnx * nnz, slot, i, tp, tq);
             mp = fabsf(tp);
             mq = fabsf(tq);
@@ -401,6 +438,7 @@ static __global__ __launch_bounds__(64 * kBY2) void fwd2db_kernel(
         p_n[i] = pn * tap;
         q_n[i] = qn * tap;
     }
+    if (sw.s16 && slot >= 0) snap_put_s16(sw, sh, nnz, slot, ix, iz, act, dp, dq);
     if (sw.amax && slot >= 0) snap_amax(sw, sh, slot, mp, mq, red);
 }
 
@@ -422,14 +460,17 @@ static __global__ __launch_bounds__(64 * kBY2) void dsnap2db_kernel(const float 
     load_tiles_b<0>(t, P + sh.f_off, Q + sh.f_off, nullptr, nullptr, nullptr, nnx, nnz, 0, x0, z0, pc, qc);
     __syncthreads();
     const int ix = x0 + threadIdx.x, iz = z0 + threadIdx.y;
-    float mp = 0.f, mq = 0.f;
-    if (ix < nnx - 4 && iz < nnz - 4) {
+    const bool act = ix < nnx - 4 && iz < nnz - 4;
+    float mp = 0.f, mq = 0.f, dp = 0.f, dq = 0.f;
+    if (act) {
         const long long i = ix + (long long)iz * nnx;
-        const float tp = dxx(t, dx2, bg.rx2), tq = dzz(t, dz2, bg.rz2);
-        snap_put(sw, sh, (long long)nnx * nnz, slot, i, tp, tq);
-        mp = fabsf(tp);
-        mq = fabsf(tq);
+        dp = dxx(t, dx2, bg.rx2);
+        dq = dzz(t, dz2, bg.rz2);
+        if (!sw.s16) snap_put(sw, sh, (long long)nnx * nnz, slot, i, dp, dq);
+        mp = fabsf(dp);
+        mq = fabsf(dq);
     }
+    if (sw.s16) snap_put_s16(sw, sh, nnz, slot, ix, iz, act, dp, dq);
     if (sw.amax) snap_amax(sw, sh, slot, mp, mq, red);
 }
 
@@ -484,10 +525,11 @@ static __global__ __launch_bounds__(64 * kBY2) void bwd2db_kernel(
     __syncthreads();
     if (!act) return;
     const float tap = taper_of(tw);
-    d0p = snap_mul<SF>(sr, sh, plane, slot0, 0, i, d0p);
-    d1p = snap_mul<SF>(sr, sh, plane, slot0 + 1, 0, i, d1p);
-    d0q = snap_mul<SF>(sr, sh, plane, slot0, 1, i, d0q);
-    d1q = snap_mul<SF>(sr, sh, plane, slot0 + 1, 1, i, d1q);
+    const long long sg = s16_seg(nnx, ix, iz);
+    d0p = snap_mul<SF>(sr, sh, nnz, slot0, 0, sg, d0p);
+    d1p = snap_mul<SF>(sr, sh, nnz, slot0 + 1, 0, sg, d1p);
+    d0q = snap_mul<SF>(sr, sh, nnz, slot0, 1, sg, d0q);
+    d1q = snap_mul<SF>(sr, sh, nnz, slot0 + 1, 1, sg, d1q);
     const float tp0 = dxx(t, dx2, bg.rx2), tq0 = dzz(t, dz2, bg.rz2);
     float pn = 2.f * pc + dt2 * tp0 - pp * tap;
     float qn = 2.f * qc + dt2 * tq0 - qp * tap;
@@ -519,12 +561,11 @@ static __global__ void xcor2db_kernel(const float *__restrict__ P, const float *
     const int ix = 4 + (tl % sh.ntx) * 64 + threadIdx.x, iz = 4 + (tl / sh.ntx) * kBY2 + threadIdx.y;
     const int nnx = sh.nnx, nnz = bg.nnz;
     if (ix >= nnx - 4 || iz >= nnz - 4) return;
-    const long long i = ix + (long long)iz * nnx, k = sh.f_off + i;
-    const long long plane = (long long)nnx * nnz;
-    const float Dp = (1.f - f) * snap_get<SF>(sr, sh, plane, slot0, 0, i) +
-                     f * snap_get<SF>(sr, sh, plane, slot0 + 1, 0, i);
-    const float Dq = (1.f - f) * snap_get<SF>(sr, sh, plane, slot0, 1, i) +
-                     f * snap_get<SF>(sr, sh, plane, slot0 + 1, 1, i);
+    const long long i = ix + (long long)iz * nnx, k = sh.f_off + i, sg = s16_seg(nnx, ix, iz);
+    const float Dp = (1.f - f) * snap_get<SF>(sr, sh, nnz, slot0, 0, i, sg) +
+                     f * snap_get<SF>(sr, sh, nnz, slot0 + 1, 0, i, sg);
+    const float Dq = (1.f - f) * snap_get<SF>(sr, sh, nnz, slot0, 1, i, sg) +
+                     f * snap_get<SF>(sr, sh, nnz, slot0 + 1, 1, i, sg);
     G11[k] += P[k] * Dp;
     G13[k] += P[k] * Dq + Q[k] * Dp;
     G33[k] += Q[k] * Dq;
@@ -765,7 +806,7 @@ __device__ __forceinline__ void fwd2db_t2_body(
                 const int i = xx + zz * nnx;
                 const float tp = qdiv(st9_row(t.a[rz], rx + 4), dx2, bg.rx2);
                 const float tq = qdiv(st9_col<72>(t.b, rz + 4, rx), dz2, bg.rz2);
-                if (slot1 >= 0 && rx >= 4 && rx < 68 && rz >= 4 && rz < BY + 4) {
+                if (slot1 >= 0 && !sw.s16 && rx >= 4 && rx < 68 && rz >= 4 && rz < BY + 4) {
                     snap_put32(sw, sh, plane, slot1, i, tp, tq);
                     mp = fmaxf(mp, fabsf(tp));
                     mq = fmaxf(mq, fabsf(tq));
@@ -789,13 +830,16 @@ __device__ __forceinline__ void fwd2db_t2_body(
     // level L+2 on the tile
     mp = 0.f;
     mq = 0.f;
+    float d2p = 0.f, d2q = 0.f;
     if (act) {
         const float tap = taper_of(tw);
         const int rx = threadIdx.x + 4, rz = threadIdx.y + 4;
         const int i = x + z * nnx;
         const float tp = qdiv(st9_row(t.p1[rz], rx), dx2, bg.rx2);
         const float tq = qdiv(st9_col<72>(t.q1, rz, rx), dz2, bg.rz2);
-        if (slot2 >= 0) {
+        d2p = tp;
+        d2q = tq;
+        if (slot2 >= 0 && !sw.s16) {
             snap_put32(sw, sh, plane, slot2, i, tp, tq);
             mp = fabsf(tp);
             mq = fabsf(tq);
@@ -811,6 +855,20 @@ __device__ __forceinline__ void fwd2db_t2_body(
         st32(Qc, i, q1);
         st32(Pd, i, pn * tap);
         st32(Qd, i, qn * tap);
+    }
+    if (sw.s16) {
+        // S16 by tile rows, one wavefront each: D of level L at the tile
+        // point again from the level-L planes in LDS (the same operations as
+        // on R1 above, so the same bits), and D of level L+1 from above
+        if (slot1 >= 0) {
+            float dp = 0.f, dq = 0.f;
+            if (act) {
+                dp = qdiv(st9_row(t.a[threadIdx.y + 4], threadIdx.x + 8), dx2, bg.rx2);
+                dq = qdiv(st9_col<72>(t.b, threadIdx.y + 8, threadIdx.x + 4), dz2, bg.rz2);
+            }
+            snap_put_s16(sw, sh, nnz, slot1, x, z, act, dp, dq);
+        }
+        if (slot2 >= 0) snap_put_s16(sw, sh, nnz, slot2, x, z, act, d2p, d2q);
     }
     if (sw.amax && slot2 >= 0) snap_amax(sw, sh, slot2, mp, mq, red);
 }
@@ -1007,20 +1065,24 @@ __device__ __forceinline__ void bwd2db_t2_body(
     st32(Qc, i, q1);
     st32(Pd, i, pn);
     st32(Qd, i, qn);
-    if (same) {  // the raw values, scaled below like the others
+    // the segment of the wavefront's row (one segment: x0 - 4 is a multiple
+    // of 64), uniform across the wave
+    const long long sg = __builtin_amdgcn_readfirstlane((int)s16_seg(nnx, x0, zo));
+    a0p = snap_mul<SF>(sr, sh, nnz, l1.slot, 0, sg, a0p);
+    a1p = snap_mul<SF>(sr, sh, nnz, l1.slot + 1, 0, sg, a1p);
+    a0q = snap_mul<SF>(sr, sh, nnz, l1.slot, 1, sg, a0q);
+    a1q = snap_mul<SF>(sr, sh, nnz, l1.slot + 1, 1, sg, a1q);
+    if (same) {  // the same slots, already scaled
         b0p = a0p;
         b1p = a1p;
         b0q = a0q;
         b1q = a1q;
+    } else {
+        b0p = snap_mul<SF>(sr, sh, nnz, l2.slot, 0, sg, b0p);
+        b1p = snap_mul<SF>(sr, sh, nnz, l2.slot + 1, 0, sg, b1p);
+        b0q = snap_mul<SF>(sr, sh, nnz, l2.slot, 1, sg, b0q);
+        b1q = snap_mul<SF>(sr, sh, nnz, l2.slot + 1, 1, sg, b1q);
     }
-    a0p = snap_mul<SF>(sr, sh, plane, l1.slot, 0, i, a0p);
-    a1p = snap_mul<SF>(sr, sh, plane, l1.slot + 1, 0, i, a1p);
-    a0q = snap_mul<SF>(sr, sh, plane, l1.slot, 1, i, a0q);
-    a1q = snap_mul<SF>(sr, sh, plane, l1.slot + 1, 1, i, a1q);
-    b0p = snap_mul<SF>(sr, sh, plane, l2.slot, 0, i, b0p);
-    b1p = snap_mul<SF>(sr, sh, plane, l2.slot + 1, 0, i, b1p);
-    b0q = snap_mul<SF>(sr, sh, plane, l2.slot, 1, i, b0q);
-    b1q = snap_mul<SF>(sr, sh, plane, l2.slot + 1, 1, i, b1q);
     const float Dp1 = (1.f - l1.f) * a0p + l1.f * a1p;
     const float Dq1 = (1.f - l1.f) * a0q + l1.f * a1q;
     const float Dp2 = (1.f - l2.f) * b0p + l2.f * b1p;
