@@ -326,17 +326,20 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
 /* The batched gradient captured once and replayed: for the iterations of an
  * inversion, which evaluate the same shots (geometry, observed data, wavelet)
  * on a new model each time.  vti2d_batch_plan_create checks the arguments as
- * vti2d_objfunc_gradient_batch does, splits the shots into waves (of at
- * most max_wave shots when > 0, and within the free device memory, as the
- * direct call does), allocates one set of wavefields, snapshots and trace
- * buffers sized for the largest wave, which the waves use one after another,
- * uploads each wave's shot layout, receiver maps and observed data (an error
- * when the whole does not fit the free device memory), and records the call's whole
- * launch sequence -- forward pass, residual, adjoint pass, FP64 accumulation,
- * every buffer clearing done by a kernel -- into a hipGraph on the context's
- * stream.  vti2d_batch_plan_run uploads c11/c13/c33, replays the graph and
- * returns what vti2d_objfunc_gradient_batch returns for the same arguments,
- * bit for bit.  The plan holds its device memory until
+ * vti2d_objfunc_gradient_batch does, splits the shots into waves as the
+ * direct call does (half of max_wave shots each when > 0, within the free
+ * device memory, two in flight), allocates two sets of wavefields,
+ * snapshots and trace buffers (one for the even waves, one for the odd
+ * ones), uploads each wave's shot layout, receiver maps and observed data
+ * (an error when the whole does not fit the free device memory), and
+ * records each wave's launch sequence -- forward pass and residual, then
+ * adjoint pass and FP64 accumulation, every buffer clearing done by a
+ * kernel -- into two hipGraphs on the context's stream.
+ * vti2d_batch_plan_run uploads c11/c13/c33 and replays the graphs, wave k
+ * on the context's stream or (odd k) a second stream the plan holds, each
+ * forward pass beside the previous wave's adjoint, and returns what
+ * vti2d_objfunc_gradient_batch returns for the same arguments, bit for
+ * bit.  The plan holds its device memory until
  * vti2d_batch_plan_destroy, which must come before vti_ctx_destroy.  Runs of
  * one plan, or of plans on one context, are serialised by the caller.
  * No counterpart in the reference, which sets every call up anew. */

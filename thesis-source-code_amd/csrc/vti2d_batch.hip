@@ -529,8 +529,12 @@ int setup_wave(BatchWave &W, const BatchArgs &a, int s0, int s1, const float *da
 // and residual are enqueued; the FP64 accumulation waits for acc_wait (the
 // previous wave's accumulation, so the global sum keeps shot order) and
 // records acc_done.
+// A captured plan passes `split` instead: called where fwd_done would be
+// recorded, it ends the capture of the wave's forward pass and begins that
+// of its adjoint.
 struct WaveSync {
     hipEvent_t fwd_done = nullptr, acc_wait = nullptr, acc_done = nullptr;
+    std::function<int()> split;
 };
 
 // A second stream and two pairs of WaveSync events for one call, released
@@ -664,6 +668,10 @@ int enqueue_wave(BatchWave &W, BatchShared &S, const BatchArgs &a, const float *
     }
 
     if (sync && sync->fwd_done) VTI_HIP(hipEventRecord(sync->fwd_done, st));
+    if (sync && sync->split) {
+        const int rc = sync->split();
+        if (rc) return rc;
+    }
 
     // ---- backward (objFuncGradient_f.cu:225-300 for every shot)
     for (int k = 0; k < 4; ++k) {
@@ -923,17 +931,22 @@ int cache_wave(const int *shots, int ns, int npad, int nz)
 }  // namespace
 
 // A captured batch: each wave's own tables, receiver map and observed data,
-// one set of wavefield/snapshot/trace buffers that the waves share (sized
-// for the largest), and the graph of the whole launch sequence, wave after
-// wave.
+// two sets of wavefield/snapshot/trace buffers (even waves share one, odd
+// waves the other, each sized for its largest wave), and two graphs per
+// wave, its forward pass and its adjoint.  A replay runs wave k on stream
+// k % 2 (the context's, or the plan's second stream) as the direct call
+// does: the forward pass of wave k after that of wave k-1, beside wave
+// k-1's adjoint, and the adjoint of wave k after that of wave k-1, so the
+// FP64 accumulations keep wave order.
 struct vti2d_batch_plan {
     vti_ctx *ctx = nullptr;
     BatchArgs a;
     BatchShared S;
-    WavePool pool;
+    WavePool pool[2];
     std::vector<std::unique_ptr<BatchWave>> W;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    std::vector<hipGraphExec_t> fwd, adj;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fwd[2] = {nullptr, nullptr}, ev_adj[2] = {nullptr, nullptr}, ev_join = nullptr;
 };
 
 extern "C" {
@@ -997,15 +1010,17 @@ int vti2d_batch_plan_create(vti_ctx *ctx, vti2d_batch_plan **plan, const float *
         }
     }
     if (!rc) {
-        // waves within the memory budget, sized on the host; the shared set
-        // fits the largest, and each wave adds its own tables and data
+        // waves within the memory budget, sized on the host, two in flight
+        // as in the direct call (half of max_wave each); each buffer set
+        // fits the largest wave that uses it, and each wave adds its own
+        // tables and data
         const long long budget = (long long)vti::device_free_bytes() - (long long)(2ULL << 30);
-        for (const auto &w : split_waves(pl->a, max_wave, budget)) {
+        for (const auto &w : split_waves(pl->a, max_wave > 0 ? (max_wave + 1) / 2 : 0, budget / 2)) {
             pl->W.emplace_back(new BatchWave);
             size_wave(*pl->W.back(), pl->a, w.first, w.second, rloc);
-            pool_fit(pl->pool, *pl->W.back(), pl->a);
+            pool_fit(pl->pool[(pl->W.size() - 1) % 2], *pl->W.back(), pl->a);
         }
-        long long need = pool_bytes(pl->pool, pl->a);
+        long long need = pool_bytes(pl->pool[0], pl->a) + (pl->W.size() > 1 ? pool_bytes(pl->pool[1], pl->a) : 0);
         for (const auto &W : pl->W)
             need += 4 * W->f_total + 4 * std::max<long long>(W->tr_total, 1) + 20LL * W->nrec +
                     4LL * (long long)(W->tile_shot.size() + W->tile_shot_t.size()) + (1LL << 16);
@@ -1016,19 +1031,47 @@ int vti2d_batch_plan_create(vti_ctx *ctx, vti2d_batch_plan **plan, const float *
         }
     }
     if (!rc) rc = setup_shared(pl->S, pl->a, st);
-    if (!rc) rc = pool_alloc(pl->pool, pl->a, st);
-    for (auto &W : pl->W)
-        if (!rc) rc = fill_wave(*W, pl->a, data, rloc, st, &pl->pool);
-    if (!rc) {
+    for (int k = 0; k < 2 && k < (int)pl->W.size(); ++k)
+        if (!rc) rc = pool_alloc(pl->pool[k], pl->a, st);
+    for (size_t k = 0; k < pl->W.size() && !rc; ++k)
+        rc = fill_wave(*pl->W[k], pl->a, data, rloc, st, &pl->pool[k % 2]);
+    if (!rc && pl->W.size() > 1) {
+        if (hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking) != hipSuccess) rc = vti::kErrHip;
+        for (int k = 0; k < 2 && !rc; ++k)
+            if (hipEventCreateWithFlags(&pl->ev_fwd[k], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&pl->ev_adj[k], hipEventDisableTiming) != hipSuccess)
+                rc = vti::kErrHip;
+        if (!rc && hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming) != hipSuccess) rc = vti::kErrHip;
+        if (rc) vti::set_error("vti2d_batch_plan_create: stream or event creation failed");
+    }
+    // each wave captured on its own, on one stream, as two graphs
+    for (size_t k = 0; k < pl->W.size() && !rc; ++k) {
+        hipGraph_t g[2] = {nullptr, nullptr};
         hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) {
-            for (int k = 0; k < 3; ++k) zero_words(pl->S.A[k].p, 2 * pl->S.A[k].n, st);
-            for (auto &W : pl->W)
-                if (!rc) rc = enqueue_wave(*W, pl->S, pl->a, wavelet, st, true);
-            const hipError_t e2 = hipStreamEndCapture(st, &pl->graph);
-            if (!rc && e2 != hipSuccess) e = e2;
+        bool capturing = e == hipSuccess;
+        WaveSync sync;
+        sync.split = [&]() -> int {
+            capturing = false;
+            hipError_t e1 = hipStreamEndCapture(st, &g[0]);
+            if (e1 == hipSuccess) e1 = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            if (e1 != hipSuccess) {
+                vti::set_error("vti2d_batch_plan_create: graph capture: %s", hipGetErrorString(e1));
+                return vti::kErrHip;
+            }
+            capturing = true;
+            return vti::kOk;
+        };
+        if (capturing) rc = enqueue_wave(*pl->W[k], pl->S, pl->a, wavelet, st, true, &sync);
+        if (capturing) {
+            const hipError_t e2 = hipStreamEndCapture(st, &g[1]);
+            if (e == hipSuccess) e = e2;
         }
-        if (!rc && e == hipSuccess) e = hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0);
+        hipGraphExec_t x[2] = {nullptr, nullptr};
+        for (int j = 0; j < 2 && !rc && e == hipSuccess; ++j) e = hipGraphInstantiate(&x[j], g[j], nullptr, nullptr, 0);
+        for (int j = 0; j < 2; ++j)
+            if (g[j]) (void)hipGraphDestroy(g[j]);
+        pl->fwd.push_back(x[0]);
+        pl->adj.push_back(x[1]);
         if (!rc && e != hipSuccess) {
             vti::set_error("vti2d_batch_plan_create: graph capture: %s", hipGetErrorString(e));
             rc = vti::kErrHip;
@@ -1050,7 +1093,24 @@ int vti2d_batch_plan_run(vti2d_batch_plan *plan, double *objfunc, double *gc11, 
     hipStream_t st = plan->ctx->stream;
     int rc = upload_model(plan->S, plan->a, c11, c13, c33, st);
     if (rc) return rc;
-    VTI_HIP(hipGraphLaunch(plan->exec, st));
+    for (int k = 0; k < 3; ++k) zero_words(plan->S.A[k].p, 2 * plan->S.A[k].n, st);
+    const size_t nw = plan->W.size();
+    if (nw > 1) {  // the second stream starts after the upload
+        VTI_HIP(hipEventRecord(plan->ev_join, st));
+        VTI_HIP(hipStreamWaitEvent(plan->side, plan->ev_join, 0));
+    }
+    for (size_t k = 0; k < nw; ++k) {
+        hipStream_t sk = k % 2 ? plan->side : st;
+        if (k >= 1) VTI_HIP(hipStreamWaitEvent(sk, plan->ev_fwd[(k - 1) % 2], 0));
+        VTI_HIP(hipGraphLaunch(plan->fwd[k], sk));
+        if (nw > 1) VTI_HIP(hipEventRecord(plan->ev_fwd[k % 2], sk));
+        if (k >= 1) VTI_HIP(hipStreamWaitEvent(sk, plan->ev_adj[(k - 1) % 2], 0));
+        VTI_HIP(hipGraphLaunch(plan->adj[k], sk));
+        if (nw > 1) VTI_HIP(hipEventRecord(plan->ev_adj[k % 2], sk));
+    }
+    if (nw > 1) {  // the context's stream waits for the last adjoint
+        VTI_HIP(hipStreamWaitEvent(st, plan->ev_adj[(nw - 1) % 2], 0));
+    }
     double obj = 0.0;
     for (const auto &W : plan->W)
         if (!rc) rc = wave_objective(*W, st, nullptr, &obj);
@@ -1065,8 +1125,17 @@ int vti2d_batch_plan_destroy(vti2d_batch_plan *plan)
     if (!plan) return vti::kOk;
     (void)hipSetDevice(plan->ctx->device);
     (void)hipStreamSynchronize(plan->ctx->stream);  // no replay still running
-    if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
-    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    if (plan->side) (void)hipStreamSynchronize(plan->side);
+    for (auto x : plan->fwd)
+        if (x) (void)hipGraphExecDestroy(x);
+    for (auto x : plan->adj)
+        if (x) (void)hipGraphExecDestroy(x);
+    for (int k = 0; k < 2; ++k) {
+        if (plan->ev_fwd[k]) (void)hipEventDestroy(plan->ev_fwd[k]);
+        if (plan->ev_adj[k]) (void)hipEventDestroy(plan->ev_adj[k]);
+    }
+    if (plan->ev_join) (void)hipEventDestroy(plan->ev_join);
+    if (plan->side) (void)hipStreamDestroy(plan->side);
     delete plan;  // the buffers go back to the allocator on the context's stream
     return vti::kOk;
 }
