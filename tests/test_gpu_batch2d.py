@@ -210,8 +210,9 @@ def test_graph_plan_replays_equal_direct_calls(vti, ctx, snap, levels, max_wave)
     three models, with other drivers' work and synchronisations between the
     replays, gives each time the direct batched call's bits on that model --
     also the third replay, back on the first model.  With max_wave > 0 the
-    plan's waves run one after another through one shared set of wavefield,
-    snapshot and trace buffers."""
+    plan holds several waves, two graphs each, replayed two at a time on two
+    streams through two alternating sets of wavefield, snapshot and trace
+    buffers."""
     B = batch_problem(vti, seed=14, ns=4)
     _say("batch2d plan", snap, "levels", levels, "max_wave", max_wave)
     data = observed(vti, ctx, B)
