@@ -883,33 +883,44 @@ int run_waves(BatchShared &S, const BatchArgs &a, const std::vector<std::pair<in
               double *obj)
 {
     int rc = vti::kOk;
+    const size_t n = waves.size();
     StreamPair sp;
-    if (waves.size() > 1) rc = sp.init(st);
-    std::unique_ptr<BatchWave> prev;
-    hipStream_t prev_st = st;
-    for (size_t k = 0; k < waves.size() && !rc; ++k) {
-        hipStream_t sk = k % 2 ? sp.side : st;
-        auto W = std::make_unique<BatchWave>();
-        // (on sk: waits for wave k-2, which ran there)
-        rc = setup_wave(*W, a, waves[k].first, waves[k].second, data, rloc, sk);
+    if (n > 1) rc = sp.init(st);
+    auto stream_of = [&](size_t k) { return k % 2 ? sp.side : st; };
+    // Wave k+1 is set up (its tables built on the host, then uploaded on its
+    // stream behind wave k-1) while waves k-1 and k run, so that it is
+    // enqueued as soon as wave k-1 is done: three waves hold memory at once.
+    std::unique_ptr<BatchWave> prev, cur;
+    if (!rc && n) {
+        cur = std::make_unique<BatchWave>();
+        rc = setup_wave(*cur, a, waves[0].first, waves[0].second, data, rloc, st);
+    }
+    for (size_t k = 0; k < n && !rc; ++k) {
+        hipStream_t sk = stream_of(k);
         WaveSync sync;
-        if (waves.size() > 1) {
+        if (n > 1) {
             sync.fwd_done = sp.fwd[k % 2];
             sync.acc_done = sp.acc[k % 2];
             if (k >= 1) {
                 sync.acc_wait = sp.acc[(k - 1) % 2];
-                if (!rc && hipStreamWaitEvent(sk, sp.fwd[(k - 1) % 2], 0) != hipSuccess) {
+                if (hipStreamWaitEvent(sk, sp.fwd[(k - 1) % 2], 0) != hipSuccess) {
                     vti::set_error("batched 2D waves: hipStreamWaitEvent failed");
                     rc = vti::kErrHip;
                 }
             }
         }
-        if (!rc) rc = enqueue_wave(*W, S, a, wavelet, sk, true, waves.size() > 1 ? &sync : nullptr);
-        if (!rc && prev) rc = wave_objective(*prev, prev_st, owner, obj);  // wave k-1 done
-        prev = std::move(W);
-        prev_st = sk;
+        if (!rc) rc = enqueue_wave(*cur, S, a, wavelet, sk, true, n > 1 ? &sync : nullptr);
+        std::unique_ptr<BatchWave> next;
+        if (!rc && k + 1 < n) {
+            next = std::make_unique<BatchWave>();
+            // (on stream k+1: waits for wave k-1, which ran there)
+            rc = setup_wave(*next, a, waves[k + 1].first, waves[k + 1].second, data, rloc, stream_of(k + 1));
+        }
+        if (!rc && prev) rc = wave_objective(*prev, stream_of(k - 1), owner, obj);  // wave k-1 done
+        prev = std::move(cur);
+        cur = std::move(next);
     }
-    if (!rc && prev) rc = wave_objective(*prev, prev_st, owner, obj);
+    if (!rc && prev) rc = wave_objective(*prev, stream_of(n - 1), owner, obj);
     return rc;  // (sp's destructor waits for both streams before it releases them)
 }
 
@@ -978,7 +989,7 @@ int vti2d_objfunc_gradient_batch(vti_ctx *ctx, double *objfunc, double *gc11, do
     // against 64.7-65.5 for waves of 7 one after the other (waves of 7
     // overlapped: 43.6-60.2).
     const size_t mfree = vti::device_free_bytes();
-    const long long budget = ((long long)mfree - (long long)(2ULL << 30)) / 2;
+    const long long budget = ((long long)mfree - (long long)(2ULL << 30)) / 3;  // three waves set up at once
     VTI_CHECK(budget > a.shot_bytes(0), "not enough device memory for one shot");
     const auto waves = split_waves(a, max_wave > 0 ? (max_wave + 1) / 2 : 0, budget);
     double obj = 0.0;
@@ -1286,6 +1297,10 @@ int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet
     const int nnt = (nt - 1) / ratio + 1;
     std::vector<int> shots, rl, owner;
     std::vector<float> data;
+    size_t ndata = 0;
+    for (int r = 0; r < n; ++r)
+        for (int k = 0; k < p[r].ns; ++k) ndata += (size_t)nnt * std::max(0, p[r].sloc[4 * k + 2]);
+    data.reserve(ndata);
     for (int r = 0; r < n; ++r)
         for (int k = 0; k < p[r].ns; ++k) {
             const int ix = p[r].sloc[4 * k], iz = p[r].sloc[4 * k + 1], nri = p[r].sloc[4 * k + 2],
@@ -1297,8 +1312,10 @@ int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet
                 rl.push_back(p[r].rloc[2 * (r0 + ir)]);
                 rl.push_back(p[r].rloc[2 * (r0 + ir) + 1]);
             }
-            for (int row = 0; row < nnt; ++row)
-                for (int ir = 0; ir < nri; ++ir) data.push_back(p[r].data[(size_t)row * p[r].nr + r0 + ir]);
+            for (int row = 0; row < nnt; ++row) {  // the shot's columns of each trace row
+                const float *src = p[r].data + (size_t)row * p[r].nr + r0;
+                data.insert(data.end(), src, src + nri);
+            }
             owner.push_back(r);
         }
     const int ns = (int)owner.size();
@@ -1327,7 +1344,7 @@ int grad2d_merged(vti_ctx *ctx, const Grad2dPart *p, int n, const float *wavelet
         const int cw = cache_wave(shots.data(), ns, npad, nz);
         const int wave = max_wave > 0 ? std::min(max_wave, cw) : cw;
         const size_t mfree = vti::device_free_bytes();
-        const long long budget = ((long long)mfree - (long long)(2ULL << 30)) / (overlap ? 2 : 1);
+        const long long budget = ((long long)mfree - (long long)(2ULL << 30)) / (overlap ? 3 : 1);
         if (overlap)
             rc = run_waves(S, a, split_waves(a, (wave + 1) / 2, budget), data.data(), rl.data(), wavelet, st,
                            owner.data(), obj.data());
