@@ -114,7 +114,8 @@ def test_launch_shapes_bit_exact(vti, ctx, levels, rows):
     assert abs(fb - fp) <= 1e-6 * fp
 
 
-@pytest.mark.parametrize("rows,snap", [(4, "fp32"), (8, "fp32"), (16, "fp32"), (16, "s16"), (16, "bf16")])
+@pytest.mark.parametrize("rows,snap", [(4, "fp32"), (8, "fp32"), (16, "fp32"), (4, "s16"), (8, "s16"), (16, "s16"),
+                                       (16, "bf16")])
 def test_inner_tiles_bit_exact(vti, ctx, rows, snap):
     """Apertures wide and deep enough that most two-level tiles lie where the
     taper is 1 (the kernels' inner-tile form: no clamps, bounds tests or
